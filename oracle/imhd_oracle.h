@@ -1,0 +1,64 @@
+/* CPU FP64 restatement of the reference's isothermal-MHD cell-update path
+ * (tsultan16/Isothermal_MHD, 3D_MPI/src).  TEST INFRASTRUCTURE ONLY: the parity tests,
+ * smoke() and bench.py's cpu_baseline leg call it as the checker; the product never
+ * links it.
+ *
+ * Arrays use the reference's layout (grid_data_mod.f90:38-42): Fortran column-major,
+ * SoA by field, ghost-padded with nb layers, i.e. element (i,j,k,f) of q_3d with
+ * i in [1-nb, nx+nb] lives at ((f*NZ + k+nb-1)*NY + j+nb-1)*NX + i+nb-1.
+ * qpass_3d has bounds [3-nb, n+nb-2] per axis (grid_data_mod.f90:42).
+ */
+#ifndef IMHD_ORACLE_H
+#define IMHD_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_cfg {
+    int nx, ny, nz;   /* interior cells */
+    int nb;           /* ghost layers (reference: 6) */
+    int npass;        /* passive scalars */
+    int mhd;          /* integrator, reference numbering: 1 TVD (K98 CTU+CT), 2 VL2+CT */
+    int riemann;      /* reference numbering inside the integrator:
+                         VL: 1 ROE, 2 HLLD, 3 HLLE;  TVD: 1 HLLE, 2 K98 Roe-TVD */
+    int recon;        /* VL corrector reconstruction: 0 PLM primitive (MC-beta),
+                         1 PPM primitive, 2 PPM characteristic (reference default) */
+    int bc;           /* 1 open (zero-gradient), 2 periodic */
+    double cs;        /* isothermal sound speed */
+    double mc_beta;   /* MC limiter beta (reference 1.01) */
+} orc_cfg;
+
+typedef struct orc_ctx orc_ctx;
+
+typedef struct orc_diag {
+    double ke, vrms_sq, momx, momy, momz;   /* driver.f90:518-553 (incl. dx^3, 0.5 on ke) */
+    double divb_sum;                         /* driver.f90:492-515, signed raw sum */
+    double divb_max;                         /* max_cell |sum of face differences| (raw) */
+    double mass;                             /* sum rho * dx^3 */
+} orc_diag;
+
+orc_ctx *orc_create(const orc_cfg *cfg);
+void orc_destroy(orc_ctx *c);
+void orc_set_state(orc_ctx *c, const double *q, const double *bface, const double *qpass);
+void orc_get_state(const orc_ctx *c, double *q, double *bface, double *qpass, double *emf_corner);
+void orc_bc(orc_ctx *c);
+void orc_step(orc_ctx *c, double dt);
+double orc_max_speed(orc_ctx *c);
+double orc_smax(const orc_ctx *c);
+void orc_copy_velocity(orc_ctx *c, int stage);
+void orc_passive_step(orc_ctx *c, double dt);
+void orc_diagnostics(const orc_ctx *c, orc_diag *out);
+int orc_floor_count(const orc_ctx *c);
+
+/* Per-interface pins: ni x nj planes (Fortran order, component slowest). */
+void orc_riemann_plane(int kind, int ni, int nj, double cs, const double *qL,
+                       const double *qR, double *F, double *smax);
+void orc_recon_plane(int kind, int ni, int nj, int nb, double cs, double mc_beta,
+                     const double *w, const double *wb, double *qL, double *qR);
+void orc_tvd_flux_plane(int kind, int ni, int nj, int nb, double cs, double dtdx,
+                        const double *q, double *F);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

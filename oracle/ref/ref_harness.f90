@@ -1,0 +1,175 @@
+! bind(C) harness around the reference Fortran solver modules (tsultan16/Isothermal_MHD,
+! 3D_MPI/src).  TEST INFRASTRUCTURE ONLY: it is compiled with the reference modules by
+! oracle/ref/build_ref.py into oracle/_ref/libimhd_ref_<cfg>.so and loaded by the CPU
+! parity tests and the golden-vector generator.  It never ships in the product.
+!
+! Every entry point only moves data into/out of the reference's own module arrays and
+! calls the reference's own procedures; no solver arithmetic lives here.
+MODULE ref_harness
+  USE ISO_C_BINDING
+  USE constants_mod
+  USE grid_data_mod
+  IMPLICIT NONE
+  LOGICAL, SAVE :: is_setup = .FALSE.
+CONTAINS
+
+  SUBROUTINE ref_dims(d) BIND(C, name='ref_dims')
+    INTEGER(C_INT), INTENT(OUT) :: d(8)
+    d = (/ nx, ny, nz, nb, npass, mhd_solver_type, riemann_solver_type, boundary_type /)
+  END SUBROUTINE ref_dims
+
+  ! driver.f90:237-266 (main_init) minus MPI, I/O and the built-in IC
+  SUBROUTINE ref_setup() BIND(C, name='ref_setup')
+    USE VanLeer_Solver_mod, ONLY: init_solver_vanleer
+    USE Iso_TVD_Solver_mod, ONLY: init_solver_tvd
+    USE passive_solver_mod, ONLY: init_passive_solver
+    IF (is_setup) RETURN
+    myrank = 0
+    my_coord = 0
+    CALL create_grid_arrays()
+    emf_corner = 0.d0
+    my_dt = 1.d-15
+    my_dx = 1.d0 / DBLE(MAX(nx, ny, nz))
+    IF (mhd_solver_type .EQ. 1) CALL init_solver_tvd()
+    IF (mhd_solver_type .EQ. 2) CALL init_solver_vanleer()
+    IF (npass .GT. 0) CALL init_passive_solver()
+    is_setup = .TRUE.
+  END SUBROUTINE ref_setup
+
+  SUBROUTINE ref_set_sound_speed(cs) BIND(C, name='ref_set_sound_speed')
+    REAL(C_DOUBLE), VALUE :: cs
+    sound_speed = cs
+  END SUBROUTINE ref_set_sound_speed
+
+  SUBROUTINE ref_set_state(q, bf, qp) BIND(C, name='ref_set_state')
+    REAL(C_DOUBLE), INTENT(IN) :: q(1-nb:nx+nb, 1-nb:ny+nb, 1-nb:nz+nb, 7)
+    REAL(C_DOUBLE), INTENT(IN) :: bf(1-nb:nx+nb, 1-nb:ny+nb, 1-nb:nz+nb, 3)
+    REAL(C_DOUBLE), INTENT(IN) :: qp(3-nb:nx+nb-2, 3-nb:ny+nb-2, 3-nb:nz+nb-2, MAX(npass,1))
+    q_3d = q
+    bface_3d = bf
+    IF (npass .GT. 0) qpass_3d = qp(:,:,:,1:npass)
+  END SUBROUTINE ref_set_state
+
+  SUBROUTINE ref_get_state(q, bf, qp, ec) BIND(C, name='ref_get_state')
+    REAL(C_DOUBLE), INTENT(OUT) :: q(1-nb:nx+nb, 1-nb:ny+nb, 1-nb:nz+nb, 7)
+    REAL(C_DOUBLE), INTENT(OUT) :: bf(1-nb:nx+nb, 1-nb:ny+nb, 1-nb:nz+nb, 3)
+    REAL(C_DOUBLE), INTENT(OUT) :: qp(3-nb:nx+nb-2, 3-nb:ny+nb-2, 3-nb:nz+nb-2, MAX(npass,1))
+    REAL(C_DOUBLE), INTENT(OUT) :: ec(1-nb:nx+nb, 1-nb:ny+nb, 1-nb:nz+nb, 3)
+    q = q_3d
+    bf = bface_3d
+    qp = 0.d0
+    IF (npass .GT. 0) qp(:,:,:,1:npass) = qpass_3d
+    ec = emf_corner
+  END SUBROUTINE ref_get_state
+
+  SUBROUTINE ref_bc() BIND(C, name='ref_bc')
+    USE boundary_conditions_mod, ONLY: boundary_conditions
+    CALL boundary_conditions()
+  END SUBROUTINE ref_bc
+
+  ! one call of the reference integrator (driver.f90:378-389)
+  SUBROUTINE ref_step(dt) BIND(C, name='ref_step')
+    USE VanLeer_Solver_mod, ONLY: vanleer_solve_3d
+    USE Iso_TVD_Solver_mod, ONLY: tvd_solve_3d
+    REAL(C_DOUBLE), VALUE :: dt
+    REAL*8 :: dtl
+    dtl = dt
+    IF (mhd_solver_type .EQ. 1) CALL tvd_solve_3d(dtl)
+    IF (mhd_solver_type .EQ. 2) CALL vanleer_solve_3d(dtl, .FALSE.)
+  END SUBROUTINE ref_step
+
+  ! driver.f90:432-448 (compute_new_time_step) returns cmax
+  SUBROUTINE ref_max_speed(cmax) BIND(C, name='ref_max_speed')
+    USE VanLeer_Solver_mod, ONLY: get_max_speed_vanleer
+    USE Iso_TVD_Solver_mod, ONLY: get_max_speed_tvd
+    REAL(C_DOUBLE), INTENT(OUT) :: cmax
+    REAL*8 :: c
+    c = 0.d0
+    IF (mhd_solver_type .EQ. 1) CALL get_max_speed_tvd(c)
+    IF (mhd_solver_type .EQ. 2) CALL get_max_speed_vanleer(c)
+    cmax = c
+  END SUBROUTINE ref_max_speed
+
+  SUBROUTINE ref_vl_smax(s) BIND(C, name='ref_vl_smax')
+    USE VanLeer_Solver_mod, ONLY: smax
+    REAL(C_DOUBLE), INTENT(OUT) :: s
+    s = smax
+  END SUBROUTINE ref_vl_smax
+
+  SUBROUTINE ref_copy_velocity(flag) BIND(C, name='ref_copy_velocity')
+    USE passive_solver_mod, ONLY: copy_velocity
+    INTEGER(C_INT), VALUE :: flag
+    CALL copy_velocity(flag .NE. 0)
+  END SUBROUTINE ref_copy_velocity
+
+  SUBROUTINE ref_passive_step(dt) BIND(C, name='ref_passive_step')
+    USE passive_solver_mod, ONLY: passive_solve_3d
+    REAL(C_DOUBLE), VALUE :: dt
+    REAL*8 :: dtl
+    dtl = dt
+    CALL passive_solve_3d(dtl)
+  END SUBROUTINE ref_passive_step
+
+  ! Per-interface Riemann fluxes of the VL module on an ni x nj plane of interface states.
+  ! kind: 1 ROE chain, 2 HLLD, 3 HLLE (vanleer_mod.f90:2264-3239).  Returns smax.
+  SUBROUTINE ref_vl_riemann(kind, ni, nj, qL, qR, F, sm) BIND(C, name='ref_vl_riemann')
+    USE VanLeer_Solver_mod, ONLY: qintL, qintR, Fs_2d, smax, compute_HLLD_fluxes_1d, &
+         compute_HLLE_fluxes_1d, compute_roe_avg_state_1d, compute_roe_eigenvalues_1d, &
+         compute_roe_eigenvectors_1d, compute_ROE_fluxes_1d
+    INTEGER(C_INT), VALUE :: kind, ni, nj
+    REAL(C_DOUBLE), INTENT(IN) :: qL(ni, nj, 7), qR(ni, nj, 7)
+    REAL(C_DOUBLE), INTENT(OUT) :: F(ni, nj, 7), sm
+    INTEGER :: ilow, ihi, jlow, jhi
+    ilow = 1
+    ihi = ni - 2*nb + 1
+    jlow = 1
+    jhi = nj - 2*nb
+    qintL = 0.d0
+    qintR = 0.d0
+    Fs_2d = 0.d0
+    qintL(1-nb:ni-nb, 1-nb:nj-nb, :) = qL
+    qintR(1-nb:ni-nb, 1-nb:nj-nb, :) = qR
+    smax = 0.d0
+    IF (kind .EQ. 2) CALL compute_HLLD_fluxes_1d(Fs_2d, ilow, ihi, jlow, jhi)
+    IF (kind .EQ. 3) CALL compute_HLLE_fluxes_1d(Fs_2d, ilow, ihi, jlow, jhi)
+    IF (kind .EQ. 1) THEN
+      CALL compute_roe_avg_state_1d(ilow, ihi, jlow, jhi)
+      CALL compute_roe_eigenvalues_1d(ilow, ihi, jlow, jhi)
+      CALL compute_roe_eigenvectors_1d(ilow, ihi, jlow, jhi)
+      CALL compute_ROE_fluxes_1d(Fs_2d, ilow, ihi, jlow, jhi)
+    END IF
+    F = Fs_2d(1-nb:ni-nb, 1-nb:nj-nb, :)
+    sm = smax
+  END SUBROUTINE ref_vl_riemann
+
+  ! Interface-state reconstruction of the VL module on an ni x nj plane of primitive
+  ! cell values w(6) and face normal field wb.  kind 1: compute_interface_states_prim as
+  ! compiled (vanleer_mod.f90:1144-1485); kind 2: compute_primitive_eigenvectors +
+  ! compute_interface_states_char (vanleer_mod.f90:941-1818).
+  SUBROUTINE ref_vl_recon(kind, ni, nj, wi, wbi, qL, qR) BIND(C, name='ref_vl_recon')
+    USE VanLeer_Solver_mod, ONLY: qintL, qintR, vw => w, wb, compute_interface_states_prim, &
+         compute_interface_states_char, compute_primitive_eigenvectors
+    INTEGER(C_INT), VALUE :: kind, ni, nj
+    REAL(C_DOUBLE), INTENT(IN) :: wi(ni, nj, 6), wbi(ni, nj)
+    REAL(C_DOUBLE), INTENT(OUT) :: qL(ni, nj, 7), qR(ni, nj, 7)
+    INTEGER :: ilow, ihi, jlow, jhi
+    ilow = 1
+    ihi = ni - 2*nb
+    jlow = 1
+    jhi = nj - 2*nb
+    qintL = 0.d0
+    qintR = 0.d0
+    vw = 0.d0
+    wb = 0.d0
+    vw(1-nb:ni-nb, 1-nb:nj-nb, :) = wi
+    wb(1-nb:ni-nb, 1-nb:nj-nb) = wbi
+    IF (kind .EQ. 1) CALL compute_interface_states_prim(ilow, ihi, jlow, jhi)
+    IF (kind .EQ. 2) THEN
+      CALL compute_primitive_eigenvectors(ilow, ihi, jlow, jhi)
+      CALL compute_interface_states_char(ilow, ihi, jlow, jhi)
+    END IF
+    qL = qintL(1-nb:ni-nb, 1-nb:nj-nb, :)
+    qR = qintR(1-nb:ni-nb, 1-nb:nj-nb, :)
+  END SUBROUTINE ref_vl_recon
+
+END MODULE ref_harness

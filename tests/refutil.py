@@ -1,0 +1,256 @@
+"""Test-side loaders for the two CPU checkers.
+
+* the CPU restatement oracle (oracle/liboracle_imhd.so, built by oracle/Makefile), and
+* the reference Fortran solver built from /root/reference by oracle/ref/build_ref.py
+  (oracle/_ref/libimhd_ref_<cfg>.so; only present where the reference is).
+
+Test infrastructure only: the product package never imports this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(REPO, "oracle", "liboracle_imhd.so")
+REF_DIR = os.path.join(REPO, "oracle", "_ref")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def P(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OrcCfg(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("nb", ctypes.c_int), ("npass", ctypes.c_int), ("mhd", ctypes.c_int),
+                ("riemann", ctypes.c_int), ("recon", ctypes.c_int), ("bc", ctypes.c_int),
+                ("cs", ctypes.c_double), ("mc_beta", ctypes.c_double)]
+
+
+class OrcDiag(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("ke", "vrms_sq", "momx", "momy", "momz", "divb_sum", "divb_max", "mass")]
+
+
+_oracle = None
+
+
+def oracle_lib():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            import subprocess
+            subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True,
+                           capture_output=True)
+        L = ctypes.CDLL(ORACLE_SO)
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.POINTER(OrcCfg)]
+        for f in ("orc_destroy", "orc_bc"):
+            getattr(L, f).argtypes = [ctypes.c_void_p]
+        L.orc_set_state.argtypes = [ctypes.c_void_p] * 4
+        L.orc_get_state.argtypes = [ctypes.c_void_p] * 5
+        L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_passive_step.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_copy_velocity.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_max_speed.argtypes = [ctypes.c_void_p]
+        L.orc_max_speed.restype = ctypes.c_double
+        L.orc_smax.argtypes = [ctypes.c_void_p]
+        L.orc_smax.restype = ctypes.c_double
+        L.orc_floor_count.argtypes = [ctypes.c_void_p]
+        L.orc_floor_count.restype = ctypes.c_int
+        L.orc_diagnostics.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcDiag)]
+        L.orc_riemann_plane.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _dp]
+        L.orc_recon_plane.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_tvd_flux_plane.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                         ctypes.c_void_p]
+        _oracle = L
+    return _oracle
+
+
+def padded_shapes(nx, ny, nz, nb, npass):
+    sh = (nx + 2 * nb, ny + 2 * nb, nz + 2 * nb)
+    shp = (nx + 2 * nb - 4, ny + 2 * nb - 4, nz + 2 * nb - 4, max(npass, 1))
+    return sh, shp
+
+
+class Oracle:
+    """The CPU restatement with the reference's array layout."""
+
+    def __init__(self, nx, ny, nz, npass=0, mhd=2, riemann=2, recon=0, bc=2, cs=1.0,
+                 nb=6, mc_beta=1.01):
+        self.L = oracle_lib()
+        self.dims = (nx, ny, nz, nb, npass)
+        cfg = OrcCfg(nx, ny, nz, nb, npass, mhd, riemann, recon, bc, cs, mc_beta)
+        self.c = self.L.orc_create(ctypes.byref(cfg))
+        self.sh, self.shp = padded_shapes(nx, ny, nz, nb, npass)
+        self.npass = npass
+
+    def __del__(self):
+        try:
+            self.L.orc_destroy(self.c)
+        except Exception:
+            pass
+
+    def set_state(self, q, bf, qp=None):
+        if qp is None:
+            qp = np.zeros(self.shp, order="F")
+        self._keep = (np.asfortranarray(q), np.asfortranarray(bf), np.asfortranarray(qp))
+        self.L.orc_set_state(self.c, *[P(a) for a in self._keep])
+
+    def get_state(self):
+        q = np.zeros(self.sh + (7,), order="F")
+        bf = np.zeros(self.sh + (3,), order="F")
+        qp = np.zeros(self.shp, order="F")
+        ec = np.zeros(self.sh + (3,), order="F")
+        self.L.orc_get_state(self.c, P(q), P(bf), P(qp), P(ec))
+        return q, bf, qp, ec
+
+    def bc(self):
+        self.L.orc_bc(self.c)
+
+    def step(self, dt):
+        self.L.orc_step(self.c, dt)
+
+    def max_speed(self):
+        return self.L.orc_max_speed(self.c)
+
+    def smax(self):
+        return self.L.orc_smax(self.c)
+
+    def copy_velocity(self, stage):
+        self.L.orc_copy_velocity(self.c, stage)
+
+    def passive_step(self, dt):
+        self.L.orc_passive_step(self.c, dt)
+
+    def diagnostics(self):
+        d = OrcDiag()
+        self.L.orc_diagnostics(self.c, ctypes.byref(d))
+        return {n: getattr(d, n) for n, _ in OrcDiag._fields_}
+
+    def floor_count(self):
+        return self.L.orc_floor_count(self.c)
+
+
+def ref_available(name):
+    return os.path.exists(os.path.join(REF_DIR, f"libimhd_ref_{name}.so"))
+
+
+_refs = {}
+
+
+class Reference:
+    """The reference Fortran modules (one process-wide instance per build config)."""
+
+    def __init__(self, name):
+        if name not in _refs:
+            L = ctypes.CDLL(os.path.join(REF_DIR, f"libimhd_ref_{name}.so"))
+            L.ref_step.argtypes = [ctypes.c_double]
+            L.ref_passive_step.argtypes = [ctypes.c_double]
+            L.ref_copy_velocity.argtypes = [ctypes.c_int]
+            L.ref_set_sound_speed.argtypes = [ctypes.c_double]
+            L.ref_vl_riemann.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, _dp]
+            L.ref_vl_recon.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+            d = (ctypes.c_int * 8)()
+            L.ref_dims(d)
+            L.ref_setup()
+            _refs[name] = (L, list(d))
+        self.L, d = _refs[name]
+        self.nx, self.ny, self.nz, self.nb, self.npass, self.mhd, self.riemann, self.bc_type = d
+        self.sh, self.shp = padded_shapes(self.nx, self.ny, self.nz, self.nb, self.npass)
+
+    def set_state(self, q, bf, qp=None):
+        if qp is None:
+            qp = np.zeros(self.shp, order="F")
+        self._keep = (np.asfortranarray(q), np.asfortranarray(bf), np.asfortranarray(qp))
+        self.L.ref_set_state(*[P(a) for a in self._keep])
+
+    def get_state(self):
+        q = np.zeros(self.sh + (7,), order="F")
+        bf = np.zeros(self.sh + (3,), order="F")
+        qp = np.zeros(self.shp, order="F")
+        ec = np.zeros(self.sh + (3,), order="F")
+        self.L.ref_get_state(P(q), P(bf), P(qp), P(ec))
+        return q, bf, qp, ec
+
+    def bc(self):
+        self.L.ref_bc()
+
+    def step(self, dt):
+        self.L.ref_step(dt)
+
+    def max_speed(self):
+        c = ctypes.c_double()
+        self.L.ref_max_speed(ctypes.byref(c))
+        return c.value
+
+    def smax(self):
+        s = ctypes.c_double()
+        self.L.ref_vl_smax(ctypes.byref(s))
+        return s.value
+
+    def copy_velocity(self, stage):
+        self.L.ref_copy_velocity(stage)
+
+    def passive_step(self, dt):
+        self.L.ref_passive_step(dt)
+
+
+def random_state(nx, ny, nz, nb=6, npass=1, seed=0, amp_v=0.5, amp_rho=0.2, b0=(0.5, 0.2, 0.1),
+                 amp_b=0.2):
+    """A smooth, divergence-free random periodic state in the reference layout.
+
+    Face B comes from a vector potential on cell edges so that the discrete (raw)
+    divergence vanishes to roundoff; cell-centred B is the face average.  Values are
+    set on the interior and the ghosts are left for the boundary fill.
+    """
+    rng = np.random.default_rng(seed)
+    sh, shp = padded_shapes(nx, ny, nz, nb, npass)
+    q = np.zeros(sh + (7,), order="F")
+    bf = np.zeros(sh + (3,), order="F")
+    qp = np.zeros(shp, order="F")
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    z = (np.arange(nz) + 0.5) / nz
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+
+    def smooth():
+        f = np.zeros((nx, ny, nz))
+        for _ in range(3):
+            kx, ky, kz = rng.integers(1, 3, size=3)
+            ph = rng.random(3) * 2 * np.pi
+            f += np.sin(2 * np.pi * kx * X + ph[0]) * np.cos(2 * np.pi * ky * Y + ph[1]) * \
+                np.sin(2 * np.pi * kz * Z + ph[2])
+        return f / 3.0
+
+    I = (slice(nb, nb + nx), slice(nb, nb + ny), slice(nb, nb + nz))
+    rho = 1.0 + amp_rho * smooth()
+    q[I + (0,)] = rho
+    for m in range(3):
+        q[I + (1 + m,)] = rho * amp_v * smooth()
+    # periodic edge vector potential A on the interior index grid
+    A = [amp_b * smooth() / (2 * np.pi) for _ in range(3)]
+    # face B = B0 + curl A (discrete, periodic), raw differences scaled by n
+    def d(a, ax):
+        return (a - np.roll(a, 1, axis=ax))
+    bx = b0[0] + (d(A[2], 1) - d(A[1], 2))
+    by = b0[1] + (d(A[0], 2) - d(A[2], 0))
+    bz = b0[2] + (d(A[1], 0) - d(A[0], 1))
+    bf[I + (0,)] = bx
+    bf[I + (1,)] = by
+    bf[I + (2,)] = bz
+    q[I + (4,)] = 0.5 * (bx + np.roll(bx, 1, axis=0))
+    q[I + (5,)] = 0.5 * (by + np.roll(by, 1, axis=1))
+    q[I + (6,)] = 0.5 * (bz + np.roll(bz, 1, axis=2))
+    Ip = (slice(nb - 2, nb - 2 + nx), slice(nb - 2, nb - 2 + ny), slice(nb - 2, nb - 2 + nz))
+    for p in range(npass):
+        qp[Ip + (p,)] = 0.5 + 0.4 * smooth()
+    return q, bf, qp
