@@ -1,0 +1,133 @@
+/* imhd: MI355X-native isothermal-MHD cell-update library (C ABI).
+ *
+ * Drop-in boundary for the hot path of tsultan16/Isothermal_MHD (3D_MPI).  The reference
+ * exposes Fortran module procedures that act on module arrays (q_3d, bface_3d,
+ * qpass_3d; 3D_MPI/src/grid_data_mod.f90:9-14); each entry point below replaces one of
+ * them and is bound from Fortran through iso_c_binding (fortran/imhd_shim.f90).
+ *
+ * Host arrays use the reference layout: FP64, Fortran column-major, SoA by field,
+ * ghost-padded with nb layers:
+ *   q_3d(1-nb:nx+nb, 1-nb:ny+nb, 1-nb:nz+nb, 7)  rho, rho vx, rho vy, rho vz, Bx, By, Bz (cell)
+ *   bface_3d(same, 3)                              Bx(i+1/2), By(j+1/2), Bz(k+1/2)
+ *   qpass_3d(3-nb:nx+nb-2, ..., npass)             passive scalars
+ * All functions return 0 on success and a negative IMHD_E* code on error;
+ * imhd_last_error() then describes it (the reference's PRINT + STOP becomes a return code).
+ * A context owns one HIP stream and its device buffers; it is not thread-safe.
+ */
+#ifndef IMHD_H
+#define IMHD_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMHD_OK 0
+#define IMHD_EINVAL (-1)   /* bad argument / configuration */
+#define IMHD_EHIP (-2)     /* HIP runtime error (allocation, launch, copy) */
+#define IMHD_ESTATE (-3)   /* call out of order (e.g. no state set) */
+#define IMHD_ENOSPEED (-4) /* max speed below 1e-20 (vanleer_mod.f90:3328-3331) */
+
+/* integrator: reference mhd_solver_type (constants_mod.f90:58) */
+#define IMHD_TVD 1
+#define IMHD_VL 2
+/* riemann: reference riemann_solver_type as read by the VL module (constants_mod.f90:57) */
+#define IMHD_ROE 1
+#define IMHD_HLLD 2
+#define IMHD_HLLE 3
+/* recon: corrector-stage reconstruction of the VL integrator */
+#define IMHD_PLM_PRIM 0   /* primitive PLM with MC-beta slope (vanleer_mod.f90:1160-1215) */
+/* bc: reference boundary_type (constants_mod.f90:36) */
+#define IMHD_BC_OPEN 1
+#define IMHD_BC_PERIODIC 2
+
+typedef struct imhd_config {
+    int nx, ny, nz;     /* interior cells (per box) */
+    int nb;             /* ghost layers; the reference uses 6 (constants_mod.f90:12) */
+    int npass;          /* passive scalars, 0..4 (constants_mod.f90:63) */
+    int integrator;     /* IMHD_VL (IMHD_TVD: see imhd_create) */
+    int riemann;        /* IMHD_ROE / IMHD_HLLD / IMHD_HLLE */
+    int recon;          /* IMHD_PLM_PRIM */
+    int bc;             /* IMHD_BC_OPEN / IMHD_BC_PERIODIC */
+    int precision;      /* 64: FP64 throughout; 32: FP32 bulk, FP64 face B, corner EMF, CFL, diagnostics */
+    int compensated;    /* precision 32 only: Kahan-compensated accumulation of rho, rho v */
+    int _reserved;
+    double cs;          /* isothermal sound speed (constants_mod.f90:35) */
+    double mc_beta;     /* MC limiter beta (vanleer_mod.f90:77), 1.01 in the reference */
+    double cour;        /* Courant number for imhd_run (constants_mod.f90:16) */
+} imhd_config;
+
+typedef struct imhd_diag {
+    double ke;          /* 0.5 sum rho v^2 dx^3 (driver.f90:518-553, same clamps) */
+    double vrms_sq;     /* sum v^2 dx^3 */
+    double momx, momy, momz;
+    double divb_sum;    /* signed raw sum of face differences (driver.f90:492-515) */
+    double divb_max;    /* max over cells of |raw face-difference sum| */
+    double mass;        /* sum rho dx^3 */
+} imhd_diag;
+
+typedef struct imhd_ctx imhd_ctx;
+
+/* Context for one box, or an ensemble of nbox boxes (cfg[b] per box; all boxes share the
+ * solver options of cfg[0]).  Replaces init_solver_vanleer (vanleer_mod.f90:127-158),
+ * init_passive_solver (passive_mod.f90:44-57) and create_grid_arrays
+ * (grid_data_mod.f90:33-67) for the device side. */
+int imhd_create(const imhd_config* cfg, imhd_ctx** out);
+int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out);
+/* destroy_solver_vanleer / destroy_passive_solver (vanleer_mod.f90:161-165) */
+int imhd_destroy(imhd_ctx* ctx);
+
+/* Upload / download the reference-layout host arrays of box `box` (qpass may be NULL
+ * when npass == 0).  get_state returns the state as the last call left it: call
+ * imhd_fill_ghosts() first for valid ghost cells. */
+int imhd_set_state(imhd_ctx* ctx, int box, const double* q_3d, const double* bface_3d, const double* qpass_3d);
+int imhd_get_state(imhd_ctx* ctx, int box, double* q_3d, double* bface_3d, double* qpass_3d);
+
+/* One integrator step of every box with the given dt:
+ * vanleer_solve_3d(dt_in, drive_next) (vanleer_mod.f90:279-426); drive_next must be 0
+ * (forcing is applied by imhd_run). */
+int imhd_step(imhd_ctx* ctx, double dt, int drive_next);
+
+/* boundary_conditions() on one rank (boundary_conditions_mod.f90:20-234) */
+int imhd_fill_ghosts(imhd_ctx* ctx);
+
+/* get_max_speed_vanleer (vanleer_mod.f90:3266-3334): max over interior cells of
+ * |v| + c_f, combined with the interface maximum smax of the last step.
+ * cmax receives one value per box. */
+int imhd_max_speed(imhd_ctx* ctx, double* cmax);
+
+/* copy_velocity(predictor_stage) (passive_mod.f90:204-244) and passive_solve_3d(dt)
+ * (passive_mod.f90:118-201) */
+int imhd_copy_velocity(imhd_ctx* ctx, int stage);
+int imhd_passive_step(imhd_ctx* ctx, double dt);
+
+/* compute_KE + compute_divB (driver.f90:492-553) and max|div B|; out[box] */
+int imhd_diagnostics(imhd_ctx* ctx, imhd_diag* out);
+
+/* Number of cells the density floor reset since creation (final_check_unphysical,
+ * vanleer_mod.f90:3243-3262, which prints instead). */
+int imhd_floor_count(imhd_ctx* ctx, unsigned int* count);
+
+/* Native, device-resident time loop of the driver (driver.f90:35-186) for nsteps steps:
+ * copy_velocity, step, passive, ghost fill, CFL dt = COUR dx / cmax with
+ * dt = MIN(t_end - t, dt).  dt stays on the device; the first call starts from
+ * dt = 1e-15 (driver.f90:249) unless imhd_set_dt was called.  With use_graph != 0 one
+ * step is captured once in a hipGraph and replayed. */
+int imhd_run(imhd_ctx* ctx, int nsteps, double t_end, int use_graph);
+int imhd_set_dt(imhd_ctx* ctx, const double* dt);          /* per box */
+int imhd_get_time(imhd_ctx* ctx, double* t, double* dt);    /* per box */
+
+/* Per-kernel device time of the last imhd_run, from HIP events on the context's stream
+ * (enable with imhd_set_profiling before the run).  Returns a ';'-separated
+ * "name=ms/launches" list. */
+int imhd_set_profiling(imhd_ctx* ctx, int on);
+const char* imhd_profile_report(imhd_ctx* ctx);
+
+/* Synchronise the context's stream. */
+int imhd_sync(imhd_ctx* ctx);
+
+const char* imhd_last_error(const imhd_ctx* ctx);
+const char* imhd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IMHD_H */

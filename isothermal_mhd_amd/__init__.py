@@ -1,0 +1,9 @@
+"""MI355X-native isothermal-MHD cell-update path (VL2 + CT, HLLD/HLLE/Roe, passive
+scalars, CFL reduction, ghost fill) behind the reference's procedure interface.
+
+The compute path is libimhd.so (hand-written HIP for gfx950, C ABI in include/imhd.h);
+this package only binds it.
+"""
+from .solver import VanLeerSolver, driver_step, padded_shapes  # noqa: F401
+
+__all__ = ["VanLeerSolver", "driver_step", "padded_shapes"]

@@ -1,0 +1,284 @@
+// Ghost fill, CFL reduction, diagnostics, passive-scalar advection and layout/precision
+// conversion kernels.
+#include "aux_kernels.hpp"
+
+namespace imhd {
+
+// ---------------------------------------------------------------------------------
+// Ghost fill (boundary_conditions_mod.f90:20-234, one rank).  The reference fills x,
+// then y, then z over whole planes; the composition equals reading every ghost cell
+// from (map_x(i), map_y(j), map_z(k)) with an independent per-axis source map, which is
+// what this kernel does in one pass (sources are always interior cells).
+__device__ __forceinline__ int bc_map(int i, int n, int bc) {
+    if (i >= 1 && i <= n) return i;
+    if (bc == 2) return i < 1 ? i + n : i - n;   // periodic: q(-g) = q(n-g), q(n+1+g) = q(1+g)
+    return i < 1 ? 1 : n;                         // open: copy the edge cell
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void k_fill_ghosts(const BoxDesc* __restrict__ boxes, Plan plan, GhostArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const int w = a.depth;   // nb for q / bface, 3 for passives (passive_mod.f90:52)
+    if (i < 1 - w || j < 1 - w || k < 1 - w || i > B.n[0] + w || j > B.n[1] + w || k > B.n[2] + w) return;
+    if (i >= 1 && i <= B.n[0] && j >= 1 && j <= B.n[1] && k >= 1 && k <= B.n[2]) return;
+    const int si = bc_map(i, B.n[0], B.bc), sj = bc_map(j, B.n[1], B.bc), sk = bc_map(k, B.n[2], B.bc);
+    const long long d = cell_index(B, i, j, k), s = cell_index(B, si, sj, sk);
+    for (int f = 0; f < a.nreal; f++) a.r[f][d] = a.r[f][s];
+    for (int f = 0; f < a.ndbl; f++) a.d[f][d] = a.d[f][s];
+}
+
+// ---------------------------------------------------------------------------------
+// CFL: max over interior cells of |v_d| + c_f,d (vanleer_mod.f90:3266-3334), FP64
+// accumulation; wavefront shuffle reduction and one integer atomicMax per wave.
+template <class real>
+__global__ __launch_bounds__(256) void k_max_speed(const BoxDesc* __restrict__ boxes, Plan plan, QPtrs<real> Q,
+                                                   double cs, unsigned long long* cmax) {
+    const real* const* q = Q.q;
+    int b = 0, i, j, k;
+    double m = 0.0;
+    bool act = plan_item(plan, b, i, j, k);
+    if (act) {
+        const BoxDesc& B = boxes[b];
+        long long c = cell_index(B, i, j, k);
+        double asqr = cs * cs;
+        double rho = q[0][c];
+        double isrho = 1.0 / sqrt(rho);
+        double bx = (double)q[4][c] * isrho, by = (double)q[5][c] * isrho, bz = (double)q[6][c] * isrho;
+        double btsqr = by * by + bz * bz, bsqr = bx * bx + btsqr, x = asqr - bsqr;
+        double tmp = sqrt(x * x + 4.0 * asqr * btsqr);
+        double cfx = sqrt(0.5 * (asqr + bsqr + tmp));
+        btsqr = bx * bx + bz * bz; bsqr = by * by + btsqr; x = asqr - bsqr;
+        tmp = sqrt(x * x + 4.0 * asqr * btsqr);
+        double cfy = sqrt(0.5 * (asqr + bsqr + tmp));
+        btsqr = bx * bx + by * by; bsqr = bz * bz + btsqr; x = asqr - bsqr;
+        tmp = sqrt(x * x + 4.0 * asqr * btsqr);
+        double cfz = sqrt(0.5 * (asqr + bsqr + tmp));
+        double vx = (double)q[1][c] / rho, vy = (double)q[2][c] / rho, vz = (double)q[3][c] / rho;
+        m = max3(fabs(vx) + cfx, fabs(vy) + cfy, fabs(vz) + cfz);
+    }
+    // interior plans are 64-aligned per box: a wave never spans two boxes
+    atomic_max_nonneg(&cmax[b], m);
+}
+
+// dt = COUR dx / max(cmax, smax + delv_max), then MIN(t_end - t, dt) (driver.f90:432-448, 475)
+__global__ void k_new_dt(int nbox, const BoxDesc* __restrict__ boxes, const unsigned long long* cmax,
+                         const unsigned long long* smax, const double* cour, double t_end, double* t, double* dt,
+                         double* dt_prev, int advance_time) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    double cm = __longlong_as_double((long long)cmax[b]);
+    double sm = __longlong_as_double((long long)smax[b]);
+    double c = fmax(cm, sm);
+    if (advance_time) t[b] = t[b] + dt[b];
+    dt_prev[b] = dt[b];
+    double dn = cour[b] * boxes[b].dx / c;
+    double rem = t_end - t[b];
+    dt[b] = rem < dn ? rem : dn;
+}
+
+__global__ void k_reset_u64(unsigned long long* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// ---------------------------------------------------------------------------------
+// Diagnostics (driver.f90:492-553) in FP64 plus total mass and max|div B|.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) for (int x = 0; x < (int)(blockDim.x >> 6); x++) s += sh[x];
+    return s;
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void k_diag(const BoxDesc* __restrict__ boxes, Plan plan, DiagArgs<real> a) {
+    __shared__ double sh[4];
+    int b = 0, i, j, k;
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    double dmax = 0.0;
+    bool act = plan_item(plan, b, i, j, k);
+    if (act) {
+        const BoxDesc& B = boxes[b];
+        long long c = cell_index(B, i, j, k);
+        const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+        double r0 = a.q[0][c];
+        double rho = fmin(fmax(r0, 1e-15), 1e10);
+        double m1 = a.q[1][c], m2 = a.q[2][c], m3 = a.q[3][c];
+        double vsqr = fmin((m1 * m1 + m2 * m2 + m3 * m3) / (rho * rho), 1e5);
+        double db = (a.b[0][c] - a.b[0][c - sx]) + (a.b[1][c] - a.b[1][c - sy]) + (a.b[2][c] - a.b[2][c - sz]);
+        v[0] = rho * vsqr; v[1] = vsqr; v[2] = m1; v[3] = m2; v[4] = m3; v[5] = db; v[6] = r0;
+        dmax = fabs(db);
+    }
+    // one box per block for diagnostics plans (256-aligned starts)
+    for (int f = 0; f < 7; f++) {
+        double s = block_sum(v[f], sh);
+        if (threadIdx.x == 0) atomicAdd(&a.out[b * DIAG_STRIDE + f], s);
+    }
+    atomic_max_nonneg(reinterpret_cast<unsigned long long*>(&a.out[b * DIAG_STRIDE + 7]), dmax);
+}
+
+// ---------------------------------------------------------------------------------
+// Passive scalars (passive_mod.f90): velocities, upwind face values, advective update.
+
+// copy_velocity (passive_mod.f90:204-244)
+template <class real>
+__global__ __launch_bounds__(256) void k_copy_velocity(const BoxDesc* __restrict__ boxes, Plan plan, VelArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    long long c = cell_index(boxes[b], i, j, k);
+    real irho = real(1) / a.q[0][c];
+    if (!a.stage) {
+        for (int d = 0; d < 3; d++) a.vel[d][c] = a.q[1 + d][c] * irho;
+    } else {
+        for (int d = 0; d < 3; d++) a.vta[d][c] = real(0.5) * (a.vel[d][c] + a.q[1 + d][c] * irho);
+    }
+}
+
+// face value by upwinding on the mean adjacent cell velocity (passive_mod.f90:594-627);
+// corrector stage reconstructs with minmod PLM (passive_mod.f90:510-557)
+template <class real>
+__global__ __launch_bounds__(256) void k_pass_flux(const BoxDesc* __restrict__ boxes, Plan plan, PassFluxArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    long long c = cell_index(B, i, j, k);
+    long long s = a.dir == 0 ? 1LL : (a.dir == 1 ? (long long)B.P[0] : (long long)B.P[0] * B.P[1]);
+    real vavg = real(0.5) * (a.v[c] + a.v[c + s]);
+    real theta = vavg >= real(0) ? real(1) : real(-1);
+    for (int p = 0; p < a.npass; p++) {
+        const real* w = a.s[p];
+        real uL, uR;
+        if (!a.stage) {
+            uL = w[c];
+            uR = w[c + s];
+        } else {
+            real wm = w[c - s], w0 = w[c], w1 = w[c + s], w2 = w[c + 2 * s];
+            real dL = w0 - wm, dR = w1 - w0;
+            real d0 = real(0.5) * (sgn1(dL) + sgn1(dR)) * min2(fabs(dL), fabs(dR));
+            dL = w1 - w0; dR = w2 - w1;
+            real d1 = real(0.5) * (sgn1(dL) + sgn1(dR)) * min2(fabs(dL), fabs(dR));
+            uL = w0 + real(0.5) * d0;
+            uR = w1 - real(0.5) * d1;
+        }
+        a.F[p][c] = real(0.5) * ((real(1) + theta) * uL + (real(1) - theta) * uR);
+    }
+}
+
+// update_passive_state_3d (passive_mod.f90:562-590): advective, not conservative
+template <class real>
+__global__ __launch_bounds__(256) void k_pass_update(const BoxDesc* __restrict__ boxes, Plan plan, PassUpdArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    long long c = cell_index(B, i, j, k);
+    const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+    real dtdx = (real)(a.factor * a.dt[b] / B.dx);
+    real vx = a.v[0][c], vy = a.v[1][c], vz = a.v[2][c];
+    for (int p = 0; p < a.npass; p++) {
+        const real* F0 = a.F[0][p];
+        const real* F1 = a.F[1][p];
+        const real* F2 = a.F[2][p];
+        a.dst[p][c] = a.src[p][c] - dtdx * (vx * (F0[c] - F0[c - sx]) + vy * (F1[c] - F1[c - sy]) + vz * (F2[c] - F2[c - sz]));
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// host <-> device layout and precision conversion (reference FP64 arrays)
+template <class real>
+__global__ void k_convert_in(const double* __restrict__ src, real* __restrict__ dst, long long n) {
+    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) dst[g] = (real)src[g];
+}
+template <class real>
+__global__ void k_convert_out(const real* __restrict__ src, double* __restrict__ dst, long long n) {
+    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) dst[g] = (double)src[g];
+}
+// passive arrays: the reference's bounds are [3-nb, n+nb-2] per axis; ours are the full
+// ghost-padded box.  dir 0: host-layout src -> device; dir 1: device -> host layout.
+template <class real>
+__global__ void k_pass_layout(BoxDesc B, const double* hsrc, double* hdst, real* dev, int to_device) {
+    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int PX = B.n[0] + 2 * B.nb - 4, PY = B.n[1] + 2 * B.nb - 4, PZ = B.n[2] + 2 * B.nb - 4;
+    long long tot = (long long)PX * PY * PZ;
+    if (g >= tot) return;
+    int ii = (int)(g % PX);
+    long long t = g / PX;
+    int jj = (int)(t % PY);
+    int kk = (int)(t / PY);
+    long long c = cell_index(B, ii + 3 - B.nb, jj + 3 - B.nb, kk + 3 - B.nb);
+    if (to_device) dev[c] = (real)hsrc[g];
+    else hdst[g] = (double)dev[c];
+}
+
+// ---------------------------------------------------------------------------------
+#define L256(k, ...) hipLaunchKernelGGL(k, dim3(L.blocks()), dim3(256), 0, L.stream, __VA_ARGS__)
+
+template <class real>
+void launch_fill_ghosts(const Launch& L, const GhostArgs<real>& a) {
+    if (L.plan.total > 0) L256((k_fill_ghosts<real>), L.boxes, L.plan, a);
+}
+template <class real>
+void launch_max_speed(const Launch& L, const QPtrs<real>& q, double cs, unsigned long long* cmax) {
+    if (L.plan.total > 0) L256((k_max_speed<real>), L.boxes, L.plan, q, cs, cmax);
+}
+template <class real>
+void launch_diag(const Launch& L, const DiagArgs<real>& a) {
+    if (L.plan.total > 0) L256((k_diag<real>), L.boxes, L.plan, a);
+}
+template <class real>
+void launch_copy_velocity(const Launch& L, const VelArgs<real>& a) {
+    if (L.plan.total > 0) L256((k_copy_velocity<real>), L.boxes, L.plan, a);
+}
+template <class real>
+void launch_pass_flux(const Launch& L, const PassFluxArgs<real>& a) {
+    if (L.plan.total > 0) L256((k_pass_flux<real>), L.boxes, L.plan, a);
+}
+template <class real>
+void launch_pass_update(const Launch& L, const PassUpdArgs<real>& a) {
+    if (L.plan.total > 0) L256((k_pass_update<real>), L.boxes, L.plan, a);
+}
+void launch_new_dt(hipStream_t st, int nbox, const BoxDesc* boxes, const unsigned long long* cmax,
+                   const unsigned long long* smax, const double* cour, double t_end, double* t, double* dt,
+                   double* dt_prev, int advance_time) {
+    hipLaunchKernelGGL(k_new_dt, dim3((nbox + 63) / 64), dim3(64), 0, st, nbox, boxes, cmax, smax, cour, t_end, t,
+                       dt, dt_prev, advance_time);
+}
+void launch_reset_u64(hipStream_t st, unsigned long long* p, int n) {
+    hipLaunchKernelGGL(k_reset_u64, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+}
+template <class real>
+void launch_convert_in(hipStream_t st, const double* src, real* dst, long long n) {
+    hipLaunchKernelGGL((k_convert_in<real>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
+}
+template <class real>
+void launch_convert_out(hipStream_t st, const real* src, double* dst, long long n) {
+    hipLaunchKernelGGL((k_convert_out<real>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
+}
+template <class real>
+void launch_pass_layout(hipStream_t st, const BoxDesc& B, const double* hsrc, double* hdst, real* dev, int to_device) {
+    long long tot = (long long)(B.n[0] + 2 * B.nb - 4) * (B.n[1] + 2 * B.nb - 4) * (B.n[2] + 2 * B.nb - 4);
+    hipLaunchKernelGGL((k_pass_layout<real>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, B, hsrc, hdst, dev,
+                       to_device);
+}
+
+#define INST_AUX(T)                                                                                           \
+    template void launch_fill_ghosts<T>(const Launch&, const GhostArgs<T>&);                                  \
+    template void launch_max_speed<T>(const Launch&, const QPtrs<T>&, double, unsigned long long*);                 \
+    template void launch_diag<T>(const Launch&, const DiagArgs<T>&);                                          \
+    template void launch_copy_velocity<T>(const Launch&, const VelArgs<T>&);                                  \
+    template void launch_pass_flux<T>(const Launch&, const PassFluxArgs<T>&);                                 \
+    template void launch_pass_update<T>(const Launch&, const PassUpdArgs<T>&);                                \
+    template void launch_convert_in<T>(hipStream_t, const double*, T*, long long);                            \
+    template void launch_convert_out<T>(hipStream_t, const T*, double*, long long);                           \
+    template void launch_pass_layout<T>(hipStream_t, const BoxDesc&, const double*, double*, T*, int);
+INST_AUX(double)
+INST_AUX(float)
+
+}  // namespace imhd

@@ -1,0 +1,71 @@
+// Argument blocks and launch entry points of the auxiliary kernels (ghost fill, CFL,
+// diagnostics, passive scalars, conversion).
+#pragma once
+#include "kernels.hpp"
+
+namespace imhd {
+
+constexpr int DIAG_STRIDE = 8;  // per box: rho v^2, v^2, mx, my, mz, sum divB, mass, max|divB|
+
+template <class real>
+struct GhostArgs {
+    real* r[7 + MAXPASS];
+    double* d[3];
+    int nreal, ndbl, depth;
+};
+
+template <class real>
+struct QPtrs {
+    const real* q[7];
+};
+
+template <class real>
+struct DiagArgs {
+    const real* q[4];
+    const double* b[3];
+    double* out;   // DIAG_STRIDE doubles per box
+};
+
+template <class real>
+struct VelArgs {
+    const real* q[4];
+    real* vel[3];
+    real* vta[3];
+    int stage;     // 0: v^n, 1: time-averaged 0.5 (v^n + v^{n+1})
+};
+
+template <class real>
+struct PassFluxArgs {
+    const real* s[MAXPASS];
+    const real* v;   // velocity component along dir
+    real* F[MAXPASS];
+    int npass, dir, stage;
+};
+
+template <class real>
+struct PassUpdArgs {
+    const real* src[MAXPASS];
+    real* dst[MAXPASS];
+    const real* v[3];
+    const real* F[3][MAXPASS];
+    const double* dt;
+    double factor;
+    int npass;
+};
+
+template <class real> void launch_fill_ghosts(const Launch& L, const GhostArgs<real>& a);
+template <class real> void launch_max_speed(const Launch& L, const QPtrs<real>& q, double cs, unsigned long long* cmax);
+template <class real> void launch_diag(const Launch& L, const DiagArgs<real>& a);
+template <class real> void launch_copy_velocity(const Launch& L, const VelArgs<real>& a);
+template <class real> void launch_pass_flux(const Launch& L, const PassFluxArgs<real>& a);
+template <class real> void launch_pass_update(const Launch& L, const PassUpdArgs<real>& a);
+void launch_new_dt(hipStream_t st, int nbox, const BoxDesc* boxes, const unsigned long long* cmax,
+                   const unsigned long long* smax, const double* cour, double t_end, double* t, double* dt,
+                   double* dt_prev, int advance_time);
+void launch_reset_u64(hipStream_t st, unsigned long long* p, int n);
+template <class real> void launch_convert_in(hipStream_t st, const double* src, real* dst, long long n);
+template <class real> void launch_convert_out(hipStream_t st, const real* src, double* dst, long long n);
+template <class real>
+void launch_pass_layout(hipStream_t st, const BoxDesc& B, const double* hsrc, double* hdst, real* dev, int to_device);
+
+}  // namespace imhd

@@ -1,0 +1,773 @@
+// Host side of the imhd C ABI: device buffers, launch plans, step orchestration, native
+// time loop with an optional hipGraph, per-kernel HIP-event timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/imhd.h"
+#include "aux_kernels.hpp"
+#include "kernels.hpp"
+
+using namespace imhd;
+
+namespace {
+
+struct PlanDev {
+    PlanBox* d = nullptr;
+    long long total = 0;
+    std::vector<PlanBox> h;
+};
+
+struct ProfRec {
+    const char* name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct imhd_ctx {
+    imhd_config cfg{};
+    int nbox = 0;
+    int prec = 64;
+    size_t esz = 8;
+    std::vector<BoxDesc> hb;
+    BoxDesc* db = nullptr;
+    long long pool = 0;  // padded cells over all boxes
+    hipStream_t st = nullptr;
+    // state buffers (two) and work arrays
+    void* q[2][7] = {};
+    double* b[2][3] = {};
+    void* qp[2][MAXPASS] = {};
+    void* F[3][4] = {};
+    void* E[6] = {};
+    double* Ec[3] = {};
+    void* vel[3] = {};
+    void* vta[3] = {};
+    void* Fp[3][MAXPASS] = {};
+    void* comp[4] = {};
+    int cur = 0, pcur = 0;
+    bool have_state = false;
+    // per-box device scalars
+    unsigned long long* smax = nullptr;
+    unsigned long long* cmax = nullptr;
+    double* dt = nullptr;
+    double* t = nullptr;
+    double* dtprev = nullptr;
+    double* cour = nullptr;
+    double* diag = nullptr;
+    unsigned int* floor_count = nullptr;
+    double* stage = nullptr;  // FP64 staging for set/get
+    size_t stage_n = 0;
+    // plans
+    PlanDev pl_pflux[3], pl_pcorner, pl_pupd, pl_cflux[3], pl_ccorner, pl_cupd;
+    PlanDev pl_ghost, pl_pghost, pl_int, pl_vel, pl_ppflux[3], pl_ppupd, pl_pcflux[3], pl_pfupd;
+    int ccb_shift = 1;
+    // native loop
+    hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};  // by state/passive buffer parity
+    double graph_t_end = 0.0;
+    bool prof = false;
+    std::vector<ProfRec> prec_list;
+    std::string prof_report;
+    std::string err;
+};
+
+#define FAIL(ctx, code, ...)                                              \
+    do {                                                                  \
+        char _b[512];                                                     \
+        snprintf(_b, sizeof(_b), __VA_ARGS__);                            \
+        if (ctx) (ctx)->err = _b;                                         \
+        return code;                                                      \
+    } while (0)
+
+#define HIPCK(ctx, call)                                                                          \
+    do {                                                                                          \
+        hipError_t _e = (call);                                                                   \
+        if (_e != hipSuccess) FAIL(ctx, IMHD_EHIP, "%s failed: %s", #call, hipGetErrorString(_e)); \
+    } while (0)
+
+namespace {
+
+int build_plan(imhd_ctx* c, PlanDev& p, int which, int dir) {
+    // which: ranges as functions of (n, nb) per axis; dir = sweep direction for face plans
+    p.h.clear();
+    long long run = 0;
+    for (int bx = 0; bx < c->nbox; bx++) {
+        const BoxDesc& B = c->hb[bx];
+        PlanBox pb{};
+        for (int a = 0; a < 3; a++) {
+            int n = B.n[a], nb = B.nb, lo = 0, hi = -1;
+            bool normal = (a == dir);
+            switch (which) {
+                case 0:  // predictor fluxes (vanleer_mod.f90:437-446, offset 0)
+                    lo = 1 - nb; hi = normal ? n + nb - 1 : n + nb; break;
+                case 1:  // predictor corner EMF (vanleer_mod.f90:1900-1902)
+                    lo = 1 - nb; hi = n + nb - 1; break;
+                case 2:  // predictor update (vanleer_mod.f90:2102-2104)
+                    lo = 2 - nb; hi = n + nb - 1; break;
+                case 3:  // corrector PLM fluxes, offset 2 (+1 for the reconstruction)
+                    lo = normal ? 4 - nb : 3 - nb; hi = normal ? n + nb - 4 : n + nb - 2; break;
+                case 4:  // corrector corner EMF, offset 3
+                    lo = 4 - nb; hi = n + nb - 4; break;
+                case 5:  // final update, offset 3
+                    lo = 5 - nb; hi = n + nb - 4; break;
+                case 6:  // whole padded box
+                    lo = 1 - nb; hi = n + nb; break;
+                case 7:  // passive ghost band
+                    lo = -2; hi = n + 3; break;
+                case 8:  // interior
+                    lo = 1; hi = n; break;
+                case 9:  // copy_velocity (passive_mod.f90:212-214)
+                    lo = -2; hi = n + 3; break;
+                case 10:  // passive predictor faces (passive_mod.f90:258-267, ilow = 1)
+                    lo = -2; hi = normal ? n + 2 : n + 3; break;
+                case 11:  // passive half update (passive_mod.f90:574-576, offset 0)
+                    lo = -1; hi = n + 2; break;
+                case 12:  // passive corrector faces (ilow = 2, reconstruction offset 1)
+                    lo = normal ? 0 : -1; hi = normal ? n : n + 2; break;
+                case 13:  // passive final update (offset 2)
+                    lo = 1; hi = n; break;
+            }
+            pb.lo[a] = lo;
+            pb.r[a] = std::max(0, hi - lo + 1);
+        }
+        pb.start = run;
+        long long cnt = (long long)pb.r[0] * pb.r[1] * pb.r[2];
+        run += (cnt + 255) / 256 * 256;
+        p.h.push_back(pb);
+    }
+    p.total = run;
+    HIPCK(c, hipMalloc(&p.d, sizeof(PlanBox) * p.h.size()));
+    HIPCK(c, hipMemcpy(p.d, p.h.data(), sizeof(PlanBox) * p.h.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+Launch mk(imhd_ctx* c, const PlanDev& p) {
+    Launch L;
+    L.boxes = c->db;
+    L.plan.boxes = p.d;
+    L.plan.nbox = c->nbox;
+    L.plan.total = p.total;
+    L.stream = c->st;
+    return L;
+}
+
+void prof_begin(imhd_ctx* c, const char* name) {
+    if (!c->prof) return;
+    ProfRec r{name, nullptr, nullptr};
+    (void)hipEventCreate(&r.a);
+    (void)hipEventCreate(&r.b);
+    (void)hipEventRecord(r.a, c->st);
+    c->prec_list.push_back(r);
+}
+void prof_end(imhd_ctx* c) {
+    if (!c->prof || c->prec_list.empty()) return;
+    (void)hipEventRecord(c->prec_list.back().b, c->st);
+}
+struct ProfScope {
+    imhd_ctx* c;
+    ProfScope(imhd_ctx* c_, const char* n) : c(c_) { prof_begin(c, n); }
+    ~ProfScope() { prof_end(c); }
+};
+
+int alloc(imhd_ctx* c, void** p, size_t bytes) {
+    HIPCK(c, hipMalloc(p, bytes));
+    HIPCK(c, hipMemset(*p, 0, bytes));
+    return 0;
+}
+
+void set_scalar_dt(imhd_ctx* c, double dt);
+
+template <class real>
+real* R(void* p) { return static_cast<real*>(p); }
+
+constexpr int ET1[3] = {4, 0, 3};  // -flux(B_t1) -> Ez@x, Ex@y, Ey@z
+constexpr int ET2[3] = {2, 5, 1};  // +flux(B_t2) -> Ey@x, Ez@y, Ex@z
+
+template <class real>
+SweepArgs<real> sweep_args(imhd_ctx* c, int s, int d) {
+    SweepArgs<real> a{};
+    for (int r = 0; r < 7; r++) a.w[r] = (r == 4) ? nullptr : R<real>(c->q[s][perm(d, r)]);
+    a.bn = c->b[s][d];
+    for (int r = 0; r < 4; r++) a.F[r] = R<real>(c->F[d][perm(d, r)]);
+    a.Et1 = R<real>(c->E[ET1[d]]);
+    a.Et2 = R<real>(c->E[ET2[d]]);
+    a.dir = d;
+    a.recon = c->cfg.recon;
+    a.cs = c->cfg.cs;
+    a.mc_beta = c->cfg.mc_beta;
+    a.smax = c->smax;
+    return a;
+}
+
+template <class real, int RS>
+void sweeps(imhd_ctx* c, int s, bool corrector) {
+    static const char* nm[2][3] = {{"flux_pred_x", "flux_pred_y", "flux_pred_z"},
+                                   {"flux_corr_x", "flux_corr_y", "flux_corr_z"}};
+    for (int d = 0; d < 3; d++) {
+        ProfScope ps(c, nm[corrector][d]);
+        SweepArgs<real> a = sweep_args<real>(c, s, d);
+        if (!corrector) launch_flux_predictor<real, RS>(mk(c, c->pl_pflux[d]), a);
+        else launch_flux_corrector<real, RS>(mk(c, c->pl_cflux[d]), a);
+    }
+}
+
+template <class real>
+void corner(imhd_ctx* c, int s, const PlanDev& p, const char* name) {
+    ProfScope ps(c, name);
+    CornerArgs<real> a{};
+    for (int f = 0; f < 7; f++) a.q[f] = R<real>(c->q[s][f]);
+    for (int e = 0; e < 6; e++) a.E[e] = R<real>(c->E[e]);
+    for (int d = 0; d < 3; d++) a.Ec[d] = c->Ec[d];
+    launch_corner<real>(mk(c, p), a);
+}
+
+template <class real>
+void update(imhd_ctx* c, int src, int dst, double factor, const PlanDev& p, bool final, const char* name) {
+    ProfScope ps(c, name);
+    UpdateArgs<real> a{};
+    for (int f = 0; f < 7; f++) { a.src.q[f] = R<real>(c->q[src][f]); a.dst.q[f] = R<real>(c->q[dst][f]); }
+    for (int d = 0; d < 3; d++) { a.src.b[d] = c->b[src][d]; a.dst.b[d] = c->b[dst][d]; a.Ec[d] = c->Ec[d]; }
+    for (int d = 0; d < 3; d++)
+        for (int f = 0; f < 4; f++) a.F[d][f] = R<real>(c->F[d][f]);
+    a.dt = c->dt;
+    a.factor = factor;
+    a.ccb_lo_shift = c->ccb_shift;
+    for (int f = 0; f < 4; f++) a.comp[f] = (final && c->comp[0]) ? R<real>(c->comp[f]) : nullptr;
+    a.floor_count = c->floor_count;
+    launch_update<real>(mk(c, p), a);
+}
+
+template <class real>
+void vl_step(imhd_ctx* c) {
+    launch_reset_u64(c->st, c->smax, c->nbox);
+    const int s0 = c->cur, s1 = 1 - c->cur;
+    switch (c->cfg.riemann) {
+        case IMHD_ROE: sweeps<real, RS_ROE>(c, s0, false); break;
+        case IMHD_HLLE: sweeps<real, RS_HLLE>(c, s0, false); break;
+        default: sweeps<real, RS_HLLD>(c, s0, false); break;
+    }
+    corner<real>(c, s0, c->pl_pcorner, "corner_pred");
+    update<real>(c, s0, s1, 0.5, c->pl_pupd, false, "update_pred");
+    switch (c->cfg.riemann) {
+        case IMHD_ROE: sweeps<real, RS_ROE>(c, s1, true); break;
+        case IMHD_HLLE: sweeps<real, RS_HLLE>(c, s1, true); break;
+        default: sweeps<real, RS_HLLD>(c, s1, true); break;
+    }
+    corner<real>(c, s1, c->pl_ccorner, "corner_corr");
+    update<real>(c, s0, s1, 1.0, c->pl_cupd, true, "update_corr");
+    c->cur = s1;
+}
+
+template <class real>
+void fill_ghosts(imhd_ctx* c) {
+    {
+        ProfScope ps(c, "fill_ghosts");
+        GhostArgs<real> g{};
+        for (int f = 0; f < 7; f++) g.r[f] = R<real>(c->q[c->cur][f]);
+        for (int d = 0; d < 3; d++) g.d[d] = c->b[c->cur][d];
+        g.nreal = 7;
+        g.ndbl = 3;
+        g.depth = c->hb[0].nb;
+        launch_fill_ghosts<real>(mk(c, c->pl_ghost), g);
+    }
+    if (c->cfg.npass > 0) {
+        ProfScope ps(c, "fill_ghosts_pass");
+        GhostArgs<real> g{};
+        for (int p = 0; p < c->cfg.npass; p++) g.r[p] = R<real>(c->qp[c->pcur][p]);
+        g.nreal = c->cfg.npass;
+        g.ndbl = 0;
+        g.depth = 3;
+        launch_fill_ghosts<real>(mk(c, c->pl_pghost), g);
+    }
+}
+
+template <class real>
+void copy_velocity(imhd_ctx* c, int stage) {
+    ProfScope ps(c, stage ? "copy_velocity_tavg" : "copy_velocity");
+    VelArgs<real> a{};
+    for (int f = 0; f < 4; f++) a.q[f] = R<real>(c->q[c->cur][f]);
+    for (int d = 0; d < 3; d++) { a.vel[d] = R<real>(c->vel[d]); a.vta[d] = R<real>(c->vta[d]); }
+    a.stage = stage;
+    launch_copy_velocity<real>(mk(c, c->pl_vel), a);
+}
+
+template <class real>
+void passive_step(imhd_ctx* c) {
+    const int np = c->cfg.npass;
+    const int p0 = c->pcur, p1 = 1 - c->pcur;
+    for (int stage = 0; stage < 2; stage++) {
+        const int src = stage ? p1 : p0;
+        void** v = stage ? c->vta : c->vel;
+        for (int d = 0; d < 3; d++) {
+            ProfScope ps(c, stage ? "pass_flux_corr" : "pass_flux_pred");
+            PassFluxArgs<real> a{};
+            for (int p = 0; p < np; p++) { a.s[p] = R<real>(c->qp[src][p]); a.F[p] = R<real>(c->Fp[d][p]); }
+            a.v = R<real>(v[d]);
+            a.npass = np;
+            a.dir = d;
+            a.stage = stage;
+            launch_pass_flux<real>(mk(c, stage ? c->pl_pcflux[d] : c->pl_ppflux[d]), a);
+        }
+        ProfScope ps(c, stage ? "pass_update_corr" : "pass_update_pred");
+        PassUpdArgs<real> u{};
+        for (int p = 0; p < np; p++) {
+            u.src[p] = R<real>(c->qp[p0][p]);
+            u.dst[p] = R<real>(c->qp[p1][p]);
+            for (int d = 0; d < 3; d++) u.F[d][p] = R<real>(c->Fp[d][p]);
+        }
+        for (int d = 0; d < 3; d++) u.v[d] = R<real>(v[d]);
+        u.dt = c->dt;
+        u.factor = stage ? 1.0 : 0.5;
+        u.npass = np;
+        launch_pass_update<real>(mk(c, stage ? c->pl_pfupd : c->pl_ppupd), u);
+    }
+    c->pcur = p1;
+}
+
+template <class real>
+void max_speed_launch(imhd_ctx* c) {
+    ProfScope ps(c, "max_speed");
+    launch_reset_u64(c->st, c->cmax, c->nbox);
+    QPtrs<real> q{};
+    for (int f = 0; f < 7; f++) q.q[f] = R<real>(c->q[c->cur][f]);
+    launch_max_speed<real>(mk(c, c->pl_int), q, c->cfg.cs, c->cmax);
+}
+
+template <class real>
+void native_step(imhd_ctx* c, double t_end) {
+    if (c->cfg.npass > 0) copy_velocity<real>(c, 0);
+    vl_step<real>(c);
+    if (c->cfg.npass > 0) {
+        copy_velocity<real>(c, 1);
+        passive_step<real>(c);
+    }
+    fill_ghosts<real>(c);
+    max_speed_launch<real>(c);
+    launch_new_dt(c->st, c->nbox, c->db, c->cmax, c->smax, c->cour, t_end, c->t, c->dt, c->dtprev, 1);
+}
+
+__global__ void k_set_dt(double* dt, int n, double v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dt[i] = v;
+}
+
+void set_scalar_dt(imhd_ctx* c, double dt) {
+    hipLaunchKernelGGL(k_set_dt, dim3((c->nbox + 63) / 64), dim3(64), 0, c->st, c->dt, c->nbox, dt);
+}
+
+int prof_collect(imhd_ctx* c) {
+    if (c->prec_list.empty()) return 0;
+    HIPCK(c, hipStreamSynchronize(c->st));
+    std::map<std::string, std::pair<double, int>> acc;
+    std::vector<std::string> order;
+    for (auto& r : c->prec_list) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        if (!acc.count(r.name)) order.push_back(r.name);
+        acc[r.name].first += ms;
+        acc[r.name].second += 1;
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    c->prec_list.clear();
+    std::string s;
+    for (auto& n : order) {
+        char b[160];
+        snprintf(b, sizeof(b), "%s=%.6f/%d;", n.c_str(), acc[n].first, acc[n].second);
+        s += b;
+    }
+    c->prof_report = s;
+    return 0;
+}
+
+}  // namespace
+
+namespace {
+
+static int ensure_stage(imhd_ctx* c, size_t n) {
+    if (c->stage_n >= n) return 0;
+    if (c->stage) HIPCK(c, hipFree(c->stage));
+    c->stage = nullptr;
+    HIPCK(c, hipMalloc(&c->stage, n * 8));
+    c->stage_n = n;
+    return 0;
+}
+
+template <class real>
+static int set_state_t(imhd_ctx* c, int bx, const double* q, const double* bf, const double* qp) {
+    const BoxDesc& B = c->hb[bx];
+    const long long nc = (long long)B.P[0] * B.P[1] * B.P[2];
+    int rc;
+    if ((rc = ensure_stage(c, (size_t)nc * 7))) return rc;
+    HIPCK(c, hipMemcpy(c->stage, q, (size_t)nc * 7 * 8, hipMemcpyHostToDevice));
+    for (int s = 0; s < 2; s++)
+        for (int f = 0; f < 7; f++) launch_convert_in<real>(c->st, c->stage + f * nc, R<real>(c->q[s][f]) + B.off, nc);
+    HIPCK(c, hipStreamSynchronize(c->st));
+    for (int s = 0; s < 2; s++)
+        for (int d = 0; d < 3; d++)
+            HIPCK(c, hipMemcpy(c->b[s][d] + B.off, bf + d * nc, (size_t)nc * 8, hipMemcpyHostToDevice));
+    if (c->cfg.npass > 0 && qp) {
+        long long pn = (long long)(B.n[0] + 2 * B.nb - 4) * (B.n[1] + 2 * B.nb - 4) * (B.n[2] + 2 * B.nb - 4);
+        HIPCK(c, hipMemcpy(c->stage, qp, (size_t)pn * c->cfg.npass * 8, hipMemcpyHostToDevice));
+        for (int s = 0; s < 2; s++)
+            for (int p = 0; p < c->cfg.npass; p++)
+                launch_pass_layout<real>(c->st, B, c->stage + p * pn, nullptr, R<real>(c->qp[s][p]), 1);
+    }
+    HIPCK(c, hipStreamSynchronize(c->st));
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
+template <class real>
+static int get_state_t(imhd_ctx* c, int bx, double* q, double* bf, double* qp) {
+    const BoxDesc& B = c->hb[bx];
+    const long long nc = (long long)B.P[0] * B.P[1] * B.P[2];
+    int rc;
+    if ((rc = ensure_stage(c, (size_t)nc * 7))) return rc;
+    HIPCK(c, hipStreamSynchronize(c->st));
+    if (q) {
+        for (int f = 0; f < 7; f++) launch_convert_out<real>(c->st, R<real>(c->q[c->cur][f]) + B.off, c->stage + f * nc, nc);
+        HIPCK(c, hipStreamSynchronize(c->st));
+        HIPCK(c, hipMemcpy(q, c->stage, (size_t)nc * 7 * 8, hipMemcpyDeviceToHost));
+    }
+    if (bf)
+        for (int d = 0; d < 3; d++)
+            HIPCK(c, hipMemcpy(bf + d * nc, c->b[c->cur][d] + B.off, (size_t)nc * 8, hipMemcpyDeviceToHost));
+    if (c->cfg.npass > 0 && qp) {
+        long long pn = (long long)(B.n[0] + 2 * B.nb - 4) * (B.n[1] + 2 * B.nb - 4) * (B.n[2] + 2 * B.nb - 4);
+        for (int p = 0; p < c->cfg.npass; p++)
+            launch_pass_layout<real>(c->st, B, nullptr, c->stage + p * pn, R<real>(c->qp[c->pcur][p]), 0);
+        HIPCK(c, hipStreamSynchronize(c->st));
+        HIPCK(c, hipMemcpy(qp, c->stage, (size_t)pn * c->cfg.npass * 8, hipMemcpyDeviceToHost));
+    }
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* imhd_version(void) { return "imhd 0.1 (gfx950)"; }
+
+const char* imhd_last_error(const imhd_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int imhd_create(const imhd_config* cfg, imhd_ctx** out) { return imhd_create_ensemble(cfg, 1, out); }
+
+int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
+    if (!cfg || !out || nbox < 1 || nbox > MAXBOX) return IMHD_EINVAL;
+    *out = nullptr;
+    imhd_ctx* c = new imhd_ctx();
+    c->cfg = cfg[0];
+    c->nbox = nbox;
+    const imhd_config& g = cfg[0];
+    auto bad = [&](const char* m) { c->err = m; return IMHD_EINVAL; };
+    if (g.integrator != IMHD_VL) { int r = bad("only the VL integrator (integrator = 2) is available"); *out = c; return r; }
+    if (g.riemann < 1 || g.riemann > 3) { int r = bad("riemann must be 1 (ROE), 2 (HLLD) or 3 (HLLE)"); *out = c; return r; }
+    if (g.recon != IMHD_PLM_PRIM) { int r = bad("recon must be 0 (PLM primitive)"); *out = c; return r; }
+    if (g.precision != 32 && g.precision != 64) { int r = bad("precision must be 32 or 64"); *out = c; return r; }
+    if (g.npass < 0 || g.npass > MAXPASS) { int r = bad("npass must be in 0..4"); *out = c; return r; }
+    if (g.bc != 1 && g.bc != 2) { int r = bad("bc must be 1 (open) or 2 (periodic)"); *out = c; return r; }
+    if (c->cfg.mc_beta == 0.0) c->cfg.mc_beta = 1.01;
+    if (c->cfg.cour == 0.0) c->cfg.cour = 0.35;
+    if (c->cfg.cs <= 0.0) { int r = bad("cs must be positive"); *out = c; return r; }
+    c->prec = g.precision;
+    c->esz = g.precision == 64 ? 8 : 4;
+    long long run = 0;
+    for (int bx = 0; bx < nbox; bx++) {
+        const imhd_config& k = cfg[bx];
+        if (k.nx < 4 || k.ny < 4 || k.nz < 4) { int r = bad("each box needs at least 4 cells per axis"); *out = c; return r; }
+        if (k.nb != g.nb || k.nb < 6) { int r = bad("nb must be >= 6 and equal for all boxes"); *out = c; return r; }
+        BoxDesc B{};
+        B.n[0] = k.nx; B.n[1] = k.ny; B.n[2] = k.nz;
+        B.nb = k.nb;
+        for (int a = 0; a < 3; a++) B.P[a] = B.n[a] + 2 * k.nb;
+        B.bc = g.bc;
+        B.off = run;
+        int nmax = std::max(k.nx, std::max(k.ny, k.nz));
+        B.dx = 1.0 / (double)nmax;
+        run += (long long)B.P[0] * B.P[1] * B.P[2];
+        c->hb.push_back(B);
+    }
+    c->pool = run;
+    *out = c;
+    HIPCK(c, hipStreamCreate(&c->st));
+    HIPCK(c, hipMalloc(&c->db, sizeof(BoxDesc) * nbox));
+    HIPCK(c, hipMemcpy(c->db, c->hb.data(), sizeof(BoxDesc) * nbox, hipMemcpyHostToDevice));
+    const size_t nr = (size_t)run * c->esz, nd = (size_t)run * 8;
+    int rc;
+    for (int s = 0; s < 2; s++) {
+        for (int f = 0; f < 7; f++) if ((rc = alloc(c, &c->q[s][f], nr))) return rc;
+        for (int d = 0; d < 3; d++) if ((rc = alloc(c, (void**)&c->b[s][d], nd))) return rc;
+        for (int p = 0; p < g.npass; p++) if ((rc = alloc(c, &c->qp[s][p], nr))) return rc;
+    }
+    for (int d = 0; d < 3; d++)
+        for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->F[d][f], nr))) return rc;
+    for (int e = 0; e < 6; e++) if ((rc = alloc(c, &c->E[e], nr))) return rc;
+    for (int d = 0; d < 3; d++) if ((rc = alloc(c, (void**)&c->Ec[d], nd))) return rc;
+    if (g.npass > 0) {
+        for (int d = 0; d < 3; d++) {
+            if ((rc = alloc(c, &c->vel[d], nr))) return rc;
+            if ((rc = alloc(c, &c->vta[d], nr))) return rc;
+            for (int p = 0; p < g.npass; p++) if ((rc = alloc(c, &c->Fp[d][p], nr))) return rc;
+        }
+    }
+    if (g.precision == 32 && g.compensated)
+        for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->comp[f], nr))) return rc;
+    if ((rc = alloc(c, (void**)&c->smax, 8 * nbox))) return rc;
+    if ((rc = alloc(c, (void**)&c->cmax, 8 * nbox))) return rc;
+    if ((rc = alloc(c, (void**)&c->dt, 8 * nbox))) return rc;
+    if ((rc = alloc(c, (void**)&c->t, 8 * nbox))) return rc;
+    if ((rc = alloc(c, (void**)&c->dtprev, 8 * nbox))) return rc;
+    if ((rc = alloc(c, (void**)&c->cour, 8 * nbox))) return rc;
+    if ((rc = alloc(c, (void**)&c->diag, 8 * DIAG_STRIDE * nbox))) return rc;
+    if ((rc = alloc(c, (void**)&c->floor_count, 4))) return rc;
+    std::vector<double> v(nbox);
+    for (int bx = 0; bx < nbox; bx++) v[bx] = cfg[bx].cour != 0.0 ? cfg[bx].cour : c->cfg.cour;
+    HIPCK(c, hipMemcpy(c->cour, v.data(), 8 * nbox, hipMemcpyHostToDevice));
+    std::fill(v.begin(), v.end(), 1.0e-15);  // driver.f90:249
+    HIPCK(c, hipMemcpy(c->dt, v.data(), 8 * nbox, hipMemcpyHostToDevice));
+    for (int d = 0; d < 3; d++) {
+        if ((rc = build_plan(c, c->pl_pflux[d], 0, d))) return rc;
+        if ((rc = build_plan(c, c->pl_cflux[d], 3, d))) return rc;
+        if ((rc = build_plan(c, c->pl_ppflux[d], 10, d))) return rc;
+        if ((rc = build_plan(c, c->pl_pcflux[d], 12, d))) return rc;
+    }
+    if ((rc = build_plan(c, c->pl_pcorner, 1, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_pupd, 2, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_ccorner, 4, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_cupd, 5, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_ghost, 6, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_pghost, 7, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_int, 8, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_vel, 9, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_ppupd, 11, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_pfupd, 13, -1))) return rc;
+    return IMHD_OK;
+}
+
+int imhd_destroy(imhd_ctx* c) {
+    if (!c) return IMHD_EINVAL;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (auto& g : c->gexec) if (g) (void)hipGraphExecDestroy(g);
+    auto fr = [](void* p) { if (p) (void)hipFree(p); };
+    for (int s = 0; s < 2; s++) {
+        for (auto p : c->q[s]) fr(p);
+        for (auto p : c->b[s]) fr(p);
+        for (auto p : c->qp[s]) fr(p);
+    }
+    for (auto& a : c->F) for (auto p : a) fr(p);
+    for (auto p : c->E) fr(p);
+    for (auto p : c->Ec) fr(p);
+    for (auto p : c->vel) fr(p);
+    for (auto p : c->vta) fr(p);
+    for (auto& a : c->Fp) for (auto p : a) fr(p);
+    for (auto p : c->comp) fr(p);
+    for (void* p : {(void*)c->smax, (void*)c->cmax, (void*)c->dt, (void*)c->t, (void*)c->dtprev, (void*)c->cour,
+                    (void*)c->diag, (void*)c->floor_count, (void*)c->stage, (void*)c->db})
+        fr(p);
+    PlanDev* pls[] = {&c->pl_pflux[0], &c->pl_pflux[1], &c->pl_pflux[2], &c->pl_pcorner, &c->pl_pupd,
+                      &c->pl_cflux[0], &c->pl_cflux[1], &c->pl_cflux[2], &c->pl_ccorner, &c->pl_cupd,
+                      &c->pl_ghost, &c->pl_pghost, &c->pl_int, &c->pl_vel, &c->pl_ppflux[0], &c->pl_ppflux[1],
+                      &c->pl_ppflux[2], &c->pl_ppupd, &c->pl_pcflux[0], &c->pl_pcflux[1], &c->pl_pcflux[2],
+                      &c->pl_pfupd};
+    for (auto p : pls) fr(p->d);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return IMHD_OK;
+}
+
+int imhd_set_state(imhd_ctx* c, int box, const double* q, const double* bf, const double* qp) {
+    if (!c || !q || !bf || box < 0 || box >= c->nbox) return IMHD_EINVAL;
+    if (c->cfg.npass > 0 && !qp) FAIL(c, IMHD_EINVAL, "qpass_3d is required when npass > 0");
+    int rc = c->prec == 64 ? set_state_t<double>(c, box, q, bf, qp) : set_state_t<float>(c, box, q, bf, qp);
+    if (rc) return rc;
+    if (c->comp[0]) for (int f = 0; f < 4; f++) HIPCK(c, hipMemset(c->comp[f], 0, (size_t)c->pool * c->esz));
+    c->have_state = true;
+    return IMHD_OK;
+}
+
+int imhd_get_state(imhd_ctx* c, int box, double* q, double* bf, double* qp) {
+    if (!c || box < 0 || box >= c->nbox) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    return c->prec == 64 ? get_state_t<double>(c, box, q, bf, qp) : get_state_t<float>(c, box, q, bf, qp);
+}
+
+int imhd_step(imhd_ctx* c, double dt, int drive_next) {
+    if (!c) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    if (drive_next) FAIL(c, IMHD_EINVAL, "forcing is applied by imhd_run; drive_next must be 0");
+    if (!(dt > 0.0)) FAIL(c, IMHD_EINVAL, "dt must be positive");
+    set_scalar_dt(c, dt);
+    if (c->prec == 64) vl_step<double>(c); else vl_step<float>(c);
+    HIPCK(c, hipGetLastError());
+    return prof_collect(c);
+}
+
+int imhd_fill_ghosts(imhd_ctx* c) {
+    if (!c) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    if (c->prec == 64) fill_ghosts<double>(c); else fill_ghosts<float>(c);
+    HIPCK(c, hipGetLastError());
+    return prof_collect(c);
+}
+
+int imhd_copy_velocity(imhd_ctx* c, int stage) {
+    if (!c) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    if (c->cfg.npass <= 0) return IMHD_OK;
+    if (c->prec == 64) copy_velocity<double>(c, stage != 0); else copy_velocity<float>(c, stage != 0);
+    HIPCK(c, hipGetLastError());
+    return prof_collect(c);
+}
+
+int imhd_passive_step(imhd_ctx* c, double dt) {
+    if (!c) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    if (c->cfg.npass <= 0) return IMHD_OK;
+    if (!(dt > 0.0)) FAIL(c, IMHD_EINVAL, "dt must be positive");
+    set_scalar_dt(c, dt);
+    if (c->prec == 64) passive_step<double>(c); else passive_step<float>(c);
+    HIPCK(c, hipGetLastError());
+    return prof_collect(c);
+}
+
+int imhd_max_speed(imhd_ctx* c, double* cmax) {
+    if (!c || !cmax) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    if (c->prec == 64) max_speed_launch<double>(c); else max_speed_launch<float>(c);
+    HIPCK(c, hipGetLastError());
+    std::vector<unsigned long long> cm(c->nbox), sm(c->nbox);
+    HIPCK(c, hipStreamSynchronize(c->st));
+    HIPCK(c, hipMemcpy(cm.data(), c->cmax, 8 * c->nbox, hipMemcpyDeviceToHost));
+    HIPCK(c, hipMemcpy(sm.data(), c->smax, 8 * c->nbox, hipMemcpyDeviceToHost));
+    for (int b = 0; b < c->nbox; b++) {
+        double a, s;
+        memcpy(&a, &cm[b], 8);
+        memcpy(&s, &sm[b], 8);
+        cmax[b] = std::max(a, s);  // MAX(max_speed, smax + delv_max), delv_max = 0
+        if (cmax[b] < ZERO) FAIL(c, IMHD_ENOSPEED, "zero max speed encountered in box %d", b);
+    }
+    prof_collect(c);
+    return IMHD_OK;
+}
+
+int imhd_diagnostics(imhd_ctx* c, imhd_diag* out) {
+    if (!c || !out) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    HIPCK(c, hipMemsetAsync(c->diag, 0, 8 * DIAG_STRIDE * c->nbox, c->st));
+    if (c->prec == 64) {
+        DiagArgs<double> a{};
+        for (int f = 0; f < 4; f++) a.q[f] = R<double>(c->q[c->cur][f]);
+        for (int d = 0; d < 3; d++) a.b[d] = c->b[c->cur][d];
+        a.out = c->diag;
+        launch_diag<double>(mk(c, c->pl_int), a);
+    } else {
+        DiagArgs<float> a{};
+        for (int f = 0; f < 4; f++) a.q[f] = R<float>(c->q[c->cur][f]);
+        for (int d = 0; d < 3; d++) a.b[d] = c->b[c->cur][d];
+        a.out = c->diag;
+        launch_diag<float>(mk(c, c->pl_int), a);
+    }
+    HIPCK(c, hipGetLastError());
+    std::vector<double> h(DIAG_STRIDE * c->nbox);
+    HIPCK(c, hipStreamSynchronize(c->st));
+    HIPCK(c, hipMemcpy(h.data(), c->diag, 8 * h.size(), hipMemcpyDeviceToHost));
+    for (int b = 0; b < c->nbox; b++) {
+        double dx = c->hb[b].dx, dx3 = dx * dx * dx;
+        const double* v = &h[b * DIAG_STRIDE];
+        out[b].ke = 0.5 * v[0] * dx3;
+        out[b].vrms_sq = v[1] * dx3;
+        out[b].momx = v[2] * dx3;
+        out[b].momy = v[3] * dx3;
+        out[b].momz = v[4] * dx3;
+        out[b].divb_sum = v[5];
+        out[b].mass = v[6] * dx3;
+        out[b].divb_max = v[7];
+    }
+    return IMHD_OK;
+}
+
+int imhd_floor_count(imhd_ctx* c, unsigned int* count) {
+    if (!c || !count) return IMHD_EINVAL;
+    HIPCK(c, hipStreamSynchronize(c->st));
+    HIPCK(c, hipMemcpy(count, c->floor_count, 4, hipMemcpyDeviceToHost));
+    return IMHD_OK;
+}
+
+int imhd_set_dt(imhd_ctx* c, const double* dt) {
+    if (!c || !dt) return IMHD_EINVAL;
+    HIPCK(c, hipStreamSynchronize(c->st));
+    HIPCK(c, hipMemcpy(c->dt, dt, 8 * c->nbox, hipMemcpyHostToDevice));
+    return IMHD_OK;
+}
+
+int imhd_get_time(imhd_ctx* c, double* t, double* dt) {
+    if (!c) return IMHD_EINVAL;
+    HIPCK(c, hipStreamSynchronize(c->st));
+    if (t) HIPCK(c, hipMemcpy(t, c->t, 8 * c->nbox, hipMemcpyDeviceToHost));
+    if (dt) HIPCK(c, hipMemcpy(dt, c->dt, 8 * c->nbox, hipMemcpyDeviceToHost));
+    return IMHD_OK;
+}
+
+int imhd_set_profiling(imhd_ctx* c, int on) {
+    if (!c) return IMHD_EINVAL;
+    c->prof = on != 0;
+    return IMHD_OK;
+}
+
+const char* imhd_profile_report(imhd_ctx* c) { return c ? c->prof_report.c_str() : ""; }
+
+int imhd_sync(imhd_ctx* c) {
+    if (!c) return IMHD_EINVAL;
+    HIPCK(c, hipStreamSynchronize(c->st));
+    return IMHD_OK;
+}
+
+int imhd_run(imhd_ctx* c, int nsteps, double t_end, int use_graph) {
+    if (!c || nsteps < 0) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    if (nsteps == 0) return IMHD_OK;
+    if (use_graph && c->prof) FAIL(c, IMHD_EINVAL, "profiling needs eager launches (use_graph = 0)");
+    auto one = [&](void) {
+        if (c->prec == 64) native_step<double>(c, t_end); else native_step<float>(c, t_end);
+    };
+    if (!use_graph) {
+        for (int s = 0; s < nsteps; s++) one();
+        HIPCK(c, hipGetLastError());
+        return prof_collect(c);
+    }
+    // A step swaps the state and passive buffers; capture one step per buffer parity
+    // once and replay it.
+    if (c->graph_t_end != t_end) {
+        for (auto& g : c->gexec) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+        c->graph_t_end = t_end;
+    }
+    for (int s = 0; s < nsteps; s++) {
+        const int par = c->cur * 2 + c->pcur;
+        if (!c->gexec[par]) {
+            hipGraph_t g;
+            HIPCK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+            int cur0 = c->cur, pcur0 = c->pcur;
+            one();
+            hipError_t e = hipStreamEndCapture(c->st, &g);
+            if (e != hipSuccess) FAIL(c, IMHD_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+            HIPCK(c, hipGraphInstantiate(&c->gexec[par], g, nullptr, nullptr, 0));
+            HIPCK(c, hipGraphDestroy(g));
+            c->cur = cur0;
+            c->pcur = pcur0;
+        }
+        HIPCK(c, hipGraphLaunch(c->gexec[par], c->st));
+        c->cur = 1 - c->cur;
+        if (c->cfg.npass > 0) c->pcur = 1 - c->pcur;
+    }
+    HIPCK(c, hipGetLastError());
+    return IMHD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// Shared device-side definitions of the MI355X isothermal-MHD cell-update library.
+//
+// Memory layout in HBM (per box): every field is its own ghost-padded 3-D array with the
+// reference's index meaning (grid_data_mod.f90:38-42): cell (i,j,k), i in [1-nb, nx+nb],
+// x fastest.  Fields of all boxes of an ensemble live in one pool per field; a box is a
+// contiguous slice at BoxDesc::off.  Work items of a launch are enumerated box by box in
+// x-fastest order (a "plan"), so one launch serves one box or an uneven ensemble.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace imhd {
+
+constexpr int MAXBOX = 256;
+constexpr int MAXPASS = 4;
+constexpr double ZERO = 1.0e-20;      // vanleer_mod.f90:75
+constexpr double MIN_DENS = 1.0e-20;  // vanleer_mod.f90:76
+
+// Riemann solver ids, reference numbering of the VL module (constants_mod.f90:57)
+enum { RS_ROE = 1, RS_HLLD = 2, RS_HLLE = 3 };
+
+struct BoxDesc {
+    int n[3];        // interior cells
+    int nb;          // ghost layers
+    int P[3];        // padded extents n + 2 nb
+    int bc;          // 1 open, 2 periodic
+    long long off;   // first padded cell of this box in each field pool
+    double dx;       // 1 / max(n)  (driver.f90:254)
+};
+
+// One box's share of a launch: cells lo .. lo + r - 1 (box index space), work items
+// start .. start + r0*r1*r2 - 1 of the launch.
+struct PlanBox {
+    int lo[3];
+    int r[3];
+    long long start;
+};
+
+struct Plan {
+    const PlanBox* boxes;  // device
+    int nbox;
+    long long total;
+};
+
+__device__ __forceinline__ long long cell_index(const BoxDesc& b, int i, int j, int k) {
+    return b.off + (long long)(i + b.nb - 1) +
+           (long long)b.P[0] * ((long long)(j + b.nb - 1) + (long long)b.P[1] * (long long)(k + b.nb - 1));
+}
+
+// Workgroup ids are dealt round-robin over the 8 XCDs; renumber them so that each XCD
+// walks one contiguous chunk of the work list (neighbouring stencil rows then share
+// that XCD's L2).
+__device__ __forceinline__ unsigned xcd_block(unsigned w, unsigned G) {
+    unsigned q = G >> 3, r = G & 7u, x = w & 7u, idx = w >> 3;
+    unsigned base = (x < r) ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+    return base + idx;
+}
+
+// Maps this thread to (box, i, j, k).  Returns false for padding threads.
+__device__ __forceinline__ bool plan_item(const Plan& p, int& box, int& i, int& j, int& k) {
+    unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    long long g = (long long)blk * blockDim.x + threadIdx.x;
+    if (g >= p.total) return false;
+    int b = 0;
+    if (p.nbox > 1) {
+        int lo = 0, hi = p.nbox - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (p.boxes[mid].start <= g) lo = mid; else hi = mid - 1;
+        }
+        b = lo;
+    }
+    const PlanBox& pb = p.boxes[b];
+    long long loc = g - pb.start;
+    long long r0 = pb.r[0], r1 = pb.r[1];
+    if (loc >= r0 * r1 * (long long)pb.r[2]) return false;
+    long long t = loc / r0;
+    i = pb.lo[0] + (int)(loc - t * r0);
+    long long u = t / r1;
+    j = pb.lo[1] + (int)(t - u * r1);
+    k = pb.lo[2] + (int)u;
+    box = b;
+    return true;
+}
+
+template <class real>
+__device__ __forceinline__ real sgn1(real x) { return copysign(real(1), x); }  // Fortran SIGN(1.d0, x)
+
+template <class real>
+__device__ __forceinline__ real min3(real a, real b, real c) { real m = a < b ? a : b; return m < c ? m : c; }
+template <class real>
+__device__ __forceinline__ real max3(real a, real b, real c) { real m = a > b ? a : b; return m > c ? m : c; }
+template <class real>
+__device__ __forceinline__ real max2(real a, real b) { return a > b ? a : b; }
+template <class real>
+__device__ __forceinline__ real min2(real a, real b) { return a < b ? a : b; }
+
+// Non-negative doubles order like their bit patterns, so the CFL/smax maxima use an
+// integer atomicMax.  Wavefront reduction first (64 lanes), then one atomic per wave.
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* addr, double v) {
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+// rotated-component permutation of a directional sweep (vanleer_mod.f90:449-464,
+// 610-625, 788-803): rotated slot r <- global field perm(d, r)
+__host__ __device__ constexpr int perm(int d, int r) {
+    return d == 0 ? r
+                  : (d == 1 ? (r == 1 ? 2 : r == 2 ? 3 : r == 3 ? 1 : r == 4 ? 5 : r == 5 ? 6 : r == 6 ? 4 : 0)
+                            : (r == 1 ? 3 : r == 2 ? 1 : r == 3 ? 2 : r == 4 ? 6 : r == 5 ? 4 : r == 6 ? 5 : 0));
+}
+
+}  // namespace imhd

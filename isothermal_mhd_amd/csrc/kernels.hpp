@@ -1,0 +1,68 @@
+// Kernel argument blocks and host-side launch entry points of the cell-update kernels.
+#pragma once
+#include "imhd_common.hpp"
+
+namespace imhd {
+
+// A directional sweep sees the state rotated so that its direction is "x":
+// w[r] = q[perm(d, r)] for r = 0..6 (w[4] unused; the normal field comes from bn).
+template <class real>
+struct SweepArgs {
+    const real* w[7];
+    const double* bn;
+    real* F[4];       // rotated hydro fluxes: rho, m_n, m_t1, m_t2
+    real* Et1;        // face EMF = -flux(B_t1)
+    real* Et2;        // face EMF = +flux(B_t2)
+    int dir;
+    int recon;        // corrector reconstruction (0 PLM primitive MC-beta)
+    double cs;
+    double mc_beta;
+    unsigned long long* smax;  // per box, bit pattern of a non-negative double
+};
+
+template <class real>
+struct StateP {
+    real* q[7];
+    double* b[3];
+};
+
+// Face EMFs in the reference's emf_3d(comp, slot) layout (vanleer_mod.f90:1868-1883):
+// E[2*comp + slot]; Ex: slot0 at y-face, slot1 at z-face; Ey: x-face, z-face;
+// Ez: x-face, y-face.
+template <class real>
+struct CornerArgs {
+    const real* q[7];      // state used for the centre EMF and the upwinding velocity
+    const real* E[6];
+    double* Ec[3];         // corner EMFs Ex(j+1/2,k+1/2), Ey(i+1/2,k+1/2), Ez(i+1/2,j+1/2)
+};
+
+template <class real>
+struct UpdateArgs {
+    StateP<real> src;      // state at t^n
+    StateP<real> dst;      // updated state
+    const real* F[3][4];   // hydro fluxes per direction, global component order
+    const double* Ec[3];
+    const double* dt;      // per-box time step
+    double factor;         // 0.5 predictor, 1 corrector
+    int ccb_lo_shift;      // cell-centred B is refreshed on [lo + shift, hi]
+    real* comp[4];         // Kahan compensation of rho, m (FP32 only; null = off)
+    unsigned int* floor_count;
+};
+
+struct Launch {
+    const BoxDesc* boxes;
+    Plan plan;
+    hipStream_t stream;
+    unsigned blocks() const { return (unsigned)((plan.total + 255) / 256); }
+};
+
+template <class real, int RS>
+void launch_flux_predictor(const Launch& L, const SweepArgs<real>& a);
+template <class real, int RS>
+void launch_flux_corrector(const Launch& L, const SweepArgs<real>& a);
+template <class real>
+void launch_corner(const Launch& L, const CornerArgs<real>& a);
+template <class real>
+void launch_update(const Launch& L, const UpdateArgs<real>& a);
+
+}  // namespace imhd

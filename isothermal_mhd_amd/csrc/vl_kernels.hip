@@ -1,0 +1,274 @@
+// VL2 + CT cell-update kernels (Stone & Gardiner 2009 unsplit predictor-corrector with
+// upwind constrained transport), following vanleer_mod.f90 of the reference.
+//
+// Per stage:  k_flux (x3 directions)  ->  k_corner  ->  k_update
+// Fluxes and face EMFs go to HBM once per face; the corner EMF is summed in FP64; the
+// update writes a second state buffer (no copy/restore of the t^n state).
+#include "kernels.hpp"
+#include "riemann.hpp"
+
+namespace imhd {
+
+__device__ __forceinline__ long long dir_stride(const BoxDesc& B, int d) {
+    return d == 0 ? 1LL : (d == 1 ? (long long)B.P[0] : (long long)B.P[0] * (long long)B.P[1]);
+}
+
+// Plan lookup that also reports the box for inactive (padding) lanes so that every lane
+// of a wavefront joins the smax reduction; box starts are 64-aligned.
+__device__ __forceinline__ bool plan_item_box(const Plan& p, int& box, int& i, int& j, int& k) {
+    box = 0;
+    bool act = plan_item(p, box, i, j, k);
+    if (!act && p.nbox > 1) {
+        unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+        long long g = (long long)blk * blockDim.x + threadIdx.x;
+        int lo = 0, hi = p.nbox - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (p.boxes[mid].start <= g) lo = mid; else hi = mid - 1;
+        }
+        box = lo;
+    }
+    return act;
+}
+
+template <class real>
+__device__ __forceinline__ void store_flux(const SweepArgs<real>& a, long long c, const real* F) {
+    a.F[0][c] = F[0];
+    a.F[1][c] = F[1];
+    a.F[2][c] = F[2];
+    a.F[3][c] = F[3];
+    a.Et1[c] = -F[5];
+    a.Et2[c] = F[6];
+}
+
+// First-order predictor fluxes: interface states are the adjacent cell values with the
+// face-centred normal field (vanleer_mod.f90:443-467).
+template <class real, int RS>
+__global__ __launch_bounds__(256) void k_flux_pred(const BoxDesc* __restrict__ boxes, Plan plan, SweepArgs<real> a) {
+    int b, i, j, k;
+    bool act = plan_item_box(plan, b, i, j, k);
+    double sm = 0.0;
+    if (act) {
+        const BoxDesc& B = boxes[b];
+        long long c = cell_index(B, i, j, k);
+        long long s = dir_stride(B, a.dir);
+        real uL[7], uR[7], F[7];
+#pragma unroll
+        for (int r = 0; r < 7; r++) {
+            if (r == 4) continue;
+            uL[r] = a.w[r][c];
+            uR[r] = a.w[r][c + s];
+        }
+        uL[4] = (real)a.bn[c];
+        uR[4] = uL[4];
+        real smax = real(0);
+        riemann<RS>(uL, uR, (real)a.cs, F, smax);
+        store_flux(a, c, F);
+        sm = (double)smax;
+    }
+    atomic_max_nonneg(&a.smax[b], sm);
+}
+
+// Corrector fluxes: primitive PLM reconstruction with the MC-beta slope
+// sign(dC) min(beta|dL|, beta|dR|, |dC|)  -- no zeroing at extrema, as in the
+// reference (vanleer_mod.f90:1160-1215, 1467-1481) -- then the Riemann solver.
+template <class real, int RS>
+__global__ __launch_bounds__(256) void k_flux_corr(const BoxDesc* __restrict__ boxes, Plan plan, SweepArgs<real> a) {
+    int b, i, j, k;
+    bool act = plan_item_box(plan, b, i, j, k);
+    double sm = 0.0;
+    if (act) {
+        const BoxDesc& B = boxes[b];
+        long long c = cell_index(B, i, j, k);
+        long long s = dir_stride(B, a.dir);
+        real W[4][6];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            long long ct = c + (long long)(t - 1) * s;
+            real rho = a.w[0][ct];
+            real irho = real(1) / rho;
+            W[t][0] = rho;
+            W[t][1] = a.w[1][ct] * irho;
+            W[t][2] = a.w[2][ct] * irho;
+            W[t][3] = a.w[3][ct] * irho;
+            W[t][4] = a.w[5][ct];
+            W[t][5] = a.w[6][ct];
+        }
+        const real beta = (real)a.mc_beta;
+        real d1[6], d2[6];
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            real dL = W[1][m] - W[0][m], dR = W[2][m] - W[1][m], dC = real(0.5) * (dL + dR);
+            d1[m] = sgn1(dC) * min3(beta * fabs(dL), beta * fabs(dR), fabs(dC));
+            dL = W[2][m] - W[1][m]; dR = W[3][m] - W[2][m]; dC = real(0.5) * (dL + dR);
+            d2[m] = sgn1(dC) * min3(beta * fabs(dL), beta * fabs(dR), fabs(dC));
+        }
+        real uL[7], uR[7], F[7];
+        real wb = (real)a.bn[c];
+        uL[0] = W[1][0] + real(0.5) * d1[0];
+        uL[1] = W[1][1] + real(0.5) * d1[1];
+        uL[2] = W[1][2] + real(0.5) * d1[2];
+        uL[3] = W[1][3] + real(0.5) * d1[3];
+        uL[4] = wb;
+        uL[5] = W[1][4] + real(0.5) * d1[4];
+        uL[6] = W[1][5] + real(0.5) * d1[5];
+        uR[0] = W[2][0] - real(0.5) * d2[0];
+        uR[1] = W[2][1] - real(0.5) * d2[1];
+        uR[2] = W[2][2] - real(0.5) * d2[2];
+        uR[3] = W[2][3] - real(0.5) * d2[3];
+        uR[4] = wb;
+        uR[5] = W[2][4] - real(0.5) * d2[4];
+        uR[6] = W[2][5] - real(0.5) * d2[5];
+        uL[1] = uL[1] * uL[0]; uL[2] = uL[2] * uL[0]; uL[3] = uL[3] * uL[0];
+        uR[1] = uR[1] * uR[0]; uR[2] = uR[2] * uR[0]; uR[3] = uR[3] * uR[0];
+        real smax = real(0);
+        riemann<RS>(uL, uR, (real)a.cs, F, smax);
+        store_flux(a, c, F);
+        sm = (double)smax;
+    }
+    atomic_max_nonneg(&a.smax[b], sm);
+}
+
+__device__ __forceinline__ void upw(double v, double L14, double R14, double L34, double R34, double& o14,
+                                    double& o34) {
+    if (v > ZERO) { o14 = L14; o34 = L34; }
+    else if (v < -ZERO) { o14 = R14; o34 = R34; }
+    else { o14 = 0.5 * (L14 + R14); o34 = 0.5 * (L34 + R34); }
+}
+
+// Gardiner-Stone upwind corner EMFs (vanleer_mod.f90:1884-2076), summed in FP64.  The
+// cell-centre reference EMF E = -v x B (vanleer_mod.f90:1823-1862) is formed on the fly.
+template <class real>
+__global__ __launch_bounds__(256) void k_corner(const BoxDesc* __restrict__ boxes, Plan plan, CornerArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const long long c = cell_index(B, i, j, k);
+    const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+    auto Q = [&](int f, long long p) -> double { return (double)a.q[f][p]; };
+    auto Exc = [&](long long p) -> double { double ir = 1.0 / Q(0, p); return (Q(5, p) * Q(3, p) - Q(6, p) * Q(2, p)) * ir; };
+    auto Eyc = [&](long long p) -> double { double ir = 1.0 / Q(0, p); return (Q(6, p) * Q(1, p) - Q(4, p) * Q(3, p)) * ir; };
+    auto Ezc = [&](long long p) -> double { double ir = 1.0 / Q(0, p); return (Q(4, p) * Q(2, p) - Q(5, p) * Q(1, p)) * ir; };
+    auto E = [&](int comp, int slot, long long p) -> double { return (double)a.E[2 * comp + slot][p]; };
+    const double vx = 0.5 * (Q(1, c) / Q(0, c) + Q(1, c + sx) / Q(0, c + sx));
+    const double vy = 0.5 * (Q(2, c) / Q(0, c) + Q(2, c + sy) / Q(0, c + sy));
+    const double vz = 0.5 * (Q(3, c) / Q(0, c) + Q(3, c + sz) / Q(0, c + sz));
+    double a14, a34, b14, b34;
+    {   // Ez at (i+1/2, j+1/2, k)
+        double e00 = Ezc(c), e10 = Ezc(c + sx), e01 = Ezc(c + sy), e11 = Ezc(c + sx + sy);
+        double y0 = E(2, 1, c), y1 = E(2, 1, c + sx), x0 = E(2, 0, c), x1 = E(2, 0, c + sy);
+        upw(vx, 2.0 * (y0 - e00), 2.0 * (y1 - e10), 2.0 * (e01 - y0), 2.0 * (e11 - y1), a14, a34);
+        upw(vy, 2.0 * (x0 - e00), 2.0 * (x1 - e01), 2.0 * (e10 - x0), 2.0 * (e11 - x1), b14, b34);
+        a.Ec[2][c] = 0.25 * (x0 + x1 + y0 + y1) + 0.125 * ((a14 - a34) + (b14 - b34));
+    }
+    {   // Ey at (i+1/2, j, k+1/2)
+        double e00 = Eyc(c), e10 = Eyc(c + sx), e01 = Eyc(c + sz), e11 = Eyc(c + sx + sz);
+        double z0 = E(1, 1, c), z1 = E(1, 1, c + sx), x0 = E(1, 0, c), x1 = E(1, 0, c + sz);
+        upw(vx, 2.0 * (z0 - e00), 2.0 * (z1 - e10), 2.0 * (e01 - z0), 2.0 * (e11 - z1), a14, a34);
+        upw(vz, 2.0 * (x0 - e00), 2.0 * (x1 - e01), 2.0 * (e10 - x0), 2.0 * (e11 - x1), b14, b34);
+        a.Ec[1][c] = 0.25 * (x0 + x1 + z0 + z1) + 0.125 * ((a14 - a34) + (b14 - b34));
+    }
+    {   // Ex at (i, j+1/2, k+1/2)
+        double e00 = Exc(c), e10 = Exc(c + sy), e01 = Exc(c + sz), e11 = Exc(c + sy + sz);
+        double z0 = E(0, 1, c), z1 = E(0, 1, c + sy), y0 = E(0, 0, c), y1 = E(0, 0, c + sz);
+        upw(vy, 2.0 * (z0 - e00), 2.0 * (z1 - e10), 2.0 * (e01 - z0), 2.0 * (e11 - z1), a14, a34);
+        upw(vz, 2.0 * (y0 - e00), 2.0 * (y1 - e01), 2.0 * (e10 - y0), 2.0 * (e11 - y1), b14, b34);
+        a.Ec[0][c] = 0.25 * (y0 + y1 + z0 + z1) + 0.125 * ((a14 - a34) + (b14 - b34));
+    }
+}
+
+// Unsplit conservative update of rho and rho v (vanleer_mod.f90:2080-2116) with the
+// density floor (:3243-3262), CT face update (:2218-2236) and face-average cell-centred B
+// (:2241-2256).  The lower faces needed by the face average are recomputed from the
+// corner EMFs, so the kernel never reads a face it writes.
+template <class real>
+__global__ __launch_bounds__(256) void k_update(const BoxDesc* __restrict__ boxes, Plan plan, UpdateArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const long long c = cell_index(B, i, j, k);
+    const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+    const double dtd = a.factor * a.dt[b] / B.dx;
+    const real dtdx = (real)dtd;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        real div = (a.F[0][f][c] - a.F[0][f][c - sx]) + (a.F[1][f][c] - a.F[1][f][c - sy]) +
+                   (a.F[2][f][c] - a.F[2][f][c - sz]);
+        real q0 = a.src.q[f][c];
+        real v;
+        if (a.comp[0] != nullptr) {  // compensated (Kahan) accumulation of the FP32 state
+            real y = -(dtdx * div) - a.comp[f][c];
+            v = q0 + y;
+            a.comp[f][c] = (v - q0) - y;
+        } else {
+            v = q0 - dtdx * div;
+        }
+        if (f == 0 && v < (real)MIN_DENS) {
+            v = (real)MIN_DENS;
+            if (a.comp[0] != nullptr) a.comp[0][c] = real(0);
+            atomicAdd(a.floor_count, 1u);
+        }
+        a.dst.q[f][c] = v;
+    }
+    const double* Ex = a.Ec[0];
+    const double* Ey = a.Ec[1];
+    const double* Ez = a.Ec[2];
+    double bx = a.src.b[0][c] - dtd * ((Ez[c] - Ez[c - sy]) - (Ey[c] - Ey[c - sz]));
+    double by = a.src.b[1][c] - dtd * ((Ex[c] - Ex[c - sz]) - (Ez[c] - Ez[c - sx]));
+    double bz = a.src.b[2][c] - dtd * ((Ey[c] - Ey[c - sx]) - (Ex[c] - Ex[c - sy]));
+    a.dst.b[0][c] = bx;
+    a.dst.b[1][c] = by;
+    a.dst.b[2][c] = bz;
+    const PlanBox& pb = plan.boxes[b];
+    const int sh = a.ccb_lo_shift;
+    if (i >= pb.lo[0] + sh && j >= pb.lo[1] + sh && k >= pb.lo[2] + sh) {
+        long long m = c - sx;
+        double bxm = a.src.b[0][m] - dtd * ((Ez[m] - Ez[m - sy]) - (Ey[m] - Ey[m - sz]));
+        m = c - sy;
+        double bym = a.src.b[1][m] - dtd * ((Ex[m] - Ex[m - sz]) - (Ez[m] - Ez[m - sx]));
+        m = c - sz;
+        double bzm = a.src.b[2][m] - dtd * ((Ey[m] - Ey[m - sx]) - (Ex[m] - Ex[m - sy]));
+        a.dst.q[4][c] = (real)(0.5 * (bx + bxm));
+        a.dst.q[5][c] = (real)(0.5 * (by + bym));
+        a.dst.q[6][c] = (real)(0.5 * (bz + bzm));
+    } else {
+        a.dst.q[4][c] = a.src.q[4][c];
+        a.dst.q[5][c] = a.src.q[5][c];
+        a.dst.q[6][c] = a.src.q[6][c];
+    }
+}
+
+template <class real, int RS>
+void launch_flux_predictor(const Launch& L, const SweepArgs<real>& a) {
+    if (L.plan.total <= 0) return;
+    hipLaunchKernelGGL((k_flux_pred<real, RS>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+template <class real, int RS>
+void launch_flux_corrector(const Launch& L, const SweepArgs<real>& a) {
+    if (L.plan.total <= 0) return;
+    hipLaunchKernelGGL((k_flux_corr<real, RS>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+template <class real>
+void launch_corner(const Launch& L, const CornerArgs<real>& a) {
+    if (L.plan.total <= 0) return;
+    hipLaunchKernelGGL((k_corner<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+template <class real>
+void launch_update(const Launch& L, const UpdateArgs<real>& a) {
+    if (L.plan.total <= 0) return;
+    hipLaunchKernelGGL((k_update<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+
+#define INST_RS(T, RS)                                                              \
+    template void launch_flux_predictor<T, RS>(const Launch&, const SweepArgs<T>&); \
+    template void launch_flux_corrector<T, RS>(const Launch&, const SweepArgs<T>&);
+#define INST(T)                                                              \
+    INST_RS(T, RS_ROE)                                                       \
+    INST_RS(T, RS_HLLD)                                                      \
+    INST_RS(T, RS_HLLE)                                                      \
+    template void launch_corner<T>(const Launch&, const CornerArgs<T>&);     \
+    template void launch_update<T>(const Launch&, const UpdateArgs<T>&);
+INST(double)
+INST(float)
+
+}  // namespace imhd

@@ -1,0 +1,180 @@
+"""Host-side mirror of the reference's solver interface over the imhd C ABI.
+
+Method names follow the reference's Fortran procedures (3D_MPI/src):
+vanleer_solve_3d, boundary_conditions, get_max_speed_vanleer, copy_velocity,
+passive_solve_3d, compute_KE/compute_divB, and the driver's time loop (driver.f90:35-186).
+Arrays are the reference's host layout (FP64, Fortran order, nb ghost layers).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+_RIEMANN = {"roe": L.IMHD_ROE, "hlld": L.IMHD_HLLD, "hlle": L.IMHD_HLLE}
+_BC = {"open": L.IMHD_BC_OPEN, "periodic": L.IMHD_BC_PERIODIC}
+
+
+def padded_shapes(nx, ny, nz, nb, npass):
+    """Host shapes of q_3d / bface_3d (without the field axis) and qpass_3d."""
+    sh = (nx + 2 * nb, ny + 2 * nb, nz + 2 * nb)
+    shp = (nx + 2 * nb - 4, ny + 2 * nb - 4, nz + 2 * nb - 4, max(npass, 1))
+    return sh, shp
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class VanLeerSolver:
+    """VL2 + CT isothermal MHD on MI355X for one box or an ensemble of boxes.
+
+    `dims` is (nx, ny, nz) or a list of such triples (ensemble: one launch per kernel
+    serves every box; each box keeps its own dt).
+    """
+
+    def __init__(self, dims, nb=6, npass=0, riemann="hlld", recon="plm", bc="periodic",
+                 precision=64, compensated=None, cs=1.0, mc_beta=1.01, cour=0.35):
+        self.lib = L.lib()
+        boxes = [tuple(dims)] if np.ndim(dims) == 1 else [tuple(d) for d in dims]
+        if recon != "plm":
+            raise ValueError("recon must be 'plm' (primitive PLM with the MC-beta limiter)")
+        if compensated is None:
+            compensated = precision == 32
+        cfgs = (L.ImhdConfig * len(boxes))()
+        for i, (nx, ny, nz) in enumerate(boxes):
+            c = cfgs[i]
+            c.nx, c.ny, c.nz, c.nb, c.npass = nx, ny, nz, nb, npass
+            c.integrator, c.riemann, c.recon = L.IMHD_VL, _RIEMANN[riemann], L.IMHD_PLM_PRIM
+            c.bc, c.precision, c.compensated = _BC[bc], precision, int(bool(compensated))
+            c.cs, c.mc_beta = cs, mc_beta
+            c.cour = cour[i] if np.ndim(cour) else cour
+        self.boxes, self.nb, self.npass, self.precision = boxes, nb, npass, precision
+        self._c = ctypes.c_void_p()
+        rc = self.lib.imhd_create_ensemble(cfgs, len(boxes), ctypes.byref(self._c))
+        self._check(rc)
+
+    # -- plumbing -------------------------------------------------------------------
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.imhd_last_error(self._c).decode() if self._c else ""
+            raise L.ImhdError(f"{L.ERRORS.get(rc, rc)}: {msg}")
+
+    def close(self):
+        if self._c:
+            self.lib.imhd_destroy(self._c)
+            self._c = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shapes(self, box=0):
+        nx, ny, nz = self.boxes[box]
+        return padded_shapes(nx, ny, nz, self.nb, self.npass)
+
+    # -- state transfer -------------------------------------------------------------
+    def set_state(self, q_3d, bface_3d, qpass_3d=None, box=0):
+        q = np.asfortranarray(q_3d, dtype=np.float64)
+        b = np.asfortranarray(bface_3d, dtype=np.float64)
+        sh, shp = self.shapes(box)
+        if q.shape != sh + (7,) or b.shape != sh + (3,):
+            raise ValueError(f"expected q_3d {sh + (7,)} and bface_3d {sh + (3,)}")
+        qp = None
+        if self.npass > 0:
+            if qpass_3d is None:
+                raise ValueError("qpass_3d required when npass > 0")
+            qp = np.asfortranarray(qpass_3d, dtype=np.float64)
+            if qp.shape[:3] != shp[:3] or qp.shape[3] < self.npass:
+                raise ValueError(f"expected qpass_3d {shp[:3] + (self.npass,)}")
+            qp = np.asfortranarray(qp[..., :self.npass])
+        self._check(self.lib.imhd_set_state(self._c, box, _p(q), _p(b), _p(qp) if qp is not None else None))
+
+    def get_state(self, box=0):
+        sh, shp = self.shapes(box)
+        q = np.zeros(sh + (7,), order="F")
+        b = np.zeros(sh + (3,), order="F")
+        qp = np.zeros(shp[:3] + (max(self.npass, 1),), order="F")
+        self._check(self.lib.imhd_get_state(self._c, box, _p(q), _p(b), _p(qp)))
+        return q, b, qp
+
+    # -- reference procedures ------------------------------------------------------
+    def vanleer_solve_3d(self, dt, drive_next=False):
+        """vanleer_mod.f90:279-426 on every box with the given dt."""
+        self._check(self.lib.imhd_step(self._c, float(dt), int(drive_next)))
+
+    def boundary_conditions(self):
+        """boundary_conditions_mod.f90:20-234 (one rank)."""
+        self._check(self.lib.imhd_fill_ghosts(self._c))
+
+    def get_max_speed_vanleer(self):
+        """vanleer_mod.f90:3266-3334; one value per box."""
+        out = np.zeros(len(self.boxes))
+        self._check(self.lib.imhd_max_speed(self._c, _p(out)))
+        return out if len(self.boxes) > 1 else float(out[0])
+
+    def copy_velocity(self, predictor_stage):
+        """passive_mod.f90:204-244."""
+        self._check(self.lib.imhd_copy_velocity(self._c, int(bool(predictor_stage))))
+
+    def passive_solve_3d(self, dt):
+        """passive_mod.f90:118-201."""
+        self._check(self.lib.imhd_passive_step(self._c, float(dt)))
+
+    def diagnostics(self):
+        """compute_KE + compute_divB (driver.f90:492-553), plus max|div B| and mass."""
+        out = (L.ImhdDiag * len(self.boxes))()
+        self._check(self.lib.imhd_diagnostics(self._c, out))
+        res = [{n: getattr(d, n) for n, _ in L.ImhdDiag._fields_} for d in out]
+        return res if len(self.boxes) > 1 else res[0]
+
+    def floor_count(self):
+        n = ctypes.c_uint()
+        self._check(self.lib.imhd_floor_count(self._c, ctypes.byref(n)))
+        return n.value
+
+    # -- native loop ----------------------------------------------------------------
+    def set_dt(self, dt):
+        v = np.broadcast_to(np.asarray(dt, dtype=np.float64), (len(self.boxes),)).copy()
+        self._check(self.lib.imhd_set_dt(self._c, _p(v)))
+
+    def get_time(self):
+        t = np.zeros(len(self.boxes))
+        dt = np.zeros(len(self.boxes))
+        self._check(self.lib.imhd_get_time(self._c, _p(t), _p(dt)))
+        return t, dt
+
+    def run(self, nsteps, t_end=1e30, graph=True):
+        """Device-resident time loop (driver.f90:35-186) for nsteps steps."""
+        self._check(self.lib.imhd_run(self._c, int(nsteps), float(t_end), int(bool(graph))))
+
+    def sync(self):
+        self._check(self.lib.imhd_sync(self._c))
+
+    def set_profiling(self, on=True):
+        self._check(self.lib.imhd_set_profiling(self._c, int(bool(on))))
+
+    def profile(self):
+        """Per-kernel HIP-event time of the last eager call: {name: (ms_total, launches)}."""
+        rep = self.lib.imhd_profile_report(self._c).decode()
+        out = {}
+        for item in filter(None, rep.split(";")):
+            name, v = item.split("=")
+            ms, n = v.split("/")
+            out[name] = (float(ms), int(n))
+        return out
+
+
+def driver_step(solver, dt):
+    """One pass of the reference driver loop body with a host dt (driver.f90:55-160):
+    copy_velocity(.FALSE.), vanleer_solve_3d, copy_velocity(.TRUE.), passive_solve_3d,
+    boundary_conditions."""
+    if solver.npass > 0:
+        solver.copy_velocity(False)
+    solver.vanleer_solve_3d(dt)
+    if solver.npass > 0:
+        solver.copy_velocity(True)
+        solver.passive_solve_3d(dt)
+    solver.boundary_conditions()
