@@ -112,12 +112,12 @@ int imhd_floor_count(imhd_ctx* ctx, unsigned int* count);
  * dt = 1e-15 (driver.f90:249) unless imhd_set_dt was called.  With use_graph != 0 one
  * step is captured once in a hipGraph and replayed. */
 int imhd_run(imhd_ctx* ctx, int nsteps, double t_end, int use_graph);
-int imhd_set_dt(imhd_ctx* ctx, const double* dt);          /* per box */
+int imhd_set_time(imhd_ctx* ctx, const double* t, const double* dt); /* per box; NULL keeps */
 int imhd_get_time(imhd_ctx* ctx, double* t, double* dt);    /* per box */
 
 /* Per-kernel device time of the last imhd_run, from HIP events on the context's stream
  * (enable with imhd_set_profiling before the run).  Returns a ';'-separated
- * "name=ms/launches" list. */
+ * "name=ms_total/launches/work_items_total" list. */
 int imhd_set_profiling(imhd_ctx* ctx, int on);
 const char* imhd_profile_report(imhd_ctx* ctx);
 

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256) void k_max_speed(const BoxDesc* __restrict__ b
         m = max3(fabs(vx) + cfx, fabs(vy) + cfy, fabs(vz) + cfz);
     }
     // interior plans are 64-aligned per box: a wave never spans two boxes
-    atomic_max_nonneg(&cmax[b], m);
+    atomic_max_nonneg(cmax, b, m);
 }
 
 // dt = COUR dx / max(cmax, smax + delv_max), then MIN(t_end - t, dt) (driver.f90:432-448, 475)
@@ -68,8 +68,8 @@ __global__ void k_new_dt(int nbox, const BoxDesc* __restrict__ boxes, const unsi
                          double* dt_prev, int advance_time) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbox) return;
-    double cm = __longlong_as_double((long long)cmax[b]);
-    double sm = __longlong_as_double((long long)smax[b]);
+    double cm = striped_max(cmax, b);
+    double sm = striped_max(smax, b);
     double c = fmax(cm, sm);
     if (advance_time) t[b] = t[b] + dt[b];
     dt_prev[b] = dt[b];
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_diag(const BoxDesc* __restrict__ boxes,
         double s = block_sum(v[f], sh);
         if (threadIdx.x == 0) atomicAdd(&a.out[b * DIAG_STRIDE + f], s);
     }
-    atomic_max_nonneg(reinterpret_cast<unsigned long long*>(&a.out[b * DIAG_STRIDE + 7]), dmax);
+    atomic_max_nonneg(a.divb_max, b, dmax);
 }
 
 // ---------------------------------------------------------------------------------

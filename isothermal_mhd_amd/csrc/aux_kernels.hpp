@@ -23,7 +23,8 @@ template <class real>
 struct DiagArgs {
     const real* q[4];
     const double* b[3];
-    double* out;   // DIAG_STRIDE doubles per box
+    double* out;                   // DIAG_STRIDE doubles per box
+    unsigned long long* divb_max;  // striped per-box maxima
 };
 
 template <class real>

@@ -20,11 +20,13 @@ namespace {
 struct PlanDev {
     PlanBox* d = nullptr;
     long long total = 0;
+    long long active = 0;  // work items (cells / faces) over all boxes
     std::vector<PlanBox> h;
 };
 
 struct ProfRec {
     const char* name;
+    long long items;
     hipEvent_t a, b;
 };
 
@@ -55,6 +57,7 @@ struct imhd_ctx {
     // per-box device scalars
     unsigned long long* smax = nullptr;
     unsigned long long* cmax = nullptr;
+    unsigned long long* dmax = nullptr;  // max|div B| stripes
     double* dt = nullptr;
     double* t = nullptr;
     double* dtprev = nullptr;
@@ -95,6 +98,7 @@ namespace {
 int build_plan(imhd_ctx* c, PlanDev& p, int which, int dir) {
     // which: ranges as functions of (n, nb) per axis; dir = sweep direction for face plans
     p.h.clear();
+    p.active = 0;
     long long run = 0;
     for (int bx = 0; bx < c->nbox; bx++) {
         const BoxDesc& B = c->hb[bx];
@@ -137,6 +141,7 @@ int build_plan(imhd_ctx* c, PlanDev& p, int which, int dir) {
         }
         pb.start = run;
         long long cnt = (long long)pb.r[0] * pb.r[1] * pb.r[2];
+        p.active += cnt;
         run += (cnt + 255) / 256 * 256;
         p.h.push_back(pb);
     }
@@ -156,9 +161,9 @@ Launch mk(imhd_ctx* c, const PlanDev& p) {
     return L;
 }
 
-void prof_begin(imhd_ctx* c, const char* name) {
+void prof_begin(imhd_ctx* c, const char* name, long long items) {
     if (!c->prof) return;
-    ProfRec r{name, nullptr, nullptr};
+    ProfRec r{name, items, nullptr, nullptr};
     (void)hipEventCreate(&r.a);
     (void)hipEventCreate(&r.b);
     (void)hipEventRecord(r.a, c->st);
@@ -170,7 +175,7 @@ void prof_end(imhd_ctx* c) {
 }
 struct ProfScope {
     imhd_ctx* c;
-    ProfScope(imhd_ctx* c_, const char* n) : c(c_) { prof_begin(c, n); }
+    ProfScope(imhd_ctx* c_, const char* n, long long items) : c(c_) { prof_begin(c, n, items); }
     ~ProfScope() { prof_end(c); }
 };
 
@@ -209,7 +214,7 @@ void sweeps(imhd_ctx* c, int s, bool corrector) {
     static const char* nm[2][3] = {{"flux_pred_x", "flux_pred_y", "flux_pred_z"},
                                    {"flux_corr_x", "flux_corr_y", "flux_corr_z"}};
     for (int d = 0; d < 3; d++) {
-        ProfScope ps(c, nm[corrector][d]);
+        ProfScope ps(c, nm[corrector][d], (corrector ? c->pl_cflux[d] : c->pl_pflux[d]).active);
         SweepArgs<real> a = sweep_args<real>(c, s, d);
         if (!corrector) launch_flux_predictor<real, RS>(mk(c, c->pl_pflux[d]), a);
         else launch_flux_corrector<real, RS>(mk(c, c->pl_cflux[d]), a);
@@ -218,7 +223,7 @@ void sweeps(imhd_ctx* c, int s, bool corrector) {
 
 template <class real>
 void corner(imhd_ctx* c, int s, const PlanDev& p, const char* name) {
-    ProfScope ps(c, name);
+    ProfScope ps(c, name, p.active);
     CornerArgs<real> a{};
     for (int f = 0; f < 7; f++) a.q[f] = R<real>(c->q[s][f]);
     for (int e = 0; e < 6; e++) a.E[e] = R<real>(c->E[e]);
@@ -228,7 +233,7 @@ void corner(imhd_ctx* c, int s, const PlanDev& p, const char* name) {
 
 template <class real>
 void update(imhd_ctx* c, int src, int dst, double factor, const PlanDev& p, bool final, const char* name) {
-    ProfScope ps(c, name);
+    ProfScope ps(c, name, p.active);
     UpdateArgs<real> a{};
     for (int f = 0; f < 7; f++) { a.src.q[f] = R<real>(c->q[src][f]); a.dst.q[f] = R<real>(c->q[dst][f]); }
     for (int d = 0; d < 3; d++) { a.src.b[d] = c->b[src][d]; a.dst.b[d] = c->b[dst][d]; a.Ec[d] = c->Ec[d]; }
@@ -244,7 +249,7 @@ void update(imhd_ctx* c, int src, int dst, double factor, const PlanDev& p, bool
 
 template <class real>
 void vl_step(imhd_ctx* c) {
-    launch_reset_u64(c->st, c->smax, c->nbox);
+    launch_reset_u64(c->st, c->smax, (int)stripe_slots(c->nbox));
     const int s0 = c->cur, s1 = 1 - c->cur;
     switch (c->cfg.riemann) {
         case IMHD_ROE: sweeps<real, RS_ROE>(c, s0, false); break;
@@ -266,7 +271,7 @@ void vl_step(imhd_ctx* c) {
 template <class real>
 void fill_ghosts(imhd_ctx* c) {
     {
-        ProfScope ps(c, "fill_ghosts");
+        ProfScope ps(c, "fill_ghosts", c->pl_ghost.active);
         GhostArgs<real> g{};
         for (int f = 0; f < 7; f++) g.r[f] = R<real>(c->q[c->cur][f]);
         for (int d = 0; d < 3; d++) g.d[d] = c->b[c->cur][d];
@@ -276,7 +281,7 @@ void fill_ghosts(imhd_ctx* c) {
         launch_fill_ghosts<real>(mk(c, c->pl_ghost), g);
     }
     if (c->cfg.npass > 0) {
-        ProfScope ps(c, "fill_ghosts_pass");
+        ProfScope ps(c, "fill_ghosts_pass", c->pl_pghost.active);
         GhostArgs<real> g{};
         for (int p = 0; p < c->cfg.npass; p++) g.r[p] = R<real>(c->qp[c->pcur][p]);
         g.nreal = c->cfg.npass;
@@ -288,7 +293,7 @@ void fill_ghosts(imhd_ctx* c) {
 
 template <class real>
 void copy_velocity(imhd_ctx* c, int stage) {
-    ProfScope ps(c, stage ? "copy_velocity_tavg" : "copy_velocity");
+    ProfScope ps(c, stage ? "copy_velocity_tavg" : "copy_velocity", c->pl_vel.active);
     VelArgs<real> a{};
     for (int f = 0; f < 4; f++) a.q[f] = R<real>(c->q[c->cur][f]);
     for (int d = 0; d < 3; d++) { a.vel[d] = R<real>(c->vel[d]); a.vta[d] = R<real>(c->vta[d]); }
@@ -304,7 +309,8 @@ void passive_step(imhd_ctx* c) {
         const int src = stage ? p1 : p0;
         void** v = stage ? c->vta : c->vel;
         for (int d = 0; d < 3; d++) {
-            ProfScope ps(c, stage ? "pass_flux_corr" : "pass_flux_pred");
+            ProfScope ps(c, stage ? "pass_flux_corr" : "pass_flux_pred",
+                         (stage ? c->pl_pcflux[d] : c->pl_ppflux[d]).active);
             PassFluxArgs<real> a{};
             for (int p = 0; p < np; p++) { a.s[p] = R<real>(c->qp[src][p]); a.F[p] = R<real>(c->Fp[d][p]); }
             a.v = R<real>(v[d]);
@@ -313,7 +319,7 @@ void passive_step(imhd_ctx* c) {
             a.stage = stage;
             launch_pass_flux<real>(mk(c, stage ? c->pl_pcflux[d] : c->pl_ppflux[d]), a);
         }
-        ProfScope ps(c, stage ? "pass_update_corr" : "pass_update_pred");
+        ProfScope ps(c, stage ? "pass_update_corr" : "pass_update_pred", (stage ? c->pl_pfupd : c->pl_ppupd).active);
         PassUpdArgs<real> u{};
         for (int p = 0; p < np; p++) {
             u.src[p] = R<real>(c->qp[p0][p]);
@@ -331,8 +337,8 @@ void passive_step(imhd_ctx* c) {
 
 template <class real>
 void max_speed_launch(imhd_ctx* c) {
-    ProfScope ps(c, "max_speed");
-    launch_reset_u64(c->st, c->cmax, c->nbox);
+    ProfScope ps(c, "max_speed", c->pl_int.active);
+    launch_reset_u64(c->st, c->cmax, (int)stripe_slots(c->nbox));
     QPtrs<real> q{};
     for (int f = 0; f < 7; f++) q.q[f] = R<real>(c->q[c->cur][f]);
     launch_max_speed<real>(mk(c, c->pl_int), q, c->cfg.cs, c->cmax);
@@ -364,6 +370,7 @@ int prof_collect(imhd_ctx* c) {
     if (c->prec_list.empty()) return 0;
     HIPCK(c, hipStreamSynchronize(c->st));
     std::map<std::string, std::pair<double, int>> acc;
+    std::map<std::string, long long> items;
     std::vector<std::string> order;
     for (auto& r : c->prec_list) {
         float ms = 0.f;
@@ -371,6 +378,7 @@ int prof_collect(imhd_ctx* c) {
         if (!acc.count(r.name)) order.push_back(r.name);
         acc[r.name].first += ms;
         acc[r.name].second += 1;
+        items[r.name] += r.items;
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
     }
@@ -378,7 +386,7 @@ int prof_collect(imhd_ctx* c) {
     std::string s;
     for (auto& n : order) {
         char b[160];
-        snprintf(b, sizeof(b), "%s=%.6f/%d;", n.c_str(), acc[n].first, acc[n].second);
+        snprintf(b, sizeof(b), "%s=%.6f/%d/%lld;", n.c_str(), acc[n].first, acc[n].second, items[n]);
         s += b;
     }
     c->prof_report = s;
@@ -519,8 +527,9 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     }
     if (g.precision == 32 && g.compensated)
         for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->comp[f], nr))) return rc;
-    if ((rc = alloc(c, (void**)&c->smax, 8 * nbox))) return rc;
-    if ((rc = alloc(c, (void**)&c->cmax, 8 * nbox))) return rc;
+    if ((rc = alloc(c, (void**)&c->smax, 8 * stripe_slots(nbox)))) return rc;
+    if ((rc = alloc(c, (void**)&c->cmax, 8 * stripe_slots(nbox)))) return rc;
+    if ((rc = alloc(c, (void**)&c->dmax, 8 * stripe_slots(nbox)))) return rc;
     if ((rc = alloc(c, (void**)&c->dt, 8 * nbox))) return rc;
     if ((rc = alloc(c, (void**)&c->t, 8 * nbox))) return rc;
     if ((rc = alloc(c, (void**)&c->dtprev, 8 * nbox))) return rc;
@@ -568,7 +577,7 @@ int imhd_destroy(imhd_ctx* c) {
     for (auto p : c->vta) fr(p);
     for (auto& a : c->Fp) for (auto p : a) fr(p);
     for (auto p : c->comp) fr(p);
-    for (void* p : {(void*)c->smax, (void*)c->cmax, (void*)c->dt, (void*)c->t, (void*)c->dtprev, (void*)c->cour,
+    for (void* p : {(void*)c->smax, (void*)c->cmax, (void*)c->dmax, (void*)c->dt, (void*)c->t, (void*)c->dtprev, (void*)c->cour,
                     (void*)c->diag, (void*)c->floor_count, (void*)c->stage, (void*)c->db})
         fr(p);
     PlanDev* pls[] = {&c->pl_pflux[0], &c->pl_pflux[1], &c->pl_pflux[2], &c->pl_pcorner, &c->pl_pupd,
@@ -642,14 +651,20 @@ int imhd_max_speed(imhd_ctx* c, double* cmax) {
     if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
     if (c->prec == 64) max_speed_launch<double>(c); else max_speed_launch<float>(c);
     HIPCK(c, hipGetLastError());
-    std::vector<unsigned long long> cm(c->nbox), sm(c->nbox);
+    const size_t ns = stripe_slots(c->nbox);
+    std::vector<unsigned long long> cm(ns), sm(ns);
     HIPCK(c, hipStreamSynchronize(c->st));
-    HIPCK(c, hipMemcpy(cm.data(), c->cmax, 8 * c->nbox, hipMemcpyDeviceToHost));
-    HIPCK(c, hipMemcpy(sm.data(), c->smax, 8 * c->nbox, hipMemcpyDeviceToHost));
+    HIPCK(c, hipMemcpy(cm.data(), c->cmax, 8 * ns, hipMemcpyDeviceToHost));
+    HIPCK(c, hipMemcpy(sm.data(), c->smax, 8 * ns, hipMemcpyDeviceToHost));
     for (int b = 0; b < c->nbox; b++) {
+        unsigned long long am = 0, smx = 0;
+        for (int st = 0; st < NSTRIPE; st++) {
+            am = std::max(am, cm[((size_t)b * NSTRIPE + st) * STRIPE_STRIDE]);
+            smx = std::max(smx, sm[((size_t)b * NSTRIPE + st) * STRIPE_STRIDE]);
+        }
         double a, s;
-        memcpy(&a, &cm[b], 8);
-        memcpy(&s, &sm[b], 8);
+        memcpy(&a, &am, 8);
+        memcpy(&s, &smx, 8);
         cmax[b] = std::max(a, s);  // MAX(max_speed, smax + delv_max), delv_max = 0
         if (cmax[b] < ZERO) FAIL(c, IMHD_ENOSPEED, "zero max speed encountered in box %d", b);
     }
@@ -661,23 +676,28 @@ int imhd_diagnostics(imhd_ctx* c, imhd_diag* out) {
     if (!c || !out) return IMHD_EINVAL;
     if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
     HIPCK(c, hipMemsetAsync(c->diag, 0, 8 * DIAG_STRIDE * c->nbox, c->st));
+    launch_reset_u64(c->st, c->dmax, (int)stripe_slots(c->nbox));
     if (c->prec == 64) {
         DiagArgs<double> a{};
         for (int f = 0; f < 4; f++) a.q[f] = R<double>(c->q[c->cur][f]);
         for (int d = 0; d < 3; d++) a.b[d] = c->b[c->cur][d];
         a.out = c->diag;
+        a.divb_max = c->dmax;
         launch_diag<double>(mk(c, c->pl_int), a);
     } else {
         DiagArgs<float> a{};
         for (int f = 0; f < 4; f++) a.q[f] = R<float>(c->q[c->cur][f]);
         for (int d = 0; d < 3; d++) a.b[d] = c->b[c->cur][d];
         a.out = c->diag;
+        a.divb_max = c->dmax;
         launch_diag<float>(mk(c, c->pl_int), a);
     }
     HIPCK(c, hipGetLastError());
     std::vector<double> h(DIAG_STRIDE * c->nbox);
+    std::vector<unsigned long long> dm(stripe_slots(c->nbox));
     HIPCK(c, hipStreamSynchronize(c->st));
     HIPCK(c, hipMemcpy(h.data(), c->diag, 8 * h.size(), hipMemcpyDeviceToHost));
+    HIPCK(c, hipMemcpy(dm.data(), c->dmax, 8 * dm.size(), hipMemcpyDeviceToHost));
     for (int b = 0; b < c->nbox; b++) {
         double dx = c->hb[b].dx, dx3 = dx * dx * dx;
         const double* v = &h[b * DIAG_STRIDE];
@@ -688,7 +708,9 @@ int imhd_diagnostics(imhd_ctx* c, imhd_diag* out) {
         out[b].momz = v[4] * dx3;
         out[b].divb_sum = v[5];
         out[b].mass = v[6] * dx3;
-        out[b].divb_max = v[7];
+        unsigned long long m = 0;
+        for (int st = 0; st < NSTRIPE; st++) m = std::max(m, dm[((size_t)b * NSTRIPE + st) * STRIPE_STRIDE]);
+        memcpy(&out[b].divb_max, &m, 8);
     }
     return IMHD_OK;
 }
@@ -700,10 +722,11 @@ int imhd_floor_count(imhd_ctx* c, unsigned int* count) {
     return IMHD_OK;
 }
 
-int imhd_set_dt(imhd_ctx* c, const double* dt) {
-    if (!c || !dt) return IMHD_EINVAL;
+int imhd_set_time(imhd_ctx* c, const double* t, const double* dt) {
+    if (!c) return IMHD_EINVAL;
     HIPCK(c, hipStreamSynchronize(c->st));
-    HIPCK(c, hipMemcpy(c->dt, dt, 8 * c->nbox, hipMemcpyHostToDevice));
+    if (t) HIPCK(c, hipMemcpy(c->t, t, 8 * c->nbox, hipMemcpyHostToDevice));
+    if (dt) HIPCK(c, hipMemcpy(c->dt, dt, 8 * c->nbox, hipMemcpyHostToDevice));
     return IMHD_OK;
 }
 

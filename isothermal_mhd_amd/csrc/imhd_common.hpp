@@ -96,14 +96,34 @@ template <class real>
 __device__ __forceinline__ real min2(real a, real b) { return a < b ? a : b; }
 
 // Non-negative doubles order like their bit patterns, so the CFL/smax maxima use an
-// integer atomicMax.  Wavefront reduction first (64 lanes), then one atomic per wave.
+// integer atomicMax.  A wave reduces with shuffles; its lane 0 then updates one of
+// NSTRIPE per-box slots (each on its own 128-byte line) and skips the atomic when the slot
+// already holds a larger value.  One address per box would serialise ~1e5 same-address
+// atomics per launch in L2.
+constexpr int NSTRIPE = 64;
+constexpr int STRIPE_STRIDE = 16;  // u64 per slot (128 B)
+__host__ __device__ constexpr size_t stripe_slots(int nbox) { return (size_t)nbox * NSTRIPE * STRIPE_STRIDE; }
+
 __device__ __forceinline__ double wave_max(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     return v;
 }
-__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* addr, double v) {
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* base, int box, double v) {
     v = wave_max(v);
-    if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+    if ((threadIdx.x & 63) == 0 && v > 0.0) {
+        unsigned stripe = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % NSTRIPE;
+        unsigned long long* a = base + ((size_t)box * NSTRIPE + stripe) * STRIPE_STRIDE;
+        unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+        if (bits > __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(a, bits);
+    }
+}
+__device__ __forceinline__ double striped_max(const unsigned long long* base, int box) {
+    unsigned long long m = 0;
+    for (int s = 0; s < NSTRIPE; s++) {
+        unsigned long long v = base[((size_t)box * NSTRIPE + s) * STRIPE_STRIDE];
+        m = v > m ? v : m;
+    }
+    return __longlong_as_double((long long)m);
 }
 
 // rotated-component permutation of a directional sweep (vanleer_mod.f90:449-464,

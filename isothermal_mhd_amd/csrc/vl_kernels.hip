@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void k_flux_pred(const BoxDesc* __restrict__ b
         store_flux(a, c, F);
         sm = (double)smax;
     }
-    atomic_max_nonneg(&a.smax[b], sm);
+    atomic_max_nonneg(a.smax, b, sm);
 }
 
 // Corrector fluxes: primitive PLM reconstruction with the MC-beta slope
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_flux_corr(const BoxDesc* __restrict__ b
         store_flux(a, c, F);
         sm = (double)smax;
     }
-    atomic_max_nonneg(&a.smax[b], sm);
+    atomic_max_nonneg(a.smax, b, sm);
 }
 
 __device__ __forceinline__ void upw(double v, double L14, double R14, double L34, double R34, double& o14,

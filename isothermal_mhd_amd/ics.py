@@ -1,0 +1,93 @@
+"""Seeded initial conditions for the benchmark workloads (the reference has only
+hard-coded test problems, init_domain_mod.f90; these random-field ICs are new).
+
+Velocity: Gaussian random Fourier modes with shell spectrum E(k) ~ k^-p on
+kmin <= |k| <= kmax (per-mode power ~ k^-(p+2)), optionally projected solenoidal, scaled
+to a target rms Mach number v_rms / c_s.  Random numbers come from numpy's PCG64 keyed by
+the seed, so every host produces the same field.  Output is the reference host layout:
+q_3d(7), bface_3d(3), qpass_3d with nb ghost layers (ghosts left for the ghost fill).
+"""
+import numpy as np
+
+from .solver import padded_shapes
+
+
+def random_velocity(n, kmin, kmax, slope, mach, cs=1.0, solenoidal=True, seed=0, dtype=np.float64):
+    nx, ny, nz = n
+    rng = np.random.default_rng(seed)
+    kx = np.fft.fftfreq(nx) * nx
+    ky = np.fft.fftfreq(ny) * ny
+    kz = np.fft.rfftfreq(nz) * nz
+    KX, KY, KZ = np.meshgrid(kx, ky, kz, indexing="ij")
+    k2 = KX ** 2 + KY ** 2 + KZ ** 2
+    k = np.sqrt(k2)
+    shell = (k >= kmin) & (k <= kmax)
+    amp = np.zeros_like(k)
+    amp[shell] = k[shell] ** (-(slope + 2.0) / 2.0)
+    vh = []
+    for _ in range(3):
+        re = rng.standard_normal(k.shape)
+        im = rng.standard_normal(k.shape)
+        vh.append((re + 1j * im) * amp)
+    if solenoidal:
+        kdotv = KX * vh[0] + KY * vh[1] + KZ * vh[2]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            f = np.where(k2 > 0, kdotv / k2, 0.0)
+        vh = [vh[0] - KX * f, vh[1] - KY * f, vh[2] - KZ * f]
+    v = [np.fft.irfftn(c, s=(nx, ny, nz)) for c in vh]
+    rms = np.sqrt(np.mean(v[0] ** 2 + v[1] ** 2 + v[2] ** 2))
+    scale = mach * cs / rms if rms > 0 else 0.0
+    return [(c * scale).astype(dtype) for c in v]
+
+
+def make_state(n, nb=6, npass=0, rho=None, v=None, b0=(0.0, 0.0, 1.0), passive=None):
+    nx, ny, nz = n
+    sh, shp = padded_shapes(nx, ny, nz, nb, npass)
+    q = np.zeros(sh + (7,), order="F")
+    bf = np.zeros(sh + (3,), order="F")
+    qp = np.zeros(shp, order="F")
+    I = (slice(nb, nb + nx), slice(nb, nb + ny), slice(nb, nb + nz))
+    r = np.ones((nx, ny, nz)) if rho is None else rho
+    q[I + (0,)] = r
+    if v is not None:
+        for m in range(3):
+            q[I + (1 + m,)] = r * v[m]
+    for m in range(3):
+        q[I + (4 + m,)] = b0[m]
+        bf[I + (m,)] = b0[m]
+    Ip = (slice(nb - 2, nb - 2 + nx), slice(nb - 2, nb - 2 + ny), slice(nb - 2, nb - 2 + nz))
+    for p in range(npass):
+        qp[Ip + (p,)] = passive[p]
+    return q, bf, qp
+
+
+def config_c2(n=256, seed=2002, nb=6):
+    """256^3 periodic, rho = 1, B0 = (0,0,1), solenoidal k^-2 velocity on 1 <= |k| <= 8 at
+    rms Mach 2, one passive scalar of uniform [0,1) seeded noise (HLLD + VL + CT)."""
+    v = random_velocity((n, n, n), 1.0, 8.0, 2.0, 2.0, solenoidal=True, seed=seed)
+    s = np.random.default_rng(seed + 1).random((n, n, n))
+    return make_state((n, n, n), nb=nb, npass=1, v=v, b0=(0.0, 0.0, 1.0), passive=[s])
+
+
+def config_c1(n=64, seed=1001, nb=6):
+    """64^3 periodic, rho = 1, B0 = (0.5,0,0), k^-2 velocity on 1 <= |k| <= 4 at rms Mach 1."""
+    v = random_velocity((n, n, n), 1.0, 4.0, 2.0, 1.0, solenoidal=False, seed=seed)
+    return make_state((n, n, n), nb=nb, npass=0, v=v, b0=(0.5, 0.0, 0.0))
+
+
+def ensemble_c4(nbox=48, seed=4004, lo=32, hi=192, nb=6):
+    """Uneven boxes: sizes ~ U{lo..hi}^3, |B0| ~ U[0.1,1] with a random direction, rms Mach
+    ~ U[0.5,4], k^-2 velocity (per box substream of the seed)."""
+    rng = np.random.default_rng(seed)
+    dims, states, machs = [], [], []
+    for b in range(nbox):
+        n = tuple(int(x) for x in rng.integers(lo, hi + 1, size=3))
+        bm = rng.uniform(0.1, 1.0)
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        mach = rng.uniform(0.5, 4.0)
+        v = random_velocity(n, 1.0, 4.0, 2.0, mach, solenoidal=False, seed=seed * 1000 + b)
+        dims.append(n)
+        machs.append(mach)
+        states.append(make_state(n, nb=nb, npass=0, v=v, b0=tuple(bm * d)))
+    return dims, states, machs

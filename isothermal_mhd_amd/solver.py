@@ -136,9 +136,12 @@ class VanLeerSolver:
         return n.value
 
     # -- native loop ----------------------------------------------------------------
-    def set_dt(self, dt):
-        v = np.broadcast_to(np.asarray(dt, dtype=np.float64), (len(self.boxes),)).copy()
-        self._check(self.lib.imhd_set_dt(self._c, _p(v)))
+    def set_time(self, t=None, dt=None):
+        nb = len(self.boxes)
+        tv = None if t is None else np.broadcast_to(np.asarray(t, dtype=np.float64), (nb,)).copy()
+        dv = None if dt is None else np.broadcast_to(np.asarray(dt, dtype=np.float64), (nb,)).copy()
+        self._check(self.lib.imhd_set_time(self._c, _p(tv) if tv is not None else None,
+                                           _p(dv) if dv is not None else None))
 
     def get_time(self):
         t = np.zeros(len(self.boxes))
@@ -157,13 +160,14 @@ class VanLeerSolver:
         self._check(self.lib.imhd_set_profiling(self._c, int(bool(on))))
 
     def profile(self):
-        """Per-kernel HIP-event time of the last eager call: {name: (ms_total, launches)}."""
+        """Per-kernel HIP-event time of the last eager call:
+        {name: (ms_total, launches, work_items_total)}."""
         rep = self.lib.imhd_profile_report(self._c).decode()
         out = {}
         for item in filter(None, rep.split(";")):
             name, v = item.split("=")
-            ms, n = v.split("/")
-            out[name] = (float(ms), int(n))
+            ms, n, items = v.split("/")
+            out[name] = (float(ms), int(n), int(items))
         return out
 
 
