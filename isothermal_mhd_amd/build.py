@@ -7,11 +7,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libimhd.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["vl_kernels.hip", "aux_kernels.hip", "imhd_api.cpp"]
+# (source, extra flags): the FP32 kernels may contract to FMA; everything else may not.
+SOURCES = [("vl_kernels_f64.hip", ["-ffp-contract=off"]),
+           ("vl_kernels_f32.hip", ["-ffp-contract=fast"]),
+           ("aux_kernels.hip", ["-ffp-contract=off"]),
+           ("imhd_api.cpp", ["-ffp-contract=off"])]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-# -ffp-contract=off: no FMA contraction, so the FP64 path reproduces the reference's
-# operation order exactly.  Packed-FP32 instructions are disabled.
-FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-ffp-contract=off",
+# FP64 sources: -ffp-contract=off (no FMA contraction), so the FP64 path reproduces the
+# reference's operation order exactly.  Packed-FP32 instructions are disabled everywhere.
+FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-command-line-argument"]
 
@@ -19,11 +23,11 @@ FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-ffp-contract=o
 def build(verbose=False, jobs=4):
     os.makedirs(OBJ, exist_ok=True)
     procs, objs = [], []
-    for s in SOURCES:
+    for s, extra in SOURCES:
         src = os.path.join(CSRC, s)
         obj = os.path.join(OBJ, s + ".o")
         objs.append(obj)
-        cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
+        cmd = [HIPCC] + FLAGS + extra + ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd))
         procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

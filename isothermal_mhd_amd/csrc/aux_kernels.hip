@@ -29,6 +29,44 @@ __global__ __launch_bounds__(256) void k_fill_ghosts(const BoxDesc* __restrict__
     for (int f = 0; f < a.ndbl; f++) a.d[f][d] = a.d[f][s];
 }
 
+// Same fill, enumerating only the ghost band of depth w around the interior: the plan gives
+// each box r[0] = number of band cells; the linear index is decoded as z-slabs, then
+// y-slabs (interior k), then x-slabs (interior j, k).
+template <class real>
+__global__ __launch_bounds__(256) void k_fill_ghost_band(const BoxDesc* __restrict__ boxes, Plan plan, GhostArgs<real> a) {
+    int b, g, dum1, dum2;
+    if (!plan_item(plan, b, g, dum1, dum2)) return;
+    const BoxDesc& B = boxes[b];
+    const int w = a.depth;
+    const long long E0 = B.n[0] + 2 * w, E1 = B.n[1] + 2 * w;
+    const long long nA = E0 * E1 * 2 * w, nB = E0 * 2 * w * B.n[2];
+    long long t = g;
+    int i, j, k;
+    if (t < nA) {
+        long long kk = t / (E0 * E1), rem = t - kk * E0 * E1;
+        k = kk < w ? 1 - w + (int)kk : B.n[2] + 1 + (int)(kk - w);
+        j = 1 - w + (int)(rem / E0);
+        i = 1 - w + (int)(rem % E0);
+    } else if (t < nA + nB) {
+        t -= nA;
+        long long kk = t / (E0 * 2 * w), rem = t - kk * E0 * 2 * w;
+        k = 1 + (int)kk;
+        long long jj = rem / E0;
+        j = jj < w ? 1 - w + (int)jj : B.n[1] + 1 + (int)(jj - w);
+        i = 1 - w + (int)(rem % E0);
+    } else {
+        t -= nA + nB;
+        long long u = t / (2 * w), ii = t - u * 2 * w;
+        i = ii < w ? 1 - w + (int)ii : B.n[0] + 1 + (int)(ii - w);
+        j = 1 + (int)(u % B.n[1]);
+        k = 1 + (int)(u / B.n[1]);
+    }
+    const int si = bc_map(i, B.n[0], B.bc), sj = bc_map(j, B.n[1], B.bc), sk = bc_map(k, B.n[2], B.bc);
+    const long long d = cell_index(B, i, j, k), s = cell_index(B, si, sj, sk);
+    for (int f = 0; f < a.nreal; f++) a.r[f][d] = a.r[f][s];
+    for (int f = 0; f < a.ndbl; f++) a.d[f][d] = a.d[f][s];
+}
+
 // ---------------------------------------------------------------------------------
 // CFL: max over interior cells of |v_d| + c_f,d (vanleer_mod.f90:3266-3334), FP64
 // accumulation; wavefront shuffle reduction and one integer atomicMax per wave.
@@ -188,6 +226,62 @@ __global__ __launch_bounds__(256) void k_pass_update(const BoxDesc* __restrict__
     }
 }
 
+template <class real>
+__device__ __forceinline__ real pass_vel(const PassStageArgs<real>& a, int d, long long p) {
+    real irho = real(1) / a.qo[0][p];
+    real v = a.qo[1 + d][p] * irho;   // copy_velocity(.FALSE.)
+    if (a.stage) {
+        real irn = real(1) / a.qn[0][p];
+        v = real(0.5) * (v + a.qn[1 + d][p] * irn);  // copy_velocity(.TRUE.)
+    }
+    return v;
+}
+
+template <class real>
+__device__ __forceinline__ real minmod_slope(real wm, real w0, real wp) {
+    real dL = w0 - wm, dR = wp - w0;
+    return real(0.5) * (sgn1(dL) + sgn1(dR)) * min2(fabs(dL), fabs(dR));
+}
+
+// face value at the face between p and p + s (passive_mod.f90:510-557, 594-627)
+template <class real>
+__device__ __forceinline__ real pass_face(const real* w, long long p, long long s, real vavg, int stage) {
+    real uL, uR;
+    if (!stage) {
+        uL = w[p];
+        uR = w[p + s];
+    } else {
+        real wm = w[p - s], w0 = w[p], w1 = w[p + s], w2 = w[p + 2 * s];
+        uL = w0 + real(0.5) * minmod_slope(wm, w0, w1);
+        uR = w1 - real(0.5) * minmod_slope(w0, w1, w2);
+    }
+    real theta = vavg >= real(0) ? real(1) : real(-1);
+    return real(0.5) * ((real(1) + theta) * uL + (real(1) - theta) * uR);
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void k_pass_stage(const BoxDesc* __restrict__ boxes, Plan plan, PassStageArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const long long c = cell_index(B, i, j, k);
+    const long long st[3] = {1, (long long)B.P[0], (long long)B.P[0] * B.P[1]};
+    real dtdx = (real)(a.factor * a.dt[b] / B.dx);
+    real vc[3], va_hi[3], va_lo[3];
+    for (int d = 0; d < 3; d++) {
+        vc[d] = pass_vel(a, d, c);
+        va_hi[d] = real(0.5) * (vc[d] + pass_vel(a, d, c + st[d]));
+        va_lo[d] = real(0.5) * (pass_vel(a, d, c - st[d]) + vc[d]);
+    }
+    for (int p = 0; p < a.npass; p++) {
+        const real* w = a.rec[p];
+        real dF[3];
+        for (int d = 0; d < 3; d++)
+            dF[d] = pass_face(w, c, st[d], va_hi[d], a.stage) - pass_face(w, c - st[d], st[d], va_lo[d], a.stage);
+        a.dst[p][c] = a.src[p][c] - dtdx * (vc[0] * dF[0] + vc[1] * dF[1] + vc[2] * dF[2]);
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // host <-> device layout and precision conversion (reference FP64 arrays)
 template <class real>
@@ -223,6 +317,14 @@ __global__ void k_pass_layout(BoxDesc B, const double* hsrc, double* hdst, real*
 template <class real>
 void launch_fill_ghosts(const Launch& L, const GhostArgs<real>& a) {
     if (L.plan.total > 0) L256((k_fill_ghosts<real>), L.boxes, L.plan, a);
+}
+template <class real>
+void launch_fill_ghost_band(const Launch& L, const GhostArgs<real>& a) {
+    if (L.plan.total > 0) L256((k_fill_ghost_band<real>), L.boxes, L.plan, a);
+}
+template <class real>
+void launch_pass_stage(const Launch& L, const PassStageArgs<real>& a) {
+    if (L.plan.total > 0) L256((k_pass_stage<real>), L.boxes, L.plan, a);
 }
 template <class real>
 void launch_max_speed(const Launch& L, const QPtrs<real>& q, double cs, unsigned long long* cmax) {
@@ -270,6 +372,8 @@ void launch_pass_layout(hipStream_t st, const BoxDesc& B, const double* hsrc, do
 
 #define INST_AUX(T)                                                                                           \
     template void launch_fill_ghosts<T>(const Launch&, const GhostArgs<T>&);                                  \
+    template void launch_fill_ghost_band<T>(const Launch&, const GhostArgs<T>&);                              \
+    template void launch_pass_stage<T>(const Launch&, const PassStageArgs<T>&);                               \
     template void launch_max_speed<T>(const Launch&, const QPtrs<T>&, double, unsigned long long*);                 \
     template void launch_diag<T>(const Launch&, const DiagArgs<T>&);                                          \
     template void launch_copy_velocity<T>(const Launch&, const VelArgs<T>&);                                  \

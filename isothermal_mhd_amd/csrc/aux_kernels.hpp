@@ -54,7 +54,25 @@ struct PassUpdArgs {
     int npass;
 };
 
+// One stage of passive_solve_3d with the face values formed inside the update (each face is
+// evaluated by both cells that share it, identically) and the velocities taken from the
+// MHD states: stage 0 uses v^n (copy_velocity(.FALSE.)), stage 1 the time average
+// 0.5 (v^n + v^{n+1}) (copy_velocity(.TRUE.)).
+template <class real>
+struct PassStageArgs {
+    const real* src[MAXPASS];   // scalars at t^n (update base)
+    const real* rec[MAXPASS];   // scalars the faces are built from (stage 1: half step)
+    real* dst[MAXPASS];
+    const real* qo[4];          // MHD rho, m at t^n
+    const real* qn[4];          // MHD rho, m at t^{n+1} (stage 1)
+    const double* dt;
+    double factor;
+    int npass, stage;
+};
+
 template <class real> void launch_fill_ghosts(const Launch& L, const GhostArgs<real>& a);
+template <class real> void launch_fill_ghost_band(const Launch& L, const GhostArgs<real>& a);
+template <class real> void launch_pass_stage(const Launch& L, const PassStageArgs<real>& a);
 template <class real> void launch_max_speed(const Launch& L, const QPtrs<real>& q, double cs, unsigned long long* cmax);
 template <class real> void launch_diag(const Launch& L, const DiagArgs<real>& a);
 template <class real> void launch_copy_velocity(const Launch& L, const VelArgs<real>& a);

@@ -68,6 +68,7 @@ struct imhd_ctx {
     size_t stage_n = 0;
     // plans
     PlanDev pl_pflux[3], pl_pcorner, pl_pupd, pl_cflux[3], pl_ccorner, pl_cupd;
+    PlanDev pl_gband, pl_pgband;
     PlanDev pl_ghost, pl_pghost, pl_int, pl_vel, pl_ppflux[3], pl_ppupd, pl_pcflux[3], pl_pfupd;
     int ccb_shift = 1;
     // native loop
@@ -138,6 +139,14 @@ int build_plan(imhd_ctx* c, PlanDev& p, int which, int dir) {
             }
             pb.lo[a] = lo;
             pb.r[a] = std::max(0, hi - lo + 1);
+        }
+        if (which == 14 || which == 15) {  // ghost band of depth w, enumerated linearly
+            long long w = which == 14 ? B.nb : 3;
+            long long E0 = B.n[0] + 2 * w, E1 = B.n[1] + 2 * w;
+            long long cnt = E0 * E1 * 2 * w + E0 * 2 * w * B.n[2] + 2 * w * (long long)B.n[1] * B.n[2];
+            pb.lo[0] = pb.lo[1] = pb.lo[2] = 0;
+            pb.r[0] = (int)cnt;
+            pb.r[1] = pb.r[2] = 1;
         }
         pb.start = run;
         long long cnt = (long long)pb.r[0] * pb.r[1] * pb.r[2];
@@ -232,7 +241,8 @@ void corner(imhd_ctx* c, int s, const PlanDev& p, const char* name) {
 }
 
 template <class real>
-void update(imhd_ctx* c, int src, int dst, double factor, const PlanDev& p, bool final, const char* name) {
+void update(imhd_ctx* c, int src, int dst, double factor, const PlanDev& p, bool final, const char* name,
+            bool fuse_cmax = false) {
     ProfScope ps(c, name, p.active);
     UpdateArgs<real> a{};
     for (int f = 0; f < 7; f++) { a.src.q[f] = R<real>(c->q[src][f]); a.dst.q[f] = R<real>(c->q[dst][f]); }
@@ -244,11 +254,13 @@ void update(imhd_ctx* c, int src, int dst, double factor, const PlanDev& p, bool
     a.ccb_lo_shift = c->ccb_shift;
     for (int f = 0; f < 4; f++) a.comp[f] = (final && c->comp[0]) ? R<real>(c->comp[f]) : nullptr;
     a.floor_count = c->floor_count;
+    a.cmax = fuse_cmax ? c->cmax : nullptr;
+    a.cs = c->cfg.cs;
     launch_update<real>(mk(c, p), a);
 }
 
 template <class real>
-void vl_step(imhd_ctx* c) {
+void vl_step(imhd_ctx* c, bool fuse_cmax = false) {
     launch_reset_u64(c->st, c->smax, (int)stripe_slots(c->nbox));
     const int s0 = c->cur, s1 = 1 - c->cur;
     switch (c->cfg.riemann) {
@@ -264,30 +276,30 @@ void vl_step(imhd_ctx* c) {
         default: sweeps<real, RS_HLLD>(c, s1, true); break;
     }
     corner<real>(c, s1, c->pl_ccorner, "corner_corr");
-    update<real>(c, s0, s1, 1.0, c->pl_cupd, true, "update_corr");
+    update<real>(c, s0, s1, 1.0, c->pl_cupd, true, fuse_cmax ? "update_corr_cfl" : "update_corr", fuse_cmax);
     c->cur = s1;
 }
 
 template <class real>
 void fill_ghosts(imhd_ctx* c) {
     {
-        ProfScope ps(c, "fill_ghosts", c->pl_ghost.active);
+        ProfScope ps(c, "fill_ghosts", c->pl_gband.active);
         GhostArgs<real> g{};
         for (int f = 0; f < 7; f++) g.r[f] = R<real>(c->q[c->cur][f]);
         for (int d = 0; d < 3; d++) g.d[d] = c->b[c->cur][d];
         g.nreal = 7;
         g.ndbl = 3;
         g.depth = c->hb[0].nb;
-        launch_fill_ghosts<real>(mk(c, c->pl_ghost), g);
+        launch_fill_ghost_band<real>(mk(c, c->pl_gband), g);
     }
     if (c->cfg.npass > 0) {
-        ProfScope ps(c, "fill_ghosts_pass", c->pl_pghost.active);
+        ProfScope ps(c, "fill_ghosts_pass", c->pl_pgband.active);
         GhostArgs<real> g{};
         for (int p = 0; p < c->cfg.npass; p++) g.r[p] = R<real>(c->qp[c->pcur][p]);
         g.nreal = c->cfg.npass;
         g.ndbl = 0;
         g.depth = 3;
-        launch_fill_ghosts<real>(mk(c, c->pl_pghost), g);
+        launch_fill_ghost_band<real>(mk(c, c->pl_pgband), g);
     }
 }
 
@@ -344,16 +356,42 @@ void max_speed_launch(imhd_ctx* c) {
     launch_max_speed<real>(mk(c, c->pl_int), q, c->cfg.cs, c->cmax);
 }
 
+// passive_solve_3d with the velocities taken from the MHD states before (qold) and after
+// (cur) the step: same arithmetic as copy_velocity + passive_step, no velocity arrays.
+// Stage 0 writes the half step into the spare buffer; stage 1 reconstructs from it and
+// updates the t^n buffer in place (a cell reads only its own t^n value), so the current
+// passive buffer does not change.
+template <class real>
+void passive_fused(imhd_ctx* c, int qold) {
+    const int np = c->cfg.npass;
+    const int p0 = c->pcur, p1 = 1 - c->pcur;
+    for (int stage = 0; stage < 2; stage++) {
+        ProfScope ps(c, stage ? "pass_stage_corr" : "pass_stage_pred", (stage ? c->pl_pfupd : c->pl_ppupd).active);
+        PassStageArgs<real> a{};
+        for (int p = 0; p < np; p++) {
+            a.src[p] = R<real>(c->qp[p0][p]);
+            a.rec[p] = R<real>(c->qp[stage ? p1 : p0][p]);
+            a.dst[p] = R<real>(c->qp[stage ? p0 : p1][p]);
+        }
+        for (int f = 0; f < 4; f++) {
+            a.qo[f] = R<real>(c->q[qold][f]);
+            a.qn[f] = R<real>(c->q[c->cur][f]);
+        }
+        a.dt = c->dt;
+        a.factor = stage ? 1.0 : 0.5;
+        a.npass = np;
+        a.stage = stage;
+        launch_pass_stage<real>(mk(c, stage ? c->pl_pfupd : c->pl_ppupd), a);
+    }
+}
+
 template <class real>
 void native_step(imhd_ctx* c, double t_end) {
-    if (c->cfg.npass > 0) copy_velocity<real>(c, 0);
-    vl_step<real>(c);
-    if (c->cfg.npass > 0) {
-        copy_velocity<real>(c, 1);
-        passive_step<real>(c);
-    }
+    launch_reset_u64(c->st, c->cmax, (int)stripe_slots(c->nbox));
+    const int qold = c->cur;
+    vl_step<real>(c, true);
+    if (c->cfg.npass > 0) passive_fused<real>(c, qold);
     fill_ghosts<real>(c);
-    max_speed_launch<real>(c);
     launch_new_dt(c->st, c->nbox, c->db, c->cmax, c->smax, c->cour, t_end, c->t, c->dt, c->dtprev, 1);
 }
 
@@ -552,6 +590,8 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     if ((rc = build_plan(c, c->pl_ccorner, 4, -1))) return rc;
     if ((rc = build_plan(c, c->pl_cupd, 5, -1))) return rc;
     if ((rc = build_plan(c, c->pl_ghost, 6, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_gband, 14, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_pgband, 15, -1))) return rc;
     if ((rc = build_plan(c, c->pl_pghost, 7, -1))) return rc;
     if ((rc = build_plan(c, c->pl_int, 8, -1))) return rc;
     if ((rc = build_plan(c, c->pl_vel, 9, -1))) return rc;
@@ -584,7 +624,7 @@ int imhd_destroy(imhd_ctx* c) {
                       &c->pl_cflux[0], &c->pl_cflux[1], &c->pl_cflux[2], &c->pl_ccorner, &c->pl_cupd,
                       &c->pl_ghost, &c->pl_pghost, &c->pl_int, &c->pl_vel, &c->pl_ppflux[0], &c->pl_ppflux[1],
                       &c->pl_ppflux[2], &c->pl_ppupd, &c->pl_pcflux[0], &c->pl_pcflux[1], &c->pl_pcflux[2],
-                      &c->pl_pfupd};
+                      &c->pl_pfupd, &c->pl_gband, &c->pl_pgband};
     for (auto p : pls) fr(p->d);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
@@ -786,8 +826,7 @@ int imhd_run(imhd_ctx* c, int nsteps, double t_end, int use_graph) {
             c->pcur = pcur0;
         }
         HIPCK(c, hipGraphLaunch(c->gexec[par], c->st));
-        c->cur = 1 - c->cur;
-        if (c->cfg.npass > 0) c->pcur = 1 - c->pcur;
+        c->cur = 1 - c->cur;  // the native step keeps the passive buffer in place
     }
     HIPCK(c, hipGetLastError());
     return IMHD_OK;

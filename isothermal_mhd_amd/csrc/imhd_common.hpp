@@ -83,6 +83,53 @@ __device__ __forceinline__ bool plan_item(const Plan& p, int& box, int& i, int& 
     return true;
 }
 
+// Division and square root.  FP64: IEEE correctly rounded (the reference's arithmetic).
+// FP32 path: hardware reciprocal / square root (about 1 ulp), as the FP32 path is held to
+// an L1 tolerance against FP64 rather than to bitwise parity.
+__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// Plan lookup that also reports the box for inactive (padding) lanes so that every lane
+// of a wavefront joins the smax reduction; box starts are 64-aligned.
+__device__ __forceinline__ bool plan_item_box(const Plan& p, int& box, int& i, int& j, int& k) {
+    box = 0;
+    bool act = plan_item(p, box, i, j, k);
+    if (!act && p.nbox > 1) {
+        unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+        long long g = (long long)blk * blockDim.x + threadIdx.x;
+        int lo = 0, hi = p.nbox - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (p.boxes[mid].start <= g) lo = mid; else hi = mid - 1;
+        }
+        box = lo;
+    }
+    return act;
+}
+
+// Largest |v_d| + c_f,d of one cell (vanleer_mod.f90:3282-3311), FP64.
+__device__ __forceinline__ double cell_fast_speed(const double* q, double cs) {
+    double asqr = cs * cs;
+    double rho = q[0];
+    double isrho = 1.0 / sqrt(rho);
+    double bx = q[4] * isrho, by = q[5] * isrho, bz = q[6] * isrho;
+    double btsqr = by * by + bz * bz, bsqr = bx * bx + btsqr, x = asqr - bsqr;
+    double tmp = sqrt(x * x + 4.0 * asqr * btsqr);
+    double cfx = sqrt(0.5 * (asqr + bsqr + tmp));
+    btsqr = bx * bx + bz * bz; bsqr = by * by + btsqr; x = asqr - bsqr;
+    tmp = sqrt(x * x + 4.0 * asqr * btsqr);
+    double cfy = sqrt(0.5 * (asqr + bsqr + tmp));
+    btsqr = bx * bx + by * by; bsqr = bz * bz + btsqr; x = asqr - bsqr;
+    tmp = sqrt(x * x + 4.0 * asqr * btsqr);
+    double cfz = sqrt(0.5 * (asqr + bsqr + tmp));
+    double vx = q[1] / rho, vy = q[2] / rho, vz = q[3] / rho;
+    double m = fabs(vx) + cfx;
+    m = m > fabs(vy) + cfy ? m : fabs(vy) + cfy;
+    return m > fabs(vz) + cfz ? m : fabs(vz) + cfz;
+}
+
 template <class real>
 __device__ __forceinline__ real sgn1(real x) { return copysign(real(1), x); }  // Fortran SIGN(1.d0, x)
 

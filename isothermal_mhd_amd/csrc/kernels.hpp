@@ -47,6 +47,8 @@ struct UpdateArgs {
     int ccb_lo_shift;      // cell-centred B is refreshed on [lo + shift, hi]
     real* comp[4];         // Kahan compensation of rho, m (FP32 only; null = off)
     unsigned int* floor_count;
+    unsigned long long* cmax;  // non-null: also reduce the CFL speed of interior cells
+    double cs;
 };
 
 struct Launch {

@@ -11,12 +11,12 @@ namespace imhd {
 
 template <class real>
 __device__ __forceinline__ real fast_speed(const real* u, real asqr) {  // vanleer_mod.f90:3048-3053
-    real bsqr = (u[4] * u[4] + u[5] * u[5] + u[6] * u[6]) / u[0];
-    real btsqr = (u[5] * u[5] + u[6] * u[6]) / u[0];
+    real bsqr = fdiv(u[4] * u[4] + u[5] * u[5] + u[6] * u[6], u[0]);
+    real btsqr = fdiv(u[5] * u[5] + u[6] * u[6], u[0]);
     real tmp1 = asqr + bsqr;
     real x = asqr - bsqr;
-    real tmp2 = sqrt(x * x + real(4) * asqr * btsqr);
-    return sqrt(real(0.5) * (tmp1 + tmp2));
+    real tmp2 = fsqrt(x * x + real(4) * asqr * btsqr);
+    return fsqrt(real(0.5) * (tmp1 + tmp2));
 }
 
 template <class real>
@@ -36,16 +36,16 @@ __device__ __forceinline__ void hlle(const real* uL, const real* uR, real asqr, 
     real uavg[7], FL[7], FR[7];
 #pragma unroll
     for (int m = 0; m < 7; m++) uavg[m] = real(0.5) * (uL[m] + uR[m]);
-    real vxL = uL[1] / uL[0], vyL = uL[2] / uL[0], vzL = uL[3] / uL[0];
-    real vxR = uR[1] / uR[0], vyR = uR[2] / uR[0], vzR = uR[3] / uR[0];
+    real vxL = fdiv(uL[1], uL[0]), vyL = fdiv(uL[2], uL[0]), vzL = fdiv(uL[3], uL[0]);
+    real vxR = fdiv(uR[1], uR[0]), vyR = fdiv(uR[2], uR[0]), vzR = fdiv(uR[3], uR[0]);
     phys_flux(uL, asqr, vxL, vyL, vzL, FL);
     phys_flux(uR, asqr, vxR, vyR, vzR, FR);
     real cfL = fast_speed(uL, asqr), cfR = fast_speed(uR, asqr), cfa = fast_speed(uavg, asqr);
-    real vxa = uavg[1] / uavg[0];
+    real vxa = fdiv(uavg[1], uavg[0]);
     real sL = min3(vxL - cfL, vxa - cfa, real(0));
     real sR = max3(vxR + cfR, vxa + cfa, real(0));
     smax = max2(smax, max3(fabs(vxL) + cfL, fabs(vxR) + cfR, fabs(vxa) + cfa));
-    real isRL = real(1) / (sR - sL);
+    real isRL = fdiv(real(1), sR - sL);
 #pragma unroll
     for (int m = 0; m < 7; m++) F[m] = (sR * FL[m] - sL * FR[m] + sR * sL * (uR[m] - uL[m])) * isRL;
 }
@@ -57,12 +57,12 @@ __device__ __forceinline__ void hlld(const real* uL, const real* uR, real asqr, 
 #pragma unroll
     for (int m = 0; m < 7; m++) uavg[m] = real(0.5) * (uL[m] + uR[m]);
     const real Bx = uL[4];
-    real vxL = uL[1] / uL[0], vyL = uL[2] / uL[0], vzL = uL[3] / uL[0];
-    real vxR = uR[1] / uR[0], vyR = uR[2] / uR[0], vzR = uR[3] / uR[0];
+    real vxL = fdiv(uL[1], uL[0]), vyL = fdiv(uL[2], uL[0]), vzL = fdiv(uL[3], uL[0]);
+    real vxR = fdiv(uR[1], uR[0]), vyR = fdiv(uR[2], uR[0]), vzR = fdiv(uR[3], uR[0]);
     phys_flux(uL, asqr, vxL, vyL, vzL, FL);
     phys_flux(uR, asqr, vxR, vyR, vzR, FR);
     real cfL = fast_speed(uL, asqr), cfR = fast_speed(uR, asqr), cfa = fast_speed(uavg, asqr);
-    real vxa = uavg[1] / uavg[0];
+    real vxa = fdiv(uavg[1], uavg[0]);
     real sL = min3(vxL - cfL, vxa - cfa, real(0));
     real sR = max3(vxR + cfR, vxa + cfa, real(0));
     smax = max2(smax, max3(fabs(vxL) + cfL, fabs(vxR) + cfR, fabs(vxa) + cfa));
@@ -76,15 +76,15 @@ __device__ __forceinline__ void hlld(const real* uL, const real* uR, real asqr, 
         for (int m = 0; m < 7; m++) F[m] = FR[m];
         return;
     }
-    real isRL = real(1) / (sR - sL);
+    real isRL = fdiv(real(1), sR - sL);
     real uHLL0 = (sR * uR[0] - sL * uL[0] - FR[0] + FL[0]) * isRL;
     real uHLL1 = (sR * uR[1] - sL * uL[1] - FR[1] + FL[1]) * isRL;
     real FHLL0 = (sR * FL[0] - sL * FR[0] + sR * sL * (uR[0] - uL[0])) * isRL;
     real FHLL1 = (sR * FL[1] - sL * FR[1] + sR * sL * (uR[1] - uL[1])) * isRL;
     const real rho_s = uHLL0, momx_s = uHLL1;
-    const real vx_s = FHLL0 / rho_s;
-    const real irho_s = real(1) / rho_s;
-    const real isrho_s = sqrt(irho_s);
+    const real vx_s = fdiv(FHLL0, rho_s);
+    const real irho_s = fdiv(real(1), rho_s);
+    const real isrho_s = fsqrt(irho_s);
     const real sLs = vx_s - fabs(Bx) * isrho_s;
     const real sRs = vx_s + fabs(Bx) * isrho_s;
     const real zero = real(ZERO);
@@ -94,7 +94,7 @@ __device__ __forceinline__ void hlld(const real* uL, const real* uR, real asqr, 
 #pragma unroll
         for (int m = 2; m < 7; m++) uLs[m] = uL[m];
     } else {
-        real is = real(1) / ((sL - sLs) * (sL - sRs));
+        real is = fdiv(real(1), (sL - sLs) * (sL - sRs));
         real t1 = Bx * (vx_s - vxL) * is;
         real d = sL - vxL;
         real t2 = (uL[0] * (d * d) - Bx * Bx) * irho_s * is;
@@ -109,7 +109,7 @@ __device__ __forceinline__ void hlld(const real* uL, const real* uR, real asqr, 
 #pragma unroll
         for (int m = 2; m < 7; m++) uRs[m] = uR[m];
     } else {
-        real is = real(1) / ((sR - sLs) * (sR - sRs));
+        real is = fdiv(real(1), (sR - sLs) * (sR - sRs));
         real t1 = Bx * (vx_s - vxR) * is;
         real d = sR - vxR;
         real t2 = (uR[0] * (d * d) - Bx * Bx) * irho_s * is;
@@ -126,8 +126,8 @@ __device__ __forceinline__ void hlld(const real* uL, const real* uR, real asqr, 
 #pragma unroll
         for (int m = 0; m < 7; m++) F[m] = FR[m] + sR * (uRs[m] - uR[m]);
     } else {
-        const real X = sqrt(rho_s) * sgn1(Bx);
-        const real iX = real(1) / X;
+        const real X = fsqrt(rho_s) * sgn1(Bx);
+        const real iX = fdiv(real(1), X);
         real uC2 = real(0.5) * (uLs[2] + uRs[2] + X * (uRs[5] - uLs[5]));
         real uC3 = real(0.5) * (uLs[3] + uRs[3] + X * (uRs[6] - uLs[6]));
         real uC5 = real(0.5) * (uLs[5] + uRs[5] + iX * (uRs[2] - uLs[2]));
@@ -137,8 +137,8 @@ __device__ __forceinline__ void hlld(const real* uL, const real* uR, real asqr, 
         F[2] = uC2 * vx_s - Bx * uC5;
         F[3] = uC3 * vx_s - Bx * uC6;
         F[4] = real(0);
-        F[5] = vx_s * uC5 - (uC2 * Bx) / rho_s;
-        F[6] = vx_s * uC6 - (uC3 * Bx) / rho_s;
+        F[5] = vx_s * uC5 - fdiv(uC2 * Bx, rho_s);
+        F[6] = vx_s * uC6 - fdiv(uC3 * Bx, rho_s);
     }
 }
 
@@ -150,11 +150,11 @@ __device__ __forceinline__ void roe(const real* qL, const real* qR, real cs, rea
     const real zero = real(ZERO);
     real qa[7];
     real rho2 = real(0.5) * (qL[0] + qR[0]);
-    real isrho = real(1) / sqrt(rho2);
+    real isrho = fdiv(real(1), fsqrt(rho2));
     qa[0] = rho2;
-    qa[1] = real(0.5) * (qL[1] / qL[0] + qR[1] / qR[0]);
-    qa[2] = real(0.5) * (qL[2] / qL[0] + qR[2] / qR[0]);
-    qa[3] = real(0.5) * (qL[3] / qL[0] + qR[3] / qR[0]);
+    qa[1] = real(0.5) * (fdiv(qL[1], qL[0]) + fdiv(qR[1], qR[0]));
+    qa[2] = real(0.5) * (fdiv(qL[2], qL[0]) + fdiv(qR[2], qR[0]));
+    qa[3] = real(0.5) * (fdiv(qL[3], qL[0]) + fdiv(qR[3], qR[0]));
     qa[4] = qL[4] * isrho;
     qa[5] = real(0.5) * (qL[5] + qR[5]) * isrho;
     qa[6] = real(0.5) * (qL[6] + qR[6]) * isrho;
@@ -162,25 +162,25 @@ __device__ __forceinline__ void roe(const real* qL, const real* qR, real cs, rea
     real bsqr = qa[4] * qa[4] + qa[5] * qa[5] + qa[6] * qa[6];
     real btsqr = qa[5] * qa[5] + qa[6] * qa[6];
     real tmp1 = asqr + bsqr, x = asqr - bsqr;
-    real tmp2 = sqrt(x * x + real(4) * asqr * btsqr);
-    real cslow = sqrt(max2(real(0), real(0.5) * (tmp1 - tmp2)));
+    real tmp2 = fsqrt(x * x + real(4) * asqr * btsqr);
+    real cslow = fsqrt(max2(real(0), real(0.5) * (tmp1 - tmp2)));
     real ca = fabs(bx);
-    real cf = sqrt(real(0.5) * (tmp1 + tmp2));
+    real cf = fsqrt(real(0.5) * (tmp1 + tmp2));
     real ev[6] = {vx2 - cf, vx2 - ca, vx2 - cslow, vx2 + cslow, vx2 + ca, vx2 + cf};
     smax = max2(smax, fabs(vx2) + cf);
-    real bt = sqrt(qa[5] * qa[5] + qa[6] * qa[6]);
+    real bt = fsqrt(qa[5] * qa[5] + qa[6] * qa[6]);
     real af, as;
     if (fabs(cf * cf - cslow * cslow) < zero) { af = real(1); as = real(1); }
     else if (fabs(cf * cf - ca * ca) < zero) { af = real(0); as = real(1); }
     else if (fabs(cf * cf - asqr) < zero) { af = real(1); as = real(0); }
     else {
-        real t = real(1) / sqrt(max2(real(0), cf * cf - cslow * cslow));
-        af = sqrt(max2(real(0), cf * cf - ca * ca)) * t;
-        as = sqrt(max2(real(0), cf * cf - asqr)) * t;
+        real t = fdiv(real(1), fsqrt(max2(real(0), cf * cf - cslow * cslow)));
+        af = fsqrt(max2(real(0), cf * cf - ca * ca)) * t;
+        as = fsqrt(max2(real(0), cf * cf - asqr)) * t;
     }
     real by_, bz_;
-    if (bt < zero) { by_ = real(1) / sqrt(real(2)); bz_ = real(1) / sqrt(real(2)); }
-    else { real t = real(1) / bt; by_ = qa[5] * t; bz_ = qa[6] * t; }
+    if (bt < zero) { by_ = fdiv(real(1), fsqrt(real(2))); bz_ = fdiv(real(1), fsqrt(real(2))); }
+    else { real t = fdiv(real(1), bt); by_ = qa[5] * t; bz_ = qa[6] * t; }
     real signbt = (fabs(qa[5]) >= zero) ? sgn1(qa[5]) : sgn1(qa[6]);
     real amc = cs * cs - ca * ca;
     real sbx = sgn1(qa[4]);
@@ -199,7 +199,7 @@ __device__ __forceinline__ void roe(const real* qL, const real* qR, real cs, rea
         R[0][4] = real(0); R[1][4] = real(0); R[2][4] = -sbx * bz_; R[3][4] = sbx * by_;
         R[4][4] = isrho * bz_; R[5][4] = -isrho * by_;
         real s1 = as * qa[2], s2 = af * by_ * cs * sbx, s3 = as * qa[3], s4 = af * bz_ * cs * sbx;
-        real s5 = af * (cs * cs) * isrho / cf;
+        real s5 = fdiv(af * (cs * cs) * isrho, cf);
         R[0][2] = as; R[1][2] = as * ev[2]; R[2][2] = s1 - s2; R[3][2] = s3 - s4; R[4][2] = -s5 * by_; R[5][2] = -s5 * bz_;
         R[0][3] = as; R[1][3] = as * ev[3]; R[2][3] = s1 + s2; R[3][3] = s3 + s4; R[4][3] = -s5 * by_; R[5][3] = -s5 * bz_;
         if (amc > zero) {
@@ -208,14 +208,14 @@ __device__ __forceinline__ void roe(const real* qL, const real* qR, real cs, rea
         }
     }
     real dq[6] = {qR[0] - qL[0], qR[1] - qL[1], qR[2] - qL[2], qR[3] - qL[3], qR[5] - qL[5], qR[6] - qL[6]};
-    real ith1 = real(1) / (real(2) * (af * af * (cs * cs) + as * as * (cf * cf)));
-    real ith2 = real(1) / (real(2) * (af * af * cs * cf + as * as * cslow * ca));
+    real ith1 = fdiv(real(1), real(2) * (af * af * (cs * cs) + as * as * (cf * cf)));
+    real ith2 = fdiv(real(1), real(2) * (af * af * cs * cf + as * as * cslow * ca));
     real ck[6];
     {
         real t1 = ith1 * af * (cs * cs);
         real t2 = ith2 * (-(af * cs * qa[1]) + as * cslow * sbx * (by_ * qa[2] + bz_ * qa[3]));
         real t3 = ith2 * as * cslow * sbx;
-        real t4 = ith1 * as * cf * sqrt(qa[0]);
+        real t4 = ith1 * as * cf * fsqrt(qa[0]);
         real Lm[6] = {t1 - t2, -(ith2 * af * cs), t3 * by_, t3 * bz_, t4 * by_, t4 * bz_};
         real Lp[6] = {t1 + t2, -Lm[1], -Lm[2], -Lm[3], Lm[4], Lm[5]};
         if (amc < zero) {
@@ -228,7 +228,7 @@ __device__ __forceinline__ void roe(const real* qL, const real* qR, real cs, rea
     {
         real t1 = real(0.5) * (bz_ * qa[2] - by_ * qa[3]) * sbx;
         real t2 = real(0.5) * sbx;
-        real t3 = real(0.5) * sqrt(qa[0]);
+        real t3 = real(0.5) * fsqrt(qa[0]);
         real Lm[6] = {-t1, real(0), t2 * bz_, -(t2 * by_), t3 * bz_, -(t3 * by_)};
         real Lp[6] = {t1, real(0), -Lm[2], -Lm[3], Lm[4], Lm[5]};
         ck[1] = dq[0] * Lm[0] + dq[1] * Lm[1] + dq[2] * Lm[2] + dq[3] * Lm[3] + dq[4] * Lm[4] + dq[5] * Lm[5];
@@ -238,7 +238,7 @@ __device__ __forceinline__ void roe(const real* qL, const real* qR, real cs, rea
         real t1 = ith1 * as * (cf * cf);
         real t2 = ith2 * (as * ca * qa[1] + af * cf * sbx * (by_ * qa[2] + bz_ * qa[3]));
         real t3 = ith2 * af * cf * sbx;
-        real t4 = ith1 * af * cf * sqrt(qa[0]);
+        real t4 = ith1 * af * cf * fsqrt(qa[0]);
         real Lm[6] = {t1 + t2, -(ith2 * as * ca), -(t3 * by_), -(t3 * bz_), -(t4 * by_), -(t4 * bz_)};
         real Lp[6] = {t1 - t2, -Lm[1], -Lm[2], -Lm[3], Lm[4], Lm[5]};
         if (amc > zero) {
@@ -250,7 +250,7 @@ __device__ __forceinline__ void roe(const real* qL, const real* qR, real cs, rea
     }
     real FLc[6], FRc[6];
     {
-        real rho = qL[0], vx = qL[1] / rho, vy = qL[2] / rho, vz = qL[3] / rho, Bxx = qL[4], By = qL[5], Bz = qL[6];
+        real rho = qL[0], vx = fdiv(qL[1], rho), vy = fdiv(qL[2], rho), vz = fdiv(qL[3], rho), Bxx = qL[4], By = qL[5], Bz = qL[6];
         FLc[0] = rho * vx;
         FLc[1] = rho * (vx * vx + cs * cs) + real(0.5) * (By * By + Bz * Bz - Bxx * Bxx);
         FLc[2] = rho * vx * vy - Bxx * By;
@@ -259,7 +259,7 @@ __device__ __forceinline__ void roe(const real* qL, const real* qR, real cs, rea
         FLc[5] = vx * Bz - vz * Bxx;
     }
     {
-        real rho = qR[0], vx = qR[1] / rho, vy = qR[2] / rho, vz = qR[3] / rho, Bxx = qR[4], By = qR[5], Bz = qR[6];
+        real rho = qR[0], vx = fdiv(qR[1], rho), vy = fdiv(qR[2], rho), vz = fdiv(qR[3], rho), Bxx = qR[4], By = qR[5], Bz = qR[6];
         FRc[0] = rho * vx;
         FRc[1] = rho * (vx * vx + cs * cs) + real(0.5) * (By * By + Bz * Bz - Bxx * Bxx);
         FRc[2] = rho * vx * vy - Bxx * By;
@@ -272,7 +272,7 @@ __device__ __forceinline__ void roe(const real* qL, const real* qR, real cs, rea
 #pragma unroll
     for (int k = 0; k < 6; k++) {
         real a = fabs(ev[k]);
-        real betak = (a >= dh[k]) ? a * ck[k] : ((a + dh[k] * dh[k]) / (real(2) * dh[k])) * ck[k];
+        real betak = (a >= dh[k]) ? a * ck[k] : fdiv(a + dh[k] * dh[k], real(2) * dh[k]) * ck[k];
 #pragma unroll
         for (int m = 0; m < 6; m++) FT[m] = FT[m] - betak * R[m][k];
     }

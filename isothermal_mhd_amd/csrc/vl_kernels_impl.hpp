@@ -1,9 +1,12 @@
 // VL2 + CT cell-update kernels (Stone & Gardiner 2009 unsplit predictor-corrector with
 // upwind constrained transport), following vanleer_mod.f90 of the reference.
+// Instantiated per precision by vl_kernels_f64.hip (no FMA contraction: bitwise with the
+// reference) and vl_kernels_f32.hip (contraction and hardware rcp/sqrt).
 //
 // Per stage:  k_flux (x3 directions)  ->  k_corner  ->  k_update
 // Fluxes and face EMFs go to HBM once per face; the corner EMF is summed in FP64; the
 // update writes a second state buffer (no copy/restore of the t^n state).
+#pragma once
 #include "kernels.hpp"
 #include "riemann.hpp"
 
@@ -11,24 +14,6 @@ namespace imhd {
 
 __device__ __forceinline__ long long dir_stride(const BoxDesc& B, int d) {
     return d == 0 ? 1LL : (d == 1 ? (long long)B.P[0] : (long long)B.P[0] * (long long)B.P[1]);
-}
-
-// Plan lookup that also reports the box for inactive (padding) lanes so that every lane
-// of a wavefront joins the smax reduction; box starts are 64-aligned.
-__device__ __forceinline__ bool plan_item_box(const Plan& p, int& box, int& i, int& j, int& k) {
-    box = 0;
-    bool act = plan_item(p, box, i, j, k);
-    if (!act && p.nbox > 1) {
-        unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-        long long g = (long long)blk * blockDim.x + threadIdx.x;
-        int lo = 0, hi = p.nbox - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (p.boxes[mid].start <= g) lo = mid; else hi = mid - 1;
-        }
-        box = lo;
-    }
-    return act;
 }
 
 template <class real>
@@ -86,7 +71,7 @@ __global__ __launch_bounds__(256) void k_flux_corr(const BoxDesc* __restrict__ b
         for (int t = 0; t < 4; t++) {
             long long ct = c + (long long)(t - 1) * s;
             real rho = a.w[0][ct];
-            real irho = real(1) / rho;
+            real irho = fdiv(real(1), rho);
             W[t][0] = rho;
             W[t][1] = a.w[1][ct] * irho;
             W[t][2] = a.w[2][ct] * irho;
@@ -182,10 +167,8 @@ __global__ __launch_bounds__(256) void k_corner(const BoxDesc* __restrict__ boxe
 // (:2241-2256).  The lower faces needed by the face average are recomputed from the
 // corner EMFs, so the kernel never reads a face it writes.
 template <class real>
-__global__ __launch_bounds__(256) void k_update(const BoxDesc* __restrict__ boxes, Plan plan, UpdateArgs<real> a) {
-    int b, i, j, k;
-    if (!plan_item(plan, b, i, j, k)) return;
-    const BoxDesc& B = boxes[b];
+__device__ __forceinline__ void update_cell(const BoxDesc& B, const PlanBox& pb, int b, int i, int j, int k,
+                                            const UpdateArgs<real>& a, double& cfl) {
     const long long c = cell_index(B, i, j, k);
     const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
     const double dtd = a.factor * a.dt[b] / B.dx;
@@ -219,7 +202,6 @@ __global__ __launch_bounds__(256) void k_update(const BoxDesc* __restrict__ boxe
     a.dst.b[0][c] = bx;
     a.dst.b[1][c] = by;
     a.dst.b[2][c] = bz;
-    const PlanBox& pb = plan.boxes[b];
     const int sh = a.ccb_lo_shift;
     if (i >= pb.lo[0] + sh && j >= pb.lo[1] + sh && k >= pb.lo[2] + sh) {
         long long m = c - sx;
@@ -236,6 +218,22 @@ __global__ __launch_bounds__(256) void k_update(const BoxDesc* __restrict__ boxe
         a.dst.q[5][c] = a.src.q[5][c];
         a.dst.q[6][c] = a.src.q[6][c];
     }
+    // CFL speed of the new state on interior cells (the driver's compute_new_time_step
+    // reads the same interior values after the passive step and ghost fill)
+    if (a.cmax != nullptr && i >= 1 && j >= 1 && k >= 1 && i <= B.n[0] && j <= B.n[1] && k <= B.n[2]) {
+        double q7[7];
+        for (int f = 0; f < 7; f++) q7[f] = (double)a.dst.q[f][c];
+        cfl = cell_fast_speed(q7, a.cs);
+    }
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void k_update(const BoxDesc* __restrict__ boxes, Plan plan, UpdateArgs<real> a) {
+    int b, i, j, k;
+    bool act = plan_item_box(plan, b, i, j, k);
+    double cfl = 0.0;
+    if (act) update_cell(boxes[b], plan.boxes[b], b, i, j, k, a, cfl);
+    if (a.cmax != nullptr) atomic_max_nonneg(a.cmax, b, cfl);
 }
 
 template <class real, int RS>
@@ -259,16 +257,13 @@ void launch_update(const Launch& L, const UpdateArgs<real>& a) {
     hipLaunchKernelGGL((k_update<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
 }
 
-#define INST_RS(T, RS)                                                              \
+#define IMHD_INST_RS(T, RS)                                                              \
     template void launch_flux_predictor<T, RS>(const Launch&, const SweepArgs<T>&); \
     template void launch_flux_corrector<T, RS>(const Launch&, const SweepArgs<T>&);
-#define INST(T)                                                              \
-    INST_RS(T, RS_ROE)                                                       \
-    INST_RS(T, RS_HLLD)                                                      \
-    INST_RS(T, RS_HLLE)                                                      \
+#define IMHD_INST_VL(T)                                                      \
+    IMHD_INST_RS(T, RS_ROE)                                                  \
+    IMHD_INST_RS(T, RS_HLLD)                                                 \
+    IMHD_INST_RS(T, RS_HLLE)                                                 \
     template void launch_corner<T>(const Launch&, const CornerArgs<T>&);     \
     template void launch_update<T>(const Launch&, const UpdateArgs<T>&);
-INST(double)
-INST(float)
-
 }  // namespace imhd
