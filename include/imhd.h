@@ -35,6 +35,8 @@ extern "C" {
 #define IMHD_HLLE 3
 /* recon: corrector-stage reconstruction of the VL integrator */
 #define IMHD_PLM_PRIM 0   /* primitive PLM with MC-beta slope (vanleer_mod.f90:1160-1215) */
+#define IMHD_PPM_PRIM 1   /* primitive PPM as compiled (vanleer_mod.f90:1217-1334) */
+#define IMHD_PPM_CHAR 2   /* characteristic PPM, the reference default (vanleer_mod.f90:941-1141, 1488-1818) */
 /* bc: reference boundary_type (constants_mod.f90:36) */
 #define IMHD_BC_OPEN 1
 #define IMHD_BC_PERIODIC 2
@@ -45,7 +47,7 @@ typedef struct imhd_config {
     int npass;          /* passive scalars, 0..4 (constants_mod.f90:63) */
     int integrator;     /* IMHD_VL (IMHD_TVD: see imhd_create) */
     int riemann;        /* IMHD_ROE / IMHD_HLLD / IMHD_HLLE */
-    int recon;          /* IMHD_PLM_PRIM */
+    int recon;          /* IMHD_PLM_PRIM / IMHD_PPM_PRIM / IMHD_PPM_CHAR */
     int bc;             /* IMHD_BC_OPEN / IMHD_BC_PERIODIC */
     int precision;      /* 64: FP64 throughout; 32: FP32 bulk, FP64 face B, corner EMF, CFL, diagnostics */
     int compensated;    /* precision 32 only: Kahan-compensated accumulation of rho, rho v */

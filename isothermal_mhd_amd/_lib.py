@@ -8,7 +8,7 @@ LIB_PATH = os.path.join(HERE, "libimhd.so")
 
 IMHD_TVD, IMHD_VL = 1, 2
 IMHD_ROE, IMHD_HLLD, IMHD_HLLE = 1, 2, 3
-IMHD_PLM_PRIM = 0
+IMHD_PLM_PRIM, IMHD_PPM_PRIM, IMHD_PPM_CHAR = 0, 1, 2
 IMHD_BC_OPEN, IMHD_BC_PERIODIC = 1, 2
 
 ERRORS = {-1: "IMHD_EINVAL", -2: "IMHD_EHIP", -3: "IMHD_ESTATE", -4: "IMHD_ENOSPEED"}

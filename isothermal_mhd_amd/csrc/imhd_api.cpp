@@ -101,6 +101,7 @@ int build_plan(imhd_ctx* c, PlanDev& p, int which, int dir) {
     p.h.clear();
     p.active = 0;
     long long run = 0;
+    const int e = c->cfg.recon == IMHD_PLM_PRIM ? 0 : 1;  // PPM needs one more cell (vanleer_mod.f90:379-383)
     for (int bx = 0; bx < c->nbox; bx++) {
         const BoxDesc& B = c->hb[bx];
         PlanBox pb{};
@@ -114,12 +115,12 @@ int build_plan(imhd_ctx* c, PlanDev& p, int which, int dir) {
                     lo = 1 - nb; hi = n + nb - 1; break;
                 case 2:  // predictor update (vanleer_mod.f90:2102-2104)
                     lo = 2 - nb; hi = n + nb - 1; break;
-                case 3:  // corrector PLM fluxes, offset 2 (+1 for the reconstruction)
-                    lo = normal ? 4 - nb : 3 - nb; hi = normal ? n + nb - 4 : n + nb - 2; break;
-                case 4:  // corrector corner EMF, offset 3
-                    lo = 4 - nb; hi = n + nb - 4; break;
-                case 5:  // final update, offset 3
-                    lo = 5 - nb; hi = n + nb - 4; break;
+                case 3:  // corrector fluxes, offset 2 (+1 PLM / +2 PPM for the reconstruction)
+                    lo = normal ? 4 - nb + e : 3 - nb; hi = normal ? n + nb - 4 - e : n + nb - 2; break;
+                case 4:  // corrector corner EMF, offset 3 (PLM) / 4 (PPM)
+                    lo = 4 - nb + e; hi = n + nb - 4 - e; break;
+                case 5:  // final update, offset 3 (PLM) / 4 (PPM)
+                    lo = 5 - nb + e; hi = n + nb - 4 - e; break;
                 case 6:  // whole padded box
                     lo = 1 - nb; hi = n + nb; break;
                 case 7:  // passive ghost band
@@ -515,7 +516,7 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     auto bad = [&](const char* m) { c->err = m; return IMHD_EINVAL; };
     if (g.integrator != IMHD_VL) { int r = bad("only the VL integrator (integrator = 2) is available"); *out = c; return r; }
     if (g.riemann < 1 || g.riemann > 3) { int r = bad("riemann must be 1 (ROE), 2 (HLLD) or 3 (HLLE)"); *out = c; return r; }
-    if (g.recon != IMHD_PLM_PRIM) { int r = bad("recon must be 0 (PLM primitive)"); *out = c; return r; }
+    if (g.recon < 0 || g.recon > 2) { int r = bad("recon must be 0 (PLM primitive), 1 (PPM primitive) or 2 (PPM characteristic)"); *out = c; return r; }
     if (g.precision != 32 && g.precision != 64) { int r = bad("precision must be 32 or 64"); *out = c; return r; }
     if (g.npass < 0 || g.npass > MAXPASS) { int r = bad("npass must be in 0..4"); *out = c; return r; }
     if (g.bc != 1 && g.bc != 2) { int r = bad("bc must be 1 (open) or 2 (periodic)"); *out = c; return r; }

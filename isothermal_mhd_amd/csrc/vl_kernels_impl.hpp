@@ -9,6 +9,7 @@
 #pragma once
 #include "kernels.hpp"
 #include "riemann.hpp"
+#include "ppm.hpp"
 
 namespace imhd {
 
@@ -106,6 +107,41 @@ __global__ __launch_bounds__(256) void k_flux_corr(const BoxDesc* __restrict__ b
         uR[6] = W[2][5] - real(0.5) * d2[5];
         uL[1] = uL[1] * uL[0]; uL[2] = uL[2] * uL[0]; uL[3] = uL[3] * uL[0];
         uR[1] = uR[1] * uR[0]; uR[2] = uR[2] * uR[0]; uR[3] = uR[3] * uR[0];
+        real smax = real(0);
+        riemann<RS>(uL, uR, (real)a.cs, F, smax);
+        store_flux(a, c, F);
+        sm = (double)smax;
+    }
+    atomic_max_nonneg(a.smax, b, sm);
+}
+
+// Corrector fluxes with PPM reconstruction (characteristic: the reference default;
+// primitive: the compiled branch), six cells c-2 .. c+3 per interface.
+template <class real, int RS, bool CHAR>
+__global__ __launch_bounds__(256) void k_flux_corr_ppm(const BoxDesc* __restrict__ boxes, Plan plan, SweepArgs<real> a) {
+    int b, i, j, k;
+    bool act = plan_item_box(plan, b, i, j, k);
+    double sm = 0.0;
+    if (act) {
+        const BoxDesc& B = boxes[b];
+        long long c = cell_index(B, i, j, k);
+        long long s = dir_stride(B, a.dir);
+        real W[6][6], wbc[6];
+#pragma unroll
+        for (int t = 0; t < 6; t++) {
+            long long ct = c + (long long)(t - 2) * s;
+            real rho = a.w[0][ct];
+            real irho = fdiv(real(1), rho);
+            W[t][0] = rho;
+            W[t][1] = a.w[1][ct] * irho;
+            W[t][2] = a.w[2][ct] * irho;
+            W[t][3] = a.w[3][ct] * irho;
+            W[t][4] = a.w[5][ct];
+            W[t][5] = a.w[6][ct];
+            wbc[t] = (real)a.bn[ct];
+        }
+        real uL[7], uR[7], F[7];
+        ppm_states<real, CHAR>(W, wbc, (real)a.cs, (real)a.mc_beta, uL, uR);
         real smax = real(0);
         riemann<RS>(uL, uR, (real)a.cs, F, smax);
         store_flux(a, c, F);
@@ -244,7 +280,12 @@ void launch_flux_predictor(const Launch& L, const SweepArgs<real>& a) {
 template <class real, int RS>
 void launch_flux_corrector(const Launch& L, const SweepArgs<real>& a) {
     if (L.plan.total <= 0) return;
-    hipLaunchKernelGGL((k_flux_corr<real, RS>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+    if (a.recon == 2)
+        hipLaunchKernelGGL((k_flux_corr_ppm<real, RS, true>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+    else if (a.recon == 1)
+        hipLaunchKernelGGL((k_flux_corr_ppm<real, RS, false>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+    else
+        hipLaunchKernelGGL((k_flux_corr<real, RS>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
 }
 template <class real>
 void launch_corner(const Launch& L, const CornerArgs<real>& a) {

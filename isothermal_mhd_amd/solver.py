@@ -13,6 +13,7 @@ from . import _lib as L
 
 _RIEMANN = {"roe": L.IMHD_ROE, "hlld": L.IMHD_HLLD, "hlle": L.IMHD_HLLE}
 _BC = {"open": L.IMHD_BC_OPEN, "periodic": L.IMHD_BC_PERIODIC}
+_RECON = {"plm": L.IMHD_PLM_PRIM, "ppm_prim": L.IMHD_PPM_PRIM, "ppm_char": L.IMHD_PPM_CHAR}
 
 
 def padded_shapes(nx, ny, nz, nb, npass):
@@ -37,15 +38,15 @@ class VanLeerSolver:
                  precision=64, compensated=None, cs=1.0, mc_beta=1.01, cour=0.35):
         self.lib = L.lib()
         boxes = [tuple(dims)] if np.ndim(dims) == 1 else [tuple(d) for d in dims]
-        if recon != "plm":
-            raise ValueError("recon must be 'plm' (primitive PLM with the MC-beta limiter)")
+        if recon not in _RECON:
+            raise ValueError("recon must be 'plm', 'ppm_prim' or 'ppm_char'")
         if compensated is None:
             compensated = precision == 32
         cfgs = (L.ImhdConfig * len(boxes))()
         for i, (nx, ny, nz) in enumerate(boxes):
             c = cfgs[i]
             c.nx, c.ny, c.nz, c.nb, c.npass = nx, ny, nz, nb, npass
-            c.integrator, c.riemann, c.recon = L.IMHD_VL, _RIEMANN[riemann], L.IMHD_PLM_PRIM
+            c.integrator, c.riemann, c.recon = L.IMHD_VL, _RIEMANN[riemann], _RECON[recon]
             c.bc, c.precision, c.compensated = _BC[bc], precision, int(bool(compensated))
             c.cs, c.mc_beta = cs, mc_beta
             c.cour = cour[i] if np.ndim(cour) else cour
