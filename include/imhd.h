@@ -29,7 +29,9 @@ extern "C" {
 /* integrator: reference mhd_solver_type (constants_mod.f90:58) */
 #define IMHD_TVD 1
 #define IMHD_VL 2
-/* riemann: reference riemann_solver_type as read by the VL module (constants_mod.f90:57) */
+/* riemann: reference riemann_solver_type as read by the VL module (constants_mod.f90:57).
+ * With IMHD_TVD: IMHD_ROE selects the K98 Roe-TVD flux (the TVD module's type 2) and
+ * IMHD_HLLE its HLLE flux (type 1) (isotvd_mod.f90:400-428); IMHD_HLLD is rejected. */
 #define IMHD_ROE 1
 #define IMHD_HLLD 2
 #define IMHD_HLLE 3
@@ -45,9 +47,9 @@ typedef struct imhd_config {
     int nx, ny, nz;     /* interior cells (per box) */
     int nb;             /* ghost layers; the reference uses 6 (constants_mod.f90:12) */
     int npass;          /* passive scalars, 0..4 (constants_mod.f90:63) */
-    int integrator;     /* IMHD_VL (IMHD_TVD: see imhd_create) */
+    int integrator;     /* IMHD_VL (vanleer_mod.f90) or IMHD_TVD (isotvd_mod.f90) */
     int riemann;        /* IMHD_ROE / IMHD_HLLD / IMHD_HLLE */
-    int recon;          /* IMHD_PLM_PRIM / IMHD_PPM_PRIM / IMHD_PPM_CHAR */
+    int recon;          /* IMHD_PLM_PRIM / IMHD_PPM_PRIM / IMHD_PPM_CHAR (VL only) */
     int bc;             /* IMHD_BC_OPEN / IMHD_BC_PERIODIC */
     int precision;      /* 64: FP64 throughout; 32: FP32 bulk, FP64 face B, corner EMF, CFL, diagnostics */
     int compensated;    /* precision 32 only: Kahan-compensated accumulation of rho, rho v */
@@ -69,9 +71,9 @@ typedef struct imhd_diag {
 typedef struct imhd_ctx imhd_ctx;
 
 /* Context for one box, or an ensemble of nbox boxes (cfg[b] per box; all boxes share the
- * solver options of cfg[0]).  Replaces init_solver_vanleer (vanleer_mod.f90:127-158),
- * init_passive_solver (passive_mod.f90:44-57) and create_grid_arrays
- * (grid_data_mod.f90:33-67) for the device side. */
+ * solver options of cfg[0]).  Replaces init_solver_vanleer (vanleer_mod.f90:127-158) or
+ * init_solver_tvd (isotvd_mod.f90:122-133, 144-202), init_passive_solver (passive_mod.f90:44-57)
+ * and create_grid_arrays (grid_data_mod.f90:33-67) for the device side. */
 int imhd_create(const imhd_config* cfg, imhd_ctx** out);
 int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out);
 /* destroy_solver_vanleer / destroy_passive_solver (vanleer_mod.f90:161-165) */
@@ -84,16 +86,20 @@ int imhd_set_state(imhd_ctx* ctx, int box, const double* q_3d, const double* bfa
 int imhd_get_state(imhd_ctx* ctx, int box, double* q_3d, double* bface_3d, double* qpass_3d);
 
 /* One integrator step of every box with the given dt:
- * vanleer_solve_3d(dt_in, drive_next) (vanleer_mod.f90:279-426); drive_next must be 0
- * (forcing is applied by imhd_run). */
+ * vanleer_solve_3d(dt_in, drive_next) (vanleer_mod.f90:279-426) or tvd_solve_3d(dt_in)
+ * (isotvd_mod.f90:226-364); drive_next must be 0 (forcing is applied by imhd_run).
+ * Cells the step does not update keep their t^n values, as the reference's restore of
+ * q_3d / bface_3d leaves them; the driver fills the ghost cells afterwards. */
 int imhd_step(imhd_ctx* ctx, double dt, int drive_next);
 
 /* boundary_conditions() on one rank (boundary_conditions_mod.f90:20-234) */
 int imhd_fill_ghosts(imhd_ctx* ctx);
 
 /* get_max_speed_vanleer (vanleer_mod.f90:3266-3334): max over interior cells of
- * |v| + c_f, combined with the interface maximum smax of the last step.
- * cmax receives one value per box. */
+ * |v| + c_f, combined with the interface maximum smax of the last step; with IMHD_TVD
+ * get_max_speed_tvd (isotvd_mod.f90:2607-2673): max of |v| + c_f of the interface-mean
+ * states over i, j, k in [0, n] (reads ghost layer 0: the driver calls it before the
+ * ghost fill, driver.f90:131-160).  cmax receives one value per box. */
 int imhd_max_speed(imhd_ctx* ctx, double* cmax);
 
 /* copy_velocity(predictor_stage) (passive_mod.f90:204-244) and passive_solve_3d(dt)
@@ -109,10 +115,10 @@ int imhd_diagnostics(imhd_ctx* ctx, imhd_diag* out);
 int imhd_floor_count(imhd_ctx* ctx, unsigned int* count);
 
 /* Native, device-resident time loop of the driver (driver.f90:35-186) for nsteps steps:
- * copy_velocity, step, passive, ghost fill, CFL dt = COUR dx / cmax with
- * dt = MIN(t_end - t, dt).  dt stays on the device; the first call starts from
- * dt = 1e-15 (driver.f90:249) unless imhd_set_dt was called.  With use_graph != 0 one
- * step is captured once in a hipGraph and replayed. */
+ * copy_velocity, step, passive, CFL dt = COUR dx / cmax with dt = MIN(t_end - t, dt),
+ * ghost fill.  dt stays on the device; the first call starts from dt = 1e-15
+ * (driver.f90:249) unless imhd_set_time set another.  With use_graph != 0 one step is
+ * captured once in a hipGraph and replayed. */
 int imhd_run(imhd_ctx* ctx, int nsteps, double t_end, int use_graph);
 int imhd_set_time(imhd_ctx* ctx, const double* t, const double* dt); /* per box; NULL keeps */
 int imhd_get_time(imhd_ctx* ctx, double* t, double* dt);    /* per box */

@@ -10,6 +10,8 @@ OBJ = os.path.join(HERE, "_build")
 # (source, extra flags): the FP32 kernels may contract to FMA; everything else may not.
 SOURCES = [("vl_kernels_f64.hip", ["-ffp-contract=off"]),
            ("vl_kernels_f32.hip", ["-ffp-contract=fast"]),
+           ("tvd_kernels_f64.hip", ["-ffp-contract=off"]),
+           ("tvd_kernels_f32.hip", ["-ffp-contract=fast"]),
            ("aux_kernels.hip", ["-ffp-contract=off"]),
            ("imhd_api.cpp", ["-ffp-contract=off"])]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -12,6 +12,7 @@
 #include "../../include/imhd.h"
 #include "aux_kernels.hpp"
 #include "kernels.hpp"
+#include "tvd_kernels.hpp"
 
 using namespace imhd;
 
@@ -52,6 +53,11 @@ struct imhd_ctx {
     void* vta[3] = {};
     void* Fp[3][MAXPASS] = {};
     void* comp[4] = {};
+    // K98 TVD integrator: corrector fluxes, corrector centre EMF, per-face limiter inputs
+    void* Fc[3][4] = {};
+    double* Ecc[3] = {};
+    void* gt[6] = {};
+    void* ck[6] = {};
     int cur = 0, pcur = 0;
     bool have_state = false;
     // per-box device scalars
@@ -70,6 +76,8 @@ struct imhd_ctx {
     PlanDev pl_pflux[3], pl_pcorner, pl_pupd, pl_cflux[3], pl_ccorner, pl_cupd;
     PlanDev pl_gband, pl_pgband;
     PlanDev pl_ghost, pl_pghost, pl_int, pl_vel, pl_ppflux[3], pl_ppupd, pl_pcflux[3], pl_pfupd;
+    PlanDev pl_teig_p[3], pl_tflux_p[3], pl_teig_c[3], pl_tflux_c[3];
+    PlanDev pl_tcorner_p, pl_tcorner_c, pl_tcenter, pl_thalf, pl_tcfl;
     int ccb_shift = 1;
     // native loop
     hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};  // by state/passive buffer parity
@@ -137,6 +145,25 @@ int build_plan(imhd_ctx* c, PlanDev& p, int which, int dir) {
                     lo = normal ? 0 : -1; hi = normal ? n : n + 2; break;
                 case 13:  // passive final update (offset 2)
                     lo = 1; hi = n; break;
+                // K98 TVD integrator (isotvd_mod.f90); offset 0 predictor, 2 corrector
+                case 16:  // predictor Roe average / HLLE faces: pencil cells [1-nb, n+nb]
+                    lo = 1 - nb; hi = normal ? n + nb - 1 : n + nb; break;
+                case 17:  // predictor TVD fluxes and face EMFs (isotvd_mod.f90:2413, 446)
+                    lo = normal ? 2 - nb : 1 - nb; hi = normal ? n + nb - 2 : n + nb; break;
+                case 18:  // corrector Roe average / HLLE faces: pencil cells [3-nb, n+nb-2]
+                    lo = 3 - nb; hi = normal ? n + nb - 3 : n + nb - 2; break;
+                case 19:  // corrector TVD fluxes and face EMFs
+                    lo = normal ? 4 - nb : 3 - nb; hi = normal ? n + nb - 4 : n + nb - 2; break;
+                case 20:  // predictor corner EMF (isotvd_mod.f90:1006-1008)
+                    lo = 2 - nb; hi = n + nb - 2; break;
+                case 21:  // corrector corner EMF
+                    lo = 4 - nb; hi = n + nb - 4; break;
+                case 22:  // corrector centre EMF (isotvd_mod.f90:880-882)
+                    lo = 4 - nb; hi = n + nb - 3; break;
+                case 23:  // CTU half updates (isotvd_mod.f90:1203-1205)
+                    lo = 3 - nb; hi = n + nb - 2; break;
+                case 24:  // get_max_speed_tvd (isotvd_mod.f90:2620-2622)
+                    lo = 0; hi = n; break;
             }
             pb.lo[a] = lo;
             pb.r[a] = std::max(0, hi - lo + 1);
@@ -281,6 +308,138 @@ void vl_step(imhd_ctx* c, bool fuse_cmax = false) {
     c->cur = s1;
 }
 
+// ---- K98 TVD integrator (tvd_solve_3d, isotvd_mod.f90:226-364) ----
+
+template <class real>
+TvdSweepArgs<real> tvd_args(imhd_ctx* c, int sq, int sb, bool corrector, int d) {
+    TvdSweepArgs<real> a{};
+    for (int r = 0; r < 7; r++) a.w[r] = (r == 4) ? nullptr : R<real>(c->q[sq][perm(d, r)]);
+    a.bn = c->b[sb][d];
+    for (int w = 0; w < 6; w++) { a.gt[w] = R<real>(c->gt[w]); a.ck[w] = R<real>(c->ck[w]); }
+    for (int r = 0; r < 4; r++) a.F[r] = R<real>(corrector ? c->Fc[d][perm(d, r)] : c->F[d][perm(d, r)]);
+    a.Et1 = R<real>(c->E[ET1[d]]);
+    a.Et2 = R<real>(c->E[ET2[d]]);
+    a.dt = c->dt;
+    a.factor = corrector ? 1.0 : 0.5;
+    a.cs = c->cfg.cs;
+    a.dir = d;
+    return a;
+}
+
+// compute_{x,y,z}fluxes_3d (isotvd_mod.f90:368-863): Roe-TVD (eig + flux) or HLLE
+template <class real>
+void tvd_sweep(imhd_ctx* c, int sq, int sb, bool corr, int d) {
+    static const char* nm[2][3][3] = {
+        {{"tvd_eig_pred_x", "tvd_eig_pred_y", "tvd_eig_pred_z"},
+         {"tvd_flux_pred_x", "tvd_flux_pred_y", "tvd_flux_pred_z"},
+         {"tvd_hlle_pred_x", "tvd_hlle_pred_y", "tvd_hlle_pred_z"}},
+        {{"tvd_eig_corr_x", "tvd_eig_corr_y", "tvd_eig_corr_z"},
+         {"tvd_flux_corr_x", "tvd_flux_corr_y", "tvd_flux_corr_z"},
+         {"tvd_hlle_corr_x", "tvd_hlle_corr_y", "tvd_hlle_corr_z"}}};
+    TvdSweepArgs<real> a = tvd_args<real>(c, sq, sb, corr, d);
+    const PlanDev& pe = corr ? c->pl_teig_c[d] : c->pl_teig_p[d];
+    if (c->cfg.riemann == IMHD_HLLE) {
+        ProfScope ps(c, nm[corr][2][d], pe.active);
+        launch_tvd_hlle<real>(mk(c, pe), a);
+        return;
+    }
+    {
+        ProfScope ps(c, nm[corr][0][d], pe.active);
+        launch_tvd_eig<real>(mk(c, pe), a);
+    }
+    const PlanDev& pf = corr ? c->pl_tflux_c[d] : c->pl_tflux_p[d];
+    ProfScope ps(c, nm[corr][1][d], pf.active);
+    launch_tvd_flux<real>(mk(c, pf), a);
+}
+
+template <class real>
+void tvd_corner(imhd_ctx* c, int s0, bool corr) {
+    const PlanDev& p = corr ? c->pl_tcorner_c : c->pl_tcorner_p;
+    ProfScope ps(c, corr ? "tvd_corner_corr" : "tvd_corner_pred", p.active);
+    TvdCornerArgs<real> a{};
+    for (int f = 0; f < 4; f++) a.q[f] = R<real>(c->q[s0][f]);
+    for (int f = 0; f < 3; f++) a.qb[f] = R<real>(c->q[s0][4 + f]);
+    for (int d = 0; d < 3; d++) a.Ecc[d] = corr ? c->Ecc[d] : nullptr;
+    for (int e = 0; e < 6; e++) a.E[e] = R<real>(c->E[e]);
+    for (int d = 0; d < 3; d++) a.Ec[d] = c->Ec[d];
+    launch_tvd_corner<real>(mk(c, p), a);
+}
+
+// One tvd_solve_3d.  Buffers: t^n in state/face set s0; the CT half-step faces, the CTU
+// half states and finally the new state in set s1.
+template <class real>
+void tvd_step(imhd_ctx* c) {
+    launch_reset_u64(c->st, c->smax, (int)stripe_slots(c->nbox));  // no smax in the TVD module
+    const int s0 = c->cur, s1 = 1 - c->cur;
+    for (int d = 0; d < 3; d++) tvd_sweep<real>(c, s0, s0, false, d);
+    tvd_corner<real>(c, s0, false);
+    {
+        ProfScope ps(c, "tvd_ct_pred", c->pl_ghost.active);
+        TvdCtArgs<real> a{};
+        for (int d = 0; d < 3; d++) { a.b0[d] = c->b[s0][d]; a.b1[d] = c->b[s1][d]; a.Ec[d] = c->Ec[d]; }
+        a.dt = c->dt;
+        launch_tvd_ct_pred<real>(mk(c, c->pl_ghost), a);
+    }
+    {
+        ProfScope ps(c, "tvd_center_corr", c->pl_tcenter.active);
+        TvdCenterArgs<real> a{};
+        for (int f = 0; f < 4; f++) a.q[f] = R<real>(c->q[s0][f]);
+        for (int d = 0; d < 3; d++) {
+            for (int f = 0; f < 4; f++) a.Fb[d][f] = R<real>(c->F[d][f]);
+            a.bh[d] = c->b[s1][d];
+            a.Ecc[d] = c->Ecc[d];
+        }
+        a.dt = c->dt;
+        launch_tvd_center<real>(mk(c, c->pl_tcenter), a);
+    }
+    static const char* hn[3] = {"tvd_half_x", "tvd_half_y", "tvd_half_z"};
+    for (int d = 0; d < 3; d++) {
+        {
+            ProfScope ps(c, hn[d], c->pl_thalf.active);
+            TvdHalfArgs<real> a{};
+            for (int f = 0; f < 7; f++) { a.q[f] = R<real>(c->q[s0][f]); a.dst[f] = R<real>(c->q[s1][f]); }
+            for (int e = 0; e < 3; e++) {
+                a.b0[e] = c->b[s0][e];
+                a.Ec[e] = c->Ec[e];
+                for (int f = 0; f < 4; f++) a.Fb[e][f] = R<real>(c->F[e][f]);
+            }
+            a.dt = c->dt;
+            a.floor_count = c->floor_count;
+            a.dir = d;
+            launch_tvd_half<real>(mk(c, c->pl_thalf), a);
+        }
+        tvd_sweep<real>(c, s1, s1, true, d);
+    }
+    tvd_corner<real>(c, s0, true);
+    {
+        ProfScope ps(c, "tvd_final", c->pl_ghost.active);
+        TvdFinalArgs<real> a{};
+        for (int f = 0; f < 7; f++) { a.src.q[f] = R<real>(c->q[s0][f]); a.dst.q[f] = R<real>(c->q[s1][f]); }
+        for (int d = 0; d < 3; d++) {
+            a.src.b[d] = c->b[s0][d];
+            a.dst.b[d] = c->b[s1][d];
+            a.Ec[d] = c->Ec[d];
+            for (int f = 0; f < 4; f++) a.F[d][f] = R<real>(c->Fc[d][f]);
+        }
+        a.dt = c->dt;
+        for (int f = 0; f < 4; f++) a.comp[f] = c->comp[0] ? R<real>(c->comp[f]) : nullptr;
+        a.floor_count = c->floor_count;
+        launch_tvd_final<real>(mk(c, c->pl_ghost), a);
+    }
+    c->cur = s1;
+}
+
+template <class real>
+void tvd_max_speed_launch(imhd_ctx* c) {
+    ProfScope ps(c, "max_speed_tvd", c->pl_tcfl.active);
+    launch_reset_u64(c->st, c->cmax, (int)stripe_slots(c->nbox));
+    QPtrs<real> q{};
+    for (int f = 0; f < 7; f++) q.q[f] = R<real>(c->q[c->cur][f]);
+    launch_tvd_max_speed<real>(mk(c, c->pl_tcfl), q, c->cfg.cs, c->cmax);
+}
+
+bool is_tvd(const imhd_ctx* c) { return c->cfg.integrator == IMHD_TVD; }
+
 template <class real>
 void fill_ghosts(imhd_ctx* c) {
     {
@@ -350,6 +509,7 @@ void passive_step(imhd_ctx* c) {
 
 template <class real>
 void max_speed_launch(imhd_ctx* c) {
+    if (is_tvd(c)) { tvd_max_speed_launch<real>(c); return; }
     ProfScope ps(c, "max_speed", c->pl_int.active);
     launch_reset_u64(c->st, c->cmax, (int)stripe_slots(c->nbox));
     QPtrs<real> q{};
@@ -388,8 +548,16 @@ void passive_fused(imhd_ctx* c, int qold) {
 
 template <class real>
 void native_step(imhd_ctx* c, double t_end) {
-    launch_reset_u64(c->st, c->cmax, (int)stripe_slots(c->nbox));
     const int qold = c->cur;
+    if (is_tvd(c)) {  // driver.f90:35-160 order: solve, passive, new dt, then ghost fill
+        tvd_step<real>(c);
+        if (c->cfg.npass > 0) passive_fused<real>(c, qold);
+        tvd_max_speed_launch<real>(c);
+        fill_ghosts<real>(c);
+        launch_new_dt(c->st, c->nbox, c->db, c->cmax, c->smax, c->cour, t_end, c->t, c->dt, c->dtprev, 1);
+        return;
+    }
+    launch_reset_u64(c->st, c->cmax, (int)stripe_slots(c->nbox));
     vl_step<real>(c, true);
     if (c->cfg.npass > 0) passive_fused<real>(c, qold);
     fill_ghosts<real>(c);
@@ -514,8 +682,9 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     c->nbox = nbox;
     const imhd_config& g = cfg[0];
     auto bad = [&](const char* m) { c->err = m; return IMHD_EINVAL; };
-    if (g.integrator != IMHD_VL) { int r = bad("only the VL integrator (integrator = 2) is available"); *out = c; return r; }
+    if (g.integrator != IMHD_VL && g.integrator != IMHD_TVD) { int r = bad("integrator must be 1 (TVD) or 2 (VL)"); *out = c; return r; }
     if (g.riemann < 1 || g.riemann > 3) { int r = bad("riemann must be 1 (ROE), 2 (HLLD) or 3 (HLLE)"); *out = c; return r; }
+    if (g.integrator == IMHD_TVD && g.riemann == IMHD_HLLD) { int r = bad("the TVD integrator takes riemann 1 (K98 Roe-TVD) or 3 (HLLE)"); *out = c; return r; }
     if (g.recon < 0 || g.recon > 2) { int r = bad("recon must be 0 (PLM primitive), 1 (PPM primitive) or 2 (PPM characteristic)"); *out = c; return r; }
     if (g.precision != 32 && g.precision != 64) { int r = bad("precision must be 32 or 64"); *out = c; return r; }
     if (g.npass < 0 || g.npass > MAXPASS) { int r = bad("npass must be in 0..4"); *out = c; return r; }
@@ -566,6 +735,17 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     }
     if (g.precision == 32 && g.compensated)
         for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->comp[f], nr))) return rc;
+    if (g.integrator == IMHD_TVD) {
+        for (int d = 0; d < 3; d++) {
+            for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->Fc[d][f], nr))) return rc;
+            if ((rc = alloc(c, (void**)&c->Ecc[d], nd))) return rc;
+        }
+        if (g.riemann == IMHD_ROE)
+            for (int w = 0; w < 6; w++) {
+                if ((rc = alloc(c, &c->gt[w], nr))) return rc;
+                if ((rc = alloc(c, &c->ck[w], nr))) return rc;
+            }
+    }
     if ((rc = alloc(c, (void**)&c->smax, 8 * stripe_slots(nbox)))) return rc;
     if ((rc = alloc(c, (void**)&c->cmax, 8 * stripe_slots(nbox)))) return rc;
     if ((rc = alloc(c, (void**)&c->dmax, 8 * stripe_slots(nbox)))) return rc;
@@ -598,6 +778,19 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     if ((rc = build_plan(c, c->pl_vel, 9, -1))) return rc;
     if ((rc = build_plan(c, c->pl_ppupd, 11, -1))) return rc;
     if ((rc = build_plan(c, c->pl_pfupd, 13, -1))) return rc;
+    if (g.integrator == IMHD_TVD) {
+        for (int d = 0; d < 3; d++) {
+            if ((rc = build_plan(c, c->pl_teig_p[d], 16, d))) return rc;
+            if ((rc = build_plan(c, c->pl_tflux_p[d], 17, d))) return rc;
+            if ((rc = build_plan(c, c->pl_teig_c[d], 18, d))) return rc;
+            if ((rc = build_plan(c, c->pl_tflux_c[d], 19, d))) return rc;
+        }
+        if ((rc = build_plan(c, c->pl_tcorner_p, 20, -1))) return rc;
+        if ((rc = build_plan(c, c->pl_tcorner_c, 21, -1))) return rc;
+        if ((rc = build_plan(c, c->pl_tcenter, 22, -1))) return rc;
+        if ((rc = build_plan(c, c->pl_thalf, 23, -1))) return rc;
+        if ((rc = build_plan(c, c->pl_tcfl, 24, -1))) return rc;
+    }
     return IMHD_OK;
 }
 
@@ -618,6 +811,10 @@ int imhd_destroy(imhd_ctx* c) {
     for (auto p : c->vta) fr(p);
     for (auto& a : c->Fp) for (auto p : a) fr(p);
     for (auto p : c->comp) fr(p);
+    for (auto& a : c->Fc) for (auto p : a) fr(p);
+    for (auto p : c->Ecc) fr(p);
+    for (auto p : c->gt) fr(p);
+    for (auto p : c->ck) fr(p);
     for (void* p : {(void*)c->smax, (void*)c->cmax, (void*)c->dmax, (void*)c->dt, (void*)c->t, (void*)c->dtprev, (void*)c->cour,
                     (void*)c->diag, (void*)c->floor_count, (void*)c->stage, (void*)c->db})
         fr(p);
@@ -625,7 +822,11 @@ int imhd_destroy(imhd_ctx* c) {
                       &c->pl_cflux[0], &c->pl_cflux[1], &c->pl_cflux[2], &c->pl_ccorner, &c->pl_cupd,
                       &c->pl_ghost, &c->pl_pghost, &c->pl_int, &c->pl_vel, &c->pl_ppflux[0], &c->pl_ppflux[1],
                       &c->pl_ppflux[2], &c->pl_ppupd, &c->pl_pcflux[0], &c->pl_pcflux[1], &c->pl_pcflux[2],
-                      &c->pl_pfupd, &c->pl_gband, &c->pl_pgband};
+                      &c->pl_pfupd, &c->pl_gband, &c->pl_pgband, &c->pl_teig_p[0], &c->pl_teig_p[1],
+                      &c->pl_teig_p[2], &c->pl_tflux_p[0], &c->pl_tflux_p[1], &c->pl_tflux_p[2],
+                      &c->pl_teig_c[0], &c->pl_teig_c[1], &c->pl_teig_c[2], &c->pl_tflux_c[0],
+                      &c->pl_tflux_c[1], &c->pl_tflux_c[2], &c->pl_tcorner_p, &c->pl_tcorner_c,
+                      &c->pl_tcenter, &c->pl_thalf, &c->pl_tcfl};
     for (auto p : pls) fr(p->d);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
@@ -654,7 +855,11 @@ int imhd_step(imhd_ctx* c, double dt, int drive_next) {
     if (drive_next) FAIL(c, IMHD_EINVAL, "forcing is applied by imhd_run; drive_next must be 0");
     if (!(dt > 0.0)) FAIL(c, IMHD_EINVAL, "dt must be positive");
     set_scalar_dt(c, dt);
-    if (c->prec == 64) vl_step<double>(c); else vl_step<float>(c);
+    if (is_tvd(c)) {
+        if (c->prec == 64) tvd_step<double>(c); else tvd_step<float>(c);
+    } else {
+        if (c->prec == 64) vl_step<double>(c); else vl_step<float>(c);
+    }
     HIPCK(c, hipGetLastError());
     return prof_collect(c);
 }

@@ -1,0 +1,625 @@
+// K98 Roe-TVD / HLLE + CTU + CT cell-update kernels (Kim et al. 1998 TVD fluxes with
+// corner-transport-upwind half updates and constrained transport), following
+// isotvd_mod.f90 of the reference.  Instantiated per precision by tvd_kernels_f64.hip (no
+// FMA contraction: bitwise with the reference) and tvd_kernels_f32.hip.
+//
+// Per step:   3 x (eig -> flux) predictor sweeps -> corner EMF -> CT half step and the
+//             corrector centre EMF -> 3 x (CTU half update -> eig -> flux) corrector
+//             sweeps -> corner EMF -> final update (hydro, CT faces, cell-centred B).
+// A Roe-TVD interface flux needs the flux limiters of its two neighbouring interfaces:
+// k_tvd_eig writes g~_k and alpha_k per face, k_tvd_flux combines three faces' values and
+// recomputes the right eigenvectors of its own face column by column (no 6x6 matrix is
+// held in registers).
+#pragma once
+#include "riemann.hpp"
+#include "tvd_kernels.hpp"
+
+namespace imhd {
+
+__device__ __forceinline__ long long tvd_stride(const BoxDesc& B, int d) {
+    return d == 0 ? 1LL : (d == 1 ? (long long)B.P[0] : (long long)B.P[0] * (long long)B.P[1]);
+}
+
+// eps(k) of the entropy fix: single-precision literals widened (isotvd_mod.f90:108)
+__device__ __forceinline__ double tvd_eps(int k) { return (k == 1 || k == 4) ? 0.0 : (double)0.2f; }
+
+// Roe (arithmetic) average, wave speeds, renormalisation factors (isotvd_mod.f90:1856-2020)
+template <class real>
+struct TvdWave {
+    real qa[7];
+    real cf, ca, cslow, af, as, by, bz, sbx, signbt, isrho, amc;
+    real ev[6];
+};
+
+template <class real>
+__device__ __forceinline__ void tvd_wave(const real* a, const real* b, real cs, TvdWave<real>& W) {
+    const real asqr = cs * cs;
+    real rho2 = real(0.5) * (a[0] + b[0]);
+    real isrho = fdiv(real(1), fsqrt(rho2));
+    W.qa[0] = rho2;
+    W.qa[1] = real(0.5) * (fdiv(a[1], a[0]) + fdiv(b[1], b[0]));
+    W.qa[2] = real(0.5) * (fdiv(a[2], a[0]) + fdiv(b[2], b[0]));
+    W.qa[3] = real(0.5) * (fdiv(a[3], a[0]) + fdiv(b[3], b[0]));
+    W.qa[4] = a[4] * isrho;
+    W.qa[5] = real(0.5) * (a[5] + b[5]) * isrho;
+    W.qa[6] = real(0.5) * (a[6] + b[6]) * isrho;
+    W.isrho = isrho;
+    const real vx2 = W.qa[1], bx = W.qa[4];
+    real bsqr = W.qa[4] * W.qa[4] + W.qa[5] * W.qa[5] + W.qa[6] * W.qa[6];
+    real btsqr = W.qa[5] * W.qa[5] + W.qa[6] * W.qa[6];
+    real tmp1 = asqr + bsqr, x = asqr - bsqr;
+    real tmp2 = fsqrt(x * x + real(4) * asqr * btsqr);
+    W.cslow = fsqrt(max2(real(0), real(0.5) * (tmp1 - tmp2)));
+    W.ca = fabs(bx);
+    W.cf = fsqrt(real(0.5) * (tmp1 + tmp2));
+    W.ev[0] = vx2 - W.cf;
+    W.ev[1] = vx2 - W.ca;
+    W.ev[2] = vx2 - W.cslow;
+    W.ev[3] = vx2 + W.cslow;
+    W.ev[4] = vx2 + W.ca;
+    W.ev[5] = vx2 + W.cf;
+    real bt = fsqrt(W.qa[5] * W.qa[5] + W.qa[6] * W.qa[6]);
+    const real cf = W.cf, ca = W.ca, csl = W.cslow;
+    // degeneracy rule of the TVD module ("MODIFIED V2", isotvd_mod.f90:1999-2009)
+    if ((bt < (real)ZERO && fabs(ca * ca - asqr) < (real)ZERO) || (fabs(cf * cf - csl * csl) < (real)ZERO)) {
+        W.af = real(1);
+        W.as = real(1);
+    } else {
+        real t = fdiv(real(1), fsqrt(max2(real(0), cf * cf - csl * csl)));
+        W.af = fsqrt(max2(real(0), cf * cf - ca * ca)) * t;
+        W.as = fsqrt(max2(real(0), cf * cf - asqr)) * t;
+    }
+    if (bt < (real)ZERO) {
+        W.by = fdiv(real(1), fsqrt(real(2)));
+        W.bz = W.by;
+    } else {
+        real t = fdiv(real(1), bt);
+        W.by = W.qa[5] * t;
+        W.bz = W.qa[6] * t;
+    }
+    W.signbt = (fabs(W.qa[5]) >= (real)ZERO) ? sgn1(W.qa[5]) : sgn1(W.qa[6]);
+    W.amc = cs * cs - ca * ca;
+    W.sbx = sgn1(W.qa[4]);
+}
+
+// characteristic amplitudes alpha_k = L_k . (q_R - q_L) (isotvd_mod.f90:2163-2333)
+template <class real>
+__device__ __forceinline__ void tvd_ck(const real* a, const real* b, real cs, const TvdWave<real>& W, real* ck) {
+    const real* qa = W.qa;
+    const real af = W.af, as = W.as, cf = W.cf, ca = W.ca, csl = W.cslow, by = W.by, bz = W.bz, sbx = W.sbx;
+    real dq[6] = {b[0] - a[0], b[1] - a[1], b[2] - a[2], b[3] - a[3], b[5] - a[5], b[6] - a[6]};
+    real ith1 = fdiv(real(1), real(2) * (af * af * (cs * cs) + as * as * (cf * cf)));
+    real ith2 = fdiv(real(1), real(2) * (af * af * cs * cf + as * as * csl * ca));
+    real sq = fsqrt(qa[0]);
+    real Lm[6], Lp[6];
+    auto dot = [&](const real* L) {
+        return dq[0] * L[0] + dq[1] * L[1] + dq[2] * L[2] + dq[3] * L[3] + dq[4] * L[4] + dq[5] * L[5];
+    };
+    {
+        real t1 = ith1 * af * (cs * cs);
+        real t2 = ith2 * (-(af * cs * qa[1]) + as * csl * sbx * (by * qa[2] + bz * qa[3]));
+        real t3 = ith2 * as * csl * sbx;
+        real t4 = ith1 * as * cf * sq;
+        Lm[0] = t1 - t2; Lp[0] = t1 + t2;
+        Lm[1] = -(ith2 * af * cs); Lp[1] = -Lm[1];
+        Lm[2] = t3 * by; Lp[2] = -Lm[2];
+        Lm[3] = t3 * bz; Lp[3] = -Lm[3];
+        Lm[4] = t4 * by; Lp[4] = Lm[4];
+        Lm[5] = t4 * bz; Lp[5] = Lm[5];
+        if (W.amc < (real)ZERO)
+            for (int m = 0; m < 6; m++) { Lm[m] = W.signbt * Lm[m]; Lp[m] = W.signbt * Lp[m]; }
+        ck[0] = dot(Lm);
+        ck[5] = dot(Lp);
+    }
+    {
+        real t1 = real(0.5) * (bz * qa[2] - by * qa[3]) * sbx, t2 = real(0.5) * sbx, t3 = real(0.5) * sq;
+        Lm[0] = -t1; Lp[0] = t1;
+        Lm[1] = real(0); Lp[1] = real(0);
+        Lm[2] = t2 * bz; Lp[2] = -Lm[2];
+        Lm[3] = -(t2 * by); Lp[3] = -Lm[3];
+        Lm[4] = t3 * bz; Lp[4] = Lm[4];
+        Lm[5] = -(t3 * by); Lp[5] = Lm[5];
+        ck[1] = dot(Lm);
+        ck[4] = dot(Lp);
+    }
+    {
+        real t1 = ith1 * as * (cf * cf);
+        real t2 = ith2 * (as * ca * qa[1] + af * cf * sbx * (by * qa[2] + bz * qa[3]));
+        real t3 = ith2 * af * cf * sbx;
+        real t4 = ith1 * af * cf * sq;
+        Lm[0] = t1 + t2; Lp[0] = t1 - t2;
+        Lm[1] = -(ith2 * as * ca); Lp[1] = -Lm[1];
+        Lm[2] = -(t3 * by); Lp[2] = -Lm[2];
+        Lm[3] = -(t3 * bz); Lp[3] = -Lm[3];
+        Lm[4] = -(t4 * by); Lp[4] = Lm[4];
+        Lm[5] = -(t4 * bz); Lp[5] = Lm[5];
+        if (W.amc > (real)ZERO)
+            for (int m = 0; m < 6; m++) { Lm[m] = W.signbt * Lm[m]; Lp[m] = W.signbt * Lp[m]; }
+        ck[2] = dot(Lm);
+        ck[3] = dot(Lp);
+    }
+}
+
+// column k of the right eigenvector matrix (isotvd_mod.f90:2026-2141)
+template <class real>
+__device__ __forceinline__ void tvd_rcol(int k, real cs, const TvdWave<real>& W, real* r) {
+    const real* qa = W.qa;
+    if (k == 0 || k == 5) {
+        real t1 = W.af * qa[2], t2 = W.as * W.by * qa[4], t3 = W.af * qa[3], t4 = W.as * W.bz * qa[4];
+        real t5 = W.as * W.cf * W.isrho;
+        r[0] = W.af;
+        r[1] = W.af * W.ev[k];
+        r[2] = k == 0 ? t1 + t2 : t1 - t2;
+        r[3] = k == 0 ? t3 + t4 : t3 - t4;
+        r[4] = t5 * W.by;
+        r[5] = t5 * W.bz;
+        if (W.amc < (real)ZERO)
+            for (int m = 0; m < 6; m++) r[m] = W.signbt * r[m];
+    } else if (k == 1 || k == 4) {
+        real s = k == 1 ? W.sbx : -W.sbx;
+        r[0] = real(0);
+        r[1] = real(0);
+        r[2] = s * W.bz;
+        r[3] = k == 1 ? -(W.sbx * W.by) : W.sbx * W.by;
+        r[4] = W.isrho * W.bz;
+        r[5] = -(W.isrho * W.by);
+    } else {
+        real s1 = W.as * qa[2], s2 = W.af * W.by * cs * W.sbx, s3 = W.as * qa[3], s4 = W.af * W.bz * cs * W.sbx;
+        real s5 = fdiv(W.af * (cs * cs) * W.isrho, W.cf);
+        r[0] = W.as;
+        r[1] = W.as * W.ev[k];
+        r[2] = k == 2 ? s1 - s2 : s1 + s2;
+        r[3] = k == 2 ? s3 - s4 : s3 + s4;
+        r[4] = -(s5 * W.by);
+        r[5] = -(s5 * W.bz);
+        if (W.amc > (real)ZERO)
+            for (int m = 0; m < 6; m++) r[m] = W.signbt * r[m];
+    }
+}
+
+// K98 eqn. 2.31 entropy function
+template <class real>
+__device__ __forceinline__ real tvd_Q(real chi, real e) {
+    if (fabs(chi) < real(2) * e) return real(0.25) * fdiv(chi * chi, e) + e;
+    return fabs(chi);
+}
+
+// minmod limiter of the TVD module (isotvd_mod.f90:2426-2429)
+template <class real>
+__device__ __forceinline__ real tvd_lim(real g1, real g2) {
+    real s = sgn1(g2);
+    return s * max2(real(0), min2(fabs(g2), g1 * s));
+}
+
+// cell flux with the cell's own face field in slot 4 (isotvd_mod.f90:2364-2382)
+template <class real>
+__device__ __forceinline__ void tvd_cell_flux(const real* u, real cs, real* f) {
+    real rho = u[0], vx = fdiv(u[1], rho), vy = fdiv(u[2], rho), vz = fdiv(u[3], rho);
+    real Bx = u[4], By = u[5], Bz = u[6];
+    f[0] = rho * vx;
+    f[1] = rho * (vx * vx + cs * cs) + real(0.5) * (By * By + Bz * Bz - Bx * Bx);
+    f[2] = rho * vx * vy - Bx * By;
+    f[3] = rho * vx * vz - Bx * Bz;
+    f[4] = vx * By - vy * Bx;
+    f[5] = vx * Bz - vz * Bx;
+}
+
+template <class real>
+__device__ __forceinline__ void tvd_load(const TvdSweepArgs<real>& a, long long c, real* u) {
+#pragma unroll
+    for (int r = 0; r < 7; r++)
+        if (r != 4) u[r] = a.w[r][c];
+    u[4] = (real)a.bn[c];
+}
+
+template <class real>
+__device__ __forceinline__ void tvd_store(const TvdSweepArgs<real>& a, long long c, const real* F) {
+    a.F[0][c] = F[0];
+    a.F[1][c] = F[1];
+    a.F[2][c] = F[2];
+    a.F[3][c] = F[3];
+    a.Et1[c] = -F[5];
+    a.Et2[c] = F[6];
+}
+
+// per face: g~_k and alpha_k (isotvd_mod.f90:1856-2333, 2400-2424)
+template <class real>
+__global__ __launch_bounds__(256) void k_tvd_eig(const BoxDesc* __restrict__ boxes, Plan plan, TvdSweepArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const long long c = cell_index(B, i, j, k), s = tvd_stride(B, a.dir);
+    real uL[7], uR[7];
+    tvd_load(a, c, uL);
+    tvd_load(a, c + s, uR);
+    const real cs = (real)a.cs;
+    TvdWave<real> W;
+    tvd_wave(uL, uR, cs, W);
+    real ck[6];
+    tvd_ck(uL, uR, cs, W, ck);
+    const double dtd = a.dt[b] / B.dx;
+    const real dtdx = (real)(a.factor == 1.0 ? dtd : a.factor * dtd);
+#pragma unroll
+    for (int w = 0; w < 6; w++) {
+        real chi = dtdx * W.ev[w];
+        real Qk = tvd_Q(chi, (real)tvd_eps(w));
+        a.gt[w][c] = real(0.5) * (Qk - chi * chi) * ck[w];
+        a.ck[w][c] = ck[w];
+    }
+}
+
+// Roe-TVD flux of face c (isotvd_mod.f90:2400-2466)
+template <class real>
+__global__ __launch_bounds__(256) void k_tvd_flux(const BoxDesc* __restrict__ boxes, Plan plan, TvdSweepArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const long long c = cell_index(B, i, j, k), s = tvd_stride(B, a.dir);
+    real uL[7], uR[7];
+    tvd_load(a, c, uL);
+    tvd_load(a, c + s, uR);
+    const real cs = (real)a.cs;
+    TvdWave<real> W;
+    tvd_wave(uL, uR, cs, W);
+    const double dtd = a.dt[b] / B.dx;
+    const real dtdx = (real)(a.factor == 1.0 ? dtd : a.factor * dtd);
+    const real dxdt = (real)(B.dx / a.dt[b]);
+    real FT[6] = {real(0), real(0), real(0), real(0), real(0), real(0)};
+#pragma unroll
+    for (int w = 0; w < 6; w++) {
+        const real g0 = a.gt[w][c - s], g1 = a.gt[w][c], g2 = a.gt[w][c + s];
+        const real ckw = a.ck[w][c];
+        const real gk0 = tvd_lim(g0, g1), gk1 = tvd_lim(g1, g2);
+        real gam = (fabs(ckw) > (real)ZERO) ? fdiv(gk1 - gk0, ckw) : real(0);
+        real chi = dtdx * W.ev[w] + gam;
+        real Qk = tvd_Q(chi, (real)tvd_eps(w));
+        real betak = Qk * ckw - (gk0 + gk1);
+        real r[6];
+        tvd_rcol(w, cs, W, r);
+#pragma unroll
+        for (int m = 0; m < 6; m++) FT[m] = FT[m] - betak * r[m];
+    }
+    real fa[6], fb[6], F[7];
+    tvd_cell_flux(uL, cs, fa);
+    tvd_cell_flux(uR, cs, fb);
+    F[0] = real(0.5) * (dxdt * FT[0] + fa[0] + fb[0]);
+    F[1] = real(0.5) * (dxdt * FT[1] + fa[1] + fb[1]);
+    F[2] = real(0.5) * (dxdt * FT[2] + fa[2] + fb[2]);
+    F[3] = real(0.5) * (dxdt * FT[3] + fa[3] + fb[3]);
+    F[4] = real(0);
+    F[5] = real(0.5) * (dxdt * FT[4] + fa[4] + fb[4]);
+    F[6] = real(0.5) * (dxdt * FT[5] + fa[5] + fb[5]);
+    tvd_store(a, c, F);
+}
+
+// HLLE of the TVD module (isotvd_mod.f90:2473-2578): adjacent cell states, each with its
+// own face field; same arithmetic as the VL module's HLLE without the smax bookkeeping.
+template <class real>
+__global__ __launch_bounds__(256) void k_tvd_hlle(const BoxDesc* __restrict__ boxes, Plan plan, TvdSweepArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const long long c = cell_index(B, i, j, k), s = tvd_stride(B, a.dir);
+    real uL[7], uR[7], F[7];
+    tvd_load(a, c, uL);
+    tvd_load(a, c + s, uR);
+    real sm = real(0);
+    hlle(uL, uR, (real)(a.cs * a.cs), F, sm);
+    tvd_store(a, c, F);
+}
+
+__device__ __forceinline__ void tvd_upw(double v, double L14, double R14, double L34, double R34, double& o14,
+                                        double& o34) {
+    if (v > ZERO) { o14 = L14; o34 = L34; }
+    else if (v < -ZERO) { o14 = R14; o34 = R34; }
+    else { o14 = 0.5 * (L14 + R14); o34 = 0.5 * (L34 + R34); }
+}
+
+// Corner EMFs of the TVD module (isotvd_mod.f90:993-1174): Gardiner-Stone upwinding with
+// gradients 2/dx (E_face - E_centre) and the correction scaled by dx/8.  Centre EMF: the
+// predictor forms (B x m)/rho from the t^n state (isotvd_mod.f90:947-953); the corrector
+// reads the half-step centre EMF.
+template <class real>
+__global__ __launch_bounds__(256) void k_tvd_corner(const BoxDesc* __restrict__ boxes, Plan plan, TvdCornerArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const long long c = cell_index(B, i, j, k);
+    const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+    const double dx = B.dx, idx2 = 2.0 / dx;
+    auto Q = [&](int f, long long p) -> double { return (double)a.q[f][p]; };
+    auto QB = [&](int f, long long p) -> double { return (double)a.qb[f][p]; };
+    auto Ecen = [&](int comp, long long p) -> double {
+        if (a.Ecc[0] != nullptr) return a.Ecc[comp][p];
+        if (comp == 0) return (QB(1, p) * Q(3, p) - QB(2, p) * Q(2, p)) / Q(0, p);
+        if (comp == 1) return (QB(2, p) * Q(1, p) - QB(0, p) * Q(3, p)) / Q(0, p);
+        return (QB(0, p) * Q(2, p) - QB(1, p) * Q(1, p)) / Q(0, p);
+    };
+    auto E = [&](int comp, int slot, long long p) -> double { return (double)a.E[2 * comp + slot][p]; };
+    const double vx = 0.5 * (Q(1, c) / Q(0, c) + Q(1, c + sx) / Q(0, c + sx));
+    const double vy = 0.5 * (Q(2, c) / Q(0, c) + Q(2, c + sy) / Q(0, c + sy));
+    const double vz = 0.5 * (Q(3, c) / Q(0, c) + Q(3, c + sz) / Q(0, c + sz));
+    double a14, a34, b14, b34;
+    {   // Ez at (i+1/2, j+1/2, k)
+        double e00 = Ecen(2, c), e10 = Ecen(2, c + sx), e01 = Ecen(2, c + sy), e11 = Ecen(2, c + sx + sy);
+        double y0 = E(2, 1, c), y1 = E(2, 1, c + sx), x0 = E(2, 0, c), x1 = E(2, 0, c + sy);
+        tvd_upw(vx, idx2 * (y0 - e00), idx2 * (y1 - e10), idx2 * (e01 - y0), idx2 * (e11 - y1), a14, a34);
+        tvd_upw(vy, idx2 * (x0 - e00), idx2 * (x1 - e01), idx2 * (e10 - x0), idx2 * (e11 - x1), b14, b34);
+        a.Ec[2][c] = 0.25 * (x0 + x1 + y0 + y1) + 0.125 * dx * ((a14 - a34) + (b14 - b34));
+    }
+    {   // Ey at (i+1/2, j, k+1/2)
+        double e00 = Ecen(1, c), e10 = Ecen(1, c + sx), e01 = Ecen(1, c + sz), e11 = Ecen(1, c + sx + sz);
+        double z0 = E(1, 1, c), z1 = E(1, 1, c + sx), x0 = E(1, 0, c), x1 = E(1, 0, c + sz);
+        tvd_upw(vx, idx2 * (z0 - e00), idx2 * (z1 - e10), idx2 * (e01 - z0), idx2 * (e11 - z1), a14, a34);
+        tvd_upw(vz, idx2 * (x0 - e00), idx2 * (x1 - e01), idx2 * (e10 - x0), idx2 * (e11 - x1), b14, b34);
+        a.Ec[1][c] = 0.25 * (x0 + x1 + z0 + z1) + 0.125 * dx * ((a14 - a34) + (b14 - b34));
+    }
+    {   // Ex at (i, j+1/2, k+1/2)
+        double e00 = Ecen(0, c), e10 = Ecen(0, c + sy), e01 = Ecen(0, c + sz), e11 = Ecen(0, c + sy + sz);
+        double z0 = E(0, 1, c), z1 = E(0, 1, c + sy), y0 = E(0, 0, c), y1 = E(0, 0, c + sz);
+        tvd_upw(vy, idx2 * (z0 - e00), idx2 * (z1 - e10), idx2 * (e01 - z0), idx2 * (e11 - z1), a14, a34);
+        tvd_upw(vz, idx2 * (y0 - e00), idx2 * (y1 - e01), idx2 * (e10 - y0), idx2 * (e11 - y1), b14, b34);
+        a.Ec[0][c] = 0.25 * (y0 + y1 + z0 + z1) + 0.125 * dx * ((a14 - a34) + (b14 - b34));
+    }
+}
+
+// Corrector centre EMF from the unsplit half update of rho, m with the predictor fluxes
+// and the face average of the half-step face fields (isotvd_mod.f90:876-906)
+template <class real>
+__global__ __launch_bounds__(256) void k_tvd_center(const BoxDesc* __restrict__ boxes, Plan plan, TvdCenterArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const long long c = cell_index(B, i, j, k);
+    const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+    const real dtdx = (real)(0.5 * a.dt[b] / B.dx);
+    real h[4];
+#pragma unroll
+    for (int f = 0; f < 4; f++)
+        h[f] = a.q[f][c] - dtdx * (a.Fb[0][f][c] - a.Fb[0][f][c - sx] + a.Fb[1][f][c] - a.Fb[1][f][c - sy] +
+                                   a.Fb[2][f][c] - a.Fb[2][f][c - sz]);
+    double vx = (double)h[1] / (double)h[0], vy = (double)h[2] / (double)h[0], vz = (double)h[3] / (double)h[0];
+    double Bx = 0.5 * (a.bh[0][c] + a.bh[0][c - sx]);
+    double By = 0.5 * (a.bh[1][c] + a.bh[1][c - sy]);
+    double Bz = 0.5 * (a.bh[2][c] + a.bh[2][c - sz]);
+    a.Ecc[0][c] = By * vz - Bz * vy;
+    a.Ecc[1][c] = Bz * vx - Bx * vz;
+    a.Ecc[2][c] = Bx * vy - By * vx;
+}
+
+// minmod-like multidimensional source of the half updates (isotvd_mod.f90:1208-1212)
+__device__ __forceinline__ double tvd_src(double v, double da, double db) {
+    if ((-da * db) > ZERO) return -v * sgn1(da) * fmin(fabs(da), fabs(db));
+    return 0.0;
+}
+
+// halfupdate_{x,y,z}_3d (isotvd_mod.f90:1179-1650) over [3-nb, n+nb-2]: 1/2 dt update of
+// rho, m with the two transverse predictor fluxes and the B_n-gradient momentum source,
+// of the two transverse cell-centred fields with the predictor corner EMFs and the
+// minmod source; then the density floor.  The normal cell-centred field is not needed by
+// the following sweep (it reads the face field) and is not written.
+template <class real>
+__global__ __launch_bounds__(256) void k_tvd_half(const BoxDesc* __restrict__ boxes, Plan plan, TvdHalfArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const long long c = cell_index(B, i, j, k);
+    const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+    const long long st[3] = {sx, sy, sz};
+    const int d = a.dir;
+    const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+    const double dtd = 0.5 * a.dt[b] / B.dx;
+    const real dtdx = (real)dtd;
+    const double dB[3] = {a.b0[0][c] - a.b0[0][c - sx], a.b0[1][c] - a.b0[1][c - sy], a.b0[2][c] - a.b0[2][c - sz]};
+    const real dn = (real)dB[d];
+    real q[7];
+#pragma unroll
+    for (int f = 0; f < 7; f++) q[f] = a.q[f][c];
+    const real sm[3] = {q[4] * dn, q[5] * dn, q[6] * dn};
+    real hq[4];
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        real sum = a.Fb[t1][f][c] - a.Fb[t1][f][c - st[t1]] + a.Fb[t2][f][c] - a.Fb[t2][f][c - st[t2]];
+        hq[f] = f == 0 ? q[0] - dtdx * sum : q[f] - dtdx * sum + dtdx * sm[f - 1];
+    }
+    if (hq[0] < (real)MIN_DENS) {
+        hq[0] = (real)MIN_DENS;
+        atomicAdd(a.floor_count, 1u);
+    }
+#pragma unroll
+    for (int f = 0; f < 4; f++) a.dst[f][c] = hq[f];
+    const double* Ec = a.Ec[d];
+    const double dBx = dB[0], dBy = dB[1], dBz = dB[2];
+    if (d == 0) {
+        double vy = (double)(q[2] / q[0]), vz = (double)(q[3] / q[0]);
+        double sBy = tvd_src(vy, dBz, dBx), sBz = tvd_src(vz, dBy, dBx);
+        double by = (double)q[5] - 0.5 * dtd * ((Ec[c] - Ec[c - sz]) + (Ec[c - sy] - Ec[c - sy - sz])) + dtd * sBy;
+        double bz = (double)q[6] + 0.5 * dtd * ((Ec[c] - Ec[c - sy]) + (Ec[c - sz] - Ec[c - sy - sz])) + dtd * sBz;
+        a.dst[5][c] = (real)by;
+        a.dst[6][c] = (real)bz;
+    } else if (d == 1) {
+        double vx = (double)(q[1] / q[0]), vz = (double)(q[3] / q[0]);
+        double sBz = tvd_src(vz, dBx, dBy), sBx = tvd_src(vx, dBz, dBy);
+        double bx = (double)q[4] + 0.5 * dtd * ((Ec[c] - Ec[c - sz]) + (Ec[c - sx] - Ec[c - sx - sz])) + dtd * sBx;
+        double bz = (double)q[6] - 0.5 * dtd * ((Ec[c] - Ec[c - sx]) + (Ec[c - sz] - Ec[c - sx - sz])) + dtd * sBz;
+        a.dst[4][c] = (real)bx;
+        a.dst[6][c] = (real)bz;
+    } else {
+        double vx = (double)(q[1] / q[0]), vy = (double)(q[2] / q[0]);
+        double sBx = tvd_src(vx, dBy, dBz), sBy = tvd_src(vy, dBx, dBz);
+        double bx = (double)q[4] - 0.5 * dtd * ((Ec[c] - Ec[c - sy]) + (Ec[c - sx] - Ec[c - sx - sy])) + dtd * sBx;
+        double by = (double)q[5] + 0.5 * dtd * ((Ec[c] - Ec[c - sx]) + (Ec[c - sy] - Ec[c - sx - sy])) + dtd * sBy;
+        a.dst[4][c] = (real)bx;
+        a.dst[5][c] = (real)by;
+    }
+}
+
+// update_CT_3d(.FALSE.) (isotvd_mod.f90:1794-1831) on [3-nb, n+nb-3], t^n faces elsewhere
+template <class real>
+__global__ __launch_bounds__(256) void k_tvd_ct_pred(const BoxDesc* __restrict__ boxes, Plan plan, TvdCtArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const long long c = cell_index(B, i, j, k);
+    const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+    const int nb = B.nb;
+    const bool in = i >= 3 - nb && j >= 3 - nb && k >= 3 - nb && i <= B.n[0] + nb - 3 && j <= B.n[1] + nb - 3 &&
+                    k <= B.n[2] + nb - 3;
+    if (!in) {
+        a.b1[0][c] = a.b0[0][c];
+        a.b1[1][c] = a.b0[1][c];
+        a.b1[2][c] = a.b0[2][c];
+        return;
+    }
+    const double dtd = 0.5 * (a.dt[b] / B.dx);
+    const double* Ex = a.Ec[0];
+    const double* Ey = a.Ec[1];
+    const double* Ez = a.Ec[2];
+    a.b1[0][c] = a.b0[0][c] - dtd * ((Ez[c] - Ez[c - sy]) - (Ey[c] - Ey[c - sz]));
+    a.b1[1][c] = a.b0[1][c] - dtd * ((Ex[c] - Ex[c - sz]) - (Ez[c] - Ez[c - sx]));
+    a.b1[2][c] = a.b0[2][c] - dtd * ((Ey[c] - Ey[c - sx]) - (Ex[c] - Ex[c - sy]));
+}
+
+// update_state_vector_3d + update_CT_3d(.TRUE.) (isotvd_mod.f90:1655-1851): rho, m on the
+// interior with the corrector fluxes and the floor; faces on [5-nb, n+nb-5] from the t^n
+// faces; cell-centred B on the interior from the new faces (lower faces recomputed, so the
+// kernel never reads a face it writes).  Everything else keeps its t^n value, as the
+// reference's restore of q_3d and bface_3d leaves it.
+template <class real>
+__global__ __launch_bounds__(256) void k_tvd_final(const BoxDesc* __restrict__ boxes, Plan plan, TvdFinalArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const long long c = cell_index(B, i, j, k);
+    const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+    const int nb = B.nb;
+    const double dtd = a.dt[b] / B.dx;
+    const real dtdx = (real)dtd;
+    const bool interior = i >= 1 && j >= 1 && k >= 1 && i <= B.n[0] && j <= B.n[1] && k <= B.n[2];
+    const bool face = i >= 5 - nb && j >= 5 - nb && k >= 5 - nb && i <= B.n[0] + nb - 5 && j <= B.n[1] + nb - 5 &&
+                      k <= B.n[2] + nb - 5;
+    const double* Ex = a.Ec[0];
+    const double* Ey = a.Ec[1];
+    const double* Ez = a.Ec[2];
+    auto ctx = [&](long long m) { return a.src.b[0][m] - dtd * ((Ez[m] - Ez[m - sy]) - (Ey[m] - Ey[m - sz])); };
+    auto cty = [&](long long m) { return a.src.b[1][m] - dtd * ((Ex[m] - Ex[m - sz]) - (Ez[m] - Ez[m - sx])); };
+    auto ctz = [&](long long m) { return a.src.b[2][m] - dtd * ((Ey[m] - Ey[m - sx]) - (Ex[m] - Ex[m - sy])); };
+    double bx, by, bz;
+    if (face) { bx = ctx(c); by = cty(c); bz = ctz(c); }
+    else { bx = a.src.b[0][c]; by = a.src.b[1][c]; bz = a.src.b[2][c]; }
+    a.dst.b[0][c] = bx;
+    a.dst.b[1][c] = by;
+    a.dst.b[2][c] = bz;
+    if (!interior) {
+#pragma unroll
+        for (int f = 0; f < 7; f++) a.dst.q[f][c] = a.src.q[f][c];
+        return;
+    }
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        real div = (a.F[0][f][c] - a.F[0][f][c - sx]) + (a.F[1][f][c] - a.F[1][f][c - sy]) +
+                   (a.F[2][f][c] - a.F[2][f][c - sz]);
+        real q0 = a.src.q[f][c];
+        real v;
+        if (a.comp[0] != nullptr) {  // compensated (Kahan) accumulation of the FP32 state
+            real y = -(dtdx * div) - a.comp[f][c];
+            v = q0 + y;
+            a.comp[f][c] = (v - q0) - y;
+        } else {
+            v = q0 - dtdx * div;
+        }
+        if (f == 0 && v < (real)MIN_DENS) {
+            v = (real)MIN_DENS;
+            if (a.comp[0] != nullptr) a.comp[0][c] = real(0);
+            atomicAdd(a.floor_count, 1u);
+        }
+        a.dst.q[f][c] = v;
+    }
+    a.dst.q[4][c] = (real)(0.5 * (bx + ctx(c - sx)));
+    a.dst.q[5][c] = (real)(0.5 * (by + cty(c - sy)));
+    a.dst.q[6][c] = (real)(0.5 * (bz + ctz(c - sz)));
+}
+
+// get_max_speed_tvd (isotvd_mod.f90:2607-2673) over [0, n]: fast speed of the
+// interface-mean state per direction
+template <class real>
+__global__ __launch_bounds__(256) void k_tvd_max_speed(const BoxDesc* __restrict__ boxes, Plan plan, QPtrs<real> Q,
+                                                       double cs, unsigned long long* cmax) {
+    int b = 0, i, j, k;
+    double m = 0.0;
+    bool act = plan_item_box(plan, b, i, j, k);
+    if (act) {
+        const BoxDesc& B = boxes[b];
+        const long long c = cell_index(B, i, j, k);
+        const long long st[3] = {1, B.P[0], (long long)B.P[0] * B.P[1]};
+        const double asqr = cs * cs;
+        auto q = [&](int f, long long p) -> double { return (double)Q.q[f][p]; };
+        double cf[3], v[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const long long n = c + st[d];
+            double rho2 = 0.5 * (q(0, c) + q(0, n));
+            double isrho = 1.0 / sqrt(rho2);
+            double bb[3];
+            for (int e = 0; e < 3; e++) bb[e] = 0.5 * (q(4 + e, c) + q(4 + e, n)) * isrho;
+            const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+            double btsqr = bb[t1] * bb[t1] + bb[t2] * bb[t2];
+            double bsqr = bb[d] * bb[d] + btsqr, x = asqr - bsqr;
+            double tmp = sqrt(x * x + 4.0 * asqr * btsqr);
+            cf[d] = sqrt(0.5 * (asqr + bsqr + tmp));
+            v[d] = 0.5 * ((q(1 + d, c) / q(0, c)) + (q(1 + d, n) / q(0, n)));
+        }
+        m = max3(fabs(v[0]) + cf[0], fabs(v[1]) + cf[1], fabs(v[2]) + cf[2]);
+    }
+    atomic_max_nonneg(cmax, b, m);
+}
+
+template <class real>
+void launch_tvd_eig(const Launch& L, const TvdSweepArgs<real>& a) {
+    if (L.plan.total > 0) hipLaunchKernelGGL((k_tvd_eig<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+template <class real>
+void launch_tvd_flux(const Launch& L, const TvdSweepArgs<real>& a) {
+    if (L.plan.total > 0) hipLaunchKernelGGL((k_tvd_flux<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+template <class real>
+void launch_tvd_hlle(const Launch& L, const TvdSweepArgs<real>& a) {
+    if (L.plan.total > 0) hipLaunchKernelGGL((k_tvd_hlle<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+template <class real>
+void launch_tvd_corner(const Launch& L, const TvdCornerArgs<real>& a) {
+    if (L.plan.total > 0) hipLaunchKernelGGL((k_tvd_corner<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+template <class real>
+void launch_tvd_center(const Launch& L, const TvdCenterArgs<real>& a) {
+    if (L.plan.total > 0) hipLaunchKernelGGL((k_tvd_center<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+template <class real>
+void launch_tvd_half(const Launch& L, const TvdHalfArgs<real>& a) {
+    if (L.plan.total > 0) hipLaunchKernelGGL((k_tvd_half<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+template <class real>
+void launch_tvd_ct_pred(const Launch& L, const TvdCtArgs<real>& a) {
+    if (L.plan.total > 0) hipLaunchKernelGGL((k_tvd_ct_pred<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+template <class real>
+void launch_tvd_final(const Launch& L, const TvdFinalArgs<real>& a) {
+    if (L.plan.total > 0) hipLaunchKernelGGL((k_tvd_final<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+template <class real>
+void launch_tvd_max_speed(const Launch& L, const QPtrs<real>& q, double cs, unsigned long long* cmax) {
+    if (L.plan.total > 0)
+        hipLaunchKernelGGL((k_tvd_max_speed<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, q, cs, cmax);
+}
+
+#define IMHD_INST_TVD(T)                                                            \
+    template void launch_tvd_eig<T>(const Launch&, const TvdSweepArgs<T>&);         \
+    template void launch_tvd_flux<T>(const Launch&, const TvdSweepArgs<T>&);        \
+    template void launch_tvd_hlle<T>(const Launch&, const TvdSweepArgs<T>&);        \
+    template void launch_tvd_corner<T>(const Launch&, const TvdCornerArgs<T>&);     \
+    template void launch_tvd_center<T>(const Launch&, const TvdCenterArgs<T>&);     \
+    template void launch_tvd_half<T>(const Launch&, const TvdHalfArgs<T>&);         \
+    template void launch_tvd_ct_pred<T>(const Launch&, const TvdCtArgs<T>&);        \
+    template void launch_tvd_final<T>(const Launch&, const TvdFinalArgs<T>&);       \
+    template void launch_tvd_max_speed<T>(const Launch&, const QPtrs<T>&, double, unsigned long long*);
+}  // namespace imhd

@@ -1,8 +1,10 @@
 """Host-side mirror of the reference's solver interface over the imhd C ABI.
 
 Method names follow the reference's Fortran procedures (3D_MPI/src):
-vanleer_solve_3d, boundary_conditions, get_max_speed_vanleer, copy_velocity,
-passive_solve_3d, compute_KE/compute_divB, and the driver's time loop (driver.f90:35-186).
+vanleer_solve_3d / get_max_speed_vanleer (VanLeerSolver, vanleer_mod.f90),
+tvd_solve_3d / get_max_speed_tvd (TvdSolver, isotvd_mod.f90), boundary_conditions,
+copy_velocity, passive_solve_3d, compute_KE/compute_divB, and the driver's time loop
+(driver.f90:35-186).
 Arrays are the reference's host layout (FP64, Fortran order, nb ghost layers).
 """
 import ctypes
@@ -34,6 +36,8 @@ class VanLeerSolver:
     serves every box; each box keeps its own dt).
     """
 
+    integrator = L.IMHD_VL
+
     def __init__(self, dims, nb=6, npass=0, riemann="hlld", recon="plm", bc="periodic",
                  precision=64, compensated=None, cs=1.0, mc_beta=1.01, cour=0.35):
         self.lib = L.lib()
@@ -46,7 +50,7 @@ class VanLeerSolver:
         for i, (nx, ny, nz) in enumerate(boxes):
             c = cfgs[i]
             c.nx, c.ny, c.nz, c.nb, c.npass = nx, ny, nz, nb, npass
-            c.integrator, c.riemann, c.recon = L.IMHD_VL, _RIEMANN[riemann], _RECON[recon]
+            c.integrator, c.riemann, c.recon = self.integrator, _RIEMANN[riemann], _RECON[recon]
             c.bc, c.precision, c.compensated = _BC[bc], precision, int(bool(compensated))
             c.cs, c.mc_beta = cs, mc_beta
             c.cour = cour[i] if np.ndim(cour) else cour
@@ -105,6 +109,10 @@ class VanLeerSolver:
     def vanleer_solve_3d(self, dt, drive_next=False):
         """vanleer_mod.f90:279-426 on every box with the given dt."""
         self._check(self.lib.imhd_step(self._c, float(dt), int(drive_next)))
+
+    def mhd_update(self, dt):
+        """driver.f90:378-389: the configured integrator's solve on every box."""
+        self.vanleer_solve_3d(dt)
 
     def boundary_conditions(self):
         """boundary_conditions_mod.f90:20-234 (one rank)."""
@@ -172,13 +180,48 @@ class VanLeerSolver:
         return out
 
 
+class TvdSolver(VanLeerSolver):
+    """K98 Roe-TVD (riemann='roe') or HLLE (riemann='hlle') + CTU + CT isothermal MHD
+    (Iso_TVD_Solver_mod, isotvd_mod.f90) on MI355X; same state layout, boundary, passive
+    and native-loop interface as VanLeerSolver."""
+
+    integrator = L.IMHD_TVD
+
+    def __init__(self, dims, nb=6, npass=0, riemann="roe", bc="periodic", precision=64,
+                 compensated=None, cs=1.0, cour=0.35):
+        if riemann not in ("roe", "hlle"):
+            raise ValueError("the TVD integrator takes riemann='roe' (K98 Roe-TVD) or 'hlle'")
+        super().__init__(dims, nb=nb, npass=npass, riemann=riemann, recon="plm", bc=bc,
+                         precision=precision, compensated=compensated, cs=cs, cour=cour)
+
+    def tvd_solve_3d(self, dt):
+        """isotvd_mod.f90:226-364 on every box with the given dt."""
+        self._check(self.lib.imhd_step(self._c, float(dt), 0))
+
+    def vanleer_solve_3d(self, dt, drive_next=False):
+        raise AttributeError("TvdSolver integrates with tvd_solve_3d")
+
+    def mhd_update(self, dt):
+        self.tvd_solve_3d(dt)
+
+    def get_max_speed_tvd(self):
+        """isotvd_mod.f90:2607-2673 (interface-mean fast speeds over [0, n]); one value per
+        box.  Like the reference driver, call it before boundary_conditions."""
+        out = np.zeros(len(self.boxes))
+        self._check(self.lib.imhd_max_speed(self._c, _p(out)))
+        return out if len(self.boxes) > 1 else float(out[0])
+
+    def get_max_speed_vanleer(self):
+        raise AttributeError("TvdSolver's CFL speed is get_max_speed_tvd")
+
+
 def driver_step(solver, dt):
     """One pass of the reference driver loop body with a host dt (driver.f90:55-160):
-    copy_velocity(.FALSE.), vanleer_solve_3d, copy_velocity(.TRUE.), passive_solve_3d,
+    copy_velocity(.FALSE.), mhd_update, copy_velocity(.TRUE.), passive_solve_3d,
     boundary_conditions."""
     if solver.npass > 0:
         solver.copy_velocity(False)
-    solver.vanleer_solve_3d(dt)
+    solver.mhd_update(dt)
     if solver.npass > 0:
         solver.copy_velocity(True)
         solver.passive_solve_3d(dt)

@@ -55,7 +55,9 @@ void orc_riemann_plane(int kind, int ni, int nj, double cs, const double *qL,
                        const double *qR, double *F, double *smax);
 void orc_recon_plane(int kind, int ni, int nj, int nb, double cs, double mc_beta,
                      const double *w, const double *wb, double *qL, double *qR);
-void orc_tvd_flux_plane(int kind, int ni, int nj, int nb, double cs, double dtdx,
+/* TVD-module interface fluxes along i of cell states q (slot 4 = the cell's own face
+ * field): kind 2 K98 Roe-TVD with dtdx = dt/dx (x 0.5 when predictor), kind 1 HLLE. */
+void orc_tvd_flux_plane(int kind, int ni, int nj, double cs, double dt, double dx, int predictor,
                         const double *q, double *F);
 
 #ifdef __cplusplus

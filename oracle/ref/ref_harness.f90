@@ -142,6 +142,40 @@ CONTAINS
     sm = smax
   END SUBROUTINE ref_vl_riemann
 
+  ! Interface fluxes of the TVD module on an ni x nj plane of cell states q(7) (slot 5 =
+  ! the cell's own face field).  kind 2: compute_roe_avg_state_1d .. compute_TVD_fluxes_1d
+  ! (isotvd_mod.f90:1856-2468); kind 1: compute_HLLE_fluxes_1d (isotvd_mod.f90:2473-2578).
+  ! The module's dt is set from dt_in; dtdx as in compute_xfluxes_3d (isotvd_mod.f90:376-378).
+  SUBROUTINE ref_tvd_flux(kind, ni, nj, dt_in, predictor, qin, F) BIND(C, name='ref_tvd_flux')
+    USE Iso_TVD_Solver_mod, ONLY: tq => q, tFs_2d => Fs_2d, tdt => dt, tdx => dx, &
+         t_roe_avg => compute_roe_avg_state_1d, compute_eigenvalues_1d, compute_eigenvectors_1d, &
+         compute_TVD_fluxes_1d, t_hlle => compute_HLLE_fluxes_1d
+    INTEGER(C_INT), VALUE :: kind, ni, nj, predictor
+    REAL(C_DOUBLE), VALUE :: dt_in
+    REAL(C_DOUBLE), INTENT(IN) :: qin(ni, nj, 7)
+    REAL(C_DOUBLE), INTENT(OUT) :: F(ni, nj, 7)
+    INTEGER :: ilow, ihi, jlow, jhi
+    REAL*8 :: dtdx
+    ilow = 1
+    ihi = ni - 2*nb
+    jlow = 1
+    jhi = nj - 2*nb
+    tdt = dt_in
+    dtdx = tdt / tdx
+    IF (predictor .NE. 0) dtdx = 0.5d0 * dtdx
+    tq = 0.d0
+    tFs_2d = 0.d0
+    tq(1-nb:ni-nb, 1-nb:nj-nb, :) = qin
+    IF (kind .EQ. 1) CALL t_hlle(tFs_2d, ilow, ihi, jlow, jhi)
+    IF (kind .EQ. 2) THEN
+      CALL t_roe_avg(ilow, ihi, jlow, jhi)
+      CALL compute_eigenvalues_1d(ilow, ihi, jlow, jhi)
+      CALL compute_eigenvectors_1d(ilow, ihi, jlow, jhi)
+      CALL compute_TVD_fluxes_1d(tFs_2d, ilow, ihi, jlow, jhi, dtdx)
+    END IF
+    F = tFs_2d(1-nb:ni-nb, 1-nb:nj-nb, :)
+  END SUBROUTINE ref_tvd_flux
+
   ! Interface-state reconstruction of the VL module on an ni x nj plane of primitive
   ! cell values w(6) and face normal field wb.  kind 1: compute_interface_states_prim as
   ! compiled (vanleer_mod.f90:1144-1485); kind 2: compute_primitive_eigenvectors +

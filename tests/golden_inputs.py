@@ -8,7 +8,33 @@ STEP_CASES = {
     "hlld_o12x10x8": ("vl_hlld_o12x10x8", (12, 10, 8), 1, 32),
     "roe_p16": ("vl_roe_p16", (16, 16, 16), 1, 33),
     "hlle_p16": ("vl_hlle_p16", (16, 16, 16), 1, 34),
+    "tvd_roe_p16": ("tvd_roe_p16", (16, 16, 16), 1, 35),
+    "tvd_hlle_p12x10x8": ("tvd_hlle_p12x10x8", (12, 10, 8), 1, 36),
+    "tvd_roe_o12x10x8": ("tvd_roe_o12x10x8", (12, 10, 8), 1, 37),
 }
+# TVD-module plane fluxes: (kind, predictor, dt); kind 2 K98 Roe-TVD, 1 HLLE; built from
+# the 16^3 TVD build (dx = 1/16)
+TVD_FLUX_CASES = ((2, 1, 2e-3), (2, 0, 2e-3), (2, 0, 1e-15), (1, 0, 2e-3))
+TVD_FLUX_DX = 1.0 / 16
+
+
+def step_case_params(name):
+    """(mhd, riemann, bc) in the reference's own numbering for a STEP_CASES entry."""
+    build = STEP_CASES[name][0]
+    bc = 1 if "_o" in build else 2
+    if build.startswith("tvd_"):
+        return 1, {"roe": 2, "hlle": 1}[build.split("_")[1]], bc
+    return 2, {"hlld": 2, "hlle": 3, "roe": 1}[build.split("_")[1]], bc
+
+
+def tvd_flux_inputs(ni=28, nj=3, seed=300):
+    """cell states with their own (independent) face fields in slot 4"""
+    A, _ = riemann_inputs(2, ni, nj)
+    rng = np.random.default_rng(seed)
+    A = np.array(A, order="F")
+    A[:, :, 4] = rng.normal(0, 1.0, (ni, nj))
+    A[3, :, 4] = 0.0
+    return A
 
 
 def riemann_inputs(kind, ni=27, nj=3):

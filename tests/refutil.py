@@ -66,9 +66,9 @@ def oracle_lib():
         L.orc_recon_plane.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
-        L.orc_tvd_flux_plane.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                         ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
-                                         ctypes.c_void_p]
+        L.orc_tvd_flux_plane.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p]
         _oracle = L
     return _oracle
 

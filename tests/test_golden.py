@@ -8,7 +8,8 @@ import os
 import numpy as np
 import pytest
 
-from golden_inputs import STEP_CASES, STEP_DTS, recon_inputs, riemann_inputs
+from golden_inputs import (STEP_CASES, STEP_DTS, TVD_FLUX_CASES, TVD_FLUX_DX, recon_inputs,
+                           riemann_inputs, step_case_params, tvd_flux_inputs)
 from refutil import Oracle, P, oracle_lib, random_state
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_vectors.npz")
@@ -41,12 +42,21 @@ def test_recon_golden(gold, kind):
     assert np.array_equal(qR[2:ni - 3], gold[f"recon_{kind}_R"])
 
 
+@pytest.mark.parametrize("n", range(len(TVD_FLUX_CASES)))
+def test_tvd_flux_golden(gold, n):
+    kind, pred, dt = TVD_FLUX_CASES[n]
+    A = tvd_flux_inputs()
+    ni, nj = A.shape[:2]
+    F = np.zeros((ni, nj, 7), order="F")
+    oracle_lib().orc_tvd_flux_plane(kind, ni, nj, 1.0, dt, TVD_FLUX_DX, pred, P(A), P(F))
+    assert np.array_equal(F, gold[f"tvdflux_{kind}_{pred}_{n}"])
+
+
 @pytest.mark.parametrize("name", list(STEP_CASES))
 def test_step_golden(gold, name):
     build, dims, npass, seed = STEP_CASES[name]
-    rs = {"hlld": 2, "hlle": 3, "roe": 1}[name.split("_")[0]]
-    bc = 1 if "_o" in name else 2
-    O = Oracle(*dims, npass=npass, mhd=2, riemann=rs, recon=2, bc=bc)
+    mhd, rs, bc = step_case_params(name)
+    O = Oracle(*dims, npass=npass, mhd=mhd, riemann=rs, recon=2, bc=bc)
     q, bf, qp = random_state(*dims, npass=npass, seed=seed)
     O.set_state(q, bf, qp)
     O.bc()

@@ -249,19 +249,26 @@ def test_gpu_matches_reference_golden_vectors():
     """The HIP FP64 path in the reference's default configuration (characteristic PPM)
     against the reference Fortran's own outputs (tests/golden/reference_vectors.npz)."""
     import os
-    from golden_inputs import STEP_CASES, STEP_DTS
+    from golden_inputs import STEP_CASES, STEP_DTS, step_case_params
+    from isothermal_mhd_amd import TvdSolver
     gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz"))
     for name, (build, dims, npass, seed) in STEP_CASES.items():
         say(f"[gpu] golden {name}")
-        riemann = name.split("_")[0]
-        bc = "open" if "_o" in name else "periodic"
+        mhd, rs, bcn = step_case_params(name)
+        bc = "open" if bcn == 1 else "periodic"
         q, bf, qp = random_state(*dims, npass=npass, seed=seed)
-        S = make_solver(dims, npass, riemann, bc, recon="ppm_char")
+        if mhd == 1:
+            S = TvdSolver(dims, npass=npass, riemann={2: "roe", 1: "hlle"}[rs], bc=bc)
+        else:
+            S = make_solver(dims, npass, {1: "roe", 2: "hlld", 3: "hlle"}[rs], bc, recon="ppm_char")
         qg, bg, pg = run_gpu(S, q, bf, qp, STEP_DTS)
         I = (slice(6, -6),) * 3
         Ip = (slice(4, -4),) * 3
         assert np.array_equal(qg[I], gold[f"step_{name}_q"]), name
         assert np.array_equal(bg[I], gold[f"step_{name}_b"]), name
         assert np.array_equal(pg[Ip][..., :max(npass, 1)], gold[f"step_{name}_p"]), name
-        assert S.get_max_speed_vanleer() == float(gold[f"step_{name}_cmax"])
+        if mhd == 1:
+            assert S.get_max_speed_tvd() == float(gold[f"step_{name}_cmax"])
+        else:
+            assert S.get_max_speed_vanleer() == float(gold[f"step_{name}_cmax"])
         S.close()
