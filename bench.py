@@ -11,11 +11,22 @@ state; `value` is the FP32 path, the FP64 path and the L1 difference are reporte
 
 Timing: W untimed warm-up steps on the initial state, then the state is reset and exactly
 K steps are timed (device synchronised on both sides).  Per-kernel durations come from HIP
-events recorded on the solver's stream around every launch of the timed region.
+events recorded on the solver's stream around every launch of the timed region.  The same
+K steps replayed from one captured hipGraph (no events) are timed beside it ("graph").
 
 Prints ONE JSON line.
+
+HBM traffic (roofline.traffic) comes from two rocprofv3 counter runs of `--pmc-run`
+(FETCH_SIZE, then WRITE_SIZE), scaled by what the same counters read for a plain copy of
+known size (k_convert_in<double> of an FP64 set_state), and reduced by `--pmc-summarize`
+into profiles/pmc_traffic.json:
+  rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -- python3 bench.py --pmc-run
+  rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -- python3 bench.py --pmc-run
+  python3 bench.py --pmc-summarize gpurun_out/pmc_fetch gpurun_out/pmc_write
 """
 import argparse
+import csv
+import glob
 import json
 import os
 import sys
@@ -37,7 +48,10 @@ def bytes_per_item(name, esz):
     if name.startswith("corner"):         # 7 state + 6 face EMFs in (real); 3 corner EMFs out (f64)
         return 13 * esz + 3 * d
     if name.startswith("update"):         # 4 state + 12 fluxes in (real), 3 face B + 3 corner EMF in (f64); 7 state (real) + 3 face B (f64) out
-        return 23 * esz + 9 * d
+        b = 23 * esz + 9 * d
+        if name.startswith("update_corr") and esz == 4:
+            b += 8 * esz                  # Kahan terms of rho, m: read and written (FP32 path)
+        return b
     return None
 
 
@@ -65,9 +79,23 @@ def run_path(ics_state, n, prec, steps, warmup, cour):
     diag = S.diagnostics()
     t, dt = S.get_time()
     floors = S.floor_count()
+    # the same K steps replayed from a captured hipGraph, without per-kernel events
+    S.set_profiling(False)
+    S.set_state(q, bf, qp)
+    S.boundary_conditions()
+    S.set_time(t=0.0, dt=1.0e-15)
+    S.run(2, graph=True)          # capture both buffer parities outside the timed region
+    S.set_state(q, bf, qp)
+    S.boundary_conditions()
+    S.set_time(t=0.0, dt=1.0e-15)
+    S.sync()
+    g0 = time.perf_counter()
+    S.run(steps, graph=True)
+    S.sync()
+    g1 = time.perf_counter()
     S.close()
-    return {"sec": t1 - t0, "prof": prof, "state": state, "diag": diag, "t": float(t[0]),
-            "dt": float(dt[0]), "floors": floors}
+    return {"sec": t1 - t0, "graph_sec": g1 - g0, "prof": prof, "state": state, "diag": diag,
+            "t": float(t[0]), "dt": float(dt[0]), "floors": floors}
 
 
 def roofline(prof, esz):
@@ -98,6 +126,104 @@ def roofline(prof, esz):
         if key in tr:
             out["traffic"] = tr[key]
     return out
+
+
+# family (bench profiler name) -> kernel symbol it launches
+FAMILY_SYMBOL = {"flux_pred": "k_flux_pred", "flux_corr": "k_flux_corr", "corner_pred": "k_corner",
+                 "corner_corr": "k_corner", "update_pred": "k_update", "update_corr_cfl": "k_update",
+                 "pass_stage_pred": "k_pass_stage", "pass_stage_corr": "k_pass_stage",
+                 "fill_ghosts": "k_fill_ghost_band", "fill_ghosts_pass": "k_fill_ghost_band"}
+
+
+def pmc_run(n=256, steps=3):
+    """Short FP32 run of the workload for counter collection, plus an FP64 set_state whose
+    k_convert_in<double> dispatches are plain copies of known size (calibration).  Writes
+    the family -> (kernel, grid size) map for --pmc-summarize."""
+    from isothermal_mhd_amd import VanLeerSolver, ics
+    q, bf, qp = ics.config_c2(n=n)
+    S = VanLeerSolver((n, n, n), npass=1, riemann="hlld", bc="periodic", precision=32)
+    S.set_state(q, bf, qp)
+    S.boundary_conditions()
+    S.set_profiling(True)
+    S.run(steps, graph=False)
+    prof = S.profile()
+    S.close()
+    S64 = VanLeerSolver((n, n, n), npass=1, riemann="hlld", bc="periodic", precision=64)
+    S64.set_state(q, bf, qp)
+    S64.sync()
+    S64.close()
+    fam = {}
+    for name, (ms, nl, items) in prof.items():
+        key = name[:-2] if name[-2:] in ("_x", "_y", "_z") else name
+        if key in FAMILY_SYMBOL:
+            grids = fam.setdefault(key, {"symbol": FAMILY_SYMBOL[key], "grids": []})
+            grid = (items // nl + 255) // 256 * 256
+            if grid not in grids["grids"]:
+                grids["grids"].append(grid)
+    nc = (n + 12) ** 3
+    out = {"n": n, "families": fam, "copy": {"symbol": "k_convert_in<double>", "grid": (nc + 255) // 256 * 256,
+                                              "bytes_read": nc * 8, "bytes_written": nc * 8}}
+    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+    with open(os.path.join(ROOT, "gpurun_out", "pmc_map.json"), "w") as f:
+        json.dump(out, f)
+    print(json.dumps({"pmc_run": "done", "families": len(fam)}))
+    return 0
+
+
+def _read_counter(dirname, counter):
+    """(kernel name, grid size) -> list of per-dispatch counter values (KiB)"""
+    rows = {}
+    files = glob.glob(os.path.join(dirname, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {dirname}")
+    for fn in files:
+        with open(fn) as f:
+            for r in csv.DictReader(f):
+                if r.get("Counter_Name") != counter:
+                    continue
+                key = (r["Kernel_Name"], int(r["Grid_Size"]))
+                rows.setdefault(key, []).append(float(r["Counter_Value"]))
+    return rows
+
+
+def _pick(rows, symbol, grid):
+    vals = []
+    for (name, g), v in rows.items():
+        if symbol in name and g == grid:
+            vals.extend(v)
+    return sum(vals) / len(vals) if vals else None
+
+
+def pmc_summarize(fetch_dir, write_dir):
+    with open(os.path.join(ROOT, "gpurun_out", "pmc_map.json")) as f:
+        mp = json.load(f)
+    fr, wr = _read_counter(fetch_dir, "FETCH_SIZE"), _read_counter(write_dir, "WRITE_SIZE")
+    cp = mp["copy"]
+    sym = "k_convert_in<double>"
+    cf, cw = _pick(fr, sym, cp["grid"]), _pick(wr, sym, cp["grid"])
+    f_fac = cf * 1024.0 / cp["bytes_read"]     # counter bytes per true byte (copy kernel)
+    w_fac = cw * 1024.0 / cp["bytes_written"]
+    out = {"_calibration": {"kernel": sym, "bytes_read": cp["bytes_read"], "bytes_written": cp["bytes_written"],
+                            "FETCH_SIZE_KiB": cf, "WRITE_SIZE_KiB": cw, "fetch_factor": f_fac, "write_factor": w_fac}}
+    lines = ["| family | kernel | grid | FETCH KiB/dispatch | WRITE KiB/dispatch | HBM bytes/launch (calibrated) |",
+             "|---|---|---|---|---|---|"]
+    for key, v in mp["families"].items():
+        tot, nf = 0.0, 0
+        for g in v["grids"]:
+            a, b = _pick(fr, v["symbol"] + "<float", g), _pick(wr, v["symbol"] + "<float", g)
+            if a is None or b is None:
+                continue
+            t = a * 1024.0 / f_fac + b * 1024.0 / w_fac
+            lines.append(f"| {key} | {v['symbol']}<float> | {g} | {a:.0f} | {b:.0f} | {t:.4g} |")
+            tot += t
+            nf += 1
+        if nf:
+            out[key] = tot / nf
+    with open(os.path.join(ROOT, "profiles", "pmc_traffic.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("\n".join(lines))
+    print(json.dumps(out["_calibration"]))
+    return 0
 
 
 def l1_errors(sa, sb, nb=6):
@@ -155,13 +281,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--pmc-run", action="store_true", help="short run for rocprofv3 counter collection")
+    ap.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     args = ap.parse_args()
+    if args.pmc_summarize:
+        return pmc_summarize(*args.pmc_summarize)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the box would shard by 3-D blocks with an RCCL halo exchange, not implemented)",
               file=sys.stderr)
         return 2
+    if args.pmc_run:
+        return pmc_run(args.n)
     from isothermal_mhd_amd import ics
     n, K, W = args.n, args.steps, args.warmup
     state0 = ics.config_c2(n=n)
@@ -190,6 +322,8 @@ def main():
                          "Kahan-compensated rho and rho v",
             "step": "copy_velocity + VL2 predictor/corrector + CT + passive + ghost fill + CFL dt",
         },
+        "graph": {"value": n ** 3 * K / r32["graph_sec"] / 1e6, "ms_per_step": r32["graph_sec"] / K * 1e3,
+                  "note": "same K FP32 steps replayed from a captured hipGraph, no per-kernel events"},
         "fp32": {"ms_per_step": r32["sec"] / K * 1e3, "t_end": r32["t"],
                  "max_divb": r32["diag"]["divb_max"], "floor_resets": r32["floors"]},
         "roofline": roofline(r32["prof"], 4),
@@ -198,6 +332,7 @@ def main():
     if r64 is not None:
         cups64 = n ** 3 * K / r64["sec"] / 1e6
         res["fp64"] = {"value": cups64, "ms_per_step": r64["sec"] / K * 1e3, "t_end": r64["t"],
+                       "graph_ms_per_step": r64["graph_sec"] / K * 1e3,
                        "max_divb": r64["diag"]["divb_max"], "floor_resets": r64["floors"],
                        "roofline": roofline(r64["prof"], 8)}
         res["fp32_over_fp64"] = cups32 / cups64
