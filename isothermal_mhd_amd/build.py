@@ -1,4 +1,6 @@
-"""Build the HIP library libimhd.so in-tree for gfx950 (hipcc, no CMake)."""
+"""Build the HIP library libimhd.so in-tree for gfx950 (hipcc, no CMake), and, where a
+Fortran compiler is present, the iso_c_binding interface and the example Fortran drivers
+that reach it through the solver shim modules (isothermal_mhd_amd/fortran/)."""
 import os
 import subprocess
 import sys
@@ -50,5 +52,50 @@ def build(verbose=False, jobs=4):
     return OUT
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FBUILD = os.path.join(FORTRAN, "_build")
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+# example configurations: (name, constants module)
+F_EXAMPLES = [("vl", "example_constants_vl.f90"), ("tvd", "example_constants_tvd.f90")]
+
+
+def fortran_compiler():
+    return FLANG if os.path.exists(FLANG) else None
+
+
+def build_fortran(verbose=False):
+    """imhd_api.f90 once, then per example configuration its constants/grid modules, the
+    shim modules and the example driver -> fortran/_build/imhd_example_<cfg> (rpath to
+    libimhd.so).  Returns the executables built, or [] without a Fortran compiler."""
+    fc = fortran_compiler()
+    if fc is None:
+        return []
+    exes = []
+    for name, consts in F_EXAMPLES:
+        bdir = os.path.join(FBUILD, name)
+        os.makedirs(bdir, exist_ok=True)
+        srcs = [os.path.join(FORTRAN, "example", consts), os.path.join(FORTRAN, "example", "example_grid_data.f90"),
+                os.path.join(FORTRAN, "imhd_api.f90"), os.path.join(FORTRAN, "imhd_shim.f90"),
+                os.path.join(FORTRAN, "example", "imhd_example_driver.f90")]
+        objs = []
+        for src in srcs:
+            obj = os.path.join(bdir, os.path.basename(src) + ".o")
+            cmd = [fc, "-O2", "-c", "-J", bdir, "-I", bdir, src, "-o", obj]
+            if verbose:
+                print(" ".join(cmd))
+            r = subprocess.run(cmd, capture_output=True, text=True, cwd=bdir)
+            if r.returncode != 0:
+                raise SystemExit(f"Fortran build failed ({src}):\n" + r.stdout + r.stderr)
+            objs.append(obj)
+        exe = os.path.join(FBUILD, f"imhd_example_{name}")
+        cmd = [fc, "-o", exe] + objs + ["-L" + HERE, "-limhd", "-Wl,-rpath,$ORIGIN/../.."]
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=bdir)
+        if r.returncode != 0:
+            raise SystemExit("Fortran link failed:\n" + r.stdout + r.stderr)
+        exes.append(exe)
+    return exes
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran(verbose="-v" in sys.argv))

@@ -67,6 +67,21 @@ __global__ __launch_bounds__(256) void k_fill_ghost_band(const BoxDesc* __restri
     for (int f = 0; f < a.ndbl; f++) a.d[f][d] = a.d[f][s];
 }
 
+// Cells outside [lo, n + hi_off] (any axis) take their t^n values: the state the
+// reference's restore (vanleer_mod.f90:404-405) leaves outside the final update range.
+template <class real>
+__global__ __launch_bounds__(256) void k_restore_shell(const BoxDesc* __restrict__ boxes, Plan plan, ShellArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const bool in = i >= a.lo && j >= a.lo && k >= a.lo && i <= B.n[0] + a.hi_off && j <= B.n[1] + a.hi_off &&
+                    k <= B.n[2] + a.hi_off;
+    if (in) return;
+    const long long c = cell_index(B, i, j, k);
+    for (int f = 0; f < 7; f++) a.dst[f][c] = a.src[f][c];
+    for (int f = 0; f < 3; f++) a.dstb[f][c] = a.srcb[f][c];
+}
+
 // ---------------------------------------------------------------------------------
 // CFL: max over interior cells of |v_d| + c_f,d (vanleer_mod.f90:3266-3334), FP64
 // accumulation; wavefront shuffle reduction and one integer atomicMax per wave.
@@ -323,6 +338,10 @@ void launch_fill_ghost_band(const Launch& L, const GhostArgs<real>& a) {
     if (L.plan.total > 0) L256((k_fill_ghost_band<real>), L.boxes, L.plan, a);
 }
 template <class real>
+void launch_restore_shell(const Launch& L, const ShellArgs<real>& a) {
+    if (L.plan.total > 0) L256((k_restore_shell<real>), L.boxes, L.plan, a);
+}
+template <class real>
 void launch_pass_stage(const Launch& L, const PassStageArgs<real>& a) {
     if (L.plan.total > 0) L256((k_pass_stage<real>), L.boxes, L.plan, a);
 }
@@ -373,6 +392,7 @@ void launch_pass_layout(hipStream_t st, const BoxDesc& B, const double* hsrc, do
 #define INST_AUX(T)                                                                                           \
     template void launch_fill_ghosts<T>(const Launch&, const GhostArgs<T>&);                                  \
     template void launch_fill_ghost_band<T>(const Launch&, const GhostArgs<T>&);                              \
+    template void launch_restore_shell<T>(const Launch&, const ShellArgs<T>&);                                \
     template void launch_pass_stage<T>(const Launch&, const PassStageArgs<T>&);                               \
     template void launch_max_speed<T>(const Launch&, const QPtrs<T>&, double, unsigned long long*);                 \
     template void launch_diag<T>(const Launch&, const DiagArgs<T>&);                                          \

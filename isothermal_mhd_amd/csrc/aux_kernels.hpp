@@ -70,6 +70,16 @@ struct PassStageArgs {
     int npass, stage;
 };
 
+template <class real>
+struct ShellArgs {
+    const real* src[7];
+    real* dst[7];
+    const double* srcb[3];
+    double* dstb[3];
+    int lo, hi_off;   // cells with every index in [lo, n + hi_off] are left alone
+};
+
+template <class real> void launch_restore_shell(const Launch& L, const ShellArgs<real>& a);
 template <class real> void launch_fill_ghosts(const Launch& L, const GhostArgs<real>& a);
 template <class real> void launch_fill_ghost_band(const Launch& L, const GhostArgs<real>& a);
 template <class real> void launch_pass_stage(const Launch& L, const PassStageArgs<real>& a);

@@ -440,6 +440,21 @@ void tvd_max_speed_launch(imhd_ctx* c) {
 
 bool is_tvd(const imhd_ctx* c) { return c->cfg.integrator == IMHD_TVD; }
 
+// Drop-in step only: cells outside the VL final-update range take their t^n values, as
+// the reference's restore leaves them (the native loop's ghost fill makes this moot).
+template <class real>
+void restore_shell(imhd_ctx* c) {
+    ProfScope ps(c, "restore_shell", c->pl_ghost.active);
+    const int e = c->cfg.recon == IMHD_PLM_PRIM ? 0 : 1;
+    const int s0 = 1 - c->cur, s1 = c->cur;   // after the step: cur is the new state
+    ShellArgs<real> a{};
+    for (int f = 0; f < 7; f++) { a.src[f] = R<real>(c->q[s0][f]); a.dst[f] = R<real>(c->q[s1][f]); }
+    for (int d = 0; d < 3; d++) { a.srcb[d] = c->b[s0][d]; a.dstb[d] = c->b[s1][d]; }
+    a.lo = 5 - c->hb[0].nb + e;       // final update range [5-nb+e, n+nb-4-e] (build_plan case 5)
+    a.hi_off = c->hb[0].nb - 4 - e;
+    launch_restore_shell<real>(mk(c, c->pl_ghost), a);
+}
+
 template <class real>
 void fill_ghosts(imhd_ctx* c) {
     {
@@ -858,7 +873,8 @@ int imhd_step(imhd_ctx* c, double dt, int drive_next) {
     if (is_tvd(c)) {
         if (c->prec == 64) tvd_step<double>(c); else tvd_step<float>(c);
     } else {
-        if (c->prec == 64) vl_step<double>(c); else vl_step<float>(c);
+        if (c->prec == 64) { vl_step<double>(c); restore_shell<double>(c); }
+        else { vl_step<float>(c); restore_shell<float>(c); }
     }
     HIPCK(c, hipGetLastError());
     return prof_collect(c);

@@ -1,0 +1,15 @@
+! Configuration module of the example driver (K98 Roe-TVD + CT, one passive scalar): the
+! names the solver modules read from an application's constants_mod.
+MODULE constants_mod
+  IMPLICIT NONE
+  INTEGER, PARAMETER :: nx = 16, ny = 12, nz = 10
+  INTEGER, PARAMETER :: nb = 6
+  INTEGER, PARAMETER :: npass = 1
+  INTEGER, PARAMETER :: boundary_type = 2          ! 1 open, 2 periodic
+  INTEGER, PARAMETER :: mhd_solver_type = 1        ! 1 TVD, 2 VL
+  INTEGER, PARAMETER :: riemann_solver_type = 2    ! TVD: 1 HLLE, 2 K98 Roe-TVD
+  REAL*8, PARAMETER :: COUR_def = 0.35d0
+  LOGICAL, PARAMETER :: drive_turbulence = .FALSE.
+  REAL*8 :: sound_speed = 1.d0
+  INTEGER :: myrank = 0
+END MODULE constants_mod

@@ -272,3 +272,25 @@ def test_gpu_matches_reference_golden_vectors():
         else:
             assert S.get_max_speed_vanleer() == float(gold[f"step_{name}_cmax"])
         S.close()
+
+
+@pytest.mark.parametrize("recon,rcode", [("plm", 0), ("ppm_char", 2)])
+def test_single_solve_whole_array(recon, rcode):
+    """One vanleer_solve_3d with no ghost fill afterwards: every padded cell equals the
+    oracle's (cells outside the final-update range keep their t^n values, as the
+    reference's restore leaves them, vanleer_mod.f90:404-405)."""
+    dims = (12, 10, 8)
+    say(f"[gpu] single solve whole arrays {recon}")
+    q, bf, qp = random_state(*dims, npass=0, seed=21)
+    O = Oracle(*dims, npass=0, mhd=2, riemann=2, recon=rcode, bc=2)
+    O.set_state(q, bf)
+    O.bc()
+    O.step(3e-3)
+    S = make_solver(dims, 0, "hlld", "periodic", recon=recon)
+    S.set_state(q, bf)
+    S.boundary_conditions()
+    S.vanleer_solve_3d(3e-3)
+    qo, bo = O.get_state()[:2]
+    qg, bg, _ = S.get_state()
+    assert np.array_equal(qg, qo) and np.array_equal(bg, bo)
+    S.close()
