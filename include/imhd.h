@@ -87,10 +87,20 @@ int imhd_get_state(imhd_ctx* ctx, int box, double* q_3d, double* bface_3d, doubl
 
 /* One integrator step of every box with the given dt:
  * vanleer_solve_3d(dt_in, drive_next) (vanleer_mod.f90:279-426) or tvd_solve_3d(dt_in)
- * (isotvd_mod.f90:226-364); drive_next must be 0 (forcing is applied by imhd_run).
- * Cells the step does not update keep their t^n values, as the reference's restore of
- * q_3d / bface_3d leaves them; the driver fills the ghost cells afterwards. */
+ * (isotvd_mod.f90:226-364).  drive_next != 0 (VL only) applies the turbulence force set
+ * by imhd_set_force as the reference does: get_rho_turb on the t^n state before the step
+ * and add_random_force after it (vanleer_mod.f90:304-305, 419-421); its delv_max enters
+ * imhd_max_speed.  Cells the step does not update keep their t^n values, as the
+ * reference's restore of q_3d / bface_3d leaves them; the driver fills the ghost cells
+ * afterwards. */
 int imhd_step(imhd_ctx* ctx, double dt, int drive_next);
+
+/* The driver's force field for box `box`: delv_re(…,3) of grid_data_mod.f90:50 on the
+ * padded grid (same layout as q_3d, 3 components), as generate_rand_force
+ * (turb_driver_mod.f90:197-391) leaves it, with Edot and dt_turb of constants_mod
+ * (constant energy input Edot * dt_turb per kick, turb_driver_mod.f90:428-442).
+ * Replaces the delv argument of add_random_force (turb_driver_mod.f90:395). */
+int imhd_set_force(imhd_ctx* ctx, int box, const double* delv_3d, double edot, double dt_turb);
 
 /* boundary_conditions() on one rank (boundary_conditions_mod.f90:20-234) */
 int imhd_fill_ghosts(imhd_ctx* ctx);

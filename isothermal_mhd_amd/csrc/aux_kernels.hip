@@ -117,12 +117,12 @@ __global__ __launch_bounds__(256) void k_max_speed(const BoxDesc* __restrict__ b
 
 // dt = COUR dx / max(cmax, smax + delv_max), then MIN(t_end - t, dt) (driver.f90:432-448, 475)
 __global__ void k_new_dt(int nbox, const BoxDesc* __restrict__ boxes, const unsigned long long* cmax,
-                         const unsigned long long* smax, const double* cour, double t_end, double* t, double* dt,
-                         double* dt_prev, int advance_time) {
+                         const unsigned long long* smax, const unsigned long long* dvmax, const double* cour,
+                         double t_end, double* t, double* dt, double* dt_prev, int advance_time) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbox) return;
     double cm = striped_max(cmax, b);
-    double sm = striped_max(smax, b);
+    double sm = striped_max(smax, b) + striped_max(dvmax, b);   // smax + delv_max (vanleer_mod.f90:3323)
     double c = fmax(cm, sm);
     if (advance_time) t[b] = t[b] + dt[b];
     dt_prev[b] = dt[b];
@@ -366,10 +366,10 @@ void launch_pass_update(const Launch& L, const PassUpdArgs<real>& a) {
     if (L.plan.total > 0) L256((k_pass_update<real>), L.boxes, L.plan, a);
 }
 void launch_new_dt(hipStream_t st, int nbox, const BoxDesc* boxes, const unsigned long long* cmax,
-                   const unsigned long long* smax, const double* cour, double t_end, double* t, double* dt,
-                   double* dt_prev, int advance_time) {
-    hipLaunchKernelGGL(k_new_dt, dim3((nbox + 63) / 64), dim3(64), 0, st, nbox, boxes, cmax, smax, cour, t_end, t,
-                       dt, dt_prev, advance_time);
+                   const unsigned long long* smax, const unsigned long long* dvmax, const double* cour, double t_end,
+                   double* t, double* dt, double* dt_prev, int advance_time) {
+    hipLaunchKernelGGL(k_new_dt, dim3((nbox + 63) / 64), dim3(64), 0, st, nbox, boxes, cmax, smax, dvmax, cour, t_end,
+                       t, dt, dt_prev, advance_time);
 }
 void launch_reset_u64(hipStream_t st, unsigned long long* p, int n) {
     hipLaunchKernelGGL(k_reset_u64, dim3((n + 255) / 256), dim3(256), 0, st, p, n);

@@ -89,8 +89,8 @@ template <class real> void launch_copy_velocity(const Launch& L, const VelArgs<r
 template <class real> void launch_pass_flux(const Launch& L, const PassFluxArgs<real>& a);
 template <class real> void launch_pass_update(const Launch& L, const PassUpdArgs<real>& a);
 void launch_new_dt(hipStream_t st, int nbox, const BoxDesc* boxes, const unsigned long long* cmax,
-                   const unsigned long long* smax, const double* cour, double t_end, double* t, double* dt,
-                   double* dt_prev, int advance_time);
+                   const unsigned long long* smax, const unsigned long long* dvmax, const double* cour, double t_end,
+                   double* t, double* dt, double* dt_prev, int advance_time);
 void launch_reset_u64(hipStream_t st, unsigned long long* p, int n);
 template <class real> void launch_convert_in(hipStream_t st, const double* src, real* dst, long long n);
 template <class real> void launch_convert_out(hipStream_t st, const real* src, double* dst, long long n);

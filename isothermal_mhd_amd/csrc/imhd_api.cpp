@@ -11,6 +11,7 @@
 
 #include "../../include/imhd.h"
 #include "aux_kernels.hpp"
+#include "forcing.hpp"
 #include "kernels.hpp"
 #include "tvd_kernels.hpp"
 
@@ -64,6 +65,14 @@ struct imhd_ctx {
     unsigned long long* smax = nullptr;
     unsigned long long* cmax = nullptr;
     unsigned long long* dmax = nullptr;  // max|div B| stripes
+    unsigned long long* dvmax = nullptr; // forcing: max |delta m| / rho0 of the last step (delv_max)
+    // forcing application (driver-supplied field, turb_driver_mod.f90)
+    double* delv[3] = {};
+    double* force_part = nullptr;
+    double* force_par = nullptr;
+    unsigned int* force_err = nullptr;
+    bool force_set = false;
+    double edot = 0.0, dt_turb = 0.0;
     double* dt = nullptr;
     double* t = nullptr;
     double* dtprev = nullptr;
@@ -290,6 +299,7 @@ void update(imhd_ctx* c, int src, int dst, double factor, const PlanDev& p, bool
 template <class real>
 void vl_step(imhd_ctx* c, bool fuse_cmax = false) {
     launch_reset_u64(c->st, c->smax, (int)stripe_slots(c->nbox));
+    launch_reset_u64(c->st, c->dvmax, (int)stripe_slots(c->nbox));   // delv_max = 0 (vanleer_mod.f90:301)
     const int s0 = c->cur, s1 = 1 - c->cur;
     switch (c->cfg.riemann) {
         case IMHD_ROE: sweeps<real, RS_ROE>(c, s0, false); break;
@@ -440,6 +450,31 @@ void tvd_max_speed_launch(imhd_ctx* c) {
 
 bool is_tvd(const imhd_ctx* c) { return c->cfg.integrator == IMHD_TVD; }
 
+// get_rho_turb (before the step, on the t^n state s0) and add_random_force (after it, on
+// the new state) with the field set by imhd_set_force (vanleer_mod.f90:304-305, 419-421)
+template <class real>
+void force_sums(imhd_ctx* c, int s0) {
+    ProfScope ps(c, "force_sums", c->pl_int.active);
+    ForceSumArgs<real> a{};
+    for (int f = 0; f < 4; f++) a.q[f] = R<real>(c->q[s0][f]);
+    for (int d = 0; d < 3; d++) a.delv[d] = c->delv[d];
+    a.part = c->force_part;
+    launch_force_sums<real>(mk(c, c->pl_int), a);
+    ForceFinArgs f{c->force_part, c->force_par, c->edot, c->dt_turb, c->force_err};
+    launch_force_finalize(c->st, c->db, c->pl_int.d, c->nbox, f);
+}
+
+template <class real>
+void force_apply(imhd_ctx* c, int s0, int s1) {
+    ProfScope ps(c, "force_apply", c->pl_int.active);
+    ForceApplyArgs<real> a{};
+    a.rho0 = R<real>(c->q[s0][0]);
+    for (int d = 0; d < 3; d++) { a.m[d] = R<real>(c->q[s1][1 + d]); a.delv[d] = c->delv[d]; }
+    a.par = c->force_par;
+    a.dvmax = c->dvmax;
+    launch_force_apply<real>(mk(c, c->pl_int), a);
+}
+
 // Drop-in step only: cells outside the VL final-update range take their t^n values, as
 // the reference's restore leaves them (the native loop's ghost fill makes this moot).
 template <class real>
@@ -569,14 +604,14 @@ void native_step(imhd_ctx* c, double t_end) {
         if (c->cfg.npass > 0) passive_fused<real>(c, qold);
         tvd_max_speed_launch<real>(c);
         fill_ghosts<real>(c);
-        launch_new_dt(c->st, c->nbox, c->db, c->cmax, c->smax, c->cour, t_end, c->t, c->dt, c->dtprev, 1);
+        launch_new_dt(c->st, c->nbox, c->db, c->cmax, c->smax, c->dvmax, c->cour, t_end, c->t, c->dt, c->dtprev, 1);
         return;
     }
     launch_reset_u64(c->st, c->cmax, (int)stripe_slots(c->nbox));
     vl_step<real>(c, true);
     if (c->cfg.npass > 0) passive_fused<real>(c, qold);
     fill_ghosts<real>(c);
-    launch_new_dt(c->st, c->nbox, c->db, c->cmax, c->smax, c->cour, t_end, c->t, c->dt, c->dtprev, 1);
+    launch_new_dt(c->st, c->nbox, c->db, c->cmax, c->smax, c->dvmax, c->cour, t_end, c->t, c->dt, c->dtprev, 1);
 }
 
 __global__ void k_set_dt(double* dt, int n, double v) {
@@ -764,6 +799,7 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     if ((rc = alloc(c, (void**)&c->smax, 8 * stripe_slots(nbox)))) return rc;
     if ((rc = alloc(c, (void**)&c->cmax, 8 * stripe_slots(nbox)))) return rc;
     if ((rc = alloc(c, (void**)&c->dmax, 8 * stripe_slots(nbox)))) return rc;
+    if ((rc = alloc(c, (void**)&c->dvmax, 8 * stripe_slots(nbox)))) return rc;
     if ((rc = alloc(c, (void**)&c->dt, 8 * nbox))) return rc;
     if ((rc = alloc(c, (void**)&c->t, 8 * nbox))) return rc;
     if ((rc = alloc(c, (void**)&c->dtprev, 8 * nbox))) return rc;
@@ -830,6 +866,8 @@ int imhd_destroy(imhd_ctx* c) {
     for (auto p : c->Ecc) fr(p);
     for (auto p : c->gt) fr(p);
     for (auto p : c->ck) fr(p);
+    for (auto p : c->delv) fr(p);
+    for (void* p : {(void*)c->force_part, (void*)c->force_par, (void*)c->force_err, (void*)c->dvmax}) fr(p);
     for (void* p : {(void*)c->smax, (void*)c->cmax, (void*)c->dmax, (void*)c->dt, (void*)c->t, (void*)c->dtprev, (void*)c->cour,
                     (void*)c->diag, (void*)c->floor_count, (void*)c->stage, (void*)c->db})
         fr(p);
@@ -867,17 +905,57 @@ int imhd_get_state(imhd_ctx* c, int box, double* q, double* bf, double* qp) {
 int imhd_step(imhd_ctx* c, double dt, int drive_next) {
     if (!c) return IMHD_EINVAL;
     if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
-    if (drive_next) FAIL(c, IMHD_EINVAL, "forcing is applied by imhd_run; drive_next must be 0");
     if (!(dt > 0.0)) FAIL(c, IMHD_EINVAL, "dt must be positive");
+    if (drive_next && is_tvd(c)) FAIL(c, IMHD_EINVAL, "tvd_solve_3d has no forcing hook (isotvd_mod.f90:226)");
+    if (drive_next && !c->force_set) FAIL(c, IMHD_ESTATE, "drive_next needs a force field (imhd_set_force)");
     set_scalar_dt(c, dt);
+    const int s0 = c->cur, s1 = 1 - c->cur;
     if (is_tvd(c)) {
         if (c->prec == 64) tvd_step<double>(c); else tvd_step<float>(c);
+    } else if (c->prec == 64) {
+        if (drive_next) force_sums<double>(c, s0);
+        vl_step<double>(c);
+        restore_shell<double>(c);
+        if (drive_next) force_apply<double>(c, s0, s1);
     } else {
-        if (c->prec == 64) { vl_step<double>(c); restore_shell<double>(c); }
-        else { vl_step<float>(c); restore_shell<float>(c); }
+        if (drive_next) force_sums<float>(c, s0);
+        vl_step<float>(c);
+        restore_shell<float>(c);
+        if (drive_next) force_apply<float>(c, s0, s1);
     }
     HIPCK(c, hipGetLastError());
+    if (drive_next) {
+        unsigned int err = 0;
+        HIPCK(c, hipMemcpyAsync(&err, c->force_err, 4, hipMemcpyDeviceToHost, c->st));
+        HIPCK(c, hipStreamSynchronize(c->st));
+        if (err) {
+            HIPCK(c, hipMemsetAsync(c->force_err, 0, 4, c->st));
+            FAIL(c, IMHD_EINVAL, "bad value in turbulence driver constant energy normalization "
+                                 "(turb_driver_mod.f90:420-425)");
+        }
+    }
     return prof_collect(c);
+}
+
+int imhd_set_force(imhd_ctx* c, int box, const double* delv, double edot, double dt_turb) {
+    if (!c || !delv || box < 0 || box >= c->nbox) return IMHD_EINVAL;
+    if (!c->delv[0]) {
+        for (int d = 0; d < 3; d++) HIPCK(c, hipMalloc(&c->delv[d], (size_t)c->pool * 8));
+        for (int d = 0; d < 3; d++) HIPCK(c, hipMemset(c->delv[d], 0, (size_t)c->pool * 8));
+        HIPCK(c, hipMalloc(&c->force_part, (size_t)(c->pl_int.total / 256 + 1) * FORCE_NSUM * 8));
+        HIPCK(c, hipMalloc(&c->force_par, (size_t)c->nbox * FORCE_NPAR * 8));
+        HIPCK(c, hipMalloc(&c->force_err, 4));
+        HIPCK(c, hipMemset(c->force_err, 0, 4));
+    }
+    const BoxDesc& B = c->hb[box];
+    const long long nc = (long long)B.P[0] * B.P[1] * B.P[2];
+    HIPCK(c, hipStreamSynchronize(c->st));
+    for (int d = 0; d < 3; d++)
+        HIPCK(c, hipMemcpy(c->delv[d] + B.off, delv + d * nc, (size_t)nc * 8, hipMemcpyHostToDevice));
+    c->edot = edot;
+    c->dt_turb = dt_turb;
+    c->force_set = true;
+    return IMHD_OK;
 }
 
 int imhd_fill_ghosts(imhd_ctx* c) {
@@ -914,20 +992,23 @@ int imhd_max_speed(imhd_ctx* c, double* cmax) {
     if (c->prec == 64) max_speed_launch<double>(c); else max_speed_launch<float>(c);
     HIPCK(c, hipGetLastError());
     const size_t ns = stripe_slots(c->nbox);
-    std::vector<unsigned long long> cm(ns), sm(ns);
+    std::vector<unsigned long long> cm(ns), sm(ns), dv(ns);
     HIPCK(c, hipStreamSynchronize(c->st));
     HIPCK(c, hipMemcpy(cm.data(), c->cmax, 8 * ns, hipMemcpyDeviceToHost));
     HIPCK(c, hipMemcpy(sm.data(), c->smax, 8 * ns, hipMemcpyDeviceToHost));
+    HIPCK(c, hipMemcpy(dv.data(), c->dvmax, 8 * ns, hipMemcpyDeviceToHost));
     for (int b = 0; b < c->nbox; b++) {
-        unsigned long long am = 0, smx = 0;
+        unsigned long long am = 0, smx = 0, dvx = 0;
         for (int st = 0; st < NSTRIPE; st++) {
             am = std::max(am, cm[((size_t)b * NSTRIPE + st) * STRIPE_STRIDE]);
             smx = std::max(smx, sm[((size_t)b * NSTRIPE + st) * STRIPE_STRIDE]);
+            dvx = std::max(dvx, dv[((size_t)b * NSTRIPE + st) * STRIPE_STRIDE]);
         }
-        double a, s;
+        double a, s, d;
         memcpy(&a, &am, 8);
         memcpy(&s, &smx, 8);
-        cmax[b] = std::max(a, s);  // MAX(max_speed, smax + delv_max), delv_max = 0
+        memcpy(&d, &dvx, 8);
+        cmax[b] = std::max(a, s + d);  // MAX(max_speed, smax + delv_max) (vanleer_mod.f90:3323)
         if (cmax[b] < ZERO) FAIL(c, IMHD_ENOSPEED, "zero max speed encountered in box %d", b);
     }
     prof_collect(c);

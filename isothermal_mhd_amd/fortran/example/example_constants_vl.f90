@@ -10,6 +10,7 @@ MODULE constants_mod
   INTEGER, PARAMETER :: riemann_solver_type = 2    ! VL: 1 ROE, 2 HLLD, 3 HLLE
   REAL*8, PARAMETER :: COUR_def = 0.35d0
   LOGICAL, PARAMETER :: drive_turbulence = .FALSE.
+  REAL*8, PARAMETER :: Edot = 1.d-3, dt_turb = 0.05d0   ! forcing (unused while not driven)
   REAL*8 :: sound_speed = 1.d0
   INTEGER :: myrank = 0
 END MODULE constants_mod

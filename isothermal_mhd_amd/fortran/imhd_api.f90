@@ -60,6 +60,14 @@ MODULE imhd_api
       INTEGER(C_INT), VALUE :: drive_next
     END FUNCTION imhd_step
 
+    INTEGER(C_INT) FUNCTION imhd_set_force(ctx, box, delv, edot, dt_turb) BIND(C, name='imhd_set_force')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: box
+      REAL(C_DOUBLE), INTENT(IN) :: delv(*)
+      REAL(C_DOUBLE), VALUE :: edot, dt_turb
+    END FUNCTION imhd_set_force
+
     INTEGER(C_INT) FUNCTION imhd_fill_ghosts(ctx) BIND(C, name='imhd_fill_ghosts')
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ctx

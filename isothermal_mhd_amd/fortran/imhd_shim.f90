@@ -11,8 +11,9 @@
 !   boundary_conditions_mod  boundary_conditions_mod.f90: boundary_conditions
 !
 ! Like the reference modules, these USE the application's constants_mod (grid size, nb,
-! npass, solver and boundary types, sound_speed, COUR_def) and grid_data_mod (q_3d,
-! bface_3d, qpass_3d), so this file is compiled together with them.  The state lives on
+! npass, solver and boundary types, sound_speed, COUR_def, drive_turbulence, Edot,
+! dt_turb) and grid_data_mod (q_3d, bface_3d, qpass_3d, delv_re), so this file is
+! compiled together with them.  The state lives on
 ! the GPU; the host arrays are uploaded when the host has changed them (after init, or
 ! after imhd_shim_host_modified) and downloaded after every call that changes the state,
 ! because the driver reads them between calls (compute_KE, file dumps).  A failing call
@@ -114,6 +115,7 @@ MODULE VanLeer_Solver_mod
   USE imhd_api
   USE imhd_shim_core
   USE constants_mod
+  USE grid_data_mod, ONLY: delv_re
   IMPLICIT NONE
   PRIVATE
   PUBLIC :: init_solver_vanleer, destroy_solver_vanleer, vanleer_solve_3d, get_max_speed_vanleer
@@ -129,17 +131,20 @@ CONTAINS
     CALL shim_destroy()
   END SUBROUTINE destroy_solver_vanleer
 
-  ! vanleer_mod.f90:279-426.  The turbulence-forcing hook (drive_next) is not available
-  ! through this library yet.
+  ! vanleer_mod.f90:279-426.  With drive_turbulence and drive_next, the force field the
+  ! driver's generate_rand_force left in delv_re is applied on the GPU as the reference
+  ! applies it (get_rho_turb before the step, add_random_force after it).
   SUBROUTINE vanleer_solve_3d(dt_in, drive_next)
     REAL*8, INTENT(INOUT) :: dt_in
     LOGICAL, INTENT(IN) :: drive_next
-    IF (drive_turbulence .AND. drive_next) THEN
-      PRINT*, 'imhd: turbulence forcing (drive_next) is not available in this build'
-      STOP
-    END IF
+    INTEGER(C_INT) :: drive
     CALL shim_upload()
-    CALL shim_check(imhd_step(shim_ctx, dt_in, 0_C_INT), 'vanleer_solve_3d')
+    drive = 0
+    IF (drive_turbulence .AND. drive_next) THEN
+      CALL shim_check(imhd_set_force(shim_ctx, 0_C_INT, delv_re, Edot, dt_turb), 'imhd_set_force')
+      drive = 1
+    END IF
+    CALL shim_check(imhd_step(shim_ctx, dt_in, drive), 'vanleer_solve_3d')
     CALL shim_download()
   END SUBROUTINE vanleer_solve_3d
 

@@ -107,8 +107,18 @@ class VanLeerSolver:
 
     # -- reference procedures ------------------------------------------------------
     def vanleer_solve_3d(self, dt, drive_next=False):
-        """vanleer_mod.f90:279-426 on every box with the given dt."""
-        self._check(self.lib.imhd_step(self._c, float(dt), int(drive_next)))
+        """vanleer_mod.f90:279-426 on every box with the given dt; drive_next applies the
+        force field of set_force (get_rho_turb before, add_random_force after the step)."""
+        self._check(self.lib.imhd_step(self._c, float(dt), int(bool(drive_next))))
+
+    def set_force(self, delv_re, edot, dt_turb, box=0):
+        """The driver's force field delv_re (padded grid, 3 components; what
+        generate_rand_force leaves in grid_data_mod) with Edot and dt_turb."""
+        sh, _ = self.shapes(box)
+        d = np.asfortranarray(delv_re, dtype=np.float64)
+        if d.shape != sh + (3,):
+            raise ValueError(f"expected delv {sh + (3,)}")
+        self._check(self.lib.imhd_set_force(self._c, box, _p(d), float(edot), float(dt_turb)))
 
     def mhd_update(self, dt):
         """driver.f90:378-389: the configured integrator's solve on every box."""

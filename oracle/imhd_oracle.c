@@ -40,6 +40,10 @@ struct orc_ctx {
     int VX, VY, VZ;
     long VC;
     double *vel, *vtavg, *Fp;
+    /* turbulence forcing (turb_driver_mod.f90): delv_re on the padded grid, rho0 */
+    double *delv, *rho0;
+    double edot, dt_turb;
+    double quad1, quad2, mavg[3];
 };
 
 /* ----------------------------------------------------------------------------------- */
@@ -766,6 +770,75 @@ static void vl_step(orc_ctx *c, double dt) {
 #include "imhd_oracle_tvd.inc"
 
 /* ----------------------------------------------------------------------------------- */
+/* forcing application of the reference's turbulence driver                            */
+
+/* get_rho_turb (turb_driver_mod.f90:130-192): rho0 = rho^n on the padded grid; the
+ * quadratic coefficients of the kinetic-energy gain and the momentum the field carries,
+ * summed over interior cells in the reference's loop order. */
+static void force_sums(orc_ctx *c) {
+    for (long p = 0; p < c->NC; p++) c->rho0[p] = c->q[p];
+    double q1 = 0.0, q2 = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+    for (int k = 1; k <= c->c.nz; k++)
+        for (int j = 1; j <= c->c.ny; j++)
+            for (int i = 1; i <= c->c.nx; i++) {
+                long p = cidx(c, i, j, k);
+                double d1 = c->delv[p], d2 = c->delv[c->NC + p], d3 = c->delv[2 * c->NC + p];
+                double rho = Q(0, i, j, k);
+                q1 = q1 + rho * (d1 * d1 + d2 * d2 + d3 * d3);
+                q2 = q2 + d1 * Q(1, i, j, k) + d2 * Q(2, i, j, k) + d3 * Q(3, i, j, k);
+                mx = mx + rho * d1;
+                my = my + rho * d2;
+                mz = mz + rho * d3;
+            }
+    c->quad1 = 0.5 * q1;
+    c->quad2 = q2;
+    c->mavg[0] = mx; c->mavg[1] = my; c->mavg[2] = mz;
+}
+
+/* add_random_force (turb_driver_mod.f90:395-457), one rank: constant-energy-input
+ * normalisation (larger root of the KE quadratic) and zero net momentum injection */
+static int force_apply(orc_ctx *c) {
+    if (c->quad1 <= 1.0e-20) return -1;
+    double dx3 = c->dx * c->dx * c->dx;
+    double ma[3] = {c->mavg[0] * dx3, c->mavg[1] * dx3, c->mavg[2] * dx3};
+    double tmp1 = c->edot * c->dt_turb / dx3;
+    double tmp2 = sqrt(c->quad2 * c->quad2 + (4.0 * tmp1 * c->quad1));
+    double constE = 0.5 * (-c->quad2 + tmp2) / c->quad1;
+    for (int k = 1; k <= c->c.nz; k++)
+        for (int j = 1; j <= c->c.ny; j++)
+            for (int i = 1; i <= c->c.nx; i++) {
+                long p = cidx(c, i, j, k);
+                double r0 = c->rho0[p], dm[3];
+                for (int d = 0; d < 3; d++) dm[d] = constE * (r0 * c->delv[d * c->NC + p] - ma[d]);
+                Q(1, i, j, k) = Q(1, i, j, k) + dm[0];
+                Q(2, i, j, k) = Q(2, i, j, k) + dm[1];
+                Q(3, i, j, k) = Q(3, i, j, k) + dm[2];
+                double m = dmax3(fabs(dm[0]) / r0, fabs(dm[1]) / r0, fabs(dm[2]) / r0);
+                c->delv_max = dmax2(c->delv_max, m);
+            }
+    return 0;
+}
+
+void orc_set_force(orc_ctx *c, const double *delv, double edot, double dt_turb) {
+    if (!c->delv) {
+        c->delv = calloc((size_t)3 * c->NC, sizeof(double));
+        c->rho0 = calloc((size_t)c->NC, sizeof(double));
+    }
+    memcpy(c->delv, delv, sizeof(double) * 3 * c->NC);
+    c->edot = edot;
+    c->dt_turb = dt_turb;
+}
+
+/* vanleer_solve_3d(dt, drive_next=.TRUE.) (vanleer_mod.f90:304-305, 419-421) */
+int orc_step_drive(orc_ctx *c, double dt) {
+    if (c->c.mhd != 2 || !c->delv) return -1;
+    c->dt = dt;
+    force_sums(c);
+    vl_step(c, dt);
+    return force_apply(c);
+}
+
+/* ----------------------------------------------------------------------------------- */
 /* ghost fill, boundary_conditions (boundary_conditions_mod.f90:20-234), one rank       */
 
 static void copy_plane(double *a, long NC, int nf, const orc_ctx *c, int d, int dst, int src, int is_pass) {
@@ -1015,6 +1088,7 @@ void orc_destroy(orc_ctx *c) {
     if (!c) return;
     free(c->q); free(c->qw); free(c->bf); free(c->bfw); free(c->ec); free(c->Fs); free(c->emf); free(c->Fbuf);
     free(c->qp); free(c->qpw); free(c->vel); free(c->vtavg); free(c->Fp);
+    free(c->delv); free(c->rho0);
     free(c);
 }
 

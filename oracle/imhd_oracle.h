@@ -43,6 +43,12 @@ void orc_set_state(orc_ctx *c, const double *q, const double *bface, const doubl
 void orc_get_state(const orc_ctx *c, double *q, double *bface, double *qpass, double *emf_corner);
 void orc_bc(orc_ctx *c);
 void orc_step(orc_ctx *c, double dt);
+/* Forcing application of the reference's turbulence driver: delv on the padded grid
+ * ((1-nb:n+nb)^3, 3; the driver's delv_re), Edot and dt_turb (constants_mod.f90).
+ * orc_step_drive = vanleer_solve_3d(dt, drive_next=.TRUE.): get_rho_turb before the step,
+ * add_random_force after it; returns -1 on the reference's bad-normalisation STOP. */
+void orc_set_force(orc_ctx *c, const double *delv, double edot, double dt_turb);
+int orc_step_drive(orc_ctx *c, double dt);
 double orc_max_speed(orc_ctx *c);
 double orc_smax(const orc_ctx *c);
 void orc_copy_velocity(orc_ctx *c, int stage);

@@ -23,7 +23,16 @@ CONTAINS
     USE VanLeer_Solver_mod, ONLY: init_solver_vanleer
     USE Iso_TVD_Solver_mod, ONLY: init_solver_tvd
     USE passive_solver_mod, ONLY: init_passive_solver
+    USE turb_driver_mod, ONLY: init_turb_driver
+    USE mpi_domain_mod, ONLY: mpi_comm3d_init
+    INTEGER :: ierr
     IF (is_setup) RETURN
+    ! the forcing path reduces over comm3d (get_rho_turb, turb_driver_mod.f90:184-188):
+    ! one-rank MPI and the Cartesian communicator of mpi_domain_mod, without its RMA set-up
+    IF (drive_turbulence) THEN
+      CALL MPI_INIT(ierr)
+      CALL mpi_comm3d_init()
+    END IF
     myrank = 0
     my_coord = 0
     CALL create_grid_arrays()
@@ -33,6 +42,7 @@ CONTAINS
     IF (mhd_solver_type .EQ. 1) CALL init_solver_tvd()
     IF (mhd_solver_type .EQ. 2) CALL init_solver_vanleer()
     IF (npass .GT. 0) CALL init_passive_solver()
+    IF (drive_turbulence) CALL init_turb_driver(s1_def, s2_def, s3_def)
     is_setup = .TRUE.
   END SUBROUTINE ref_setup
 
@@ -141,6 +151,35 @@ CONTAINS
     F = Fs_2d(1-nb:ni-nb, 1-nb:nj-nb, :)
     sm = smax
   END SUBROUTINE ref_vl_riemann
+
+  ! Turbulence forcing (builds with drive_turbulence = .TRUE.): the driver's force field
+  ! delv_re on the padded grid (grid_data_mod.f90:50; turb_driver_mod.f90:379 index shift),
+  ! and vanleer_solve_3d(dt, drive_next=.TRUE.) (get_rho_turb + add_random_force).
+  SUBROUTINE ref_set_force(delv) BIND(C, name='ref_set_force')
+    REAL(C_DOUBLE), INTENT(IN) :: delv(1-nb:nx+nb, 1-nb:ny+nb, 1-nb:nz+nb, 3)
+    delv_re(-nb-nx/2:nb-1+nx/2, -nb-ny/2:nb-1+ny/2, -nb-nz/2:nb-1+nz/2, :) = delv
+  END SUBROUTINE ref_set_force
+
+  SUBROUTINE ref_step_drive(dt) BIND(C, name='ref_step_drive')
+    USE VanLeer_Solver_mod, ONLY: vanleer_solve_3d
+    REAL(C_DOUBLE), VALUE :: dt
+    REAL*8 :: dtl
+    dtl = dt
+    CALL vanleer_solve_3d(dtl, .TRUE.)
+  END SUBROUTINE ref_step_drive
+
+  ! taus88 stream of the driver (turb_driver_mod.f90:95-126)
+  SUBROUTINE ref_rand_taus(seeds, n, out) BIND(C, name='ref_rand_taus')
+    USE turb_driver_mod, ONLY: set_taus_seed, rand_taus
+    INTEGER(C_INT), INTENT(IN) :: seeds(3)
+    INTEGER(C_INT), VALUE :: n
+    REAL(C_DOUBLE), INTENT(OUT) :: out(n)
+    INTEGER :: i
+    CALL set_taus_seed(seeds(1), seeds(2), seeds(3))
+    DO i = 1, n
+      out(i) = rand_taus()
+    END DO
+  END SUBROUTINE ref_rand_taus
 
   ! Interface fluxes of the TVD module on an ni x nj plane of cell states q(7) (slot 5 =
   ! the cell's own face field).  kind 2: compute_roe_avg_state_1d .. compute_TVD_fluxes_1d

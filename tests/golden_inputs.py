@@ -67,3 +67,15 @@ def recon_inputs(kind, ni=28, nj=3):
     w[3] = w[2]
     wb = 0.5 + 0.5 * rng.standard_normal((ni, nj))
     return np.asfortranarray(w), np.asfortranarray(wb)
+
+
+# driven VL steps (reference build with drive_turbulence = .TRUE., Edot = 1e-3,
+# dt_turb = 0.05): (reference build, dims, npass, state seed, force seed, dts, drive flags)
+DRIVE_CASE = ("vl_hlld_p16_drive", (16, 16, 16), 1, 71, 72, (1e-15, 2e-3, 1.5e-3), (True, False, True))
+DRIVE_EDOT, DRIVE_DT_TURB = 1e-3, 0.05
+
+
+def drive_force(shape, seed):
+    """a seeded force field on the padded grid, the shape of the driver's delv_re"""
+    rng = np.random.default_rng(seed)
+    return np.asfortranarray(rng.standard_normal(tuple(shape) + (3,)))

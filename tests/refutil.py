@@ -66,6 +66,9 @@ def oracle_lib():
         L.orc_recon_plane.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_set_force.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.orc_step_drive.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_step_drive.restype = ctypes.c_int
         L.orc_tvd_flux_plane.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p]
@@ -145,6 +148,21 @@ def ref_available(name):
 _refs = {}
 
 
+def _oracle_force_methods():
+    def set_force(self, delv, edot=1e-3, dt_turb=0.05):
+        self._delv = np.asfortranarray(delv, dtype=np.float64)
+        self.L.orc_set_force(self.c, P(self._delv), edot, dt_turb)
+
+    def step_drive(self, dt):
+        rc = self.L.orc_step_drive(self.c, dt)
+        assert rc == 0, "bad forcing normalisation"
+    Oracle.set_force = set_force
+    Oracle.step_drive = step_drive
+
+
+_oracle_force_methods()
+
+
 class Reference:
     """The reference Fortran modules (one process-wide instance per build config)."""
 
@@ -152,6 +170,8 @@ class Reference:
         if name not in _refs:
             L = ctypes.CDLL(os.path.join(REF_DIR, f"libimhd_ref_{name}.so"))
             L.ref_step.argtypes = [ctypes.c_double]
+            if hasattr(L, "ref_step_drive"):
+                L.ref_step_drive.argtypes = [ctypes.c_double]
             L.ref_passive_step.argtypes = [ctypes.c_double]
             L.ref_copy_velocity.argtypes = [ctypes.c_int]
             L.ref_set_sound_speed.argtypes = [ctypes.c_double]
@@ -202,6 +222,14 @@ class Reference:
 
     def passive_step(self, dt):
         self.L.ref_passive_step(dt)
+
+    def set_force(self, delv, edot=None, dt_turb=None):
+        """edot / dt_turb are the build's constants (1e-3, 0.05)."""
+        self._delv = np.asfortranarray(delv, dtype=np.float64)
+        self.L.ref_set_force(P(self._delv))
+
+    def step_drive(self, dt):
+        self.L.ref_step_drive(ctypes.c_double(dt))
 
 
 def random_state(nx, ny, nz, nb=6, npass=1, seed=0, amp_v=0.5, amp_rho=0.2, b0=(0.5, 0.2, 0.1),

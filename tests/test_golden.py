@@ -8,8 +8,9 @@ import os
 import numpy as np
 import pytest
 
-from golden_inputs import (STEP_CASES, STEP_DTS, TVD_FLUX_CASES, TVD_FLUX_DX, recon_inputs,
-                           riemann_inputs, step_case_params, tvd_flux_inputs)
+from golden_inputs import (DRIVE_CASE, DRIVE_DT_TURB, DRIVE_EDOT, STEP_CASES, STEP_DTS, TVD_FLUX_CASES,
+                           TVD_FLUX_DX, drive_force, recon_inputs, riemann_inputs, step_case_params,
+                           tvd_flux_inputs)
 from refutil import Oracle, P, oracle_lib, random_state
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_vectors.npz")
@@ -76,3 +77,24 @@ def test_step_golden(gold, name):
     assert np.array_equal(po[Ip][..., :max(npass, 1)], gold[f"step_{name}_p"])
     assert O.smax() == float(gold[f"step_{name}_smax"])
     assert O.max_speed() == float(gold[f"step_{name}_cmax"])
+
+
+def test_drive_golden(gold):
+    """forcing application (get_rho_turb + add_random_force) against the reference's output"""
+    build, dims, npass, seed, fseed, dts, drives = DRIVE_CASE
+    O = Oracle(*dims, npass=npass, mhd=2, riemann=2, recon=2, bc=2)
+    q, bf, qp = random_state(*dims, npass=npass, seed=seed)
+    O.set_state(q, bf, qp)
+    O.bc()
+    O.set_force(drive_force(O.sh, fseed), DRIVE_EDOT, DRIVE_DT_TURB)
+    for dt, drive in zip(dts, drives):
+        O.copy_velocity(0)
+        O.step_drive(dt) if drive else O.step(dt)
+        O.copy_velocity(1)
+        O.passive_step(dt)
+        O.bc()
+    qo, bo, po, _ = O.get_state()
+    assert np.array_equal(qo, gold["drive_q"])
+    assert np.array_equal(bo, gold["drive_b"])
+    assert np.array_equal(po, gold["drive_p"])
+    assert O.max_speed() == float(gold["drive_cmax"])

@@ -16,6 +16,11 @@ Parity status of the oracle's functions:
     corrector dtdx, incl. the 1e-15 first step) and whole driver steps (CTU half updates
     with multidimensional source terms, centre/corner EMFs, CT, final update, CFL speed)
     -- bitwise for Roe/HLLE, periodic/open, cubic and non-cubic boxes.
+  * Forcing application of the turbulence driver (get_rho_turb + add_random_force inside
+    vanleer_solve_3d(dt, drive_next=.TRUE.)): bitwise over driven and undriven steps,
+    incl. the CFL speed with delv_max.  generate_rand_force itself is not restated (it reads
+    uninitialised Fourier modes, turb_driver_mod.f90:232-233); the driver keeps producing
+    delv_re and the library applies it.
   * PLM-primitive reconstruction (the benchmark's reconstruction) is not reachable in the
     reference build (vanleer_mod.f90:83-84 fixes piecewise_parabolic=.TRUE.); it is pinned by
     tests/test_oracle_properties.py.
@@ -189,3 +194,24 @@ def test_ghost_fill_bitwise_open_and_periodic():
             X.bc()
         for a, b in zip(R.get_state()[:3], O.get_state()[:3]):
             assert np.array_equal(a, b)
+
+
+def test_forcing_application_bitwise():
+    R = need("vl_hlld_p16_drive")
+    dims = (16, 16, 16)
+    O = Oracle(*dims, npass=1, mhd=2, riemann=2, recon=2, bc=2)
+    q, bf, qp = random_state(*dims, npass=1, seed=8)
+    delv = np.asfortranarray(np.random.default_rng(9).standard_normal(R.sh + (3,)))
+    for X in (R, O):
+        X.set_state(q, bf, qp)
+        X.bc()
+        X.set_force(delv)
+        for dt, drive in ((1e-15, True), (2e-3, False), (2e-3, True)):
+            X.copy_velocity(0)
+            X.step_drive(dt) if drive else X.step(dt)
+            X.copy_velocity(1)
+            X.passive_step(dt)
+            X.bc()
+    for a, b, nm in zip(R.get_state(), O.get_state(), ["q", "bface", "qpass", "emf_corner"]):
+        assert np.array_equal(a, b), nm
+    assert R.max_speed() == O.max_speed()
