@@ -102,6 +102,30 @@ int imhd_step(imhd_ctx* ctx, double dt, int drive_next);
  * Replaces the delv argument of add_random_force (turb_driver_mod.f90:395). */
 int imhd_set_force(imhd_ctx* ctx, int box, const double* delv_3d, double edot, double dt_turb);
 
+/* Ornstein-Uhlenbeck turbulence forcing of the native loop (imhd_run); no counterpart in
+ * the reference, whose driver kicks impulsively (turb_driver_mod.f90).  Modes: integer
+ * wave vectors n with kmin <= |n| <= kmax (one of each +/- pair; k = 2 pi n_d / L_d),
+ * amplitudes a_n (3 complex components) following
+ *     a <- f a + sigma sqrt(1 - f^2) P(k) xi,   f = exp(-dt / t_corr),
+ * xi complex standard normal from Philox4x32-10 keyed by (seed, box) and counted by
+ * (OU step, mode, component); P(k) projects onto the plane normal to k when solenoidal.
+ * Each step after the MHD update: acceleration a(x) = sum 2 Re(a_n e^{i k.x}) and
+ * delta(rho v) = dt (rho a - <rho a>) on interior cells (no net momentum injection), before
+ * the passive step, CFL reduction and ghost fill.  Amplitudes start from the stationary
+ * distribution.  sigma <= 0 (or ou == NULL) switches the forcing off. */
+typedef struct imhd_ou {
+    double kmin, kmax;  /* mode-number shell, e.g. 1 and 2 */
+    double t_corr;      /* correlation time */
+    double sigma;       /* stationary rms of each real amplitude component (acceleration units) */
+    unsigned long long seed;
+    int solenoidal;     /* 1: project the forcing onto solenoidal modes */
+    int _reserved;
+} imhd_ou;
+int imhd_set_ou_forcing(imhd_ctx* ctx, const imhd_ou* ou);
+/* Number of OU modes; with non-NULL arrays also their integer wave vectors (3 per mode)
+ * and the current amplitudes of box `box` (per mode: 3 real parts, then 3 imaginary). */
+int imhd_get_ou_modes(imhd_ctx* ctx, int box, int* nmode, int* kvec, double* amp);
+
 /* boundary_conditions() on one rank (boundary_conditions_mod.f90:20-234) */
 int imhd_fill_ghosts(imhd_ctx* ctx);
 

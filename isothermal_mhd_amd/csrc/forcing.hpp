@@ -31,6 +31,45 @@ struct ForceApplyArgs {
     unsigned long long* dvmax; // striped per-box max |delta m| / rho0
 };
 
+// ---- Ornstein-Uhlenbeck forcing (native loop) ----
+// Mode m of box b: integer wave vector n_m (one of each +/- pair), complex amplitude
+// a_m (3 components).  Per step: a <- f a + sigma sqrt(1 - f^2) P(k) xi, f = exp(-dt/T),
+// xi complex Gaussian from Philox4x32-10 keyed by (seed, box), counter (step, m, d, 0).
+// Field a(x) = sum_m 2 Re(a_m e^{i k_m.x}) from per-axis twiddle tables; applied as
+// dm = dt (rho a - <rho a>) on interior cells (no net momentum injection).
+constexpr int OU_MAXMODE = 128;
+
+struct OuDev {
+    const int* n;            // 3 per mode
+    double* are;             // [box][mode][3] real parts
+    double* aim;             // imaginary parts
+    unsigned int* step;      // per box OU step counter
+    const double* tw;        // twiddles: per box, axis-major blocks of (2K+1) x N_axis complex
+    const long long* twoff;  // per box offset into tw
+    int nmode, K;
+    double t_corr, sigma;
+    unsigned int seed_lo, seed_hi;
+    int solenoidal;
+    const double* dt;        // per box
+};
+
+template <class real>
+struct OuApplyArgs {
+    OuDev ou;
+    real* q[7];              // new state (rho, m read; m written; all read for the CFL)
+    double* part;            // 3 sums per 256-item block
+    double* mean;            // 3 per box
+    unsigned long long* cmax;// non-null: fused CFL speed of the forced cells
+    double cs;
+    int pass;                // 0: sums of rho a; 1: apply
+};
+
+void launch_ou_update(hipStream_t st, int nbox, const BoxDesc* boxes, const OuDev& ou, int init);
+template <class real>
+void launch_ou_pass(const Launch& L, const OuApplyArgs<real>& a);
+void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, const PlanBox* plan, int nbox, const double* part,
+                        double* mean);
+
 template <class real>
 void launch_force_sums(const Launch& L, const ForceSumArgs<real>& a);
 void launch_force_finalize(hipStream_t st, const BoxDesc* boxes, const PlanBox* plan, int nbox, const ForceFinArgs& a);

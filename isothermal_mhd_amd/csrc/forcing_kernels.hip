@@ -111,6 +111,172 @@ void launch_force_apply(const Launch& L, const ForceApplyArgs<real>& a) {
     if (L.plan.total > 0) hipLaunchKernelGGL((k_force_apply<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
 }
 
+// ---------------------------------------------------------------------------------
+// Ornstein-Uhlenbeck forcing
+
+// Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11): 10 rounds of two 32x32->64
+// multiplies with the Weyl key schedule.
+__device__ __forceinline__ void philox4x32(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0,
+                                           unsigned k1, unsigned* o) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        unsigned long long p0 = (unsigned long long)0xD2511F53u * c0;
+        unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c2;
+        unsigned y0 = (unsigned)(p1 >> 32) ^ c1 ^ k0, y1 = (unsigned)p1;
+        unsigned y2 = (unsigned)(p0 >> 32) ^ c3 ^ k1, y3 = (unsigned)p0;
+        c0 = y0; c1 = y1; c2 = y2; c3 = y3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    o[0] = c0; o[1] = c1; o[2] = c2; o[3] = c3;
+}
+
+// Box-Muller (cosine branch) from two 32-bit words mapped to (0, 1)
+__device__ __forceinline__ double box_muller(unsigned a, unsigned b) {
+    const double s = 2.3283064365386963e-10;   // 2^-32
+    double u1 = ((double)a + 0.5) * s, u2 = ((double)b + 0.5) * s;
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// one thread per (box, mode): OU update of the 3 complex components (init: draw from
+// the stationary distribution, counter word 3 = 1).  The box's step counter is advanced
+// afterwards by k_ou_bump.
+__global__ void k_ou_update(int nbox, const BoxDesc* __restrict__ boxes, OuDev ou, int init) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbox * ou.nmode) return;
+    const int b = t / ou.nmode, m = t - b * ou.nmode;
+    const unsigned step = ou.step[b];
+    double xr[3], xi[3];
+    for (int d = 0; d < 3; d++) {
+        unsigned o[4];
+        philox4x32(step, (unsigned)m, (unsigned)d, init ? 1u : 0u, ou.seed_lo, ou.seed_hi ^ (unsigned)b, o);
+        xr[d] = box_muller(o[0], o[1]);
+        xi[d] = box_muller(o[2], o[3]);
+    }
+    if (ou.solenoidal) {   // project onto the plane normal to the physical k = 2 pi n_d / L_d
+        const int* n = ou.n + 3 * m;
+        const BoxDesc& B = boxes[b];
+        double k[3];
+        for (int d = 0; d < 3; d++) k[d] = (double)n[d] / ((double)B.n[d] * B.dx);
+        double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        double dr = k[0] * xr[0] + k[1] * xr[1] + k[2] * xr[2];
+        double di = k[0] * xi[0] + k[1] * xi[1] + k[2] * xi[2];
+        for (int d = 0; d < 3; d++) {
+            xr[d] = xr[d] - k[d] * dr / k2;
+            xi[d] = xi[d] - k[d] * di / k2;
+        }
+    }
+    double* ar = ou.are + ((size_t)b * ou.nmode + m) * 3;
+    double* ai = ou.aim + ((size_t)b * ou.nmode + m) * 3;
+    if (init) {
+        for (int d = 0; d < 3; d++) { ar[d] = ou.sigma * xr[d]; ai[d] = ou.sigma * xi[d]; }
+    } else {
+        const double f = exp(-ou.dt[b] / ou.t_corr);
+        const double g = ou.sigma * sqrt(1.0 - f * f);
+        for (int d = 0; d < 3; d++) {
+            ar[d] = f * ar[d] + g * xr[d];
+            ai[d] = f * ai[d] + g * xi[d];
+        }
+    }
+}
+
+__global__ void k_ou_bump(int nbox, unsigned int* step) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nbox) step[b] = step[b] + 1u;
+}
+
+// a(x) of cell (i, j, k) of box b (interior indices from 1)
+__device__ __forceinline__ void ou_field(const OuDev& ou, const BoxDesc& B, int b, int i, int j, int k, double* a) {
+    const int W = 2 * ou.K + 1;
+    const double* tx = ou.tw + ou.twoff[b];
+    const double* ty = tx + (size_t)2 * W * B.n[0];
+    const double* tz = ty + (size_t)2 * W * B.n[1];
+    const double* ar = ou.are + (size_t)b * ou.nmode * 3;
+    const double* ai = ou.aim + (size_t)b * ou.nmode * 3;
+    a[0] = a[1] = a[2] = 0.0;
+    for (int m = 0; m < ou.nmode; m++) {
+        const int* n = ou.n + 3 * m;
+        const double* ex = tx + 2 * ((size_t)(n[0] + ou.K) * B.n[0] + (i - 1));
+        const double* ey = ty + 2 * ((size_t)(n[1] + ou.K) * B.n[1] + (j - 1));
+        const double* ez = tz + 2 * ((size_t)(n[2] + ou.K) * B.n[2] + (k - 1));
+        double pr = ex[0] * ey[0] - ex[1] * ey[1], pi = ex[0] * ey[1] + ex[1] * ey[0];
+        double er = pr * ez[0] - pi * ez[1], ei = pr * ez[1] + pi * ez[0];
+        for (int d = 0; d < 3; d++) a[d] = a[d] + 2.0 * (ar[3 * m + d] * er - ai[3 * m + d] * ei);
+    }
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void k_ou_pass(const BoxDesc* __restrict__ boxes, Plan plan, OuApplyArgs<real> a) {
+    __shared__ double sh[FORCE_NSUM][256];
+    int b, i, j, k;
+    const bool act = plan_item_box(plan, b, i, j, k);
+    double v[FORCE_NSUM] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double cfl = 0.0;
+    if (act) {
+        const BoxDesc& B = boxes[b];
+        const long long c = cell_index(B, i, j, k);
+        double f[3];
+        ou_field(a.ou, B, b, i, j, k, f);
+        const double rho = (double)a.q[0][c];
+        if (a.pass == 0) {
+            v[0] = rho * f[0];
+            v[1] = rho * f[1];
+            v[2] = rho * f[2];
+        } else {
+            const double dt = a.ou.dt[b];
+            const double* mean = a.mean + 3 * b;
+            for (int d = 0; d < 3; d++) a.q[1 + d][c] = (real)((double)a.q[1 + d][c] + dt * (rho * f[d] - mean[d]));
+            if (a.cmax != nullptr) {
+                double q7[7];
+                for (int g = 0; g < 7; g++) q7[g] = (double)a.q[g][c];
+                cfl = cell_fast_speed(q7, a.cs);
+            }
+        }
+    }
+    if (a.pass == 0) {
+        block_sum5(sh, v);
+        if (threadIdx.x == 0) {
+            const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+            for (int s = 0; s < 3; s++) a.part[(size_t)blk * 3 + s] = sh[s][0];
+        }
+    } else if (a.cmax != nullptr) {
+        atomic_max_nonneg(a.cmax, b, cfl);
+    }
+}
+
+// per box: <rho a> = sum / cells, fixed summation order
+__global__ __launch_bounds__(256) void k_ou_finalize(const BoxDesc* __restrict__ boxes, const PlanBox* __restrict__ plan,
+                                                     const double* part, double* mean) {
+    __shared__ double sh[FORCE_NSUM][256];
+    const int b = blockIdx.x;
+    const PlanBox& pb = plan[b];
+    const long long first = pb.start / 256;
+    const long long ncell = (long long)pb.r[0] * pb.r[1] * pb.r[2];
+    const long long nblk = (ncell + 255) / 256;
+    double v[FORCE_NSUM] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (long long q = threadIdx.x; q < nblk; q += 256)
+        for (int s = 0; s < 3; s++) v[s] = v[s] + part[(size_t)(first + q) * 3 + s];
+    block_sum5(sh, v);
+    if (threadIdx.x == 0)
+        for (int s = 0; s < 3; s++) mean[3 * b + s] = sh[s][0] / (double)ncell;
+}
+
+void launch_ou_update(hipStream_t st, int nbox, const BoxDesc* boxes, const OuDev& ou, int init) {
+    const int n = nbox * ou.nmode;
+    hipLaunchKernelGGL(k_ou_update, dim3((n + 255) / 256), dim3(256), 0, st, nbox, boxes, ou, init);
+    if (!init) hipLaunchKernelGGL(k_ou_bump, dim3((nbox + 255) / 256), dim3(256), 0, st, nbox, ou.step);
+}
+template <class real>
+void launch_ou_pass(const Launch& L, const OuApplyArgs<real>& a) {
+    if (L.plan.total > 0) hipLaunchKernelGGL((k_ou_pass<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, const PlanBox* plan, int nbox, const double* part,
+                        double* mean) {
+    hipLaunchKernelGGL(k_ou_finalize, dim3(nbox), dim3(256), 0, st, boxes, plan, part, mean);
+}
+template void launch_ou_pass<double>(const Launch&, const OuApplyArgs<double>&);
+template void launch_ou_pass<float>(const Launch&, const OuApplyArgs<float>&);
+
 template void launch_force_sums<double>(const Launch&, const ForceSumArgs<double>&);
 template void launch_force_sums<float>(const Launch&, const ForceSumArgs<float>&);
 template void launch_force_apply<double>(const Launch&, const ForceApplyArgs<double>&);

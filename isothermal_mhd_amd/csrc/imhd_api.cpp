@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -73,6 +74,19 @@ struct imhd_ctx {
     unsigned int* force_err = nullptr;
     bool force_set = false;
     double edot = 0.0, dt_turb = 0.0;
+    // OU forcing of the native loop
+    bool ou_on = false;
+    imhd_ou ou_cfg{};
+    std::vector<int> ou_k;        // 3 per mode (host copy)
+    int* ou_n = nullptr;
+    double* ou_are = nullptr;
+    double* ou_aim = nullptr;
+    unsigned int* ou_step = nullptr;
+    double* ou_tw = nullptr;
+    long long* ou_twoff = nullptr;
+    double* ou_part = nullptr;
+    double* ou_mean = nullptr;
+    int ou_K = 0;
     double* dt = nullptr;
     double* t = nullptr;
     double* dtprev = nullptr;
@@ -475,6 +489,46 @@ void force_apply(imhd_ctx* c, int s0, int s1) {
     launch_force_apply<real>(mk(c, c->pl_int), a);
 }
 
+OuDev ou_dev(imhd_ctx* c) {
+    OuDev o{};
+    o.n = c->ou_n;
+    o.are = c->ou_are;
+    o.aim = c->ou_aim;
+    o.step = c->ou_step;
+    o.tw = c->ou_tw;
+    o.twoff = c->ou_twoff;
+    o.nmode = (int)(c->ou_k.size() / 3);
+    o.K = c->ou_K;
+    o.t_corr = c->ou_cfg.t_corr;
+    o.sigma = c->ou_cfg.sigma;
+    o.seed_lo = (unsigned)(c->ou_cfg.seed & 0xffffffffull);
+    o.seed_hi = (unsigned)(c->ou_cfg.seed >> 32);
+    o.solenoidal = c->ou_cfg.solenoidal;
+    o.dt = c->dt;
+    return o;
+}
+
+// OU step on the current (new) state: mode update, <rho a>, kick (+ the CFL speed of the
+// kicked cells when fuse_cmax)
+template <class real>
+void ou_step(imhd_ctx* c, bool fuse_cmax) {
+    ProfScope ps(c, "ou_forcing", c->pl_int.active);
+    OuDev o = ou_dev(c);
+    launch_ou_update(c->st, c->nbox, c->db, o, 0);
+    OuApplyArgs<real> a{};
+    a.ou = o;
+    for (int f = 0; f < 7; f++) a.q[f] = R<real>(c->q[c->cur][f]);
+    a.part = c->ou_part;
+    a.mean = c->ou_mean;
+    a.cs = c->cfg.cs;
+    a.pass = 0;
+    launch_ou_pass<real>(mk(c, c->pl_int), a);
+    launch_ou_finalize(c->st, c->db, c->pl_int.d, c->nbox, c->ou_part, c->ou_mean);
+    a.pass = 1;
+    a.cmax = fuse_cmax ? c->cmax : nullptr;
+    launch_ou_pass<real>(mk(c, c->pl_int), a);
+}
+
 // Drop-in step only: cells outside the VL final-update range take their t^n values, as
 // the reference's restore leaves them (the native loop's ghost fill makes this moot).
 template <class real>
@@ -601,6 +655,7 @@ void native_step(imhd_ctx* c, double t_end) {
     const int qold = c->cur;
     if (is_tvd(c)) {  // driver.f90:35-160 order: solve, passive, new dt, then ghost fill
         tvd_step<real>(c);
+        if (c->ou_on) ou_step<real>(c, false);
         if (c->cfg.npass > 0) passive_fused<real>(c, qold);
         tvd_max_speed_launch<real>(c);
         fill_ghosts<real>(c);
@@ -608,7 +663,8 @@ void native_step(imhd_ctx* c, double t_end) {
         return;
     }
     launch_reset_u64(c->st, c->cmax, (int)stripe_slots(c->nbox));
-    vl_step<real>(c, true);
+    vl_step<real>(c, !c->ou_on);
+    if (c->ou_on) ou_step<real>(c, true);   // the CFL speed is taken after the kick
     if (c->cfg.npass > 0) passive_fused<real>(c, qold);
     fill_ghosts<real>(c);
     launch_new_dt(c->st, c->nbox, c->db, c->cmax, c->smax, c->dvmax, c->cour, t_end, c->t, c->dt, c->dtprev, 1);
@@ -867,6 +923,9 @@ int imhd_destroy(imhd_ctx* c) {
     for (auto p : c->gt) fr(p);
     for (auto p : c->ck) fr(p);
     for (auto p : c->delv) fr(p);
+    for (void* p : {(void*)c->ou_n, (void*)c->ou_are, (void*)c->ou_aim, (void*)c->ou_step, (void*)c->ou_tw,
+                    (void*)c->ou_twoff, (void*)c->ou_part, (void*)c->ou_mean})
+        fr(p);
     for (void* p : {(void*)c->force_part, (void*)c->force_par, (void*)c->force_err, (void*)c->dvmax}) fr(p);
     for (void* p : {(void*)c->smax, (void*)c->cmax, (void*)c->dmax, (void*)c->dt, (void*)c->t, (void*)c->dtprev, (void*)c->cour,
                     (void*)c->diag, (void*)c->floor_count, (void*)c->stage, (void*)c->db})
@@ -935,6 +994,91 @@ int imhd_step(imhd_ctx* c, double dt, int drive_next) {
         }
     }
     return prof_collect(c);
+}
+
+int imhd_set_ou_forcing(imhd_ctx* c, const imhd_ou* ou) {
+    if (!c) return IMHD_EINVAL;
+    HIPCK(c, hipStreamSynchronize(c->st));
+    for (auto& g : c->gexec) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }   // the step changes shape
+    if (!ou || !(ou->sigma > 0.0)) { c->ou_on = false; return IMHD_OK; }
+    if (!(ou->t_corr > 0.0) || !(ou->kmax >= ou->kmin) || !(ou->kmax >= 1.0) || ou->kmax > 16.0)
+        FAIL(c, IMHD_EINVAL, "OU forcing needs t_corr > 0 and 1 <= kmax <= 16, kmin <= kmax");
+    const int K = (int)floor(ou->kmax + 1e-12);
+    std::vector<int> kv;
+    for (int nz = -K; nz <= K; nz++)
+        for (int ny = -K; ny <= K; ny++)
+            for (int nx = -K; nx <= K; nx++) {
+                bool half = nx > 0 || (nx == 0 && ny > 0) || (nx == 0 && ny == 0 && nz > 0);
+                double mag = sqrt((double)(nx * nx + ny * ny + nz * nz));
+                if (half && mag >= ou->kmin - 1e-12 && mag <= ou->kmax + 1e-12) {
+                    kv.push_back(nx); kv.push_back(ny); kv.push_back(nz);
+                }
+            }
+    const int nm = (int)kv.size() / 3;
+    if (nm == 0 || nm > OU_MAXMODE) FAIL(c, IMHD_EINVAL, "OU forcing: %d modes in the shell (1..%d allowed)", nm, OU_MAXMODE);
+    for (int b = 0; b < c->nbox; b++)
+        for (int d = 0; d < 3; d++)
+            if (2 * K >= c->hb[b].n[d]) FAIL(c, IMHD_EINVAL, "OU forcing: kmax must be below n/2 on every axis");
+    auto fr = [](void* p) { if (p) (void)hipFree(p); };
+    for (void* p : {(void*)c->ou_n, (void*)c->ou_are, (void*)c->ou_aim, (void*)c->ou_step, (void*)c->ou_tw,
+                    (void*)c->ou_twoff, (void*)c->ou_part, (void*)c->ou_mean})
+        fr(p);
+    // per-axis twiddles e^{i 2 pi n (i - 1/2) / N}, n in [-K, K], interior i (host libm)
+    std::vector<double> tw;
+    std::vector<long long> off;
+    const double two_pi = 6.283185307179586;
+    for (int b = 0; b < c->nbox; b++) {
+        off.push_back((long long)tw.size());
+        for (int d = 0; d < 3; d++) {
+            const int N = c->hb[b].n[d];
+            for (int n = -K; n <= K; n++)
+                for (int i = 1; i <= N; i++) {
+                    double th = two_pi * (double)n * ((double)i - 0.5) / (double)N;
+                    tw.push_back(cos(th));
+                    tw.push_back(sin(th));
+                }
+        }
+    }
+    HIPCK(c, hipMalloc(&c->ou_n, sizeof(int) * kv.size()));
+    HIPCK(c, hipMemcpy(c->ou_n, kv.data(), sizeof(int) * kv.size(), hipMemcpyHostToDevice));
+    HIPCK(c, hipMalloc(&c->ou_are, 8 * (size_t)c->nbox * nm * 3));
+    HIPCK(c, hipMalloc(&c->ou_aim, 8 * (size_t)c->nbox * nm * 3));
+    HIPCK(c, hipMalloc(&c->ou_step, 4 * (size_t)c->nbox));
+    HIPCK(c, hipMemset(c->ou_step, 0, 4 * (size_t)c->nbox));
+    HIPCK(c, hipMalloc(&c->ou_tw, 8 * tw.size()));
+    HIPCK(c, hipMemcpy(c->ou_tw, tw.data(), 8 * tw.size(), hipMemcpyHostToDevice));
+    HIPCK(c, hipMalloc(&c->ou_twoff, 8 * off.size()));
+    HIPCK(c, hipMemcpy(c->ou_twoff, off.data(), 8 * off.size(), hipMemcpyHostToDevice));
+    HIPCK(c, hipMalloc(&c->ou_part, 8 * 3 * (size_t)(c->pl_int.total / 256 + 1)));
+    HIPCK(c, hipMalloc(&c->ou_mean, 8 * 3 * (size_t)c->nbox));
+    c->ou_cfg = *ou;
+    c->ou_k = kv;
+    c->ou_K = K;
+    c->ou_on = true;
+    launch_ou_update(c->st, c->nbox, c->db, ou_dev(c), 1);   // stationary start
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipStreamSynchronize(c->st));
+    return IMHD_OK;
+}
+
+int imhd_get_ou_modes(imhd_ctx* c, int box, int* nmode, int* kvec, double* amp) {
+    if (!c || !nmode || box < 0 || box >= c->nbox) return IMHD_EINVAL;
+    const int nm = c->ou_on ? (int)c->ou_k.size() / 3 : 0;
+    *nmode = nm;
+    if (nm == 0) return IMHD_OK;
+    if (kvec) memcpy(kvec, c->ou_k.data(), sizeof(int) * 3 * nm);
+    if (amp) {
+        std::vector<double> re(3 * nm), im(3 * nm);
+        HIPCK(c, hipStreamSynchronize(c->st));
+        HIPCK(c, hipMemcpy(re.data(), c->ou_are + (size_t)box * nm * 3, 8 * 3 * nm, hipMemcpyDeviceToHost));
+        HIPCK(c, hipMemcpy(im.data(), c->ou_aim + (size_t)box * nm * 3, 8 * 3 * nm, hipMemcpyDeviceToHost));
+        for (int m = 0; m < nm; m++)
+            for (int d = 0; d < 3; d++) {
+                amp[6 * m + d] = re[3 * m + d];
+                amp[6 * m + 3 + d] = im[3 * m + d];
+            }
+    }
+    return IMHD_OK;
 }
 
 int imhd_set_force(imhd_ctx* c, int box, const double* delv, double edot, double dt_turb) {

@@ -154,6 +154,22 @@ class VanLeerSolver:
         self._check(self.lib.imhd_floor_count(self._c, ctypes.byref(n)))
         return n.value
 
+    def set_ou_forcing(self, kmin=1.0, kmax=2.0, t_corr=0.5, sigma=1.0, seed=3003, solenoidal=True):
+        """Ornstein-Uhlenbeck forcing of the native loop (see imhd.h); sigma <= 0 switches
+        it off."""
+        o = L.ImhdOU(kmin, kmax, t_corr, sigma, seed, int(bool(solenoidal)), 0)
+        self._check(self.lib.imhd_set_ou_forcing(self._c, ctypes.byref(o)))
+
+    def ou_modes(self, box=0):
+        """(integer wave vectors (nmode, 3), complex amplitudes (nmode, 3))"""
+        n = ctypes.c_int()
+        self._check(self.lib.imhd_get_ou_modes(self._c, box, ctypes.byref(n), None, None))
+        k = np.zeros((n.value, 3), dtype=np.int32)
+        a = np.zeros((n.value, 6))
+        if n.value:
+            self._check(self.lib.imhd_get_ou_modes(self._c, box, ctypes.byref(n), _p(k), _p(a)))
+        return k, a[:, :3] + 1j * a[:, 3:]
+
     # -- native loop ----------------------------------------------------------------
     def set_time(self, t=None, dt=None):
         nb = len(self.boxes)

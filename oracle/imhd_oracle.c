@@ -15,6 +15,7 @@
 #include "imhd_oracle.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -44,6 +45,7 @@ struct orc_ctx {
     double *delv, *rho0;
     double edot, dt_turb;
     double quad1, quad2, mavg[3];
+    struct orc_ou_s *ou;
 };
 
 /* ----------------------------------------------------------------------------------- */
@@ -769,6 +771,10 @@ static void vl_step(orc_ctx *c, double dt) {
 
 #include "imhd_oracle_tvd.inc"
 
+#define orc_ou struct orc_ou_s
+#include "imhd_oracle_ou.inc"
+#undef orc_ou
+
 /* ----------------------------------------------------------------------------------- */
 /* forcing application of the reference's turbulence driver                            */
 
@@ -1089,6 +1095,7 @@ void orc_destroy(orc_ctx *c) {
     free(c->q); free(c->qw); free(c->bf); free(c->bfw); free(c->ec); free(c->Fs); free(c->emf); free(c->Fbuf);
     free(c->qp); free(c->qpw); free(c->vel); free(c->vtavg); free(c->Fp);
     free(c->delv); free(c->rho0);
+    if (c->ou) { for (int d = 0; d < 3; d++) free(c->ou->tw[d]); free(c->ou); }
     free(c);
 }
 

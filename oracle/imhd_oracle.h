@@ -48,6 +48,16 @@ void orc_step(orc_ctx *c, double dt);
  * orc_step_drive = vanleer_solve_3d(dt, drive_next=.TRUE.): get_rho_turb before the step,
  * add_random_force after it; returns -1 on the reference's bad-normalisation STOP. */
 void orc_set_force(orc_ctx *c, const double *delv, double edot, double dt_turb);
+/* Ornstein-Uhlenbeck forcing of the library's native loop (imhd_set_ou_forcing):
+ * setup returns the number of modes; orc_ou_step updates the amplitudes with dt and kicks
+ * the current state; orc_ou_modes_step only updates the amplitudes. */
+int orc_ou_setup(orc_ctx *c, double kmin, double kmax, double t_corr, double sigma,
+                 unsigned long long seed, int solenoidal, int box);
+void orc_ou_step(orc_ctx *c, double dt);
+void orc_ou_modes_step(orc_ctx *c, double dt);
+void orc_ou_get(const orc_ctx *c, int *kvec, double *amp);
+void orc_ou_field(const orc_ctx *c, double *out);
+void orc_philox4x32(const unsigned int *ctr, const unsigned int *key, unsigned int *out);
 int orc_step_drive(orc_ctx *c, double dt);
 double orc_max_speed(orc_ctx *c);
 double orc_smax(const orc_ctx *c);

@@ -66,6 +66,14 @@ def oracle_lib():
         L.orc_recon_plane.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_ou_setup.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_ulonglong, ctypes.c_int, ctypes.c_int]
+        L.orc_ou_setup.restype = ctypes.c_int
+        L.orc_ou_step.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_ou_modes_step.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_ou_get.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_ou_field.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_philox4x32.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_set_force.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         L.orc_step_drive.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_step_drive.restype = ctypes.c_int
@@ -158,6 +166,33 @@ def _oracle_force_methods():
         assert rc == 0, "bad forcing normalisation"
     Oracle.set_force = set_force
     Oracle.step_drive = step_drive
+
+    def ou_setup(self, kmin=1.0, kmax=2.0, t_corr=0.5, sigma=1.0, seed=3003, solenoidal=True, box=0):
+        self.ou_nmode = self.L.orc_ou_setup(self.c, kmin, kmax, t_corr, sigma, seed, int(solenoidal), box)
+        return self.ou_nmode
+
+    def ou_step(self, dt):
+        self.L.orc_ou_step(self.c, dt)
+
+    def ou_modes_step(self, dt):
+        self.L.orc_ou_modes_step(self.c, dt)
+
+    def ou_modes(self):
+        k = np.zeros((self.ou_nmode, 3), dtype=np.int32)
+        a = np.zeros((self.ou_nmode, 6))
+        self.L.orc_ou_get(self.c, P(k), P(a))
+        return k, a[:, :3] + 1j * a[:, 3:]
+
+    def ou_field(self):
+        nx, ny, nz = self.dims[:3]
+        out = np.zeros((nx, ny, nz, 3), order="F")
+        self.L.orc_ou_field(self.c, P(out))
+        return out
+    Oracle.ou_setup = ou_setup
+    Oracle.ou_step = ou_step
+    Oracle.ou_modes_step = ou_modes_step
+    Oracle.ou_modes = ou_modes
+    Oracle.ou_field = ou_field
 
 
 _oracle_force_methods()
