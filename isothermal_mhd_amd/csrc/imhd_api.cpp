@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -24,6 +25,7 @@ struct PlanDev {
     PlanBox* d = nullptr;
     long long total = 0;
     long long active = 0;  // work items (cells / faces) over all boxes
+    bool column = false;   // (i, j, z-segment) enumeration for the z-marching kernels
     std::vector<PlanBox> h;
 };
 
@@ -100,6 +102,9 @@ struct imhd_ctx {
     PlanDev pl_gband, pl_pgband;
     PlanDev pl_ghost, pl_pghost, pl_int, pl_vel, pl_ppflux[3], pl_ppupd, pl_pcflux[3], pl_pfupd;
     PlanDev pl_teig_p[3], pl_tflux_p[3], pl_teig_c[3], pl_tflux_c[3];
+    int zseg = 4;    // z-segment length of the column (z-marching) kernels
+    bool fuse3 = true;   // the three directional flux sweeps of a stage in one launch
+    PlanDev pl_pflux3, pl_cflux3;
     PlanDev pl_tcorner_p, pl_tcorner_c, pl_tcenter, pl_thalf, pl_tcfl;
     int ccb_shift = 1;
     // native loop
@@ -127,10 +132,14 @@ struct imhd_ctx {
 
 namespace {
 
-int build_plan(imhd_ctx* c, PlanDev& p, int which, int dir) {
+// seg > 0: a column plan of the same cell range for the z-marching kernels, segments of
+// seg cells along z
+int build_plan(imhd_ctx* c, PlanDev& p, int which, int dir, int seg = 0) {
     // which: ranges as functions of (n, nb) per axis; dir = sweep direction for face plans
     p.h.clear();
     p.active = 0;
+    p.column = seg > 0;
+    if (p.d) { (void)hipFree(p.d); p.d = nullptr; }
     long long run = 0;
     const int e = c->cfg.recon == IMHD_PLM_PRIM ? 0 : 1;  // PPM needs one more cell (vanleer_mod.f90:379-383)
     for (int bx = 0; bx < c->nbox; bx++) {
@@ -169,6 +178,10 @@ int build_plan(imhd_ctx* c, PlanDev& p, int which, int dir) {
                 case 13:  // passive final update (offset 2)
                     lo = 1; hi = n; break;
                 // K98 TVD integrator (isotvd_mod.f90); offset 0 predictor, 2 corrector
+                case 25:  // union of the three predictor flux ranges (fused sweeps)
+                    lo = 1 - nb; hi = n + nb; break;
+                case 26:  // union of the three corrector flux ranges (their transverse range)
+                    lo = 3 - nb; hi = n + nb - 2; break;
                 case 16:  // predictor Roe average / HLLE faces: pencil cells [1-nb, n+nb]
                     lo = 1 - nb; hi = normal ? n + nb - 1 : n + nb; break;
                 case 17:  // predictor TVD fluxes and face EMFs (isotvd_mod.f90:2413, 446)
@@ -199,9 +212,16 @@ int build_plan(imhd_ctx* c, PlanDev& p, int which, int dir) {
             pb.r[0] = (int)cnt;
             pb.r[1] = pb.r[2] = 1;
         }
+        if (seg > 0) {
+            pb.zlo = pb.lo[2];
+            pb.zn = pb.r[2];
+            pb.seg = seg;
+            pb.lo[2] = 0;
+            pb.r[2] = (pb.zn + seg - 1) / seg;
+        }
         pb.start = run;
         long long cnt = (long long)pb.r[0] * pb.r[1] * pb.r[2];
-        p.active += cnt;
+        p.active += seg > 0 ? (long long)pb.r[0] * pb.r[1] * pb.zn : cnt;   // cells
         run += (cnt + 255) / 256 * 256;
         p.h.push_back(pb);
     }
@@ -218,6 +238,7 @@ Launch mk(imhd_ctx* c, const PlanDev& p) {
     L.plan.nbox = c->nbox;
     L.plan.total = p.total;
     L.stream = c->st;
+    L.column = p.column;
     return L;
 }
 
@@ -271,6 +292,18 @@ SweepArgs<real> sweep_args(imhd_ctx* c, int s, int d) {
 
 template <class real, int RS>
 void sweeps(imhd_ctx* c, int s, bool corrector) {
+    if (c->fuse3) {   // the three directions in one launch over their union range
+        const PlanDev& p = corrector ? c->pl_cflux3 : c->pl_pflux3;
+        ProfScope ps(c, corrector ? "flux_corr" : "flux_pred", p.active);
+        Sweep3Args<real> a{};
+        for (int d = 0; d < 3; d++) a.d[d] = sweep_args<real>(c, s, d);
+        const int nb = c->hb[0].nb, e = c->cfg.recon == IMHD_PLM_PRIM ? 0 : 1;
+        a.lo_off = corrector ? 4 - nb + e : 1 - nb;
+        a.hi_off = corrector ? nb - 4 - e : nb - 1;
+        const int kind = !corrector ? 0 : (c->cfg.recon == IMHD_PLM_PRIM ? 1 : (c->cfg.recon == IMHD_PPM_PRIM ? 2 : 3));
+        launch_flux3<real, RS>(mk(c, p), a, kind);
+        return;
+    }
     static const char* nm[2][3] = {{"flux_pred_x", "flux_pred_y", "flux_pred_z"},
                                    {"flux_corr_x", "flux_corr_y", "flux_corr_z"}};
     for (int d = 0; d < 3; d++) {
@@ -800,6 +833,8 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     if (c->cfg.cs <= 0.0) { int r = bad("cs must be positive"); *out = c; return r; }
     c->prec = g.precision;
     c->esz = g.precision == 64 ? 8 : 4;
+    if (const char* e = getenv("IMHD_ZSEG")) c->zseg = std::max(0, atoi(e));   // 0: per-cell kernels
+    if (const char* e = getenv("IMHD_FUSE3")) c->fuse3 = atoi(e) != 0;      // 0: one launch per direction
     long long run = 0;
     for (int bx = 0; bx < nbox; bx++) {
         const imhd_config& k = cfg[bx];
@@ -873,10 +908,12 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
         if ((rc = build_plan(c, c->pl_ppflux[d], 10, d))) return rc;
         if ((rc = build_plan(c, c->pl_pcflux[d], 12, d))) return rc;
     }
+    if ((rc = build_plan(c, c->pl_pflux3, 25, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_cflux3, 26, -1))) return rc;
     if ((rc = build_plan(c, c->pl_pcorner, 1, -1))) return rc;
-    if ((rc = build_plan(c, c->pl_pupd, 2, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_pupd, 2, -1, c->zseg))) return rc;
     if ((rc = build_plan(c, c->pl_ccorner, 4, -1))) return rc;
-    if ((rc = build_plan(c, c->pl_cupd, 5, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_cupd, 5, -1, c->zseg))) return rc;
     if ((rc = build_plan(c, c->pl_ghost, 6, -1))) return rc;
     if ((rc = build_plan(c, c->pl_gband, 14, -1))) return rc;
     if ((rc = build_plan(c, c->pl_pgband, 15, -1))) return rc;
@@ -938,7 +975,7 @@ int imhd_destroy(imhd_ctx* c) {
                       &c->pl_teig_p[2], &c->pl_tflux_p[0], &c->pl_tflux_p[1], &c->pl_tflux_p[2],
                       &c->pl_teig_c[0], &c->pl_teig_c[1], &c->pl_teig_c[2], &c->pl_tflux_c[0],
                       &c->pl_tflux_c[1], &c->pl_tflux_c[2], &c->pl_tcorner_p, &c->pl_tcorner_c,
-                      &c->pl_tcenter, &c->pl_thalf, &c->pl_tcfl};
+                      &c->pl_tcenter, &c->pl_thalf, &c->pl_tcfl, &c->pl_pflux3, &c->pl_cflux3};
     for (auto p : pls) fr(p->d);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;

@@ -29,11 +29,13 @@ struct BoxDesc {
 };
 
 // One box's share of a launch: cells lo .. lo + r - 1 (box index space), work items
-// start .. start + r0*r1*r2 - 1 of the launch.
+// start .. start + r0*r1*r2 - 1 of the launch.  Column plans (z-marching kernels) enumerate
+// (i, j, segment) instead: r[2] segments of seg cells each cover z cells zlo .. zlo + zn - 1.
 struct PlanBox {
     int lo[3];
     int r[3];
     long long start;
+    int zlo, zn, seg, _pad;
 };
 
 struct Plan {

@@ -55,8 +55,19 @@ struct Launch {
     const BoxDesc* boxes;
     Plan plan;
     hipStream_t stream;
+    bool column = false;   // the plan is a column plan (z-marching kernel variant)
     unsigned blocks() const { return (unsigned)((plan.total + 255) / 256); }
 };
+
+// All three directional sweeps of a stage in one launch (see k_flux3): a cell computes
+// its d-face when its index along d lies in [lo_off, n_d + hi_off].
+template <class real>
+struct Sweep3Args {
+    SweepArgs<real> d[3];
+    int lo_off, hi_off;
+};
+template <class real, int RS>
+void launch_flux3(const Launch& L, const Sweep3Args<real>& a, int kind);
 
 template <class real, int RS>
 void launch_flux_predictor(const Launch& L, const SweepArgs<real>& a);

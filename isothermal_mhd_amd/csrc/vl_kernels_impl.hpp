@@ -30,27 +30,30 @@ __device__ __forceinline__ void store_flux(const SweepArgs<real>& a, long long c
 // First-order predictor fluxes: interface states are the adjacent cell values with the
 // face-centred normal field (vanleer_mod.f90:443-467).
 template <class real, int RS>
+__device__ __forceinline__ double flux_pred_face(const SweepArgs<real>& a, long long c, long long s) {
+    real uL[7], uR[7], F[7];
+#pragma unroll
+    for (int r = 0; r < 7; r++) {
+        if (r == 4) continue;
+        uL[r] = a.w[r][c];
+        uR[r] = a.w[r][c + s];
+    }
+    uL[4] = (real)a.bn[c];
+    uR[4] = uL[4];
+    real smax = real(0);
+    riemann<RS>(uL, uR, (real)a.cs, F, smax);
+    store_flux(a, c, F);
+    return (double)smax;
+}
+
+template <class real, int RS>
 __global__ __launch_bounds__(256) void k_flux_pred(const BoxDesc* __restrict__ boxes, Plan plan, SweepArgs<real> a) {
     int b, i, j, k;
     bool act = plan_item_box(plan, b, i, j, k);
     double sm = 0.0;
     if (act) {
         const BoxDesc& B = boxes[b];
-        long long c = cell_index(B, i, j, k);
-        long long s = dir_stride(B, a.dir);
-        real uL[7], uR[7], F[7];
-#pragma unroll
-        for (int r = 0; r < 7; r++) {
-            if (r == 4) continue;
-            uL[r] = a.w[r][c];
-            uR[r] = a.w[r][c + s];
-        }
-        uL[4] = (real)a.bn[c];
-        uR[4] = uL[4];
-        real smax = real(0);
-        riemann<RS>(uL, uR, (real)a.cs, F, smax);
-        store_flux(a, c, F);
-        sm = (double)smax;
+        sm = flux_pred_face<real, RS>(a, cell_index(B, i, j, k), dir_stride(B, a.dir));
     }
     atomic_max_nonneg(a.smax, b, sm);
 }
@@ -59,58 +62,61 @@ __global__ __launch_bounds__(256) void k_flux_pred(const BoxDesc* __restrict__ b
 // sign(dC) min(beta|dL|, beta|dR|, |dC|)  -- no zeroing at extrema, as in the
 // reference (vanleer_mod.f90:1160-1215, 1467-1481) -- then the Riemann solver.
 template <class real, int RS>
+__device__ __forceinline__ double flux_corr_face(const SweepArgs<real>& a, long long c, long long s) {
+    real W[4][6];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        long long ct = c + (long long)(t - 1) * s;
+        real rho = a.w[0][ct];
+        real irho = fdiv(real(1), rho);
+        W[t][0] = rho;
+        W[t][1] = a.w[1][ct] * irho;
+        W[t][2] = a.w[2][ct] * irho;
+        W[t][3] = a.w[3][ct] * irho;
+        W[t][4] = a.w[5][ct];
+        W[t][5] = a.w[6][ct];
+    }
+    const real beta = (real)a.mc_beta;
+    real d1[6], d2[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        real dL = W[1][m] - W[0][m], dR = W[2][m] - W[1][m], dC = real(0.5) * (dL + dR);
+        d1[m] = sgn1(dC) * min3(beta * fabs(dL), beta * fabs(dR), fabs(dC));
+        dL = W[2][m] - W[1][m]; dR = W[3][m] - W[2][m]; dC = real(0.5) * (dL + dR);
+        d2[m] = sgn1(dC) * min3(beta * fabs(dL), beta * fabs(dR), fabs(dC));
+    }
+    real uL[7], uR[7], F[7];
+    real wb = (real)a.bn[c];
+    uL[0] = W[1][0] + real(0.5) * d1[0];
+    uL[1] = W[1][1] + real(0.5) * d1[1];
+    uL[2] = W[1][2] + real(0.5) * d1[2];
+    uL[3] = W[1][3] + real(0.5) * d1[3];
+    uL[4] = wb;
+    uL[5] = W[1][4] + real(0.5) * d1[4];
+    uL[6] = W[1][5] + real(0.5) * d1[5];
+    uR[0] = W[2][0] - real(0.5) * d2[0];
+    uR[1] = W[2][1] - real(0.5) * d2[1];
+    uR[2] = W[2][2] - real(0.5) * d2[2];
+    uR[3] = W[2][3] - real(0.5) * d2[3];
+    uR[4] = wb;
+    uR[5] = W[2][4] - real(0.5) * d2[4];
+    uR[6] = W[2][5] - real(0.5) * d2[5];
+    uL[1] = uL[1] * uL[0]; uL[2] = uL[2] * uL[0]; uL[3] = uL[3] * uL[0];
+    uR[1] = uR[1] * uR[0]; uR[2] = uR[2] * uR[0]; uR[3] = uR[3] * uR[0];
+    real smax = real(0);
+    riemann<RS>(uL, uR, (real)a.cs, F, smax);
+    store_flux(a, c, F);
+    return (double)smax;
+}
+
+template <class real, int RS>
 __global__ __launch_bounds__(256) void k_flux_corr(const BoxDesc* __restrict__ boxes, Plan plan, SweepArgs<real> a) {
     int b, i, j, k;
     bool act = plan_item_box(plan, b, i, j, k);
     double sm = 0.0;
     if (act) {
         const BoxDesc& B = boxes[b];
-        long long c = cell_index(B, i, j, k);
-        long long s = dir_stride(B, a.dir);
-        real W[4][6];
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            long long ct = c + (long long)(t - 1) * s;
-            real rho = a.w[0][ct];
-            real irho = fdiv(real(1), rho);
-            W[t][0] = rho;
-            W[t][1] = a.w[1][ct] * irho;
-            W[t][2] = a.w[2][ct] * irho;
-            W[t][3] = a.w[3][ct] * irho;
-            W[t][4] = a.w[5][ct];
-            W[t][5] = a.w[6][ct];
-        }
-        const real beta = (real)a.mc_beta;
-        real d1[6], d2[6];
-#pragma unroll
-        for (int m = 0; m < 6; m++) {
-            real dL = W[1][m] - W[0][m], dR = W[2][m] - W[1][m], dC = real(0.5) * (dL + dR);
-            d1[m] = sgn1(dC) * min3(beta * fabs(dL), beta * fabs(dR), fabs(dC));
-            dL = W[2][m] - W[1][m]; dR = W[3][m] - W[2][m]; dC = real(0.5) * (dL + dR);
-            d2[m] = sgn1(dC) * min3(beta * fabs(dL), beta * fabs(dR), fabs(dC));
-        }
-        real uL[7], uR[7], F[7];
-        real wb = (real)a.bn[c];
-        uL[0] = W[1][0] + real(0.5) * d1[0];
-        uL[1] = W[1][1] + real(0.5) * d1[1];
-        uL[2] = W[1][2] + real(0.5) * d1[2];
-        uL[3] = W[1][3] + real(0.5) * d1[3];
-        uL[4] = wb;
-        uL[5] = W[1][4] + real(0.5) * d1[4];
-        uL[6] = W[1][5] + real(0.5) * d1[5];
-        uR[0] = W[2][0] - real(0.5) * d2[0];
-        uR[1] = W[2][1] - real(0.5) * d2[1];
-        uR[2] = W[2][2] - real(0.5) * d2[2];
-        uR[3] = W[2][3] - real(0.5) * d2[3];
-        uR[4] = wb;
-        uR[5] = W[2][4] - real(0.5) * d2[4];
-        uR[6] = W[2][5] - real(0.5) * d2[5];
-        uL[1] = uL[1] * uL[0]; uL[2] = uL[2] * uL[0]; uL[3] = uL[3] * uL[0];
-        uR[1] = uR[1] * uR[0]; uR[2] = uR[2] * uR[0]; uR[3] = uR[3] * uR[0];
-        real smax = real(0);
-        riemann<RS>(uL, uR, (real)a.cs, F, smax);
-        store_flux(a, c, F);
-        sm = (double)smax;
+        sm = flux_corr_face<real, RS>(a, cell_index(B, i, j, k), dir_stride(B, a.dir));
     }
     atomic_max_nonneg(a.smax, b, sm);
 }
@@ -118,36 +124,68 @@ __global__ __launch_bounds__(256) void k_flux_corr(const BoxDesc* __restrict__ b
 // Corrector fluxes with PPM reconstruction (characteristic: the reference default;
 // primitive: the compiled branch), six cells c-2 .. c+3 per interface.
 template <class real, int RS, bool CHAR>
+__device__ __forceinline__ double flux_corr_ppm_face(const SweepArgs<real>& a, long long c, long long s) {
+    real W[6][6], wbc[6];
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        long long ct = c + (long long)(t - 2) * s;
+        real rho = a.w[0][ct];
+        real irho = fdiv(real(1), rho);
+        W[t][0] = rho;
+        W[t][1] = a.w[1][ct] * irho;
+        W[t][2] = a.w[2][ct] * irho;
+        W[t][3] = a.w[3][ct] * irho;
+        W[t][4] = a.w[5][ct];
+        W[t][5] = a.w[6][ct];
+        wbc[t] = (real)a.bn[ct];
+    }
+    real uL[7], uR[7], F[7];
+    ppm_states<real, CHAR>(W, wbc, (real)a.cs, (real)a.mc_beta, uL, uR);
+    real smax = real(0);
+    riemann<RS>(uL, uR, (real)a.cs, F, smax);
+    store_flux(a, c, F);
+    return (double)smax;
+}
+
+template <class real, int RS, bool CHAR>
 __global__ __launch_bounds__(256) void k_flux_corr_ppm(const BoxDesc* __restrict__ boxes, Plan plan, SweepArgs<real> a) {
     int b, i, j, k;
     bool act = plan_item_box(plan, b, i, j, k);
     double sm = 0.0;
     if (act) {
         const BoxDesc& B = boxes[b];
-        long long c = cell_index(B, i, j, k);
-        long long s = dir_stride(B, a.dir);
-        real W[6][6], wbc[6];
-#pragma unroll
-        for (int t = 0; t < 6; t++) {
-            long long ct = c + (long long)(t - 2) * s;
-            real rho = a.w[0][ct];
-            real irho = fdiv(real(1), rho);
-            W[t][0] = rho;
-            W[t][1] = a.w[1][ct] * irho;
-            W[t][2] = a.w[2][ct] * irho;
-            W[t][3] = a.w[3][ct] * irho;
-            W[t][4] = a.w[5][ct];
-            W[t][5] = a.w[6][ct];
-            wbc[t] = (real)a.bn[ct];
-        }
-        real uL[7], uR[7], F[7];
-        ppm_states<real, CHAR>(W, wbc, (real)a.cs, (real)a.mc_beta, uL, uR);
-        real smax = real(0);
-        riemann<RS>(uL, uR, (real)a.cs, F, smax);
-        store_flux(a, c, F);
-        sm = (double)smax;
+        sm = flux_corr_ppm_face<real, RS, CHAR>(a, cell_index(B, i, j, k), dir_stride(B, a.dir));
     }
     atomic_max_nonneg(a.smax, b, sm);
+}
+
+// All three directional sweeps of a stage in one launch: the plan is the union of the
+// three directions' ranges (their common transverse range), and a cell computes its
+// d-face only when its index along d is inside direction d's normal range
+// [lo_off, n_d + hi_off].  Same per-face arithmetic as the single-direction kernels; the
+// cell's own state is fetched from HBM once for all three faces.
+template <class real, int RS, int KIND>   // KIND 0 predictor, 1 PLM, 2 PPM primitive, 3 PPM characteristic
+__global__ __launch_bounds__(256) void k_flux3(const BoxDesc* __restrict__ boxes, Plan plan, Sweep3Args<real> a) {
+    int b, i, j, k;
+    bool act = plan_item_box(plan, b, i, j, k);
+    double sm = 0.0;
+    if (act) {
+        const BoxDesc& B = boxes[b];
+        const long long c = cell_index(B, i, j, k);
+        const int idx[3] = {i, j, k};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (idx[d] < a.lo_off || idx[d] > B.n[d] + a.hi_off) continue;
+            const long long s = dir_stride(B, d);
+            double f;
+            if (KIND == 0) f = flux_pred_face<real, RS>(a.d[d], c, s);
+            else if (KIND == 1) f = flux_corr_face<real, RS>(a.d[d], c, s);
+            else if (KIND == 2) f = flux_corr_ppm_face<real, RS, false>(a.d[d], c, s);
+            else f = flux_corr_ppm_face<real, RS, true>(a.d[d], c, s);
+            sm = fmax(sm, f);
+        }
+    }
+    atomic_max_nonneg(a.d[0].smax, b, sm);
 }
 
 __device__ __forceinline__ void upw(double v, double L14, double R14, double L34, double R34, double& o14,
@@ -272,6 +310,98 @@ __global__ __launch_bounds__(256) void k_update(const BoxDesc* __restrict__ boxe
     if (a.cmax != nullptr) atomic_max_nonneg(a.cmax, b, cfl);
 }
 
+// Column (z-marching) form of k_update: one thread walks seg cells of an (i, j) column and
+// carries the k-1 values it needs -- the z-fluxes F_z(k-1), the corner EMFs Ex, Ey at k-1
+// and at the (i-1)/(j-1) neighbours, and the freshly updated B_z face of k-1, which is the
+// lower face of the cell-centred average -- instead of re-reading the k-1 plane.  The
+// arithmetic per cell is update_cell's.
+template <class real>
+__global__ __launch_bounds__(256) void k_update_z(const BoxDesc* __restrict__ boxes, Plan plan, UpdateArgs<real> a) {
+    int b, i, j, sg;
+    bool act = plan_item_box(plan, b, i, j, sg);
+    double cfl = 0.0;
+    if (act) {
+        const BoxDesc& B = boxes[b];
+        const PlanBox& pb = plan.boxes[b];
+        const int kb = pb.zlo + sg * pb.seg, ke = min(kb + pb.seg, pb.zlo + pb.zn);
+        const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+        const double dtd = a.factor * a.dt[b] / B.dx;
+        const real dtdx = (real)dtd;
+        const double* Ex = a.Ec[0];
+        const double* Ey = a.Ec[1];
+        const double* Ez = a.Ec[2];
+        const int sh = a.ccb_lo_shift;
+        const bool ccb_ij = i >= pb.lo[0] + sh && j >= pb.lo[1] + sh;
+        const bool int_ij = i >= 1 && j >= 1 && i <= B.n[0] && j <= B.n[1];
+        long long c = cell_index(B, i, j, kb);
+        // carried k-1 values, loaded once per segment
+        real fz[4];
+#pragma unroll
+        for (int f = 0; f < 4; f++) fz[f] = a.F[2][f][c - sz];
+        double ey_m = Ey[c - sz], ex_m = Ex[c - sz], eyx_m = Ey[c - sx - sz], exy_m = Ex[c - sy - sz];
+        double bz_m = a.src.b[2][c - sz] - dtd * ((ey_m - eyx_m) - (ex_m - exy_m));
+#pragma unroll 1
+        for (int k = kb; k < ke; k++, c += sz) {
+            // keep one live cell index instead of a strength-reduced pointer per array and
+            // neighbour offset (register pressure): addresses are formed from c each step
+            __asm__ volatile("" : "+v"(c));
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                const real fzc = a.F[2][f][c];
+                real div = (a.F[0][f][c] - a.F[0][f][c - sx]) + (a.F[1][f][c] - a.F[1][f][c - sy]) + (fzc - fz[f]);
+                fz[f] = fzc;
+                real q0 = a.src.q[f][c];
+                real v;
+                if (a.comp[0] != nullptr) {
+                    real y = -(dtdx * div) - a.comp[f][c];
+                    v = q0 + y;
+                    a.comp[f][c] = (v - q0) - y;
+                } else {
+                    v = q0 - dtdx * div;
+                }
+                if (f == 0 && v < (real)MIN_DENS) {
+                    v = (real)MIN_DENS;
+                    if (a.comp[0] != nullptr) a.comp[0][c] = real(0);
+                    atomicAdd(a.floor_count, 1u);
+                }
+                a.dst.q[f][c] = v;
+            }
+            const double ex = Ex[c], ey = Ey[c], ez = Ez[c];
+            const double ez_y = Ez[c - sy], ez_x = Ez[c - sx], ey_x = Ey[c - sx], ex_y = Ex[c - sy];
+            double bx = a.src.b[0][c] - dtd * ((ez - ez_y) - (ey - ey_m));
+            double by = a.src.b[1][c] - dtd * ((ex - ex_m) - (ez - ez_x));
+            double bz = a.src.b[2][c] - dtd * ((ey - ey_x) - (ex - ex_y));
+            a.dst.b[0][c] = bx;
+            a.dst.b[1][c] = by;
+            a.dst.b[2][c] = bz;
+            if (ccb_ij && k >= pb.zlo + sh) {
+                long long m = c - sx;
+                double bxm = a.src.b[0][m] - dtd * ((ez_x - Ez[m - sy]) - (ey_x - eyx_m));
+                m = c - sy;
+                double bym = a.src.b[1][m] - dtd * ((ex_y - exy_m) - (ez_y - Ez[m - sx]));
+                a.dst.q[4][c] = (real)(0.5 * (bx + bxm));
+                a.dst.q[5][c] = (real)(0.5 * (by + bym));
+                a.dst.q[6][c] = (real)(0.5 * (bz + bz_m));
+            } else {
+                a.dst.q[4][c] = a.src.q[4][c];
+                a.dst.q[5][c] = a.src.q[5][c];
+                a.dst.q[6][c] = a.src.q[6][c];
+            }
+            ey_m = ey;
+            ex_m = ex;
+            eyx_m = ey_x;
+            exy_m = ex_y;
+            bz_m = bz;
+            if (a.cmax != nullptr && int_ij && k >= 1 && k <= B.n[2]) {
+                double q7[7];
+                for (int f = 0; f < 7; f++) q7[f] = (double)a.dst.q[f][c];
+                cfl = fmax(cfl, cell_fast_speed(q7, a.cs));
+            }
+        }
+    }
+    if (a.cmax != nullptr) atomic_max_nonneg(a.cmax, b, cfl);
+}
+
 template <class real, int RS>
 void launch_flux_predictor(const Launch& L, const SweepArgs<real>& a) {
     if (L.plan.total <= 0) return;
@@ -287,6 +417,15 @@ void launch_flux_corrector(const Launch& L, const SweepArgs<real>& a) {
     else
         hipLaunchKernelGGL((k_flux_corr<real, RS>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
 }
+template <class real, int RS>
+void launch_flux3(const Launch& L, const Sweep3Args<real>& a, int kind) {
+    if (L.plan.total <= 0) return;
+    const dim3 g(L.blocks()), t(256);
+    if (kind == 0) hipLaunchKernelGGL((k_flux3<real, RS, 0>), g, t, 0, L.stream, L.boxes, L.plan, a);
+    else if (kind == 1) hipLaunchKernelGGL((k_flux3<real, RS, 1>), g, t, 0, L.stream, L.boxes, L.plan, a);
+    else if (kind == 2) hipLaunchKernelGGL((k_flux3<real, RS, 2>), g, t, 0, L.stream, L.boxes, L.plan, a);
+    else hipLaunchKernelGGL((k_flux3<real, RS, 3>), g, t, 0, L.stream, L.boxes, L.plan, a);
+}
 template <class real>
 void launch_corner(const Launch& L, const CornerArgs<real>& a) {
     if (L.plan.total <= 0) return;
@@ -295,12 +434,16 @@ void launch_corner(const Launch& L, const CornerArgs<real>& a) {
 template <class real>
 void launch_update(const Launch& L, const UpdateArgs<real>& a) {
     if (L.plan.total <= 0) return;
-    hipLaunchKernelGGL((k_update<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+    if (L.column)
+        hipLaunchKernelGGL((k_update_z<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+    else
+        hipLaunchKernelGGL((k_update<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
 }
 
 #define IMHD_INST_RS(T, RS)                                                              \
     template void launch_flux_predictor<T, RS>(const Launch&, const SweepArgs<T>&); \
-    template void launch_flux_corrector<T, RS>(const Launch&, const SweepArgs<T>&);
+    template void launch_flux_corrector<T, RS>(const Launch&, const SweepArgs<T>&); \
+    template void launch_flux3<T, RS>(const Launch&, const Sweep3Args<T>&, int);
 #define IMHD_INST_VL(T)                                                      \
     IMHD_INST_RS(T, RS_ROE)                                                  \
     IMHD_INST_RS(T, RS_HLLD)                                                 \
