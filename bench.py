@@ -226,6 +226,54 @@ def pmc_summarize(fetch_dir, write_dir):
     return 0
 
 
+def run_ensemble(steps, cour, warmup=2):
+    """Batched-ensemble configuration: 48 periodic boxes of uneven sizes drawn from
+    {32..192}^3 (seed 4004), FP32, HLLE + VL2 + CT, every kernel one launch for all boxes,
+    the whole step in a hipGraph.  Returns throughput over all boxes."""
+    from isothermal_mhd_amd import VanLeerSolver, ics
+    dims, states, _ = ics.ensemble_c4()
+    S = VanLeerSolver(dims, npass=0, riemann="hlle", precision=32, cour=cour)
+    for b, (q, bf, qp) in enumerate(states):
+        S.set_state(q, bf, None, box=b)
+    del states
+    S.boundary_conditions()
+    S.run(warmup, graph=True)
+    S.sync()
+    t0 = time.perf_counter()
+    S.run(steps, graph=True)
+    S.sync()
+    sec = time.perf_counter() - t0
+    cells = sum(a * b * c for a, b, c in dims)
+    S.close()
+    return {"value": cells * steps / sec / 1e6, "unit": "Mcell-updates/s", "boxes": len(dims), "cells": cells,
+            "steps": steps, "ms_per_step": sec / steps * 1e3, "dtype": "f32",
+            "workload": "48 periodic boxes, sizes U{32..192}^3 (seed 4004), |B0| U[0.1,1] random direction, "
+                        "rms Mach U[0.5,4], k^-2 velocity; HLLE + VL2 + CT, PLM; one launch per kernel for all "
+                        "boxes, whole step in a hipGraph"}
+
+
+def run_tvd_c1(steps, cour):
+    """64^3 periodic, rho = 1, B0 = (0.5,0,0), k^-2 velocity 1<=|k|<=4 at rms Mach 1 (seed 1001),
+    K98 Roe-TVD + CTU + CT, FP64, native loop in a hipGraph."""
+    from isothermal_mhd_amd import TvdSolver, ics
+    n = 64
+    q, bf, qp = ics.config_c1(n=n)
+    S = TvdSolver((n, n, n), npass=0, riemann="roe", precision=64, cour=cour)
+    S.set_state(q, bf, None)
+    S.boundary_conditions()
+    S.run(2, graph=True)
+    S.sync()
+    t0 = time.perf_counter()
+    S.run(steps, graph=True)
+    S.sync()
+    sec = time.perf_counter() - t0
+    S.close()
+    return {"value": n ** 3 * steps / sec / 1e6, "unit": "Mcell-updates/s", "steps": steps,
+            "ms_per_step": sec / steps * 1e3, "dtype": "f64",
+            "workload": "64^3 periodic, rho=1, B0=(0.5,0,0), k^-2 velocity 1<=|k|<=4 rms Mach 1 (seed 1001); "
+                        "K98 Roe-TVD + CTU + CT, FP64, hipGraph native loop"}
+
+
 def l1_errors(sa, sb, nb=6):
     import numpy as np
     qa, ba, pa = sa
@@ -283,6 +331,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--pmc-run", action="store_true", help="short run for rocprofv3 counter collection")
+    ap.add_argument("--no-extra", action="store_true", help="skip the ensemble and TVD side measurements")
+    ap.add_argument("--extra-steps", type=int, default=50)
     ap.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     args = ap.parse_args()
     if args.pmc_summarize:
@@ -338,6 +388,12 @@ def main():
         res["fp32_over_fp64"] = cups32 / cups64
         res["l1_fp32_vs_fp64"] = l1_errors(r32["state"], r64["state"])
         res["kernels_fp64_ms"] = {k: round(v[0] / v[1], 5) for k, v in r64["prof"].items()}
+    if not args.no_extra:
+        for key, fn in (("ensemble_c4", run_ensemble), ("tvd_c1", run_tvd_c1)):
+            try:
+                res[key] = fn(args.extra_steps, args.cour)
+            except Exception as e:   # a side measurement never hides the main line
+                res[key] = {"error": repr(e)}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps, args.cour)
     print(json.dumps(res))

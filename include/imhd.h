@@ -163,6 +163,13 @@ int imhd_get_time(imhd_ctx* ctx, double* t, double* dt);    /* per box */
 int imhd_set_profiling(imhd_ctx* ctx, int on);
 const char* imhd_profile_report(imhd_ctx* ctx);
 
+/* Launch-shape autotuner (no counterpart in the reference): times `steps` native-loop steps
+ * (imhd_run without a graph) from the current state for each candidate z-segment length of
+ * the column update kernels and y-strip height of the work enumeration, keeps the fastest,
+ * and restores state, time and forcing state exactly.  `report` (may be NULL) receives the
+ * per-candidate timings and the choice. */
+int imhd_autotune(imhd_ctx* ctx, int steps, char* report, int report_len);
+
 /* Synchronise the context's stream. */
 int imhd_sync(imhd_ctx* ctx);
 

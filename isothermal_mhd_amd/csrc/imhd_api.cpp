@@ -103,6 +103,7 @@ struct imhd_ctx {
     PlanDev pl_ghost, pl_pghost, pl_int, pl_vel, pl_ppflux[3], pl_ppupd, pl_pcflux[3], pl_pfupd;
     PlanDev pl_teig_p[3], pl_tflux_p[3], pl_teig_c[3], pl_tflux_c[3];
     int zseg = 4;    // z-segment length of the column (z-marching) kernels
+    int strip = 0;   // y-strip height of the work enumeration (0: whole planes)
     bool fuse3 = true;   // the three directional flux sweeps of a stage in one launch
     PlanDev pl_pflux3, pl_cflux3;
     PlanDev pl_tcorner_p, pl_tcorner_c, pl_tcenter, pl_thalf, pl_tcfl;
@@ -239,6 +240,7 @@ Launch mk(imhd_ctx* c, const PlanDev& p) {
     L.plan.total = p.total;
     L.stream = c->st;
     L.column = p.column;
+    L.plan.sy = c->strip;
     return L;
 }
 
@@ -835,6 +837,7 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     c->esz = g.precision == 64 ? 8 : 4;
     if (const char* e = getenv("IMHD_ZSEG")) c->zseg = std::max(0, atoi(e));   // 0: per-cell kernels
     if (const char* e = getenv("IMHD_FUSE3")) c->fuse3 = atoi(e) != 0;      // 0: one launch per direction
+    if (const char* e = getenv("IMHD_STRIP")) c->strip = std::max(0, atoi(e));
     long long run = 0;
     for (int bx = 0; bx < nbox; bx++) {
         const imhd_config& k = cfg[bx];
@@ -1031,6 +1034,106 @@ int imhd_step(imhd_ctx* c, double dt, int drive_next) {
         }
     }
     return prof_collect(c);
+}
+
+// Launch-shape autotuner: times `steps` native steps for each candidate (z-segment length
+// of the column kernels x y-strip height of the work enumeration) from a snapshot of the
+// current state, keeps the fastest, and restores the state and time.
+int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
+    if (!c || steps < 1) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    if (c->prof) FAIL(c, IMHD_EINVAL, "switch profiling off before autotuning");
+    HIPCK(c, hipStreamSynchronize(c->st));
+    const size_t nr = (size_t)c->pool * c->esz, nd = (size_t)c->pool * 8;
+    const int cur0 = c->cur, pcur0 = c->pcur;
+    std::vector<std::pair<void*, void*>> saved;   // (live buffer, snapshot)
+    std::vector<size_t> sizes;
+    auto keep = [&](void* live, size_t bytes) -> int {
+        if (!live) return 0;
+        void* snap = nullptr;
+        HIPCK(c, hipMalloc(&snap, bytes));
+        HIPCK(c, hipMemcpy(snap, live, bytes, hipMemcpyDeviceToDevice));
+        saved.push_back({live, snap});
+        sizes.push_back(bytes);
+        return 0;
+    };
+    int rc = 0;
+    for (int f = 0; f < 7 && !rc; f++) rc = keep(c->q[cur0][f], nr);
+    for (int d = 0; d < 3 && !rc; d++) rc = keep(c->b[cur0][d], nd);
+    for (int p = 0; p < c->cfg.npass && !rc; p++) rc = keep(c->qp[pcur0][p], nr);
+    for (int f = 0; f < 4 && !rc; f++) rc = keep(c->comp[f], nr);
+    if (!rc) rc = keep(c->t, 8 * c->nbox);
+    if (!rc) rc = keep(c->dt, 8 * c->nbox);
+    if (!rc) rc = keep(c->dtprev, 8 * c->nbox);
+    if (!rc) rc = keep(c->floor_count, 4);
+    if (!rc && c->ou_on) {
+        const size_t na = 8 * (size_t)c->nbox * (c->ou_k.size() / 3) * 3;
+        rc = keep(c->ou_are, na);
+        if (!rc) rc = keep(c->ou_aim, na);
+        if (!rc) rc = keep(c->ou_step, 4 * (size_t)c->nbox);
+    }
+    auto restore = [&]() -> int {
+        for (size_t n = 0; n < saved.size(); n++)
+            HIPCK(c, hipMemcpy(saved[n].first, saved[n].second, sizes[n], hipMemcpyDeviceToDevice));
+        c->cur = cur0;
+        c->pcur = pcur0;
+        return 0;
+    };
+    auto set_shape = [&](int zseg, int strip) -> int {
+        c->zseg = zseg;
+        c->strip = strip;
+        int r = build_plan(c, c->pl_pupd, 2, -1, zseg);
+        if (!r) r = build_plan(c, c->pl_cupd, 5, -1, zseg);
+        for (auto& g : c->gexec) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+        return r;
+    };
+    std::string rep;
+    int best_z = c->zseg, best_s = c->strip;
+    float best = 1e30f;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc) { HIPCK(c, hipEventCreate(&e0)); HIPCK(c, hipEventCreate(&e1)); }
+    auto trial = [&](int zseg, int strip) -> int {
+        int r = set_shape(zseg, strip);
+        if (!r) r = restore();
+        if (r) return r;
+        for (int s = 0; s < 1; s++) { if (c->prec == 64) native_step<double>(c, 1e300); else native_step<float>(c, 1e300); }
+        HIPCK(c, hipEventRecord(e0, c->st));
+        for (int s = 0; s < steps; s++) { if (c->prec == 64) native_step<double>(c, 1e300); else native_step<float>(c, 1e300); }
+        HIPCK(c, hipEventRecord(e1, c->st));
+        HIPCK(c, hipEventSynchronize(e1));
+        HIPCK(c, hipGetLastError());
+        float ms = 0.f;
+        HIPCK(c, hipEventElapsedTime(&ms, e0, e1));
+        ms /= (float)steps;
+        char b[96];
+        snprintf(b, sizeof(b), "zseg=%d strip=%d: %.4f ms/step; ", zseg, strip, ms);
+        rep += b;
+        if (ms < best) { best = ms; best_z = zseg; best_s = strip; }
+        return 0;
+    };
+    // coordinate search: strip height at the current z-segment, then the z-segment
+    static const int strips[] = {0, 8, 16, 32, 64};
+    static const int zsegs[] = {0, 2, 4, 8, 16};
+    const int z0 = c->zseg;
+    for (int s : strips) if (!rc) rc = trial(z0, s);
+    const int s_best = best_s;
+    for (int z : zsegs) if (!rc && z != z0) rc = trial(z, s_best);
+    if (!rc) rc = set_shape(best_z, best_s);
+    if (!rc) rc = restore();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (auto& sv : saved) (void)hipFree(sv.second);
+    if (rc) return rc;
+    char b[96];
+    snprintf(b, sizeof(b), "chosen zseg=%d strip=%d (%.4f ms/step)", best_z, best_s, best);
+    rep += b;
+    c->prof_report = rep;
+    if (report && report_len > 0) {
+        strncpy(report, rep.c_str(), (size_t)report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    HIPCK(c, hipStreamSynchronize(c->st));
+    return IMHD_OK;
 }
 
 int imhd_set_ou_forcing(imhd_ctx* c, const imhd_ou* ou) {

@@ -42,6 +42,7 @@ struct Plan {
     const PlanBox* boxes;  // device
     int nbox;
     long long total;
+    int sy;                // > 0: rows enumerated in y-strips of sy rows, all k per strip
 };
 
 __device__ __forceinline__ long long cell_index(const BoxDesc& b, int i, int j, int k) {
@@ -74,13 +75,33 @@ __device__ __forceinline__ bool plan_item(const Plan& p, int& box, int& i, int& 
     }
     const PlanBox& pb = p.boxes[b];
     long long loc = g - pb.start;
-    long long r0 = pb.r[0], r1 = pb.r[1];
-    if (loc >= r0 * r1 * (long long)pb.r[2]) return false;
+    const long long r0 = pb.r[0], r1 = pb.r[1], r2 = pb.r[2];
+    if (loc >= r0 * r1 * r2) return false;
     long long t = loc / r0;
     i = pb.lo[0] + (int)(loc - t * r0);
-    long long u = t / r1;
-    j = pb.lo[1] + (int)(t - u * r1);
-    k = pb.lo[2] + (int)u;
+    const long long sy = p.sy;
+    if (sy > 0 && r1 > sy) {
+        // y-strip order: strip s holds rows [s sy, (s+1) sy) of every k, k outer within it,
+        // so a strip's k-1 / k+1 planes stay in the XCD's L2 while the strip is swept
+        const long long full = r1 / sy, per = sy * r2;
+        const long long s = t / per;
+        long long jj, kk;
+        if (s < full) {
+            const long long rem = t - s * per;
+            kk = rem / sy;
+            jj = s * sy + (rem - kk * sy);
+        } else {
+            const long long h = r1 - full * sy, rem = t - full * per;
+            kk = rem / h;
+            jj = full * sy + (rem - kk * h);
+        }
+        j = pb.lo[1] + (int)jj;
+        k = pb.lo[2] + (int)kk;
+    } else {
+        long long u = t / r1;
+        j = pb.lo[1] + (int)(t - u * r1);
+        k = pb.lo[2] + (int)u;
+    }
     box = b;
     return true;
 }

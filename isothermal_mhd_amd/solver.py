@@ -188,6 +188,13 @@ class VanLeerSolver:
         """Device-resident time loop (driver.f90:35-186) for nsteps steps."""
         self._check(self.lib.imhd_run(self._c, int(nsteps), float(t_end), int(bool(graph))))
 
+    def autotune(self, steps=5):
+        """Pick the fastest launch shape on the current state (state and time restored);
+        returns the report string."""
+        buf = ctypes.create_string_buffer(2048)
+        self._check(self.lib.imhd_autotune(self._c, int(steps), buf, len(buf)))
+        return buf.value.decode()
+
     def sync(self):
         self._check(self.lib.imhd_sync(self._c))
 

@@ -294,3 +294,34 @@ def test_single_solve_whole_array(recon, rcode):
     qg, bg, _ = S.get_state()
     assert np.array_equal(qg, qo) and np.array_equal(bg, bo)
     S.close()
+
+
+def test_autotune_restores_state_and_keeps_results():
+    """imhd_autotune times candidate launch shapes on the live state, then restores it:
+    the state and time afterwards are unchanged and the tuned run equals an untuned one
+    bit for bit (launch shapes never change the arithmetic)."""
+    dims = (24, 20, 16)
+    say("[gpu] autotune")
+    q, bf, qp = random_state(*dims, npass=1, seed=31)
+    S = make_solver(dims, 1, "hlld", "periodic")
+    S.set_state(q, bf, qp)
+    S.boundary_conditions()
+    S.run(2, graph=False)
+    before = S.get_state(), S.get_time()
+    rep = S.autotune(steps=2)
+    say("  " + rep)
+    assert "chosen" in rep
+    after = S.get_state(), S.get_time()
+    for x, y in zip(before[0], after[0]):
+        assert np.array_equal(x, y)
+    assert np.array_equal(before[1][0], after[1][0]) and np.array_equal(before[1][1], after[1][1])
+    S.run(5, graph=True)
+    tuned = S.get_state()
+    S.close()
+    U = make_solver(dims, 1, "hlld", "periodic")
+    U.set_state(q, bf, qp)
+    U.boundary_conditions()
+    U.run(7, graph=False)
+    for x, y in zip(tuned, U.get_state()):
+        assert np.array_equal(x, y)
+    U.close()
