@@ -297,6 +297,85 @@ __global__ __launch_bounds__(256) void k_pass_stage(const BoxDesc* __restrict__ 
     }
 }
 
+// z-marching form of k_pass_stage: one thread walks a z-segment of an (i, j) column.  At
+// the z-neighbours only v_z is needed, and the lower z-face value of cell k is the upper
+// one of cell k-1 (same operands in the same order, so the same bits); both are carried
+// in registers, with the scalar's z-window, so each plane of rho, m and the scalar is
+// loaded once per segment instead of once per neighbour.
+template <class real, int STAGE>
+__device__ __forceinline__ real pass_face_v(real wm, real w0, real w1, real w2, real vavg) {
+    real uL, uR;
+    if (!STAGE) {
+        uL = w0;
+        uR = w1;
+    } else {
+        uL = w0 + real(0.5) * minmod_slope(wm, w0, w1);
+        uR = w1 - real(0.5) * minmod_slope(w0, w1, w2);
+    }
+    real theta = vavg >= real(0) ? real(1) : real(-1);
+    return real(0.5) * ((real(1) + theta) * uL + (real(1) - theta) * uR);
+}
+
+template <class real, int STAGE>
+__global__ __launch_bounds__(256) void k_pass_stage_z(const BoxDesc* __restrict__ boxes, Plan plan, PassStageArgs<real> a) {
+    int b, i, j, sg;
+    if (!plan_item(plan, b, i, j, sg)) return;
+    const BoxDesc& B = boxes[b];
+    const PlanBox& pb = plan.boxes[b];
+    const int kb = pb.zlo + sg * pb.seg, ke = min(kb + pb.seg, pb.zlo + pb.zn);
+    const long long sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+    const real dtdx = (real)(a.factor * a.dt[b] / B.dx);
+    const long long c0 = cell_index(B, i, j, kb);
+    for (int p = 0; p < a.npass; p++) {
+        const real* w = a.rec[p];
+        long long c = c0;
+        // carried: v_z at k, the scalar at k-1, k, k+1 (k, k+1 for the predictor), and the
+        // value at the face between k-1 and k
+        real vz1 = pass_vel(a, 2, c);
+        real flo, wa, wb, wc;
+        {
+            const real vz0 = pass_vel(a, 2, c - sz);
+            const real va = real(0.5) * (vz0 + vz1);
+            if (STAGE) {
+                const real wmm = w[c - 2 * sz];
+                wa = w[c - sz]; wb = w[c]; wc = w[c + sz];
+                flo = pass_face_v<real, 1>(wmm, wa, wb, wc, va);
+            } else {
+                wa = w[c - sz]; wb = w[c]; wc = wb;
+                flo = pass_face_v<real, 0>(wa, wa, wb, wb, va);
+            }
+        }
+#pragma unroll 1
+        for (int k = kb; k < ke; k++, c += sz) {
+            // one live cell index instead of a strength-reduced pointer per array and offset
+            __asm__ volatile("" : "+v"(c));
+            const real vx = pass_vel(a, 0, c), vy = pass_vel(a, 1, c);
+            const real vz2 = pass_vel(a, 2, c + sz);
+            const real vxh = real(0.5) * (vx + pass_vel(a, 0, c + 1));
+            const real vxl = real(0.5) * (pass_vel(a, 0, c - 1) + vx);
+            const real vyh = real(0.5) * (vy + pass_vel(a, 1, c + sy));
+            const real vyl = real(0.5) * (pass_vel(a, 1, c - sy) + vy);
+            const real vzh = real(0.5) * (vz1 + vz2);
+            real fhi;
+            if (STAGE) {
+                const real wd = w[c + 2 * sz];
+                fhi = pass_face_v<real, 1>(wa, wb, wc, wd, vzh);
+                wa = wb; wb = wc; wc = wd;
+            } else {
+                const real wn = w[c + sz];
+                fhi = pass_face_v<real, 0>(wb, wb, wn, wn, vzh);
+                wb = wn;
+            }
+            const real dF0 = pass_face(w, c, (long long)1, vxh, STAGE) - pass_face(w, c - 1, (long long)1, vxl, STAGE);
+            const real dF1 = pass_face(w, c, sy, vyh, STAGE) - pass_face(w, c - sy, sy, vyl, STAGE);
+            const real dF2 = fhi - flo;
+            a.dst[p][c] = a.src[p][c] - dtdx * (vx * dF0 + vy * dF1 + vz1 * dF2);
+            flo = fhi;
+            vz1 = vz2;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // host <-> device layout and precision conversion (reference FP64 arrays)
 template <class real>
@@ -343,7 +422,10 @@ void launch_restore_shell(const Launch& L, const ShellArgs<real>& a) {
 }
 template <class real>
 void launch_pass_stage(const Launch& L, const PassStageArgs<real>& a) {
-    if (L.plan.total > 0) L256((k_pass_stage<real>), L.boxes, L.plan, a);
+    if (L.plan.total <= 0) return;
+    if (!L.column) L256((k_pass_stage<real>), L.boxes, L.plan, a);
+    else if (a.stage) L256((k_pass_stage_z<real, 1>), L.boxes, L.plan, a);
+    else L256((k_pass_stage_z<real, 0>), L.boxes, L.plan, a);
 }
 template <class real>
 void launch_max_speed(const Launch& L, const QPtrs<real>& q, double cs, unsigned long long* cmax) {

@@ -101,6 +101,7 @@ struct imhd_ctx {
     PlanDev pl_pflux[3], pl_pcorner, pl_pupd, pl_cflux[3], pl_ccorner, pl_cupd;
     PlanDev pl_gband, pl_pgband;
     PlanDev pl_ghost, pl_pghost, pl_int, pl_vel, pl_ppflux[3], pl_ppupd, pl_pcflux[3], pl_pfupd;
+    PlanDev pl_pstage[2];   // fused passive stages (column plans when zseg > 0)
     PlanDev pl_teig_p[3], pl_tflux_p[3], pl_teig_c[3], pl_tflux_c[3];
     int zseg = 4;    // z-segment length of the column (z-marching) kernels
     int strip = 0;   // y-strip height of the work enumeration (0: whole planes)
@@ -666,7 +667,7 @@ void passive_fused(imhd_ctx* c, int qold) {
     const int np = c->cfg.npass;
     const int p0 = c->pcur, p1 = 1 - c->pcur;
     for (int stage = 0; stage < 2; stage++) {
-        ProfScope ps(c, stage ? "pass_stage_corr" : "pass_stage_pred", (stage ? c->pl_pfupd : c->pl_ppupd).active);
+        ProfScope ps(c, stage ? "pass_stage_corr" : "pass_stage_pred", c->pl_pstage[stage].active);
         PassStageArgs<real> a{};
         for (int p = 0; p < np; p++) {
             a.src[p] = R<real>(c->qp[p0][p]);
@@ -681,7 +682,7 @@ void passive_fused(imhd_ctx* c, int qold) {
         a.factor = stage ? 1.0 : 0.5;
         a.npass = np;
         a.stage = stage;
-        launch_pass_stage<real>(mk(c, stage ? c->pl_pfupd : c->pl_ppupd), a);
+        launch_pass_stage<real>(mk(c, c->pl_pstage[stage]), a);
     }
 }
 
@@ -913,9 +914,9 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     }
     if ((rc = build_plan(c, c->pl_pflux3, 25, -1))) return rc;
     if ((rc = build_plan(c, c->pl_cflux3, 26, -1))) return rc;
-    if ((rc = build_plan(c, c->pl_pcorner, 1, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_pcorner, 1, -1, c->zseg))) return rc;
     if ((rc = build_plan(c, c->pl_pupd, 2, -1, c->zseg))) return rc;
-    if ((rc = build_plan(c, c->pl_ccorner, 4, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_ccorner, 4, -1, c->zseg))) return rc;
     if ((rc = build_plan(c, c->pl_cupd, 5, -1, c->zseg))) return rc;
     if ((rc = build_plan(c, c->pl_ghost, 6, -1))) return rc;
     if ((rc = build_plan(c, c->pl_gband, 14, -1))) return rc;
@@ -925,6 +926,8 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     if ((rc = build_plan(c, c->pl_vel, 9, -1))) return rc;
     if ((rc = build_plan(c, c->pl_ppupd, 11, -1))) return rc;
     if ((rc = build_plan(c, c->pl_pfupd, 13, -1))) return rc;
+    if ((rc = build_plan(c, c->pl_pstage[0], 11, -1, c->zseg))) return rc;
+    if ((rc = build_plan(c, c->pl_pstage[1], 13, -1, c->zseg))) return rc;
     if (g.integrator == IMHD_TVD) {
         for (int d = 0; d < 3; d++) {
             if ((rc = build_plan(c, c->pl_teig_p[d], 16, d))) return rc;
@@ -978,7 +981,8 @@ int imhd_destroy(imhd_ctx* c) {
                       &c->pl_teig_p[2], &c->pl_tflux_p[0], &c->pl_tflux_p[1], &c->pl_tflux_p[2],
                       &c->pl_teig_c[0], &c->pl_teig_c[1], &c->pl_teig_c[2], &c->pl_tflux_c[0],
                       &c->pl_tflux_c[1], &c->pl_tflux_c[2], &c->pl_tcorner_p, &c->pl_tcorner_c,
-                      &c->pl_tcenter, &c->pl_thalf, &c->pl_tcfl, &c->pl_pflux3, &c->pl_cflux3};
+                      &c->pl_tcenter, &c->pl_thalf, &c->pl_tcfl, &c->pl_pflux3, &c->pl_cflux3,
+                      &c->pl_pstage[0], &c->pl_pstage[1]};
     for (auto p : pls) fr(p->d);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
@@ -1084,6 +1088,10 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
         c->strip = strip;
         int r = build_plan(c, c->pl_pupd, 2, -1, zseg);
         if (!r) r = build_plan(c, c->pl_cupd, 5, -1, zseg);
+        if (!r) r = build_plan(c, c->pl_pcorner, 1, -1, zseg);
+        if (!r) r = build_plan(c, c->pl_ccorner, 4, -1, zseg);
+        if (!r) r = build_plan(c, c->pl_pstage[0], 11, -1, zseg);
+        if (!r) r = build_plan(c, c->pl_pstage[1], 13, -1, zseg);
         for (auto& g : c->gexec) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
         return r;
     };

@@ -236,6 +236,64 @@ __global__ __launch_bounds__(256) void k_corner(const BoxDesc* __restrict__ boxe
     }
 }
 
+// z-marching form of k_corner: one thread walks a z-segment of an (i, j) column.  What
+// cell k reads at plane k+1 (the centre EMFs Ex, Ey and v_z there, and the slot-0 face
+// EMFs of Ey and Ex) is what cell k+1 reads at its own plane, so those values are carried
+// in registers; every value is formed by the same expression as in k_corner.
+template <class real>
+__global__ __launch_bounds__(256) void k_corner_z(const BoxDesc* __restrict__ boxes, Plan plan, CornerArgs<real> a) {
+    int b, i, j, sg;
+    if (!plan_item(plan, b, i, j, sg)) return;
+    const BoxDesc& B = boxes[b];
+    const PlanBox& pb = plan.boxes[b];
+    const int kb = pb.zlo + sg * pb.seg, ke = min(kb + pb.seg, pb.zlo + pb.zn);
+    const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+    long long c = cell_index(B, i, j, kb);
+    auto Q = [&](int f, long long p) -> double { return (double)a.q[f][p]; };
+    auto Exc = [&](long long p) -> double { double ir = 1.0 / Q(0, p); return (Q(5, p) * Q(3, p) - Q(6, p) * Q(2, p)) * ir; };
+    auto Eyc = [&](long long p) -> double { double ir = 1.0 / Q(0, p); return (Q(6, p) * Q(1, p) - Q(4, p) * Q(3, p)) * ir; };
+    auto Ezc = [&](long long p) -> double { double ir = 1.0 / Q(0, p); return (Q(4, p) * Q(2, p) - Q(5, p) * Q(1, p)) * ir; };
+    auto E = [&](int comp, int slot, long long p) -> double { return (double)a.E[2 * comp + slot][p]; };
+    // carried plane-k values
+    double wz = Q(3, c) / Q(0, c);
+    double ey0 = Eyc(c), ey1 = Eyc(c + sx), ex0 = Exc(c), ex1 = Exc(c + sy);
+    double fy = E(1, 0, c), fx = E(0, 0, c);
+#pragma unroll 1
+    for (int k = kb; k < ke; k++, c += sz) {
+        // one live cell index instead of a strength-reduced pointer per array and offset
+        __asm__ volatile("" : "+v"(c));
+        const double wzn = Q(3, c + sz) / Q(0, c + sz);
+        const double vx = 0.5 * (Q(1, c) / Q(0, c) + Q(1, c + sx) / Q(0, c + sx));
+        const double vy = 0.5 * (Q(2, c) / Q(0, c) + Q(2, c + sy) / Q(0, c + sy));
+        const double vz = 0.5 * (wz + wzn);
+        double a14, a34, b14, b34;
+        {   // Ez at (i+1/2, j+1/2, k)
+            double e00 = Ezc(c), e10 = Ezc(c + sx), e01 = Ezc(c + sy), e11 = Ezc(c + sx + sy);
+            double y0 = E(2, 1, c), y1 = E(2, 1, c + sx), x0 = E(2, 0, c), x1 = E(2, 0, c + sy);
+            upw(vx, 2.0 * (y0 - e00), 2.0 * (y1 - e10), 2.0 * (e01 - y0), 2.0 * (e11 - y1), a14, a34);
+            upw(vy, 2.0 * (x0 - e00), 2.0 * (x1 - e01), 2.0 * (e10 - x0), 2.0 * (e11 - x1), b14, b34);
+            a.Ec[2][c] = 0.25 * (x0 + x1 + y0 + y1) + 0.125 * ((a14 - a34) + (b14 - b34));
+        }
+        {   // Ey at (i+1/2, j, k+1/2)
+            double e00 = ey0, e10 = ey1, e01 = Eyc(c + sz), e11 = Eyc(c + sx + sz);
+            double z0 = E(1, 1, c), z1 = E(1, 1, c + sx), x0 = fy, x1 = E(1, 0, c + sz);
+            upw(vx, 2.0 * (z0 - e00), 2.0 * (z1 - e10), 2.0 * (e01 - z0), 2.0 * (e11 - z1), a14, a34);
+            upw(vz, 2.0 * (x0 - e00), 2.0 * (x1 - e01), 2.0 * (e10 - x0), 2.0 * (e11 - x1), b14, b34);
+            a.Ec[1][c] = 0.25 * (x0 + x1 + z0 + z1) + 0.125 * ((a14 - a34) + (b14 - b34));
+            ey0 = e01; ey1 = e11; fy = x1;
+        }
+        {   // Ex at (i, j+1/2, k+1/2)
+            double e00 = ex0, e10 = ex1, e01 = Exc(c + sz), e11 = Exc(c + sy + sz);
+            double z0 = E(0, 1, c), z1 = E(0, 1, c + sy), y0 = fx, y1 = E(0, 0, c + sz);
+            upw(vy, 2.0 * (z0 - e00), 2.0 * (z1 - e10), 2.0 * (e01 - z0), 2.0 * (e11 - z1), a14, a34);
+            upw(vz, 2.0 * (y0 - e00), 2.0 * (y1 - e01), 2.0 * (e10 - y0), 2.0 * (e11 - y1), b14, b34);
+            a.Ec[0][c] = 0.25 * (y0 + y1 + z0 + z1) + 0.125 * ((a14 - a34) + (b14 - b34));
+            ex0 = e01; ex1 = e11; fx = y1;
+        }
+        wz = wzn;
+    }
+}
+
 // Unsplit conservative update of rho and rho v (vanleer_mod.f90:2080-2116) with the
 // density floor (:3243-3262), CT face update (:2218-2236) and face-average cell-centred B
 // (:2241-2256).  The lower faces needed by the face average are recomputed from the
@@ -429,7 +487,10 @@ void launch_flux3(const Launch& L, const Sweep3Args<real>& a, int kind) {
 template <class real>
 void launch_corner(const Launch& L, const CornerArgs<real>& a) {
     if (L.plan.total <= 0) return;
-    hipLaunchKernelGGL((k_corner<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+    if (L.column)
+        hipLaunchKernelGGL((k_corner_z<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+    else
+        hipLaunchKernelGGL((k_corner<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
 }
 template <class real>
 void launch_update(const Launch& L, const UpdateArgs<real>& a) {
