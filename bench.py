@@ -43,6 +43,13 @@ def bytes_per_item(name, esz):
     """Algorithmic (compulsory) HBM bytes per work item of each kernel: every input field
     read once and every output written once per cell or face."""
     d = 8
+    if name.startswith("stage_pred"):     # fused predictor: 7 state + 3 face B in (t^n), 7 state + 3 face B out (half step)
+        return 14 * esz + 6 * d
+    if name.startswith("stage_corr"):     # fused corrector: half step 7 + 3 face B, t^n rho, m + 3 face B in; 7 + 3 out
+        b = 18 * esz + 9 * d
+        if esz == 4:
+            b += 8 * esz                  # Kahan terms of rho, m: read and written (FP32 path)
+        return b
     if name.startswith("flux_"):          # fused sweeps, per cell: 7 state (real) + 3 face B (f64) in; 3 x (4 fluxes + 2 face EMFs) out
         return 25 * esz + 3 * d
     if name.startswith("corner"):         # 7 state + 6 face EMFs in (real); 3 corner EMFs out (f64)
@@ -131,7 +138,8 @@ def roofline(prof, esz):
 
 # family (bench profiler name) -> kernel symbol it launches (fused three-direction sweeps,
 # z-marching column updates)
-FAMILY_SYMBOL = {"flux_pred": "k_flux3", "flux_corr": "k_flux3", "corner_pred": "k_corner",
+FAMILY_SYMBOL = {"stage_pred": "k_vl_stage", "stage_corr_cfl": "k_vl_stage",
+                 "flux_pred": "k_flux3", "flux_corr": "k_flux3", "corner_pred": "k_corner",
                  "corner_corr": "k_corner", "update_pred": "k_update_z", "update_corr_cfl": "k_update_z",
                  "pass_stage_pred": "k_pass_stage", "pass_stage_corr": "k_pass_stage",
                  "fill_ghosts": "k_fill_ghost_band", "fill_ghosts_pass": "k_fill_ghost_band"}

@@ -29,6 +29,13 @@ struct PlanDev {
     std::vector<PlanBox> h;
 };
 
+struct TilePlanDev {
+    TileBox* d = nullptr;
+    long long total = 0;
+    long long active = 0;  // cells updated over all boxes
+    std::vector<TileBox> h;
+};
+
 struct ProfRec {
     const char* name;
     long long items;
@@ -47,8 +54,8 @@ struct imhd_ctx {
     long long pool = 0;  // padded cells over all boxes
     hipStream_t st = nullptr;
     // state buffers (two) and work arrays
-    void* q[2][7] = {};
-    double* b[2][3] = {};
+    void* q[3][7] = {};    // set 2: half-step state of the fused VL stages
+    double* b[3][3] = {};
     void* qp[2][MAXPASS] = {};
     void* F[3][4] = {};
     void* E[6] = {};
@@ -107,6 +114,9 @@ struct imhd_ctx {
     int strip = 0;   // y-strip height of the work enumeration (0: whole planes)
     bool fuse3 = true;   // the three directional flux sweeps of a stage in one launch
     PlanDev pl_pflux3, pl_cflux3;
+    bool vstage = true;  // VL stages as single fused launches (k_vl_stage)
+    int vseg = 16;       // z-segment length of the fused stages
+    TilePlanDev tp_pred, tp_corr;
     PlanDev pl_tcorner_p, pl_tcorner_c, pl_tcenter, pl_thalf, pl_tcfl;
     int ccb_shift = 1;
     // native loop
@@ -233,6 +243,58 @@ int build_plan(imhd_ctx* c, PlanDev& p, int which, int dir, int seg = 0) {
     return 0;
 }
 
+// Index ranges of one VL stage per axis, as [lo, n + hi]: the flux sweeps' normal and
+// transverse ranges, the corner EMFs and the updated cells (the ranges of build_plan's
+// cases 0-2 for the predictor and 3-5 for the corrector).
+struct StageRanges {
+    int fn_lo, fn_hi, ft_lo, ft_hi, c_lo, c_hi, u_lo, u_hi;
+};
+StageRanges stage_ranges(const imhd_ctx* c, bool corr) {
+    const int nb = c->hb[0].nb, e = c->cfg.recon == IMHD_PLM_PRIM ? 0 : 1;
+    if (!corr) return {1 - nb, nb - 1, 1 - nb, nb, 1 - nb, nb - 1, 2 - nb, nb - 1};
+    return {4 - nb + e, nb - 4 - e, 3 - nb, nb - 2, 4 - nb + e, nb - 4 - e, 5 - nb + e, nb - 4 - e};
+}
+
+// Tiles and z-segments of a fused stage.  Thread t of tile T sits at lo + T w + t (w = tile
+// width - 2); threads 1 .. width-2 update, threads 0 .. width-1 solve faces, threads
+// 0 .. width-2 form corners.  The tile count covers every face, corner and cell of the
+// stage's ranges; the march covers planes zlo-1 .. zhi+1.
+int build_tiles(imhd_ctx* c, TilePlanDev& p, bool corr) {
+    p.h.clear();
+    p.active = 0;
+    if (p.d) { (void)hipFree(p.d); p.d = nullptr; }
+    const StageRanges R = stage_ranges(c, corr);
+    const int lo = std::min(std::min(R.ft_lo, R.fn_lo), R.u_lo - 1);
+    const int fhi = std::max(R.ft_hi, R.fn_hi);
+    const int seg = std::max(1, c->vseg);
+    long long run = 0;
+    for (int bx = 0; bx < c->nbox; bx++) {
+        const BoxDesc& B = c->hb[bx];
+        TileBox tb{};
+        tb.x0 = lo;
+        tb.y0 = lo;
+        const int wx[2] = {VS_TX - 2, VS_TY - 2};
+        for (int a = 0; a < 2; a++) {
+            const int need = std::max(B.n[a] + fhi - 1, B.n[a] + R.u_hi) - lo;
+            tb.nt[a] = (need + wx[a] - 1) / wx[a];
+        }
+        tb.zlo = lo + 1;
+        tb.zhi = B.n[2] + std::max(std::max(R.ft_hi - 1, R.fn_hi), R.u_hi);
+        tb.seg = seg;
+        tb.nseg = (tb.zhi - tb.zlo + 1 + seg - 1) / seg;
+        tb.start = run;
+        run += (long long)tb.nt[0] * tb.nt[1] * tb.nseg;
+        long long cells = 1;
+        for (int a = 0; a < 3; a++) cells *= std::max(0, B.n[a] + R.u_hi - R.u_lo + 1);
+        p.active += cells;
+        p.h.push_back(tb);
+    }
+    p.total = run;
+    HIPCK(c, hipMalloc(&p.d, sizeof(TileBox) * p.h.size()));
+    HIPCK(c, hipMemcpy(p.d, p.h.data(), sizeof(TileBox) * p.h.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
 Launch mk(imhd_ctx* c, const PlanDev& p) {
     Launch L;
     L.boxes = c->db;
@@ -346,11 +408,64 @@ void update(imhd_ctx* c, int src, int dst, double factor, const PlanDev& p, bool
     launch_update<real>(mk(c, p), a);
 }
 
+// One whole stage in one launch (k_vl_stage): fluxes from set sflux, update src -> dst.
+template <class real>
+void vl_stage(imhd_ctx* c, int sflux, int src, int dst, bool corr, bool fuse_cmax, const char* name) {
+    const TilePlanDev& p = corr ? c->tp_corr : c->tp_pred;
+    ProfScope ps(c, name, p.active);
+    StageArgs<real> a{};
+    for (int d = 0; d < 3; d++) a.d[d] = sweep_args<real>(c, sflux, d);
+    for (int f = 0; f < 7; f++) {
+        a.qf[f] = R<real>(c->q[sflux][f]);
+        a.src.q[f] = R<real>(c->q[src][f]);
+        a.dst.q[f] = R<real>(c->q[dst][f]);
+    }
+    for (int d = 0; d < 3; d++) { a.src.b[d] = c->b[src][d]; a.dst.b[d] = c->b[dst][d]; }
+    a.dt = c->dt;
+    a.factor = corr ? 1.0 : 0.5;
+    a.ccb_lo_shift = c->ccb_shift;
+    for (int f = 0; f < 4; f++) a.comp[f] = (corr && c->comp[0]) ? R<real>(c->comp[f]) : nullptr;
+    a.floor_count = c->floor_count;
+    a.cmax = fuse_cmax ? c->cmax : nullptr;
+    a.smax = c->smax;
+    a.cs = c->cfg.cs;
+    const StageRanges r = stage_ranges(c, corr);
+    a.fn_lo = r.fn_lo; a.fn_hi = r.fn_hi; a.ft_lo = r.ft_lo; a.ft_hi = r.ft_hi;
+    a.c_lo = r.c_lo; a.c_hi = r.c_hi; a.u_lo = r.u_lo; a.u_hi = r.u_hi;
+    const TilePlan tp{p.d, c->nbox, p.total};
+    switch (c->cfg.riemann) {
+        case IMHD_ROE: launch_vl_stage<real, RS_ROE>(c->st, c->db, tp, a, corr); break;
+        case IMHD_HLLE: launch_vl_stage<real, RS_HLLE>(c->st, c->db, tp, a, corr); break;
+        default: launch_vl_stage<real, RS_HLLD>(c->st, c->db, tp, a, corr); break;
+    }
+}
+
 template <class real>
 void vl_step(imhd_ctx* c, bool fuse_cmax = false) {
     launch_reset_u64(c->st, c->smax, (int)stripe_slots(c->nbox));
     launch_reset_u64(c->st, c->dvmax, (int)stripe_slots(c->nbox));   // delv_max = 0 (vanleer_mod.f90:301)
     const int s0 = c->cur, s1 = 1 - c->cur;
+    if (c->vstage && c->q[2][0]) {
+        // Fused stages.  The corrector's fluxes read the half step while its update writes
+        // the new state, so the half step goes to its own set (2) when the corrector is
+        // fused too (PLM); the PPM corrector keeps the three-kernel path on set s1.
+        const bool fcorr = c->cfg.recon == IMHD_PLM_PRIM;
+        const int sh = fcorr ? 2 : s1;
+        vl_stage<real>(c, s0, s0, sh, false, false, "stage_pred");
+        if (fcorr) {
+            vl_stage<real>(c, 2, s0, s1, true, fuse_cmax, fuse_cmax ? "stage_corr_cfl" : "stage_corr");
+        } else {
+            switch (c->cfg.riemann) {
+                case IMHD_ROE: sweeps<real, RS_ROE>(c, s1, true); break;
+                case IMHD_HLLE: sweeps<real, RS_HLLE>(c, s1, true); break;
+                default: sweeps<real, RS_HLLD>(c, s1, true); break;
+            }
+            corner<real>(c, s1, c->pl_ccorner, "corner_corr");
+            update<real>(c, s0, s1, 1.0, c->pl_cupd, true, fuse_cmax ? "update_corr_cfl" : "update_corr", fuse_cmax);
+        }
+        c->cur = s1;
+        return;
+    }
     switch (c->cfg.riemann) {
         case IMHD_ROE: sweeps<real, RS_ROE>(c, s0, false); break;
         case IMHD_HLLE: sweeps<real, RS_HLLE>(c, s0, false); break;
@@ -839,6 +954,8 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     if (const char* e = getenv("IMHD_ZSEG")) c->zseg = std::max(0, atoi(e));   // 0: per-cell kernels
     if (const char* e = getenv("IMHD_FUSE3")) c->fuse3 = atoi(e) != 0;      // 0: one launch per direction
     if (const char* e = getenv("IMHD_STRIP")) c->strip = std::max(0, atoi(e));
+    if (const char* e = getenv("IMHD_VSTAGE")) c->vstage = atoi(e) != 0;    // 0: three kernels per stage
+    if (const char* e = getenv("IMHD_VSEG")) c->vseg = std::max(1, atoi(e));
     long long run = 0;
     for (int bx = 0; bx < nbox; bx++) {
         const imhd_config& k = cfg[bx];
@@ -866,6 +983,10 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
         for (int f = 0; f < 7; f++) if ((rc = alloc(c, &c->q[s][f], nr))) return rc;
         for (int d = 0; d < 3; d++) if ((rc = alloc(c, (void**)&c->b[s][d], nd))) return rc;
         for (int p = 0; p < g.npass; p++) if ((rc = alloc(c, &c->qp[s][p], nr))) return rc;
+    }
+    if (g.integrator == IMHD_VL) {   // half-step set of the fused stages
+        for (int f = 0; f < 7; f++) if ((rc = alloc(c, &c->q[2][f], nr))) return rc;
+        for (int d = 0; d < 3; d++) if ((rc = alloc(c, (void**)&c->b[2][d], nd))) return rc;
     }
     for (int d = 0; d < 3; d++)
         for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->F[d][f], nr))) return rc;
@@ -928,6 +1049,10 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     if ((rc = build_plan(c, c->pl_pfupd, 13, -1))) return rc;
     if ((rc = build_plan(c, c->pl_pstage[0], 11, -1, c->zseg))) return rc;
     if ((rc = build_plan(c, c->pl_pstage[1], 13, -1, c->zseg))) return rc;
+    if (g.integrator == IMHD_VL) {
+        if ((rc = build_tiles(c, c->tp_pred, false))) return rc;
+        if ((rc = build_tiles(c, c->tp_corr, true))) return rc;
+    }
     if (g.integrator == IMHD_TVD) {
         for (int d = 0; d < 3; d++) {
             if ((rc = build_plan(c, c->pl_teig_p[d], 16, d))) return rc;
@@ -954,6 +1079,10 @@ int imhd_destroy(imhd_ctx* c) {
         for (auto p : c->b[s]) fr(p);
         for (auto p : c->qp[s]) fr(p);
     }
+    for (auto p : c->q[2]) fr(p);
+    for (auto p : c->b[2]) fr(p);
+    fr(c->tp_pred.d);
+    fr(c->tp_corr.d);
     for (auto& a : c->F) for (auto p : a) fr(p);
     for (auto p : c->E) fr(p);
     for (auto p : c->Ec) fr(p);

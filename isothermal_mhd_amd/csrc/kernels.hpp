@@ -78,4 +78,40 @@ void launch_corner(const Launch& L, const CornerArgs<real>& a);
 template <class real>
 void launch_update(const Launch& L, const UpdateArgs<real>& a);
 
+// One whole VL stage (fluxes, corner EMFs, update) in one launch: k_vl_stage.  A workgroup
+// owns a tile of VS_TX x VS_TY (i, j) columns, of which the inner (VS_TX-2) x (VS_TY-2)
+// are updated, and marches a z-segment; fluxes and EMFs stay in LDS and registers.
+constexpr int VS_TX = 16, VS_TY = 16;
+struct TileBox {
+    int x0, y0;        // cell index of thread (0, 0) of tile (0, 0)
+    int zlo, zhi;      // planes marched (updates masked to the update range)
+    int nt[2];         // tiles along x, y (tile t starts at x0 + t (VS_TX - 2))
+    int nseg, seg;     // z-segments of seg planes
+    long long start;   // first workgroup of this box
+};
+struct TilePlan {
+    const TileBox* boxes;
+    int nbox;
+    long long total;   // workgroups
+};
+template <class real>
+struct StageArgs {
+    SweepArgs<real> d[3];   // flux state per direction (rotated fields, normal face B); F, Et unused
+    const real* qf[7];      // flux state, global order (centre EMFs, velocities)
+    StateP<real> src, dst;  // update base and output
+    const double* dt;
+    double factor;          // 0.5 predictor, 1 corrector
+    int ccb_lo_shift;
+    real* comp[4];          // Kahan terms (final stage of the FP32 path) or null
+    unsigned int* floor_count;
+    unsigned long long* cmax;   // non-null: CFL speed of the new interior cells
+    unsigned long long* smax;
+    double cs;
+    // index ranges per axis a, as [lo, n_a + hi]: faces normal / transverse, corners, updates
+    int fn_lo, fn_hi, ft_lo, ft_hi, c_lo, c_hi, u_lo, u_hi;
+};
+template <class real, int RS>
+void launch_vl_stage(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a,
+                     bool corrector);
+
 }  // namespace imhd

@@ -30,8 +30,8 @@ __device__ __forceinline__ void store_flux(const SweepArgs<real>& a, long long c
 // First-order predictor fluxes: interface states are the adjacent cell values with the
 // face-centred normal field (vanleer_mod.f90:443-467).
 template <class real, int RS>
-__device__ __forceinline__ double flux_pred_face(const SweepArgs<real>& a, long long c, long long s) {
-    real uL[7], uR[7], F[7];
+__device__ __forceinline__ real flux_pred_calc(const SweepArgs<real>& a, long long c, long long s, real* F) {
+    real uL[7], uR[7];
 #pragma unroll
     for (int r = 0; r < 7; r++) {
         if (r == 4) continue;
@@ -42,6 +42,12 @@ __device__ __forceinline__ double flux_pred_face(const SweepArgs<real>& a, long 
     uR[4] = uL[4];
     real smax = real(0);
     riemann<RS>(uL, uR, (real)a.cs, F, smax);
+    return smax;
+}
+template <class real, int RS>
+__device__ __forceinline__ double flux_pred_face(const SweepArgs<real>& a, long long c, long long s) {
+    real F[7];
+    const real smax = flux_pred_calc<real, RS>(a, c, s, F);
     store_flux(a, c, F);
     return (double)smax;
 }
@@ -62,7 +68,7 @@ __global__ __launch_bounds__(256) void k_flux_pred(const BoxDesc* __restrict__ b
 // sign(dC) min(beta|dL|, beta|dR|, |dC|)  -- no zeroing at extrema, as in the
 // reference (vanleer_mod.f90:1160-1215, 1467-1481) -- then the Riemann solver.
 template <class real, int RS>
-__device__ __forceinline__ double flux_corr_face(const SweepArgs<real>& a, long long c, long long s) {
+__device__ __forceinline__ real flux_corr_calc(const SweepArgs<real>& a, long long c, long long s, real* F) {
     real W[4][6];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
@@ -85,7 +91,7 @@ __device__ __forceinline__ double flux_corr_face(const SweepArgs<real>& a, long 
         dL = W[2][m] - W[1][m]; dR = W[3][m] - W[2][m]; dC = real(0.5) * (dL + dR);
         d2[m] = sgn1(dC) * min3(beta * fabs(dL), beta * fabs(dR), fabs(dC));
     }
-    real uL[7], uR[7], F[7];
+    real uL[7], uR[7];
     real wb = (real)a.bn[c];
     uL[0] = W[1][0] + real(0.5) * d1[0];
     uL[1] = W[1][1] + real(0.5) * d1[1];
@@ -105,6 +111,12 @@ __device__ __forceinline__ double flux_corr_face(const SweepArgs<real>& a, long 
     uR[1] = uR[1] * uR[0]; uR[2] = uR[2] * uR[0]; uR[3] = uR[3] * uR[0];
     real smax = real(0);
     riemann<RS>(uL, uR, (real)a.cs, F, smax);
+    return smax;
+}
+template <class real, int RS>
+__device__ __forceinline__ double flux_corr_face(const SweepArgs<real>& a, long long c, long long s) {
+    real F[7];
+    const real smax = flux_corr_calc<real, RS>(a, c, s, F);
     store_flux(a, c, F);
     return (double)smax;
 }
@@ -501,10 +513,15 @@ void launch_update(const Launch& L, const UpdateArgs<real>& a) {
         hipLaunchKernelGGL((k_update<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
 }
 
+}  // namespace imhd
+#include "vl_stage_impl.hpp"
+namespace imhd {
+
 #define IMHD_INST_RS(T, RS)                                                              \
     template void launch_flux_predictor<T, RS>(const Launch&, const SweepArgs<T>&); \
     template void launch_flux_corrector<T, RS>(const Launch&, const SweepArgs<T>&); \
-    template void launch_flux3<T, RS>(const Launch&, const Sweep3Args<T>&, int);
+    template void launch_flux3<T, RS>(const Launch&, const Sweep3Args<T>&, int);     \
+    template void launch_vl_stage<T, RS>(hipStream_t, const BoxDesc*, const TilePlan&, const StageArgs<T>&, bool);
 #define IMHD_INST_VL(T)                                                      \
     IMHD_INST_RS(T, RS_ROE)                                                  \
     IMHD_INST_RS(T, RS_HLLD)                                                 \

@@ -1,0 +1,259 @@
+// One VL2 + CT stage in one launch: directional fluxes (vanleer_mod.f90:429-938), upwind
+// corner EMFs (:1884-2076) and the conservative + CT update (:2080-2259), with the per-face
+// and per-corner arithmetic of flux_*_calc, k_corner and k_update_z, so every value has
+// the same bits as on the three-kernel path.  Fluxes, face EMFs and corner EMFs never go
+// to HBM: a workgroup holds a VS_TX x VS_TY tile of (i, j) columns (one-cell halo) and
+// marches a z-segment; per plane it
+//   A  solves the x- and y-faces of plane k+1 and the z-face k+1/2, and forms the centre
+//      EMFs and velocities of plane k+1 (LDS, double-buffered by plane parity);
+//   D  forms the corner EMFs Ez(k), Ey(k+1/2), Ex(k+1/2) and the x/y flux differences;
+//   G  updates the cells of plane k.
+// Every face, corner and cell is masked to the reference's own index range for that
+// quantity, so the reads and the signal-speed maximum are those of the separate kernels.
+#pragma once
+#include "kernels.hpp"
+
+namespace imhd {
+
+template <class real, int RS, int KIND, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_vl_stage(const BoxDesc* __restrict__ boxes, TilePlan plan,
+                                                      StageArgs<real> a) {
+    constexpr int N = TX * TY;
+    __shared__ real sF[2][2][4][N];    // [plane parity][x, y face][flux, global order]
+    __shared__ real sEf[2][2][N];      // [plane parity][x face: E(2,0); y face: E(2,1)]
+    __shared__ real sEz3[2][N];        // z face k+1/2: E(1,1), E(0,1)
+    __shared__ double sC[2][5][N];     // [plane parity][centre Ex, Ey, Ez, m_x/rho, m_y/rho]
+    __shared__ double sEz[N];          // corner Ez of plane k
+    __shared__ double sEc[2][2][N];    // [corner index parity][Ex, Ey] at k+1/2
+
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    int b = 0;
+    if (plan.nbox > 1) {
+        int lo = 0, hi = plan.nbox - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (plan.boxes[mid].start <= (long long)blk) lo = mid; else hi = mid - 1;
+        }
+        b = lo;
+    }
+    const TileBox& tb = plan.boxes[b];
+    const long long loc = (long long)blk - tb.start;
+    const int tX = (int)(loc % tb.nt[0]);
+    const long long l1 = loc / tb.nt[0];
+    const int tY = (int)(l1 % tb.nt[1]);
+    const int sg = (int)(l1 / tb.nt[1]);
+    const BoxDesc& B = boxes[b];
+    const int t = threadIdx.x, tx = t % TX, ty = t / TX;
+    const int i = tb.x0 + tX * (TX - 2) + tx, j = tb.y0 + tY * (TY - 2) + ty;
+    const int ks = tb.zlo + sg * tb.seg, ke = min(ks + tb.seg, tb.zhi + 1);
+    const int nb = B.nb, n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+    const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+    auto inr = [](int v, int lo, int hi) { return v >= lo && v <= hi; };
+    const bool cell_ij = inr(i, 1 - nb, n0 + nb) && inr(j, 1 - nb, n1 + nb);
+    const bool fx_ij = inr(i, a.fn_lo, n0 + a.fn_hi) && inr(j, a.ft_lo, n1 + a.ft_hi);
+    const bool fy_ij = inr(j, a.fn_lo, n1 + a.fn_hi) && inr(i, a.ft_lo, n0 + a.ft_hi);
+    const bool fz_ij = inr(i, a.ft_lo, n0 + a.ft_hi) && inr(j, a.ft_lo, n1 + a.ft_hi);
+    const bool c_ij = tx < TX - 1 && ty < TY - 1 && inr(i, a.c_lo, n0 + a.c_hi) && inr(j, a.c_lo, n1 + a.c_hi);
+    const bool u_ij = tx >= 1 && tx < TX - 1 && ty >= 1 && ty < TY - 1 && inr(i, a.u_lo, n0 + a.u_hi) &&
+                      inr(j, a.u_lo, n1 + a.u_hi);
+    const int sh = a.ccb_lo_shift;
+    const bool ccb_ij = i >= a.u_lo + sh && j >= a.u_lo + sh;
+    const bool int_ij = i >= 1 && j >= 1 && i <= n0 && j <= n1;
+    const double dtd = a.factor * a.dt[b] / B.dx;
+    const real dtdx = (real)dtd;
+
+    // centre EMFs and velocities of plane p (k_corner's Exc, Eyc, Ezc, m/rho); m_z/rho returned
+    auto cells = [&](int p) -> double {
+        double vz = 0.0;
+        const int q = p & 1;
+        if (cell_ij && inr(p, 1 - nb, n2 + nb)) {
+            const long long c = cell_index(B, i, j, p);
+            const double r = (double)a.qf[0][c], m1 = (double)a.qf[1][c], m2 = (double)a.qf[2][c],
+                         m3 = (double)a.qf[3][c], b4 = (double)a.qf[4][c], b5 = (double)a.qf[5][c],
+                         b6 = (double)a.qf[6][c];
+            const double ir = 1.0 / r;
+            sC[q][0][t] = (b5 * m3 - b6 * m2) * ir;
+            sC[q][1][t] = (b6 * m1 - b4 * m3) * ir;
+            sC[q][2][t] = (b4 * m2 - b5 * m1) * ir;
+            sC[q][3][t] = m1 / r;
+            sC[q][4][t] = m2 / r;
+            vz = m3 / r;
+        }
+        return vz;
+    };
+    auto flux = [&](int d, long long c, long long s, real* F) -> real {
+        if (KIND == 0) return flux_pred_calc<real, RS>(a.d[d], c, s, F);
+        return flux_corr_calc<real, RS>(a.d[d], c, s, F);
+    };
+    // x- and y-faces of plane p: fluxes and Ez slots to LDS; the Ey (x-face) and Ex (y-face)
+    // slots stay with the thread (only its own column reads them)
+    auto xyfaces = [&](int p, real& ey_x, real& ex_y, double& sm) {
+        const int q = p & 1;
+        const long long c = cell_index(B, i, j, p);
+        const bool pt = inr(p, a.ft_lo, n2 + a.ft_hi);
+        real F[7];
+        if (fx_ij && pt) {
+            sm = fmax(sm, (double)flux(0, c, sx, F));
+            sF[q][0][0][t] = F[0]; sF[q][0][1][t] = F[1]; sF[q][0][2][t] = F[2]; sF[q][0][3][t] = F[3];
+            sEf[q][0][t] = -F[5];
+            ey_x = F[6];
+        } else {
+            sF[q][0][0][t] = sF[q][0][1][t] = sF[q][0][2][t] = sF[q][0][3][t] = real(0);
+            sEf[q][0][t] = real(0);
+            ey_x = real(0);
+        }
+        if (fy_ij && pt) {
+            sm = fmax(sm, (double)flux(1, c, sy, F));
+            sF[q][1][0][t] = F[0]; sF[q][1][2][t] = F[1]; sF[q][1][3][t] = F[2]; sF[q][1][1][t] = F[3];
+            ex_y = -F[5];
+            sEf[q][1][t] = F[6];
+        } else {
+            sF[q][1][0][t] = sF[q][1][1][t] = sF[q][1][2][t] = sF[q][1][3][t] = real(0);
+            sEf[q][1][t] = real(0);
+            ex_y = real(0);
+        }
+    };
+    // z-face between planes p and p+1
+    auto zface = [&](int p, real* fz, double& sm) {
+        if (fz_ij && inr(p, a.fn_lo, n2 + a.fn_hi)) {
+            real F[7];
+            sm = fmax(sm, (double)flux(2, cell_index(B, i, j, p), sz, F));
+            fz[0] = F[0]; fz[3] = F[1]; fz[1] = F[2]; fz[2] = F[3];
+            sEz3[0][t] = -F[5];
+            sEz3[1][t] = F[6];
+        } else {
+            fz[0] = fz[1] = fz[2] = fz[3] = real(0);
+            sEz3[0][t] = sEz3[1][t] = real(0);
+        }
+    };
+
+    double sm = 0.0, cfl = 0.0;
+    real fz_lo[4], fz_hi[4];
+    real eyx_lo, eyx_hi, exy_lo, exy_hi;   // own x-face Ey slot and y-face Ex slot, planes k, k+1
+    double vz_lo = cells(ks - 1), vz_hi;
+    xyfaces(ks - 1, eyx_lo, exy_lo, sm);
+    double bz_m = 0.0;
+    bool have_bzm = false;
+#pragma unroll 1
+    for (int k = ks - 1; k < ke; k++) {
+        // A: plane k+1 and the z-face k+1/2
+        vz_hi = cells(k + 1);
+        xyfaces(k + 1, eyx_hi, exy_hi, sm);
+        zface(k, fz_hi, sm);
+        __syncthreads();
+        // D: corner EMFs of index k, x/y flux differences of plane k
+        const int q0 = k & 1, q1 = (k + 1) & 1;
+        if (c_ij && inr(k, a.c_lo, n2 + a.c_hi)) {
+            const double vx = 0.5 * (sC[q0][3][t] + sC[q0][3][t + 1]);
+            const double vy = 0.5 * (sC[q0][4][t] + sC[q0][4][t + TX]);
+            const double vz = 0.5 * (vz_lo + vz_hi);
+            double a14, a34, b14, b34;
+            {   // Ez at (i+1/2, j+1/2, k)
+                double e00 = sC[q0][2][t], e10 = sC[q0][2][t + 1], e01 = sC[q0][2][t + TX], e11 = sC[q0][2][t + TX + 1];
+                double y0 = sEf[q0][1][t], y1 = sEf[q0][1][t + 1], x0 = sEf[q0][0][t], x1 = sEf[q0][0][t + TX];
+                upw(vx, 2.0 * (y0 - e00), 2.0 * (y1 - e10), 2.0 * (e01 - y0), 2.0 * (e11 - y1), a14, a34);
+                upw(vy, 2.0 * (x0 - e00), 2.0 * (x1 - e01), 2.0 * (e10 - x0), 2.0 * (e11 - x1), b14, b34);
+                sEz[t] = 0.25 * (x0 + x1 + y0 + y1) + 0.125 * ((a14 - a34) + (b14 - b34));
+            }
+            {   // Ey at (i+1/2, j, k+1/2)
+                double e00 = sC[q0][1][t], e10 = sC[q0][1][t + 1], e01 = sC[q1][1][t], e11 = sC[q1][1][t + 1];
+                double z0 = sEz3[0][t], z1 = sEz3[0][t + 1], x0 = eyx_lo, x1 = eyx_hi;
+                upw(vx, 2.0 * (z0 - e00), 2.0 * (z1 - e10), 2.0 * (e01 - z0), 2.0 * (e11 - z1), a14, a34);
+                upw(vz, 2.0 * (x0 - e00), 2.0 * (x1 - e01), 2.0 * (e10 - x0), 2.0 * (e11 - x1), b14, b34);
+                sEc[q0][1][t] = 0.25 * (x0 + x1 + z0 + z1) + 0.125 * ((a14 - a34) + (b14 - b34));
+            }
+            {   // Ex at (i, j+1/2, k+1/2)
+                double e00 = sC[q0][0][t], e10 = sC[q0][0][t + TX], e01 = sC[q1][0][t], e11 = sC[q1][0][t + TX];
+                double z0 = sEz3[1][t], z1 = sEz3[1][t + TX], y0 = exy_lo, y1 = exy_hi;
+                upw(vy, 2.0 * (z0 - e00), 2.0 * (z1 - e10), 2.0 * (e01 - z0), 2.0 * (e11 - z1), a14, a34);
+                upw(vz, 2.0 * (y0 - e00), 2.0 * (y1 - e01), 2.0 * (e10 - y0), 2.0 * (e11 - y1), b14, b34);
+                sEc[q0][0][t] = 0.25 * (y0 + y1 + z0 + z1) + 0.125 * ((a14 - a34) + (b14 - b34));
+            }
+        }
+        const bool upd = k >= ks && u_ij && inr(k, a.u_lo, n2 + a.u_hi);
+        real dxy[4];
+        if (upd) {
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                dxy[f] = (sF[q0][0][f][t] - sF[q0][0][f][t - 1]) + (sF[q0][1][f][t] - sF[q0][1][f][t - TX]);
+        }
+        __syncthreads();
+        // G: update of plane k (k_update_z's arithmetic)
+        if (upd) {
+            const long long c = cell_index(B, i, j, k);
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                real div = dxy[f] + (fz_hi[f] - fz_lo[f]);
+                real q0v = a.src.q[f][c];
+                real v;
+                if (a.comp[0] != nullptr) {
+                    real y = -(dtdx * div) - a.comp[f][c];
+                    v = q0v + y;
+                    a.comp[f][c] = (v - q0v) - y;
+                } else {
+                    v = q0v - dtdx * div;
+                }
+                if (f == 0 && v < (real)MIN_DENS) {
+                    v = (real)MIN_DENS;
+                    if (a.comp[0] != nullptr) a.comp[0][c] = real(0);
+                    atomicAdd(a.floor_count, 1u);
+                }
+                a.dst.q[f][c] = v;
+            }
+            const int qm = (k - 1) & 1;
+            const double ex = sEc[q0][0][t], ey = sEc[q0][1][t], ez = sEz[t];
+            const double ez_y = sEz[t - TX], ez_x = sEz[t - 1], ey_x = sEc[q0][1][t - 1], ex_y = sEc[q0][0][t - TX];
+            const double ey_m = sEc[qm][1][t], ex_m = sEc[qm][0][t], eyx_m = sEc[qm][1][t - 1],
+                         exy_m = sEc[qm][0][t - TX];
+            double bx = a.src.b[0][c] - dtd * ((ez - ez_y) - (ey - ey_m));
+            double by = a.src.b[1][c] - dtd * ((ex - ex_m) - (ez - ez_x));
+            double bz = a.src.b[2][c] - dtd * ((ey - ey_x) - (ex - ex_y));
+            a.dst.b[0][c] = bx;
+            a.dst.b[1][c] = by;
+            a.dst.b[2][c] = bz;
+            if (ccb_ij && k >= a.u_lo + sh) {
+                const double ez_xy = sEz[t - 1 - TX];
+                double bxm = a.src.b[0][c - sx] - dtd * ((ez_x - ez_xy) - (ey_x - eyx_m));
+                double bym = a.src.b[1][c - sy] - dtd * ((ex_y - exy_m) - (ez_y - ez_xy));
+                if (!have_bzm) bz_m = a.src.b[2][c - sz] - dtd * ((ey_m - eyx_m) - (ex_m - exy_m));
+                a.dst.q[4][c] = (real)(0.5 * (bx + bxm));
+                a.dst.q[5][c] = (real)(0.5 * (by + bym));
+                a.dst.q[6][c] = (real)(0.5 * (bz + bz_m));
+            } else {
+                a.dst.q[4][c] = a.src.q[4][c];
+                a.dst.q[5][c] = a.src.q[5][c];
+                a.dst.q[6][c] = a.src.q[6][c];
+            }
+            bz_m = bz;
+            have_bzm = true;
+            if (a.cmax != nullptr && int_ij && k >= 1 && k <= n2) {
+                double q7[7];
+                for (int f = 0; f < 7; f++) q7[f] = (double)a.dst.q[f][c];
+                cfl = fmax(cfl, cell_fast_speed(q7, a.cs));
+            }
+        } else {
+            have_bzm = false;
+        }
+#pragma unroll
+        for (int f = 0; f < 4; f++) fz_lo[f] = fz_hi[f];
+        eyx_lo = eyx_hi;
+        exy_lo = exy_hi;
+        vz_lo = vz_hi;
+    }
+    atomic_max_nonneg(a.smax, b, sm);
+    if (a.cmax != nullptr) atomic_max_nonneg(a.cmax, b, cfl);
+}
+
+template <class real, int RS>
+void launch_vl_stage(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a,
+                     bool corrector) {
+    if (plan.total <= 0) return;
+    if (corrector)
+        hipLaunchKernelGGL((k_vl_stage<real, RS, 1, VS_TX, VS_TY>), dim3((unsigned)plan.total), dim3(VS_TX * VS_TY), 0,
+                           st, boxes, plan, a);
+    else
+        hipLaunchKernelGGL((k_vl_stage<real, RS, 0, VS_TX, VS_TY>), dim3((unsigned)plan.total), dim3(VS_TX * VS_TY), 0,
+                           st, boxes, plan, a);
+}
+
+}  // namespace imhd
