@@ -141,7 +141,7 @@ def roofline(prof, esz):
 FAMILY_SYMBOL = {"stage_pred": "k_vl_stage", "stage_corr_cfl": "k_vl_stage",
                  "flux_pred": "k_flux3", "flux_corr": "k_flux3", "corner_pred": "k_corner",
                  "corner_corr": "k_corner", "update_pred": "k_update_z", "update_corr_cfl": "k_update_z",
-                 "pass_stage_pred": "k_pass_stage", "pass_stage_corr": "k_pass_stage",
+                 "pass_stage_pred": "k_pass_stage_z", "pass_stage_corr": "k_pass_stage_z",
                  "fill_ghosts": "k_fill_ghost_band", "fill_ghosts_pass": "k_fill_ghost_band"}
 
 

@@ -116,6 +116,7 @@ struct imhd_ctx {
     PlanDev pl_pflux3, pl_cflux3;
     bool vstage = true;  // VL stages as single fused launches (k_vl_stage)
     int vseg = 16;       // z-segment length of the fused stages
+    int vshape = 0;      // tile shape of the fused stages (0: 16 x 16, 1: 32 x 16; FP32 default 1)
     TilePlanDev tp_pred, tp_corr;
     PlanDev pl_tcorner_p, pl_tcorner_c, pl_tcenter, pl_thalf, pl_tcfl;
     int ccb_shift = 1;
@@ -273,7 +274,7 @@ int build_tiles(imhd_ctx* c, TilePlanDev& p, bool corr) {
         TileBox tb{};
         tb.x0 = lo;
         tb.y0 = lo;
-        const int wx[2] = {VS_TX - 2, VS_TY - 2};
+        const int wx[2] = {vs_tx(c->vshape) - 2, vs_ty(c->vshape) - 2};
         for (int a = 0; a < 2; a++) {
             const int need = std::max(B.n[a] + fhi - 1, B.n[a] + R.u_hi) - lo;
             tb.nt[a] = (need + wx[a] - 1) / wx[a];
@@ -434,9 +435,9 @@ void vl_stage(imhd_ctx* c, int sflux, int src, int dst, bool corr, bool fuse_cma
     a.c_lo = r.c_lo; a.c_hi = r.c_hi; a.u_lo = r.u_lo; a.u_hi = r.u_hi;
     const TilePlan tp{p.d, c->nbox, p.total};
     switch (c->cfg.riemann) {
-        case IMHD_ROE: launch_vl_stage<real, RS_ROE>(c->st, c->db, tp, a, corr); break;
-        case IMHD_HLLE: launch_vl_stage<real, RS_HLLE>(c->st, c->db, tp, a, corr); break;
-        default: launch_vl_stage<real, RS_HLLD>(c->st, c->db, tp, a, corr); break;
+        case IMHD_ROE: launch_vl_stage<real, RS_ROE>(c->st, c->db, tp, a, corr, c->vshape); break;
+        case IMHD_HLLE: launch_vl_stage<real, RS_HLLE>(c->st, c->db, tp, a, corr, c->vshape); break;
+        default: launch_vl_stage<real, RS_HLLD>(c->st, c->db, tp, a, corr, c->vshape); break;
     }
 }
 
@@ -956,6 +957,10 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     if (const char* e = getenv("IMHD_STRIP")) c->strip = std::max(0, atoi(e));
     if (const char* e = getenv("IMHD_VSTAGE")) c->vstage = atoi(e) != 0;    // 0: three kernels per stage
     if (const char* e = getenv("IMHD_VSEG")) c->vseg = std::max(1, atoi(e));
+    // FP32: 32 x 16 tiles (2 workgroups of 80 KB LDS per CU); FP64: 16 x 16 (a 32 x 16
+    // FP64 tile would leave one workgroup per CU)
+    c->vshape = g.precision == 32 ? 1 : 0;
+    if (const char* e = getenv("IMHD_VSHAPE")) c->vshape = std::min(VS_SHAPES - 1, std::max(0, atoi(e)));
     long long run = 0;
     for (int bx = 0; bx < nbox; bx++) {
         const imhd_config& k = cfg[bx];
@@ -1212,25 +1217,35 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
         c->pcur = pcur0;
         return 0;
     };
-    auto set_shape = [&](int zseg, int strip) -> int {
-        c->zseg = zseg;
-        c->strip = strip;
-        int r = build_plan(c, c->pl_pupd, 2, -1, zseg);
-        if (!r) r = build_plan(c, c->pl_cupd, 5, -1, zseg);
-        if (!r) r = build_plan(c, c->pl_pcorner, 1, -1, zseg);
-        if (!r) r = build_plan(c, c->pl_ccorner, 4, -1, zseg);
-        if (!r) r = build_plan(c, c->pl_pstage[0], 11, -1, zseg);
-        if (!r) r = build_plan(c, c->pl_pstage[1], 13, -1, zseg);
+    // launch shape: z-segment of the column kernels, y-strip of the work enumeration, and
+    // tile shape and z-segment of the fused VL stages
+    struct Shape {
+        int zseg, strip, vshape, vseg;
+    };
+    const bool vl = c->tp_pred.d != nullptr;
+    auto set_shape = [&](const Shape& sh) -> int {
+        c->zseg = sh.zseg;
+        c->strip = sh.strip;
+        c->vshape = sh.vshape;
+        c->vseg = sh.vseg;
+        int r = build_plan(c, c->pl_pupd, 2, -1, sh.zseg);
+        if (!r) r = build_plan(c, c->pl_cupd, 5, -1, sh.zseg);
+        if (!r) r = build_plan(c, c->pl_pcorner, 1, -1, sh.zseg);
+        if (!r) r = build_plan(c, c->pl_ccorner, 4, -1, sh.zseg);
+        if (!r) r = build_plan(c, c->pl_pstage[0], 11, -1, sh.zseg);
+        if (!r) r = build_plan(c, c->pl_pstage[1], 13, -1, sh.zseg);
+        if (!r && vl) r = build_tiles(c, c->tp_pred, false);
+        if (!r && vl) r = build_tiles(c, c->tp_corr, true);
         for (auto& g : c->gexec) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
         return r;
     };
     std::string rep;
-    int best_z = c->zseg, best_s = c->strip;
+    Shape bsh{c->zseg, c->strip, c->vshape, c->vseg};
     float best = 1e30f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!rc) { HIPCK(c, hipEventCreate(&e0)); HIPCK(c, hipEventCreate(&e1)); }
-    auto trial = [&](int zseg, int strip) -> int {
-        int r = set_shape(zseg, strip);
+    auto trial = [&](const Shape& sh) -> int {
+        int r = set_shape(sh);
         if (!r) r = restore();
         if (r) return r;
         for (int s = 0; s < 1; s++) { if (c->prec == 64) native_step<double>(c, 1e300); else native_step<float>(c, 1e300); }
@@ -1242,27 +1257,32 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
         float ms = 0.f;
         HIPCK(c, hipEventElapsedTime(&ms, e0, e1));
         ms /= (float)steps;
-        char b[96];
-        snprintf(b, sizeof(b), "zseg=%d strip=%d: %.4f ms/step; ", zseg, strip, ms);
+        char b[128];
+        snprintf(b, sizeof(b), "zseg=%d strip=%d tile=%d vseg=%d: %.4f ms/step; ", sh.zseg, sh.strip, sh.vshape, sh.vseg, ms);
         rep += b;
-        if (ms < best) { best = ms; best_z = zseg; best_s = strip; }
+        if (ms < best) { best = ms; bsh = sh; }
         return 0;
     };
-    // coordinate search: strip height at the current z-segment, then the z-segment
+    // coordinate search, one parameter at a time from the current best
     static const int strips[] = {0, 8, 16, 32, 64};
     static const int zsegs[] = {0, 2, 4, 8, 16};
-    const int z0 = c->zseg;
-    for (int s : strips) if (!rc) rc = trial(z0, s);
-    const int s_best = best_s;
-    for (int z : zsegs) if (!rc && z != z0) rc = trial(z, s_best);
-    if (!rc) rc = set_shape(best_z, best_s);
+    static const int vsegs[] = {8, 16, 32};
+    const Shape s0 = bsh;
+    for (int s : strips) if (!rc) rc = trial({s0.zseg, s, s0.vshape, s0.vseg});
+    { const Shape b0 = bsh; for (int z : zsegs) if (!rc && z != b0.zseg) rc = trial({z, b0.strip, b0.vshape, b0.vseg}); }
+    if (vl) {
+        { const Shape b0 = bsh; for (int v = 0; v < VS_SHAPES; v++) if (!rc && v != b0.vshape) rc = trial({b0.zseg, b0.strip, v, b0.vseg}); }
+        { const Shape b0 = bsh; for (int v : vsegs) if (!rc && v != b0.vseg) rc = trial({b0.zseg, b0.strip, b0.vshape, v}); }
+    }
+    if (!rc) rc = set_shape(bsh);
     if (!rc) rc = restore();
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     for (auto& sv : saved) (void)hipFree(sv.second);
     if (rc) return rc;
-    char b[96];
-    snprintf(b, sizeof(b), "chosen zseg=%d strip=%d (%.4f ms/step)", best_z, best_s, best);
+    char b[128];
+    snprintf(b, sizeof(b), "chosen zseg=%d strip=%d tile=%d vseg=%d (%.4f ms/step)", bsh.zseg, bsh.strip, bsh.vshape,
+             bsh.vseg, best);
     rep += b;
     c->prof_report = rep;
     if (report && report_len > 0) {

@@ -79,13 +79,16 @@ template <class real>
 void launch_update(const Launch& L, const UpdateArgs<real>& a);
 
 // One whole VL stage (fluxes, corner EMFs, update) in one launch: k_vl_stage.  A workgroup
-// owns a tile of VS_TX x VS_TY (i, j) columns, of which the inner (VS_TX-2) x (VS_TY-2)
-// are updated, and marches a z-segment; fluxes and EMFs stay in LDS and registers.
-constexpr int VS_TX = 16, VS_TY = 16;
+// owns a tile of TX x TY (i, j) columns, of which the inner (TX-2) x (TY-2) are updated,
+// and marches a z-segment; fluxes and EMFs stay in LDS and registers.  Tile shapes:
+// 0 = 16 x 16 (256 threads), 1 = 32 x 16 (512 threads).
+constexpr int VS_SHAPES = 2;
+constexpr int vs_tx(int shape) { return shape == 1 ? 32 : 16; }
+constexpr int vs_ty(int shape) { return 16; }
 struct TileBox {
     int x0, y0;        // cell index of thread (0, 0) of tile (0, 0)
     int zlo, zhi;      // planes marched (updates masked to the update range)
-    int nt[2];         // tiles along x, y (tile t starts at x0 + t (VS_TX - 2))
+    int nt[2];         // tiles along x, y (tile t starts at x0 + t (TX - 2))
     int nseg, seg;     // z-segments of seg planes
     long long start;   // first workgroup of this box
 };
@@ -112,6 +115,6 @@ struct StageArgs {
 };
 template <class real, int RS>
 void launch_vl_stage(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a,
-                     bool corrector);
+                     bool corrector, int shape);
 
 }  // namespace imhd

@@ -2,12 +2,15 @@
 // corner EMFs (:1884-2076) and the conservative + CT update (:2080-2259), with the per-face
 // and per-corner arithmetic of flux_*_calc, k_corner and k_update_z, so every value has
 // the same bits as on the three-kernel path.  Fluxes, face EMFs and corner EMFs never go
-// to HBM: a workgroup holds a VS_TX x VS_TY tile of (i, j) columns (one-cell halo) and
+// to HBM: a workgroup holds a TX x TY tile of (i, j) columns (one-cell halo) and
 // marches a z-segment; per plane it
 //   A  solves the x- and y-faces of plane k+1 and the z-face k+1/2, and forms the centre
-//      EMFs and velocities of plane k+1 (LDS, double-buffered by plane parity);
-//   D  forms the corner EMFs Ez(k), Ey(k+1/2), Ex(k+1/2) and the x/y flux differences;
-//   G  updates the cells of plane k.
+//      EMFs and velocities of plane k+1 (face EMFs and centre values double-buffered by
+//      plane parity in LDS);
+//   D  forms the corner EMFs Ez(k), Ey(k+1/2), Ex(k+1/2), and the x/y flux differences of
+//      plane k+1 while its fluxes are in LDS;
+//   G  updates the cells of plane k, keeping the index-k corner EMFs it reads for plane k+1.
+// Two barriers per plane; FP32 LDS: 40 KB per 16 x 16 tile, 80 KB per 32 x 16 tile.
 // Every face, corner and cell is masked to the reference's own index range for that
 // quantity, so the reads and the signal-speed maximum are those of the separate kernels.
 #pragma once
@@ -15,16 +18,29 @@
 
 namespace imhd {
 
+// waves per SIMD the register allocation must allow: in FP32 the LDS of a 16 x 16 tile
+// (40 KB) fits 4 workgroups per CU and that of a 32 x 16 tile (80 KB) 2, 16 waves per CU
+// either way; FP64 needs more of both and gets 2 waves per SIMD
+template <class real>
+constexpr int vs_waves() { return sizeof(real) == 4 ? 4 : 1; }
+#define VS_WAVES vs_waves<real>()
+
 template <class real, int RS, int KIND, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void k_vl_stage(const BoxDesc* __restrict__ boxes, TilePlan plan,
+__global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* __restrict__ boxes, TilePlan plan,
                                                       StageArgs<real> a) {
     constexpr int N = TX * TY;
-    __shared__ real sF[2][2][4][N];    // [plane parity][x, y face][flux, global order]
+    // FP32 keeps one buffer of the x/y fluxes and of the corner Ex, Ey (forming the flux
+    // differences one plane ahead and carrying the index k-1 corners in registers): 40 KB of
+    // LDS per 16 x 16 tile.  FP64, which registers limit to 2 waves per SIMD anyway, keeps
+    // two (by plane / index parity) and fewer live registers.
+    constexpr bool ONE = sizeof(real) == 4;
+    constexpr int NB = ONE ? 1 : 2;
+    __shared__ real sF[NB][2][4][N];   // [buffer][x, y face][flux, global order]
     __shared__ real sEf[2][2][N];      // [plane parity][x face: E(2,0); y face: E(2,1)]
     __shared__ real sEz3[2][N];        // z face k+1/2: E(1,1), E(0,1)
     __shared__ double sC[2][5][N];     // [plane parity][centre Ex, Ey, Ez, m_x/rho, m_y/rho]
     __shared__ double sEz[N];          // corner Ez of plane k
-    __shared__ double sEc[2][2][N];    // [corner index parity][Ex, Ey] at k+1/2
+    __shared__ double sEc[NB][2][N];   // [buffer][Ex, Ey] corner EMFs of index k (at k+1/2)
 
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
     int b = 0;
@@ -88,27 +104,27 @@ __global__ __launch_bounds__(TX * TY) void k_vl_stage(const BoxDesc* __restrict_
     // x- and y-faces of plane p: fluxes and Ez slots to LDS; the Ey (x-face) and Ex (y-face)
     // slots stay with the thread (only its own column reads them)
     auto xyfaces = [&](int p, real& ey_x, real& ex_y, double& sm) {
-        const int q = p & 1;
+        const int q = p & 1, fq = ONE ? 0 : q;
         const long long c = cell_index(B, i, j, p);
         const bool pt = inr(p, a.ft_lo, n2 + a.ft_hi);
         real F[7];
         if (fx_ij && pt) {
             sm = fmax(sm, (double)flux(0, c, sx, F));
-            sF[q][0][0][t] = F[0]; sF[q][0][1][t] = F[1]; sF[q][0][2][t] = F[2]; sF[q][0][3][t] = F[3];
+            sF[fq][0][0][t] = F[0]; sF[fq][0][1][t] = F[1]; sF[fq][0][2][t] = F[2]; sF[fq][0][3][t] = F[3];
             sEf[q][0][t] = -F[5];
             ey_x = F[6];
         } else {
-            sF[q][0][0][t] = sF[q][0][1][t] = sF[q][0][2][t] = sF[q][0][3][t] = real(0);
+            sF[fq][0][0][t] = sF[fq][0][1][t] = sF[fq][0][2][t] = sF[fq][0][3][t] = real(0);
             sEf[q][0][t] = real(0);
             ey_x = real(0);
         }
         if (fy_ij && pt) {
             sm = fmax(sm, (double)flux(1, c, sy, F));
-            sF[q][1][0][t] = F[0]; sF[q][1][2][t] = F[1]; sF[q][1][3][t] = F[2]; sF[q][1][1][t] = F[3];
+            sF[fq][1][0][t] = F[0]; sF[fq][1][2][t] = F[1]; sF[fq][1][3][t] = F[2]; sF[fq][1][1][t] = F[3];
             ex_y = -F[5];
             sEf[q][1][t] = F[6];
         } else {
-            sF[q][1][0][t] = sF[q][1][1][t] = sF[q][1][2][t] = sF[q][1][3][t] = real(0);
+            sF[fq][1][0][t] = sF[fq][1][1][t] = sF[fq][1][2][t] = sF[fq][1][3][t] = real(0);
             sEf[q][1][t] = real(0);
             ex_y = real(0);
         }
@@ -134,6 +150,11 @@ __global__ __launch_bounds__(TX * TY) void k_vl_stage(const BoxDesc* __restrict_
     xyfaces(ks - 1, eyx_lo, exy_lo, sm);
     double bz_m = 0.0;
     bool have_bzm = false;
+    // carried from the update phase of plane k-1: corner EMFs of index k-1 (own and the
+    // i-1 / j-1 neighbours), and the x/y flux differences of plane k (formed one plane ahead,
+    // while the fluxes of that plane are in LDS)
+    double ey_m = 0.0, ex_m = 0.0, eyx_m = 0.0, exy_m = 0.0;
+    real dxy[4] = {real(0), real(0), real(0), real(0)}, dxy_n[4];
 #pragma unroll 1
     for (int k = ks - 1; k < ke; k++) {
         // A: plane k+1 and the z-face k+1/2
@@ -141,8 +162,8 @@ __global__ __launch_bounds__(TX * TY) void k_vl_stage(const BoxDesc* __restrict_
         xyfaces(k + 1, eyx_hi, exy_hi, sm);
         zface(k, fz_hi, sm);
         __syncthreads();
-        // D: corner EMFs of index k, x/y flux differences of plane k
-        const int q0 = k & 1, q1 = (k + 1) & 1;
+        // D: corner EMFs of index k, x/y flux differences
+        const int q0 = k & 1, q1 = (k + 1) & 1, eq = ONE ? 0 : q0;
         if (c_ij && inr(k, a.c_lo, n2 + a.c_hi)) {
             const double vx = 0.5 * (sC[q0][3][t] + sC[q0][3][t + 1]);
             const double vy = 0.5 * (sC[q0][4][t] + sC[q0][4][t + TX]);
@@ -160,19 +181,24 @@ __global__ __launch_bounds__(TX * TY) void k_vl_stage(const BoxDesc* __restrict_
                 double z0 = sEz3[0][t], z1 = sEz3[0][t + 1], x0 = eyx_lo, x1 = eyx_hi;
                 upw(vx, 2.0 * (z0 - e00), 2.0 * (z1 - e10), 2.0 * (e01 - z0), 2.0 * (e11 - z1), a14, a34);
                 upw(vz, 2.0 * (x0 - e00), 2.0 * (x1 - e01), 2.0 * (e10 - x0), 2.0 * (e11 - x1), b14, b34);
-                sEc[q0][1][t] = 0.25 * (x0 + x1 + z0 + z1) + 0.125 * ((a14 - a34) + (b14 - b34));
+                sEc[eq][1][t] = 0.25 * (x0 + x1 + z0 + z1) + 0.125 * ((a14 - a34) + (b14 - b34));
             }
             {   // Ex at (i, j+1/2, k+1/2)
                 double e00 = sC[q0][0][t], e10 = sC[q0][0][t + TX], e01 = sC[q1][0][t], e11 = sC[q1][0][t + TX];
                 double z0 = sEz3[1][t], z1 = sEz3[1][t + TX], y0 = exy_lo, y1 = exy_hi;
                 upw(vy, 2.0 * (z0 - e00), 2.0 * (z1 - e10), 2.0 * (e01 - z0), 2.0 * (e11 - z1), a14, a34);
                 upw(vz, 2.0 * (y0 - e00), 2.0 * (y1 - e01), 2.0 * (e10 - y0), 2.0 * (e11 - y1), b14, b34);
-                sEc[q0][0][t] = 0.25 * (y0 + y1 + z0 + z1) + 0.125 * ((a14 - a34) + (b14 - b34));
+                sEc[eq][0][t] = 0.25 * (y0 + y1 + z0 + z1) + 0.125 * ((a14 - a34) + (b14 - b34));
             }
         }
         const bool upd = k >= ks && u_ij && inr(k, a.u_lo, n2 + a.u_hi);
-        real dxy[4];
-        if (upd) {
+        if (ONE) {   // plane k+1, while its fluxes are in the single buffer
+            if (u_ij) {
+#pragma unroll
+                for (int f = 0; f < 4; f++)
+                    dxy_n[f] = (sF[0][0][f][t] - sF[0][0][f][t - 1]) + (sF[0][1][f][t] - sF[0][1][f][t - TX]);
+            }
+        } else if (upd) {   // plane k, from its parity buffer
 #pragma unroll
             for (int f = 0; f < 4; f++)
                 dxy[f] = (sF[q0][0][f][t] - sF[q0][0][f][t - 1]) + (sF[q0][1][f][t] - sF[q0][1][f][t - TX]);
@@ -200,11 +226,12 @@ __global__ __launch_bounds__(TX * TY) void k_vl_stage(const BoxDesc* __restrict_
                 }
                 a.dst.q[f][c] = v;
             }
-            const int qm = (k - 1) & 1;
-            const double ex = sEc[q0][0][t], ey = sEc[q0][1][t], ez = sEz[t];
-            const double ez_y = sEz[t - TX], ez_x = sEz[t - 1], ey_x = sEc[q0][1][t - 1], ex_y = sEc[q0][0][t - TX];
-            const double ey_m = sEc[qm][1][t], ex_m = sEc[qm][0][t], eyx_m = sEc[qm][1][t - 1],
-                         exy_m = sEc[qm][0][t - TX];
+            const double ex = sEc[eq][0][t], ey = sEc[eq][1][t], ez = sEz[t];
+            const double ez_y = sEz[t - TX], ez_x = sEz[t - 1], ey_x = sEc[eq][1][t - 1], ex_y = sEc[eq][0][t - TX];
+            if (!ONE) {
+                const int qm = (k - 1) & 1;
+                ey_m = sEc[qm][1][t]; ex_m = sEc[qm][0][t]; eyx_m = sEc[qm][1][t - 1]; exy_m = sEc[qm][0][t - TX];
+            }
             double bx = a.src.b[0][c] - dtd * ((ez - ez_y) - (ey - ey_m));
             double by = a.src.b[1][c] - dtd * ((ex - ex_m) - (ez - ez_x));
             double bz = a.src.b[2][c] - dtd * ((ey - ey_x) - (ex - ex_y));
@@ -226,6 +253,7 @@ __global__ __launch_bounds__(TX * TY) void k_vl_stage(const BoxDesc* __restrict_
             }
             bz_m = bz;
             have_bzm = true;
+            ey_m = ey; ex_m = ex; eyx_m = ey_x; exy_m = ex_y;
             if (a.cmax != nullptr && int_ij && k >= 1 && k <= n2) {
                 double q7[7];
                 for (int f = 0; f < 7; f++) q7[f] = (double)a.dst.q[f][c];
@@ -233,9 +261,13 @@ __global__ __launch_bounds__(TX * TY) void k_vl_stage(const BoxDesc* __restrict_
             }
         } else {
             have_bzm = false;
+            if (ONE && u_ij) { ey_m = sEc[0][1][t]; ex_m = sEc[0][0][t]; eyx_m = sEc[0][1][t - 1]; exy_m = sEc[0][0][t - TX]; }
         }
 #pragma unroll
-        for (int f = 0; f < 4; f++) fz_lo[f] = fz_hi[f];
+        for (int f = 0; f < 4; f++) {
+            fz_lo[f] = fz_hi[f];
+            if (ONE) dxy[f] = dxy_n[f];
+        }
         eyx_lo = eyx_hi;
         exy_lo = exy_hi;
         vz_lo = vz_hi;
@@ -244,16 +276,23 @@ __global__ __launch_bounds__(TX * TY) void k_vl_stage(const BoxDesc* __restrict_
     if (a.cmax != nullptr) atomic_max_nonneg(a.cmax, b, cfl);
 }
 
+template <class real, int RS, int SHAPE>
+void launch_vl_stage_t(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a,
+                       bool corrector) {
+    constexpr int TX = vs_tx(SHAPE), TY = vs_ty(SHAPE);
+    if (corrector)
+        hipLaunchKernelGGL((k_vl_stage<real, RS, 1, TX, TY>), dim3((unsigned)plan.total), dim3(TX * TY), 0, st, boxes,
+                           plan, a);
+    else
+        hipLaunchKernelGGL((k_vl_stage<real, RS, 0, TX, TY>), dim3((unsigned)plan.total), dim3(TX * TY), 0, st, boxes,
+                           plan, a);
+}
 template <class real, int RS>
 void launch_vl_stage(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a,
-                     bool corrector) {
+                     bool corrector, int shape) {
     if (plan.total <= 0) return;
-    if (corrector)
-        hipLaunchKernelGGL((k_vl_stage<real, RS, 1, VS_TX, VS_TY>), dim3((unsigned)plan.total), dim3(VS_TX * VS_TY), 0,
-                           st, boxes, plan, a);
-    else
-        hipLaunchKernelGGL((k_vl_stage<real, RS, 0, VS_TX, VS_TY>), dim3((unsigned)plan.total), dim3(VS_TX * VS_TY), 0,
-                           st, boxes, plan, a);
+    if (shape == 1) launch_vl_stage_t<real, RS, 1>(st, boxes, plan, a, corrector);
+    else launch_vl_stage_t<real, RS, 0>(st, boxes, plan, a, corrector);
 }
 
 }  // namespace imhd
