@@ -1,0 +1,85 @@
+"""The fused VL stage kernel (one launch per stage: directional fluxes, corner EMFs and the
+update) against the three-kernel path it replaces (k_flux3 -> k_corner -> k_update_z), and
+its two tile shapes against each other.  FP64 must agree bit for bit over native-loop runs
+(ghost fills, passive scalars, the CFL dt sequence and the density-floor count included);
+FP32 within rounding.  The path and shape are chosen per context from IMHD_VSTAGE and
+IMHD_VSHAPE, which imhd_create reads."""
+import sys
+
+import numpy as np
+import pytest
+
+from refutil import random_state
+
+pytestmark = pytest.mark.gpu
+
+# dims, riemann, boundary, reconstruction, passives: multi-tile boxes in x and y, several
+# z-segments, non-cubic and thin boxes, both boundaries, all three solvers, PPM (fused
+# predictor with the three-kernel corrector) and PLM (both stages fused)
+CASES = [((40, 36, 40), "hlld", "periodic", "plm", 2),
+         ((13, 9, 11), "roe", "open", "plm", 1),
+         ((34, 17, 9), "hlle", "open", "plm", 0),
+         ((16, 16, 16), "hlld", "periodic", "ppm_char", 1),
+         ((12, 20, 7), "roe", "periodic", "ppm_prim", 0)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run(monkeypatch, env, dims, riemann, bc, recon, npass, precision=64, steps=6, seed=41):
+    from isothermal_mhd_amd import VanLeerSolver
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    try:
+        S = VanLeerSolver(dims, npass=npass, riemann=riemann, recon=recon, bc=bc, precision=precision)
+    finally:
+        for k in env:
+            monkeypatch.delenv(k)
+    q, bf, qp = random_state(*dims, npass=npass, seed=seed)
+    S.set_state(q, bf, qp if npass else None)
+    S.boundary_conditions()
+    S.run(steps, graph=False)
+    out = S.get_state(), S.get_time(), S.floor_count()
+    S.close()
+    return out
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[1]}_{c[2]}_{c[3]}_{'x'.join(map(str, c[0]))}" for c in CASES])
+def test_fused_stage_equals_three_kernel_path_fp64(monkeypatch, case):
+    dims, riemann, bc, recon, npass = case
+    say(f"[gpu] fused vs three-kernel {case}")
+    ref = run(monkeypatch, {"IMHD_VSTAGE": "0"}, *case)
+    for shape in ("0", "1"):
+        got = run(monkeypatch, {"IMHD_VSTAGE": "1", "IMHD_VSHAPE": shape}, *case)
+        for x, y in zip(ref[0], got[0]):
+            if x is not None:
+                assert np.array_equal(x, y), f"state differs (tile shape {shape})"
+        assert np.array_equal(ref[1][0], got[1][0]) and np.array_equal(ref[1][1], got[1][1]), "t / dt differ"
+        assert ref[2] == got[2]
+
+
+def test_fused_stage_segments_fp64(monkeypatch):
+    """z-segment length changes only the work split (segment overlap planes are solved
+    twice); results are identical."""
+    case = CASES[0]
+    say("[gpu] fused stage z-segments")
+    ref = run(monkeypatch, {"IMHD_VSEG": "16"}, *case)
+    for seg in ("1", "5", "64"):
+        got = run(monkeypatch, {"IMHD_VSEG": seg}, *case)
+        for x, y in zip(ref[0], got[0]):
+            if x is not None:
+                assert np.array_equal(x, y), f"state differs (segment {seg})"
+
+
+def test_fused_stage_fp32_matches_three_kernel_path(monkeypatch):
+    """FP32: the fused kernel's arithmetic per value is the three-kernel path's; FMA
+    contraction may differ between the two compilations, so agreement is to rounding."""
+    dims, riemann, bc, recon, npass = CASES[0]
+    say("[gpu] fused vs three-kernel fp32")
+    ref = run(monkeypatch, {"IMHD_VSTAGE": "0"}, dims, riemann, bc, recon, npass, precision=32)
+    got = run(monkeypatch, {"IMHD_VSTAGE": "1"}, dims, riemann, bc, recon, npass, precision=32)
+    for x, y in zip(ref[0][:2], got[0][:2]):
+        assert np.max(np.abs(x - y)) <= 1e-5 * max(1.0, np.max(np.abs(x)))
+    assert abs(ref[1][1][0] - got[1][1][0]) <= 1e-5 * ref[1][1][0]
