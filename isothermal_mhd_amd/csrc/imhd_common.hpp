@@ -153,6 +153,27 @@ __device__ __forceinline__ double cell_fast_speed(const double* q, double cs) {
     return m > fabs(vz) + cfz ? m : fabs(vz) + cfz;
 }
 
+// The same speed in FP32 with the hardware reciprocal, reciprocal square root and square
+// root: the FP32 path's per-cell CFL speed (its maximum over cells is still taken in FP64).
+__device__ __forceinline__ float cell_fast_speed_f(const float* q, float cs) {
+    float asqr = cs * cs;
+    float ir = __builtin_amdgcn_rcpf(q[0]);
+    float isrho = __builtin_amdgcn_rsqf(q[0]);
+    float bx = q[4] * isrho, by = q[5] * isrho, bz = q[6] * isrho;
+    float btsqr = by * by + bz * bz, bsqr = bx * bx + btsqr, x = asqr - bsqr;
+    float tmp = __builtin_amdgcn_sqrtf(x * x + 4.0f * asqr * btsqr);
+    float cfx = __builtin_amdgcn_sqrtf(0.5f * (asqr + bsqr + tmp));
+    btsqr = bx * bx + bz * bz; bsqr = by * by + btsqr; x = asqr - bsqr;
+    tmp = __builtin_amdgcn_sqrtf(x * x + 4.0f * asqr * btsqr);
+    float cfy = __builtin_amdgcn_sqrtf(0.5f * (asqr + bsqr + tmp));
+    btsqr = bx * bx + by * by; bsqr = bz * bz + btsqr; x = asqr - bsqr;
+    tmp = __builtin_amdgcn_sqrtf(x * x + 4.0f * asqr * btsqr);
+    float cfz = __builtin_amdgcn_sqrtf(0.5f * (asqr + bsqr + tmp));
+    float m = fabsf(q[1] * ir) + cfx;
+    m = fmaxf(m, fabsf(q[2] * ir) + cfy);
+    return fmaxf(m, fabsf(q[3] * ir) + cfz);
+}
+
 template <class real>
 __device__ __forceinline__ real sgn1(real x) { return copysign(real(1), x); }  // Fortran SIGN(1.d0, x)
 

@@ -91,9 +91,15 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             sC[q][0][t] = (b5 * m3 - b6 * m2) * ir;
             sC[q][1][t] = (b6 * m1 - b4 * m3) * ir;
             sC[q][2][t] = (b4 * m2 - b5 * m1) * ir;
-            sC[q][3][t] = m1 / r;
-            sC[q][4][t] = m2 / r;
-            vz = m3 / r;
+            if (ONE) {   // FP32 path: m (1/rho) for the velocities (FP64 keeps the division)
+                sC[q][3][t] = m1 * ir;
+                sC[q][4][t] = m2 * ir;
+                vz = m3 * ir;
+            } else {
+                sC[q][3][t] = m1 / r;
+                sC[q][4][t] = m2 / r;
+                vz = m3 / r;
+            }
         }
         return vz;
     };
@@ -207,6 +213,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         // G: update of plane k (k_update_z's arithmetic)
         if (upd) {
             const long long c = cell_index(B, i, j, k);
+            real nq[7];   // the new state of the cell, for its CFL speed
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 real div = dxy[f] + (fz_hi[f] - fz_lo[f]);
@@ -225,6 +232,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                     atomicAdd(a.floor_count, 1u);
                 }
                 a.dst.q[f][c] = v;
+                nq[f] = v;
             }
             const double ex = sEc[eq][0][t], ey = sEc[eq][1][t], ez = sEz[t];
             const double ez_y = sEz[t - TX], ez_x = sEz[t - 1], ey_x = sEc[eq][1][t - 1], ex_y = sEc[eq][0][t - TX];
@@ -243,21 +251,28 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 double bxm = a.src.b[0][c - sx] - dtd * ((ez_x - ez_xy) - (ey_x - eyx_m));
                 double bym = a.src.b[1][c - sy] - dtd * ((ex_y - exy_m) - (ez_y - ez_xy));
                 if (!have_bzm) bz_m = a.src.b[2][c - sz] - dtd * ((ey_m - eyx_m) - (ex_m - exy_m));
-                a.dst.q[4][c] = (real)(0.5 * (bx + bxm));
-                a.dst.q[5][c] = (real)(0.5 * (by + bym));
-                a.dst.q[6][c] = (real)(0.5 * (bz + bz_m));
+                nq[4] = (real)(0.5 * (bx + bxm));
+                nq[5] = (real)(0.5 * (by + bym));
+                nq[6] = (real)(0.5 * (bz + bz_m));
             } else {
-                a.dst.q[4][c] = a.src.q[4][c];
-                a.dst.q[5][c] = a.src.q[5][c];
-                a.dst.q[6][c] = a.src.q[6][c];
+                nq[4] = a.src.q[4][c];
+                nq[5] = a.src.q[5][c];
+                nq[6] = a.src.q[6][c];
             }
+            a.dst.q[4][c] = nq[4];
+            a.dst.q[5][c] = nq[5];
+            a.dst.q[6][c] = nq[6];
             bz_m = bz;
             have_bzm = true;
             ey_m = ey; ex_m = ex; eyx_m = ey_x; exy_m = ex_y;
             if (a.cmax != nullptr && int_ij && k >= 1 && k <= n2) {
-                double q7[7];
-                for (int f = 0; f < 7; f++) q7[f] = (double)a.dst.q[f][c];
-                cfl = fmax(cfl, cell_fast_speed(q7, a.cs));
+                if constexpr (ONE) {
+                    cfl = fmax(cfl, (double)cell_fast_speed_f(nq, (float)a.cs));
+                } else {
+                    double q7[7];
+                    for (int f = 0; f < 7; f++) q7[f] = (double)nq[f];
+                    cfl = fmax(cfl, cell_fast_speed(q7, a.cs));
+                }
             }
         } else {
             have_bzm = false;
