@@ -133,6 +133,16 @@ def roofline(prof, esz):
             tr = json.load(f)
         if key in tr:
             out["traffic"] = tr[key]
+    # the fused stage kernels are VALU-bound: their VALU issue rate against the MI355X issue
+    # peak, from the counted instructions per launch and this run's launch time
+    sq = os.path.join(ROOT, "profiles", "sq_valu.json")
+    if esz == 4 and os.path.exists(sq):
+        with open(sq) as f:
+            vq = json.load(f)
+        if key in vq:
+            ach = vq[key]["valu_wave_instr"] / (avg_ms * 1e-3) / 1e9
+            out["valu_issue"] = {"achieved": ach, "peak": VALU_PEAK_GWI, "unit": "G wave-instr/s",
+                                 "frac": ach / VALU_PEAK_GWI, "wave_instr_per_launch": vq[key]["valu_wave_instr"]}
     return out
 
 
@@ -235,6 +245,43 @@ def pmc_summarize(fetch_dir, write_dir):
         json.dump(out, f, indent=1)
     print("\n".join(lines))
     print(json.dumps(out["_calibration"]))
+    return 0
+
+
+# VALU issue peak of MI355X: 256 CUs x 4 SIMDs, a wave64 VALU instruction every 4 cycles per
+# SIMD (16 lanes), 2.4 GHz engine clock
+VALU_PEAK_GWI = 256 * 4 / 4 * 2.4
+
+
+def sq_summarize(sq_dir):
+    """VALU wave-instructions per launch of each family (SQ_INSTS_VALU from a rocprofv3 run
+    of --pmc-run), matched to families by dispatch order as in pmc_summarize; written to
+    profiles/sq_valu.json, from which bench.py reports each kernel's VALU issue rate."""
+    with open(os.path.join(ROOT, "gpurun_out", "pmc_map.json")) as f:
+        mp = json.load(f)
+    counters = {}
+    for c in ("SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES", "SQ_WAVES"):
+        counters[c] = _read_counter(sq_dir, c)
+    fams = sorted(mp["families"].items(), key=lambda kv: kv[1]["order"])
+    out = {"_peak_G_wave_instr_per_s": VALU_PEAK_GWI}
+    lines = ["| family | kernel | VALU wave-instr / launch | VALU instr / wave | active VALU / wave-cycles |",
+             "|---|---|---|---|---|"]
+    for key, v in fams:
+        same = [k for k, w in fams if w["symbol"] == v["symbol"]]
+        pos = same.index(key)
+        per = {}
+        for c, rows in counters.items():
+            vals = [val for n, val in rows if v["symbol"] + "<float" in n][pos::len(same)]
+            per[c] = sum(vals) / len(vals) if vals else 0.0
+        if not per["SQ_WAVES"]:
+            continue
+        out[key] = {"valu_wave_instr": per["SQ_INSTS_VALU"], "waves": per["SQ_WAVES"],
+                    "active_valu_per_wave_cycle": per["SQ_ACTIVE_INST_VALU"] / per["SQ_WAVE_CYCLES"]}
+        lines.append(f"| {key} | {v['symbol']}<float> | {per['SQ_INSTS_VALU']:.4g} | "
+                     f"{per['SQ_INSTS_VALU'] / per['SQ_WAVES']:.0f} | {out[key]['active_valu_per_wave_cycle']:.3f} |")
+    with open(os.path.join(ROOT, "profiles", "sq_valu.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("\n".join(lines))
     return 0
 
 
@@ -346,9 +393,12 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the ensemble and TVD side measurements")
     ap.add_argument("--extra-steps", type=int, default=50)
     ap.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
+    ap.add_argument("--sq-summarize", metavar="SQ_DIR")
     args = ap.parse_args()
     if args.pmc_summarize:
         return pmc_summarize(*args.pmc_summarize)
+    if args.sq_summarize:
+        return sq_summarize(args.sq_summarize)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the box would shard by 3-D blocks with an RCCL halo exchange, not implemented)",
