@@ -241,12 +241,20 @@ __global__ __launch_bounds__(256) void k_pass_update(const BoxDesc* __restrict__
     }
 }
 
+// 1/rho of the velocity reads: IEEE division in FP64 (bitwise with the reference), the
+// hardware reciprocal in the FP32 path
+template <class real>
+__device__ __forceinline__ real pass_inv(real r) {
+    if constexpr (sizeof(real) == 4) return __builtin_amdgcn_rcpf(r);
+    else return real(1) / r;
+}
+
 template <class real>
 __device__ __forceinline__ real pass_vel(const PassStageArgs<real>& a, int d, long long p) {
-    real irho = real(1) / a.qo[0][p];
+    real irho = pass_inv(a.qo[0][p]);
     real v = a.qo[1 + d][p] * irho;   // copy_velocity(.FALSE.)
     if (a.stage) {
-        real irn = real(1) / a.qn[0][p];
+        real irn = pass_inv(a.qn[0][p]);
         v = real(0.5) * (v + a.qn[1 + d][p] * irn);  // copy_velocity(.TRUE.)
     }
     return v;

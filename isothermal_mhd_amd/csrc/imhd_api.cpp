@@ -1266,7 +1266,7 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
     // coordinate search, one parameter at a time from the current best
     static const int strips[] = {0, 8, 16, 32, 64};
     static const int zsegs[] = {0, 2, 4, 8, 16};
-    static const int vsegs[] = {8, 16, 32};
+    static const int vsegs[] = {8, 12, 16, 20, 24, 32};
     const Shape s0 = bsh;
     for (int s : strips) if (!rc) rc = trial({s0.zseg, s, s0.vshape, s0.vseg});
     { const Shape b0 = bsh; for (int z : zsegs) if (!rc && z != b0.zseg) rc = trial({z, b0.strip, b0.vshape, b0.vseg}); }
