@@ -285,6 +285,37 @@ def sq_summarize(sq_dir):
     return 0
 
 
+def run_big(n, steps, cour, warmup=2):
+    """Throughput of one large FP32 box (HLLD + VL2 + CT, PLM, fused stages, whole step in
+    a hipGraph) on the separable solenoidal IC of ics.separable_large, with the context's
+    device memory: the single-GPU capacity line (512^3, 1024^3)."""
+    from isothermal_mhd_amd import VanLeerSolver, ics
+    t0 = time.perf_counter()
+    q, bf = ics.separable_large(n)
+    t_ic = time.perf_counter() - t0
+    S = VanLeerSolver((n, n, n), npass=0, riemann="hlld", bc="periodic", precision=32, cour=cour)
+    t0 = time.perf_counter()
+    S.set_state(q, bf)
+    S.sync()
+    t_up = time.perf_counter() - t0
+    del q, bf
+    S.boundary_conditions()
+    S.run(warmup, graph=True)
+    S.sync()
+    t0 = time.perf_counter()
+    S.run(steps, graph=True)
+    S.sync()
+    sec = time.perf_counter() - t0
+    d = S.diagnostics()
+    mem = S.device_bytes()
+    S.close()
+    return {"n": n, "value": n ** 3 * steps / sec / 1e6, "unit": "Mcell-updates/s", "steps": steps,
+            "ms_per_step": sec / steps * 1e3, "device_GB": mem / 1e9, "upload_s": t_up, "ic_s": t_ic,
+            "max_divb": float(d["divb_max"]), "dtype": "f32",
+            "workload": f"{n}^3 periodic, rho=1, B0=(0.05,0.05,0), separable solenoidal velocity (6 modes per "
+                        "component, |k|<=16, rms Mach 2, seed 5005); HLLD + VL2 + CT, PLM, FP32, hipGraph"}
+
+
 def run_ensemble(steps, cour, warmup=2):
     """Batched-ensemble configuration: 48 periodic boxes of uneven sizes drawn from
     {32..192}^3 (seed 4004), FP32, HLLE + VL2 + CT, every kernel one launch for all boxes,
@@ -394,6 +425,7 @@ def main():
     ap.add_argument("--extra-steps", type=int, default=50)
     ap.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--sq-summarize", metavar="SQ_DIR")
+    ap.add_argument("--big", metavar="N[,N]", help="only the large-box capacity lines, e.g. 512,1024")
     args = ap.parse_args()
     if args.pmc_summarize:
         return pmc_summarize(*args.pmc_summarize)
@@ -406,6 +438,12 @@ def main():
         return 2
     if args.pmc_run:
         return pmc_run(args.n)
+    if args.big:
+        lines = []
+        for nb_ in args.big.split(","):
+            lines.append(run_big(int(nb_), args.steps, args.cour))
+            print(json.dumps({"big": lines[-1]}), flush=True)
+        return 0
     from isothermal_mhd_amd import ics
     n, K, W = args.n, args.steps, args.warmup
     state0 = ics.config_c2(n=n)
@@ -451,7 +489,8 @@ def main():
         res["l1_fp32_vs_fp64"] = l1_errors(r32["state"], r64["state"])
         res["kernels_fp64_ms"] = {k: round(v[0] / v[1], 5) for k, v in r64["prof"].items()}
     if not args.no_extra:
-        for key, fn in (("ensemble_c4", run_ensemble), ("tvd_c1", run_tvd_c1)):
+        for key, fn in (("ensemble_c4", run_ensemble), ("tvd_c1", run_tvd_c1),
+                        ("big_512", lambda k, c: run_big(512, k // 2, c))):
             try:
                 res[key] = fn(args.extra_steps, args.cour)
             except Exception as e:   # a side measurement never hides the main line

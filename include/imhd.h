@@ -148,6 +148,13 @@ int imhd_diagnostics(imhd_ctx* ctx, imhd_diag* out);
  * vanleer_mod.f90:3243-3262, which prints instead). */
 int imhd_floor_count(imhd_ctx* ctx, unsigned int* count);
 
+/* Device memory held by the context's field and work arrays, in bytes (no counterpart in
+ * the reference, whose arrays live in module memory).  Work arrays a configuration never
+ * uses are not allocated: with both VL stages fused (PLM) the flux, face-EMF and
+ * corner-EMF arrays are absent, and the drop-in passive path's velocity and flux arrays
+ * appear on the first imhd_copy_velocity / imhd_passive_step call. */
+int imhd_device_bytes(imhd_ctx* ctx, unsigned long long* bytes);
+
 /* Native, device-resident time loop of the driver (driver.f90:35-186) for nsteps steps:
  * copy_velocity, step, passive, CFL dt = COUR dx / cmax with dt = MIN(t_end - t, dt),
  * ghost fill.  dt stays on the device; the first call starts from dt = 1e-15

@@ -17,7 +17,7 @@ ERRORS = {-1: "IMHD_EINVAL", -2: "IMHD_EHIP", -3: "IMHD_ESTATE", -4: "IMHD_ENOSP
 EXPORTS = [
     "imhd_create", "imhd_create_ensemble", "imhd_destroy", "imhd_set_state", "imhd_get_state",
     "imhd_autotune", "imhd_step", "imhd_set_force", "imhd_set_ou_forcing", "imhd_get_ou_modes", "imhd_fill_ghosts", "imhd_max_speed", "imhd_copy_velocity", "imhd_passive_step",
-    "imhd_diagnostics", "imhd_floor_count", "imhd_run", "imhd_set_time", "imhd_get_time",
+    "imhd_diagnostics", "imhd_floor_count", "imhd_device_bytes", "imhd_run", "imhd_set_time", "imhd_get_time",
     "imhd_set_profiling", "imhd_profile_report", "imhd_sync", "imhd_last_error", "imhd_version",
 ]
 
@@ -75,6 +75,7 @@ def lib():
     L.imhd_passive_step.argtypes = [vp, dp]
     L.imhd_diagnostics.argtypes = [vp, ctypes.POINTER(ImhdDiag)]
     L.imhd_floor_count.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
+    L.imhd_device_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong)]
     L.imhd_run.argtypes = [vp, ip, dp, ip]
     L.imhd_set_time.argtypes = [vp, vp, vp]
     L.imhd_get_time.argtypes = [vp, vp, vp]

@@ -398,19 +398,21 @@ __global__ void k_convert_out(const real* __restrict__ src, double* __restrict__
 }
 // passive arrays: the reference's bounds are [3-nb, n+nb-2] per axis; ours are the full
 // ghost-padded box.  dir 0: host-layout src -> device; dir 1: device -> host layout.
+// one chunk [g0, g0 + m) of the host-layout index; hsrc / hdst hold that chunk only
 template <class real>
-__global__ void k_pass_layout(BoxDesc B, const double* hsrc, double* hdst, real* dev, int to_device) {
-    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int PX = B.n[0] + 2 * B.nb - 4, PY = B.n[1] + 2 * B.nb - 4, PZ = B.n[2] + 2 * B.nb - 4;
-    long long tot = (long long)PX * PY * PZ;
-    if (g >= tot) return;
+__global__ void k_pass_layout(BoxDesc B, const double* hsrc, double* hdst, real* dev, int to_device, long long g0,
+                              long long m) {
+    const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= m) return;
+    const long long g = g0 + l;
+    int PX = B.n[0] + 2 * B.nb - 4, PY = B.n[1] + 2 * B.nb - 4;
     int ii = (int)(g % PX);
     long long t = g / PX;
     int jj = (int)(t % PY);
     int kk = (int)(t / PY);
     long long c = cell_index(B, ii + 3 - B.nb, jj + 3 - B.nb, kk + 3 - B.nb);
-    if (to_device) dev[c] = (real)hsrc[g];
-    else hdst[g] = (double)dev[c];
+    if (to_device) dev[c] = (real)hsrc[l];
+    else hdst[l] = (double)dev[c];
 }
 
 // ---------------------------------------------------------------------------------
@@ -473,10 +475,11 @@ void launch_convert_out(hipStream_t st, const real* src, double* dst, long long 
     hipLaunchKernelGGL((k_convert_out<real>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
 }
 template <class real>
-void launch_pass_layout(hipStream_t st, const BoxDesc& B, const double* hsrc, double* hdst, real* dev, int to_device) {
-    long long tot = (long long)(B.n[0] + 2 * B.nb - 4) * (B.n[1] + 2 * B.nb - 4) * (B.n[2] + 2 * B.nb - 4);
-    hipLaunchKernelGGL((k_pass_layout<real>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, B, hsrc, hdst, dev,
-                       to_device);
+void launch_pass_layout(hipStream_t st, const BoxDesc& B, const double* hsrc, double* hdst, real* dev, int to_device,
+                        long long g0, long long m) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL((k_pass_layout<real>), dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, B, hsrc, hdst, dev,
+                       to_device, g0, m);
 }
 
 #define INST_AUX(T)                                                                                           \
@@ -491,7 +494,7 @@ void launch_pass_layout(hipStream_t st, const BoxDesc& B, const double* hsrc, do
     template void launch_pass_update<T>(const Launch&, const PassUpdArgs<T>&);                                \
     template void launch_convert_in<T>(hipStream_t, const double*, T*, long long);                            \
     template void launch_convert_out<T>(hipStream_t, const T*, double*, long long);                           \
-    template void launch_pass_layout<T>(hipStream_t, const BoxDesc&, const double*, double*, T*, int);
+    template void launch_pass_layout<T>(hipStream_t, const BoxDesc&, const double*, double*, T*, int, long long, long long);
 INST_AUX(double)
 INST_AUX(float)
 

@@ -95,6 +95,7 @@ void launch_reset_u64(hipStream_t st, unsigned long long* p, int n);
 template <class real> void launch_convert_in(hipStream_t st, const double* src, real* dst, long long n);
 template <class real> void launch_convert_out(hipStream_t st, const real* src, double* dst, long long n);
 template <class real>
-void launch_pass_layout(hipStream_t st, const BoxDesc& B, const double* hsrc, double* hdst, real* dev, int to_device);
+void launch_pass_layout(hipStream_t st, const BoxDesc& B, const double* hsrc, double* hdst, real* dev, int to_device,
+                        long long g0, long long m);
 
 }  // namespace imhd

@@ -102,6 +102,7 @@ struct imhd_ctx {
     double* cour = nullptr;
     double* diag = nullptr;
     unsigned int* floor_count = nullptr;
+    unsigned long long dev_bytes = 0;   // field and work arrays allocated by alloc()
     double* stage = nullptr;  // FP64 staging for set/get
     size_t stage_n = 0;
     // plans
@@ -329,6 +330,7 @@ struct ProfScope {
 int alloc(imhd_ctx* c, void** p, size_t bytes) {
     HIPCK(c, hipMalloc(p, bytes));
     HIPCK(c, hipMemset(*p, 0, bytes));
+    c->dev_bytes += bytes;
     return 0;
 }
 
@@ -446,11 +448,11 @@ void vl_step(imhd_ctx* c, bool fuse_cmax = false) {
     launch_reset_u64(c->st, c->smax, (int)stripe_slots(c->nbox));
     launch_reset_u64(c->st, c->dvmax, (int)stripe_slots(c->nbox));   // delv_max = 0 (vanleer_mod.f90:301)
     const int s0 = c->cur, s1 = 1 - c->cur;
-    if (c->vstage && c->q[2][0]) {
+    if (c->vstage) {
         // Fused stages.  The corrector's fluxes read the half step while its update writes
         // the new state, so the half step goes to its own set (2) when the corrector is
         // fused too (PLM); the PPM corrector keeps the three-kernel path on set s1.
-        const bool fcorr = c->cfg.recon == IMHD_PLM_PRIM;
+        const bool fcorr = c->q[2][0] != nullptr;
         const int sh = fcorr ? 2 : s1;
         vl_stage<real>(c, s0, s0, sh, false, false, "stage_pred");
         if (fcorr) {
@@ -871,25 +873,69 @@ static int ensure_stage(imhd_ctx* c, size_t n) {
     return 0;
 }
 
+// Host FP64 <-> device arrays through a bounded staging buffer (STAGE_CHUNK doubles), so a
+// 1024^3 box needs no whole-field FP64 copy on the device.  FP64 contexts copy directly.
+constexpr long long STAGE_CHUNK = 1LL << 25;   // 256 MB
+
+template <class real>
+static int upload_field(imhd_ctx* c, const double* h, real* d0, real* d1, long long n) {
+    if (sizeof(real) == 8) {
+        HIPCK(c, hipMemcpy(d0, h, (size_t)n * 8, hipMemcpyHostToDevice));
+        if (d1) HIPCK(c, hipMemcpy(d1, d0, (size_t)n * 8, hipMemcpyDeviceToDevice));
+        return 0;
+    }
+    int rc;
+    if ((rc = ensure_stage(c, (size_t)std::min(n, STAGE_CHUNK)))) return rc;
+    for (long long o = 0; o < n; o += STAGE_CHUNK) {
+        const long long m = std::min(STAGE_CHUNK, n - o);
+        HIPCK(c, hipMemcpy(c->stage, h + o, (size_t)m * 8, hipMemcpyHostToDevice));
+        launch_convert_in<real>(c->st, c->stage, d0 + o, m);
+        if (d1) launch_convert_in<real>(c->st, c->stage, d1 + o, m);
+        HIPCK(c, hipStreamSynchronize(c->st));
+    }
+    return 0;
+}
+
+template <class real>
+static int download_field(imhd_ctx* c, const real* d, double* h, long long n) {
+    if (sizeof(real) == 8) {
+        HIPCK(c, hipMemcpy(h, d, (size_t)n * 8, hipMemcpyDeviceToHost));
+        return 0;
+    }
+    int rc;
+    if ((rc = ensure_stage(c, (size_t)std::min(n, STAGE_CHUNK)))) return rc;
+    for (long long o = 0; o < n; o += STAGE_CHUNK) {
+        const long long m = std::min(STAGE_CHUNK, n - o);
+        launch_convert_out<real>(c->st, d + o, c->stage, m);
+        HIPCK(c, hipStreamSynchronize(c->st));
+        HIPCK(c, hipMemcpy(h + o, c->stage, (size_t)m * 8, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
 template <class real>
 static int set_state_t(imhd_ctx* c, int bx, const double* q, const double* bf, const double* qp) {
     const BoxDesc& B = c->hb[bx];
     const long long nc = (long long)B.P[0] * B.P[1] * B.P[2];
     int rc;
-    if ((rc = ensure_stage(c, (size_t)nc * 7))) return rc;
-    HIPCK(c, hipMemcpy(c->stage, q, (size_t)nc * 7 * 8, hipMemcpyHostToDevice));
-    for (int s = 0; s < 2; s++)
-        for (int f = 0; f < 7; f++) launch_convert_in<real>(c->st, c->stage + f * nc, R<real>(c->q[s][f]) + B.off, nc);
     HIPCK(c, hipStreamSynchronize(c->st));
+    for (int f = 0; f < 7; f++)
+        if ((rc = upload_field<real>(c, q + f * nc, R<real>(c->q[0][f]) + B.off, R<real>(c->q[1][f]) + B.off, nc)))
+            return rc;
     for (int s = 0; s < 2; s++)
         for (int d = 0; d < 3; d++)
             HIPCK(c, hipMemcpy(c->b[s][d] + B.off, bf + d * nc, (size_t)nc * 8, hipMemcpyHostToDevice));
     if (c->cfg.npass > 0 && qp) {
-        long long pn = (long long)(B.n[0] + 2 * B.nb - 4) * (B.n[1] + 2 * B.nb - 4) * (B.n[2] + 2 * B.nb - 4);
-        HIPCK(c, hipMemcpy(c->stage, qp, (size_t)pn * c->cfg.npass * 8, hipMemcpyHostToDevice));
-        for (int s = 0; s < 2; s++)
-            for (int p = 0; p < c->cfg.npass; p++)
-                launch_pass_layout<real>(c->st, B, c->stage + p * pn, nullptr, R<real>(c->qp[s][p]), 1);
+        const long long pn = (long long)(B.n[0] + 2 * B.nb - 4) * (B.n[1] + 2 * B.nb - 4) * (B.n[2] + 2 * B.nb - 4);
+        if ((rc = ensure_stage(c, (size_t)std::min(pn, STAGE_CHUNK)))) return rc;
+        for (int p = 0; p < c->cfg.npass; p++)
+            for (long long o = 0; o < pn; o += STAGE_CHUNK) {
+                const long long m = std::min(STAGE_CHUNK, pn - o);
+                HIPCK(c, hipMemcpy(c->stage, qp + p * pn + o, (size_t)m * 8, hipMemcpyHostToDevice));
+                for (int s = 0; s < 2; s++)
+                    launch_pass_layout<real>(c->st, B, c->stage, nullptr, R<real>(c->qp[s][p]), 1, o, m);
+                HIPCK(c, hipStreamSynchronize(c->st));
+            }
     }
     HIPCK(c, hipStreamSynchronize(c->st));
     HIPCK(c, hipGetLastError());
@@ -901,24 +947,39 @@ static int get_state_t(imhd_ctx* c, int bx, double* q, double* bf, double* qp) {
     const BoxDesc& B = c->hb[bx];
     const long long nc = (long long)B.P[0] * B.P[1] * B.P[2];
     int rc;
-    if ((rc = ensure_stage(c, (size_t)nc * 7))) return rc;
     HIPCK(c, hipStreamSynchronize(c->st));
-    if (q) {
-        for (int f = 0; f < 7; f++) launch_convert_out<real>(c->st, R<real>(c->q[c->cur][f]) + B.off, c->stage + f * nc, nc);
-        HIPCK(c, hipStreamSynchronize(c->st));
-        HIPCK(c, hipMemcpy(q, c->stage, (size_t)nc * 7 * 8, hipMemcpyDeviceToHost));
-    }
+    if (q)
+        for (int f = 0; f < 7; f++)
+            if ((rc = download_field<real>(c, R<real>(c->q[c->cur][f]) + B.off, q + f * nc, nc))) return rc;
     if (bf)
         for (int d = 0; d < 3; d++)
             HIPCK(c, hipMemcpy(bf + d * nc, c->b[c->cur][d] + B.off, (size_t)nc * 8, hipMemcpyDeviceToHost));
     if (c->cfg.npass > 0 && qp) {
-        long long pn = (long long)(B.n[0] + 2 * B.nb - 4) * (B.n[1] + 2 * B.nb - 4) * (B.n[2] + 2 * B.nb - 4);
+        const long long pn = (long long)(B.n[0] + 2 * B.nb - 4) * (B.n[1] + 2 * B.nb - 4) * (B.n[2] + 2 * B.nb - 4);
+        if ((rc = ensure_stage(c, (size_t)std::min(pn, STAGE_CHUNK)))) return rc;
         for (int p = 0; p < c->cfg.npass; p++)
-            launch_pass_layout<real>(c->st, B, nullptr, c->stage + p * pn, R<real>(c->qp[c->pcur][p]), 0);
-        HIPCK(c, hipStreamSynchronize(c->st));
-        HIPCK(c, hipMemcpy(qp, c->stage, (size_t)pn * c->cfg.npass * 8, hipMemcpyDeviceToHost));
+            for (long long o = 0; o < pn; o += STAGE_CHUNK) {
+                const long long m = std::min(STAGE_CHUNK, pn - o);
+                launch_pass_layout<real>(c->st, B, nullptr, c->stage, R<real>(c->qp[c->pcur][p]), 0, o, m);
+                HIPCK(c, hipStreamSynchronize(c->st));
+                HIPCK(c, hipMemcpy(qp + p * pn + o, c->stage, (size_t)m * 8, hipMemcpyDeviceToHost));
+            }
     }
     HIPCK(c, hipGetLastError());
+    return 0;
+}
+
+// Work arrays of the drop-in passive path (copy_velocity + passive_solve_3d), allocated on
+// its first use: the native loop's fused passive stages never need them.
+static int ensure_passive_work(imhd_ctx* c) {
+    if (c->cfg.npass <= 0 || c->vel[0]) return 0;
+    const size_t nr = (size_t)c->pool * c->esz;
+    int rc;
+    for (int d = 0; d < 3; d++) {
+        if ((rc = alloc(c, &c->vel[d], nr))) return rc;
+        if ((rc = alloc(c, &c->vta[d], nr))) return rc;
+        for (int p = 0; p < c->cfg.npass; p++) if ((rc = alloc(c, &c->Fp[d][p], nr))) return rc;
+    }
     return 0;
 }
 
@@ -989,20 +1050,18 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
         for (int d = 0; d < 3; d++) if ((rc = alloc(c, (void**)&c->b[s][d], nd))) return rc;
         for (int p = 0; p < g.npass; p++) if ((rc = alloc(c, &c->qp[s][p], nr))) return rc;
     }
-    if (g.integrator == IMHD_VL) {   // half-step set of the fused stages
+    // Both VL stages fused (PLM): a half-step set, and no flux / face-EMF / corner-EMF work
+    // arrays (18 real + 3 FP64 per cell, the largest share of a 1024^3 box).  Otherwise
+    // (TVD, PPM corrector, IMHD_VSTAGE=0) the three-kernel stages need the work arrays.
+    const bool fused = g.integrator == IMHD_VL && c->vstage && g.recon == IMHD_PLM_PRIM;
+    if (fused) {
         for (int f = 0; f < 7; f++) if ((rc = alloc(c, &c->q[2][f], nr))) return rc;
         for (int d = 0; d < 3; d++) if ((rc = alloc(c, (void**)&c->b[2][d], nd))) return rc;
-    }
-    for (int d = 0; d < 3; d++)
-        for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->F[d][f], nr))) return rc;
-    for (int e = 0; e < 6; e++) if ((rc = alloc(c, &c->E[e], nr))) return rc;
-    for (int d = 0; d < 3; d++) if ((rc = alloc(c, (void**)&c->Ec[d], nd))) return rc;
-    if (g.npass > 0) {
-        for (int d = 0; d < 3; d++) {
-            if ((rc = alloc(c, &c->vel[d], nr))) return rc;
-            if ((rc = alloc(c, &c->vta[d], nr))) return rc;
-            for (int p = 0; p < g.npass; p++) if ((rc = alloc(c, &c->Fp[d][p], nr))) return rc;
-        }
+    } else {
+        for (int d = 0; d < 3; d++)
+            for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->F[d][f], nr))) return rc;
+        for (int e = 0; e < 6; e++) if ((rc = alloc(c, &c->E[e], nr))) return rc;
+        for (int d = 0; d < 3; d++) if ((rc = alloc(c, (void**)&c->Ec[d], nd))) return rc;
     }
     if (g.precision == 32 && g.compensated)
         for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->comp[f], nr))) return rc;
@@ -1411,6 +1470,8 @@ int imhd_copy_velocity(imhd_ctx* c, int stage) {
     if (!c) return IMHD_EINVAL;
     if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
     if (c->cfg.npass <= 0) return IMHD_OK;
+    int rc;
+    if ((rc = ensure_passive_work(c))) return rc;
     if (c->prec == 64) copy_velocity<double>(c, stage != 0); else copy_velocity<float>(c, stage != 0);
     HIPCK(c, hipGetLastError());
     return prof_collect(c);
@@ -1421,6 +1482,8 @@ int imhd_passive_step(imhd_ctx* c, double dt) {
     if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
     if (c->cfg.npass <= 0) return IMHD_OK;
     if (!(dt > 0.0)) FAIL(c, IMHD_EINVAL, "dt must be positive");
+    int rc;
+    if ((rc = ensure_passive_work(c))) return rc;
     set_scalar_dt(c, dt);
     if (c->prec == 64) passive_step<double>(c); else passive_step<float>(c);
     HIPCK(c, hipGetLastError());
@@ -1503,6 +1566,12 @@ int imhd_floor_count(imhd_ctx* c, unsigned int* count) {
     if (!c || !count) return IMHD_EINVAL;
     HIPCK(c, hipStreamSynchronize(c->st));
     HIPCK(c, hipMemcpy(count, c->floor_count, 4, hipMemcpyDeviceToHost));
+    return IMHD_OK;
+}
+
+int imhd_device_bytes(imhd_ctx* c, unsigned long long* bytes) {
+    if (!c || !bytes) return IMHD_EINVAL;
+    *bytes = c->dev_bytes;
     return IMHD_OK;
 }
 

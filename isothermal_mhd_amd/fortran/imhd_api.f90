@@ -103,6 +103,12 @@ MODULE imhd_api
       INTEGER(C_INT), INTENT(OUT) :: count
     END FUNCTION imhd_floor_count
 
+    INTEGER(C_INT) FUNCTION imhd_device_bytes(ctx, bytes) BIND(C, name='imhd_device_bytes')
+      IMPORT :: C_INT, C_PTR, C_LONG_LONG
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_LONG_LONG), INTENT(OUT) :: bytes
+    END FUNCTION imhd_device_bytes
+
     INTEGER(C_INT) FUNCTION imhd_run(ctx, nsteps, t_end, use_graph) BIND(C, name='imhd_run')
       IMPORT :: C_INT, C_PTR, C_DOUBLE
       TYPE(C_PTR), VALUE :: ctx

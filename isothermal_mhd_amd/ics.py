@@ -91,3 +91,36 @@ def ensemble_c4(nbox=48, seed=4004, lo=32, hi=192, nb=6):
         machs.append(mach)
         states.append(make_state(n, nb=nb, npass=0, v=v, b0=tuple(bm * d)))
     return dims, states, machs
+
+
+def separable_large(n, nb=6, nmodes=6, kmax=16, mach=2.0, b0=(0.05, 0.05, 0.0), seed=5005):
+    """Large-box throughput IC built without 3-D FFTs: rho = 1, uniform B0, and a velocity
+    whose x-component depends on (y, z) only, y on (z, x) and z on (x, y) -- divergence-free
+    by construction -- each a sum of `nmodes` seeded products of sines with integer wave
+    numbers 1..kmax, scaled to rms Mach `mach`.  Returns q_3d, bface_3d (no passives); the
+    host arrays are filled by broadcasting the 2-D component fields along their third axis."""
+    rng = np.random.default_rng(seed)
+    sh = padded_shapes(n, n, n, nb, 0)[0]
+    x = (np.arange(n) + 0.5) / n
+    comps = []
+    for c in range(3):
+        f = np.zeros((n, n))
+        for _ in range(nmodes):
+            ka, kb = rng.integers(1, kmax + 1, size=2)
+            pa, pb = rng.random(2)
+            amp = rng.standard_normal() * (ka * ka + kb * kb) ** -0.5
+            f += amp * np.outer(np.sin(2 * np.pi * (ka * x + pa)), np.sin(2 * np.pi * (kb * x + pb)))
+        comps.append(f)
+    rms = np.sqrt(sum(np.mean(f * f) for f in comps))
+    comps = [f * (mach / rms) for f in comps]
+    q = np.zeros(sh + (7,), order="F")
+    bf = np.zeros(sh + (3,), order="F")
+    I = slice(nb, nb + n)
+    q[I, I, I, 0] = 1.0
+    q[I, I, I, 1] = comps[0][None, :, :]          # v_x(y, z)
+    q[I, I, I, 2] = comps[1].T[:, None, :]        # v_y(z, x), stored as (x, z)
+    q[I, I, I, 3] = comps[2][:, :, None]          # v_z(x, y)
+    for m in range(3):
+        q[I, I, I, 4 + m] = b0[m]
+        bf[I, I, I, m] = b0[m]
+    return q, bf

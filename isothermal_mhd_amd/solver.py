@@ -154,6 +154,12 @@ class VanLeerSolver:
         self._check(self.lib.imhd_floor_count(self._c, ctypes.byref(n)))
         return n.value
 
+    def device_bytes(self):
+        """Device memory held by this context's field and work arrays (bytes)."""
+        n = ctypes.c_ulonglong()
+        self._check(self.lib.imhd_device_bytes(self._c, ctypes.byref(n)))
+        return n.value
+
     def set_ou_forcing(self, kmin=1.0, kmax=2.0, t_corr=0.5, sigma=1.0, seed=3003, solenoidal=True):
         """Ornstein-Uhlenbeck forcing of the native loop (see imhd.h); sigma <= 0 switches
         it off."""
