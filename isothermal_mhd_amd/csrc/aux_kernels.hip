@@ -324,8 +324,10 @@ __device__ __forceinline__ real pass_face_v(real wm, real w0, real w1, real w2, 
     return real(0.5) * ((real(1) + theta) * uL + (real(1) - theta) * uR);
 }
 
-template <class real, int STAGE>
+template <class real, int STAGE, bool VARR>
 __global__ __launch_bounds__(256) void k_pass_stage_z(const BoxDesc* __restrict__ boxes, Plan plan, PassStageArgs<real> a) {
+    // velocities: read from a.vel (formed by the fused MHD stages) or formed here from rho, m
+    auto V = [&](int d, long long p) -> real { return VARR ? a.vel[d][p] : pass_vel(a, d, p); };
     int b, i, j, sg;
     if (!plan_item(plan, b, i, j, sg)) return;
     const BoxDesc& B = boxes[b];
@@ -339,10 +341,10 @@ __global__ __launch_bounds__(256) void k_pass_stage_z(const BoxDesc* __restrict_
         long long c = c0;
         // carried: v_z at k, the scalar at k-1, k, k+1 (k, k+1 for the predictor), and the
         // value at the face between k-1 and k
-        real vz1 = pass_vel(a, 2, c);
+        real vz1 = V(2, c);
         real flo, wa, wb, wc;
         {
-            const real vz0 = pass_vel(a, 2, c - sz);
+            const real vz0 = V(2, c - sz);
             const real va = real(0.5) * (vz0 + vz1);
             if (STAGE) {
                 const real wmm = w[c - 2 * sz];
@@ -357,12 +359,12 @@ __global__ __launch_bounds__(256) void k_pass_stage_z(const BoxDesc* __restrict_
         for (int k = kb; k < ke; k++, c += sz) {
             // one live cell index instead of a strength-reduced pointer per array and offset
             __asm__ volatile("" : "+v"(c));
-            const real vx = pass_vel(a, 0, c), vy = pass_vel(a, 1, c);
-            const real vz2 = pass_vel(a, 2, c + sz);
-            const real vxh = real(0.5) * (vx + pass_vel(a, 0, c + 1));
-            const real vxl = real(0.5) * (pass_vel(a, 0, c - 1) + vx);
-            const real vyh = real(0.5) * (vy + pass_vel(a, 1, c + sy));
-            const real vyl = real(0.5) * (pass_vel(a, 1, c - sy) + vy);
+            const real vx = V(0, c), vy = V(1, c);
+            const real vz2 = V(2, c + sz);
+            const real vxh = real(0.5) * (vx + V(0, c + 1));
+            const real vxl = real(0.5) * (V(0, c - 1) + vx);
+            const real vyh = real(0.5) * (vy + V(1, c + sy));
+            const real vyl = real(0.5) * (V(1, c - sy) + vy);
             const real vzh = real(0.5) * (vz1 + vz2);
             real fhi;
             if (STAGE) {
@@ -433,9 +435,13 @@ void launch_restore_shell(const Launch& L, const ShellArgs<real>& a) {
 template <class real>
 void launch_pass_stage(const Launch& L, const PassStageArgs<real>& a) {
     if (L.plan.total <= 0) return;
+    // the per-cell form forms its velocities itself (a.vel only caches them)
     if (!L.column) L256((k_pass_stage<real>), L.boxes, L.plan, a);
-    else if (a.stage) L256((k_pass_stage_z<real, 1>), L.boxes, L.plan, a);
-    else L256((k_pass_stage_z<real, 0>), L.boxes, L.plan, a);
+    else if (a.vel[0] != nullptr) {
+        if (a.stage) L256((k_pass_stage_z<real, 1, true>), L.boxes, L.plan, a);
+        else L256((k_pass_stage_z<real, 0, true>), L.boxes, L.plan, a);
+    } else if (a.stage) L256((k_pass_stage_z<real, 1, false>), L.boxes, L.plan, a);
+    else L256((k_pass_stage_z<real, 0, false>), L.boxes, L.plan, a);
 }
 template <class real>
 void launch_max_speed(const Launch& L, const QPtrs<real>& q, double cs, unsigned long long* cmax) {

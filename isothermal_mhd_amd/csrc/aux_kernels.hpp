@@ -65,6 +65,7 @@ struct PassStageArgs {
     real* dst[MAXPASS];
     const real* qo[4];          // MHD rho, m at t^n
     const real* qn[4];          // MHD rho, m at t^{n+1} (stage 1)
+    const real* vel[3];         // non-null: the stage's velocities, already formed per cell
     const double* dt;
     double factor;
     int npass, stage;

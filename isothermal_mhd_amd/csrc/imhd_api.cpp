@@ -54,6 +54,7 @@ struct imhd_ctx {
     long long pool = 0;  // padded cells over all boxes
     hipStream_t st = nullptr;
     // state buffers (two) and work arrays
+    void* pvel[3] = {};    // passive velocities written by the fused VL stages
     void* q[3][7] = {};    // set 2: half-step state of the fused VL stages
     double* b[3][3] = {};
     void* qp[2][MAXPASS] = {};
@@ -413,7 +414,8 @@ void update(imhd_ctx* c, int src, int dst, double factor, const PlanDev& p, bool
 
 // One whole stage in one launch (k_vl_stage): fluxes from set sflux, update src -> dst.
 template <class real>
-void vl_stage(imhd_ctx* c, int sflux, int src, int dst, bool corr, bool fuse_cmax, const char* name) {
+void vl_stage(imhd_ctx* c, int sflux, int src, int dst, bool corr, bool fuse_cmax, const char* name,
+              bool pvel = false) {
     const TilePlanDev& p = corr ? c->tp_corr : c->tp_pred;
     ProfScope ps(c, name, p.active);
     StageArgs<real> a{};
@@ -435,6 +437,7 @@ void vl_stage(imhd_ctx* c, int sflux, int src, int dst, bool corr, bool fuse_cma
     const StageRanges r = stage_ranges(c, corr);
     a.fn_lo = r.fn_lo; a.fn_hi = r.fn_hi; a.ft_lo = r.ft_lo; a.ft_hi = r.ft_hi;
     a.c_lo = r.c_lo; a.c_hi = r.c_hi; a.u_lo = r.u_lo; a.u_hi = r.u_hi;
+    for (int d = 0; d < 3; d++) a.pvel[d] = pvel ? R<real>(c->pvel[d]) : nullptr;
     const TilePlan tp{p.d, c->nbox, p.total};
     switch (c->cfg.riemann) {
         case IMHD_ROE: launch_vl_stage<real, RS_ROE>(c->st, c->db, tp, a, corr, c->vshape); break;
@@ -444,10 +447,25 @@ void vl_stage(imhd_ctx* c, int sflux, int src, int dst, bool corr, bool fuse_cma
 }
 
 template <class real>
-void vl_step(imhd_ctx* c, bool fuse_cmax = false) {
+void passive_stage(imhd_ctx* c, int qold, int qnew, int stage, bool vel);
+
+// with_passive (both stages fused, passives, no forcing kick between the MHD step and the
+// passive step): the passive stages run inline -- the predictor right after the MHD
+// predictor (it needs only t^n data), the corrector after the MHD corrector -- and read
+// the velocities the fused MHD stages write, instead of re-deriving them from rho and m
+template <class real>
+void vl_step(imhd_ctx* c, bool fuse_cmax = false, bool with_passive = false) {
     launch_reset_u64(c->st, c->smax, (int)stripe_slots(c->nbox));
     launch_reset_u64(c->st, c->dvmax, (int)stripe_slots(c->nbox));   // delv_max = 0 (vanleer_mod.f90:301)
     const int s0 = c->cur, s1 = 1 - c->cur;
+    if (with_passive) {
+        vl_stage<real>(c, s0, s0, 2, false, false, "stage_pred", true);
+        passive_stage<real>(c, s0, s0, 0, true);
+        vl_stage<real>(c, 2, s0, s1, true, fuse_cmax, fuse_cmax ? "stage_corr_cfl" : "stage_corr", true);
+        passive_stage<real>(c, s0, s1, 1, true);
+        c->cur = s1;
+        return;
+    }
     if (c->vstage) {
         // Fused stages.  The corrector's fluxes read the half step while its update writes
         // the new state, so the half step goes to its own set (2) when the corrector is
@@ -780,28 +798,34 @@ void max_speed_launch(imhd_ctx* c) {
 // Stage 0 writes the half step into the spare buffer; stage 1 reconstructs from it and
 // updates the t^n buffer in place (a cell reads only its own t^n value), so the current
 // passive buffer does not change.
+// One passive stage: velocities from the MHD states qold (t^n) and qnew (t^{n+1}), or, with
+// vel, read from the per-cell velocity set the fused MHD stage has just written.
 template <class real>
-void passive_fused(imhd_ctx* c, int qold) {
+void passive_stage(imhd_ctx* c, int qold, int qnew, int stage, bool vel) {
     const int np = c->cfg.npass;
     const int p0 = c->pcur, p1 = 1 - c->pcur;
-    for (int stage = 0; stage < 2; stage++) {
-        ProfScope ps(c, stage ? "pass_stage_corr" : "pass_stage_pred", c->pl_pstage[stage].active);
-        PassStageArgs<real> a{};
-        for (int p = 0; p < np; p++) {
-            a.src[p] = R<real>(c->qp[p0][p]);
-            a.rec[p] = R<real>(c->qp[stage ? p1 : p0][p]);
-            a.dst[p] = R<real>(c->qp[stage ? p0 : p1][p]);
-        }
-        for (int f = 0; f < 4; f++) {
-            a.qo[f] = R<real>(c->q[qold][f]);
-            a.qn[f] = R<real>(c->q[c->cur][f]);
-        }
-        a.dt = c->dt;
-        a.factor = stage ? 1.0 : 0.5;
-        a.npass = np;
-        a.stage = stage;
-        launch_pass_stage<real>(mk(c, c->pl_pstage[stage]), a);
+    ProfScope ps(c, stage ? "pass_stage_corr" : "pass_stage_pred", c->pl_pstage[stage].active);
+    PassStageArgs<real> a{};
+    for (int p = 0; p < np; p++) {
+        a.src[p] = R<real>(c->qp[p0][p]);
+        a.rec[p] = R<real>(c->qp[stage ? p1 : p0][p]);
+        a.dst[p] = R<real>(c->qp[stage ? p0 : p1][p]);
     }
+    for (int f = 0; f < 4; f++) {
+        a.qo[f] = R<real>(c->q[qold][f]);
+        a.qn[f] = R<real>(c->q[qnew][f]);
+    }
+    for (int d = 0; d < 3; d++) a.vel[d] = vel ? R<real>(c->pvel[d]) : nullptr;
+    a.dt = c->dt;
+    a.factor = stage ? 1.0 : 0.5;
+    a.npass = np;
+    a.stage = stage;
+    launch_pass_stage<real>(mk(c, c->pl_pstage[stage]), a);
+}
+
+template <class real>
+void passive_fused(imhd_ctx* c, int qold) {
+    for (int stage = 0; stage < 2; stage++) passive_stage<real>(c, qold, c->cur, stage, false);
 }
 
 template <class real>
@@ -817,9 +841,10 @@ void native_step(imhd_ctx* c, double t_end) {
         return;
     }
     launch_reset_u64(c->st, c->cmax, (int)stripe_slots(c->nbox));
-    vl_step<real>(c, !c->ou_on);
+    const bool inl = c->pvel[0] != nullptr && !c->ou_on;
+    vl_step<real>(c, !c->ou_on, inl);
     if (c->ou_on) ou_step<real>(c, true);   // the CFL speed is taken after the kick
-    if (c->cfg.npass > 0) passive_fused<real>(c, qold);
+    if (c->cfg.npass > 0 && !inl) passive_fused<real>(c, qold);
     fill_ghosts<real>(c);
     launch_new_dt(c->st, c->nbox, c->db, c->cmax, c->smax, c->dvmax, c->cour, t_end, c->t, c->dt, c->dtprev, 1);
 }
@@ -1057,6 +1082,8 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     if (fused) {
         for (int f = 0; f < 7; f++) if ((rc = alloc(c, &c->q[2][f], nr))) return rc;
         for (int d = 0; d < 3; d++) if ((rc = alloc(c, (void**)&c->b[2][d], nd))) return rc;
+        if (g.npass > 0)
+            for (int d = 0; d < 3; d++) if ((rc = alloc(c, &c->pvel[d], nr))) return rc;
     } else {
         for (int d = 0; d < 3; d++)
             for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->F[d][f], nr))) return rc;
@@ -1144,6 +1171,7 @@ int imhd_destroy(imhd_ctx* c) {
         for (auto p : c->qp[s]) fr(p);
     }
     for (auto p : c->q[2]) fr(p);
+    for (auto p : c->pvel) fr(p);
     for (auto p : c->b[2]) fr(p);
     fr(c->tp_pred.d);
     fr(c->tp_corr.d);

@@ -112,6 +112,10 @@ struct StageArgs {
     double cs;
     // index ranges per axis a, as [lo, n_a + hi]: faces normal / transverse, corners, updates
     int fn_lo, fn_hi, ft_lo, ft_hi, c_lo, c_hi, u_lo, u_hi;
+    // non-null: the passive scalars' velocities of every updated cell, in the arithmetic of
+    // copy_velocity -- the predictor writes m/rho at t^n, the corrector the average of t^n
+    // and the new state
+    real* pvel[3];
 };
 template <class real, int RS>
 void launch_vl_stage(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a,

@@ -214,10 +214,12 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         if (upd) {
             const long long c = cell_index(B, i, j, k);
             real nq[7];   // the new state of the cell, for its CFL speed
+            real oq[4];   // rho, m at t^n
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 real div = dxy[f] + (fz_hi[f] - fz_lo[f]);
                 real q0v = a.src.q[f][c];
+                oq[f] = q0v;
                 real v;
                 if (a.comp[0] != nullptr) {
                     real y = -(dtdx * div) - a.comp[f][c];
@@ -265,6 +267,20 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             bz_m = bz;
             have_bzm = true;
             ey_m = ey; ex_m = ex; eyx_m = ey_x; exy_m = ex_y;
+            if (a.pvel[0] != nullptr) {   // passive velocities (copy_velocity's arithmetic)
+                const real io = ONE ? (real)__builtin_amdgcn_rcpf((float)oq[0]) : real(1) / oq[0];
+                if (KIND == 0) {
+#pragma unroll
+                    for (int d = 0; d < 3; d++) a.pvel[d][c] = oq[1 + d] * io;
+                } else {
+                    const real in = ONE ? (real)__builtin_amdgcn_rcpf((float)nq[0]) : real(1) / nq[0];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const real vo = oq[1 + d] * io;
+                        a.pvel[d][c] = real(0.5) * (vo + nq[1 + d] * in);
+                    }
+                }
+            }
             if (a.cmax != nullptr && int_ij && k >= 1 && k <= n2) {
                 if constexpr (ONE) {
                     cfl = fmax(cfl, (double)cell_fast_speed_f(nq, (float)a.cs));
