@@ -117,7 +117,9 @@ struct imhd_ctx {
     bool fuse3 = true;   // the three directional flux sweeps of a stage in one launch
     PlanDev pl_pflux3, pl_cflux3;
     bool vstage = true;  // VL stages as single fused launches (k_vl_stage)
-    int vseg = 16;       // z-segment length of the fused stages
+    int vseg = 16;       // z-segment length of the fused stages when vrounds = 0
+    int vrounds = 4;     // > 0: fused-stage workgroups = vrounds x resident workgroups of the GPU
+    int ncu = 0;         // compute units of the device
     int vshape = 0;      // tile shape of the fused stages (0: 16 x 16, 1: 32 x 16; FP32 default 1)
     TilePlanDev tp_pred, tp_corr;
     PlanDev pl_tcorner_p, pl_tcorner_c, pl_tcenter, pl_thalf, pl_tcfl;
@@ -262,6 +264,22 @@ StageRanges stage_ranges(const imhd_ctx* c, bool corr) {
 // width - 2); threads 1 .. width-2 update, threads 0 .. width-1 solve faces, threads
 // 0 .. width-2 form corners.  The tile count covers every face, corner and cell of the
 // stage's ranges; the march covers planes zlo-1 .. zhi+1.
+template <class real>
+int stage_blocks_per_cu(const imhd_ctx* c, bool corr) {
+    switch (c->cfg.riemann) {
+        case IMHD_ROE: return vl_stage_blocks_per_cu<real, RS_ROE>(corr, c->vshape);
+        case IMHD_HLLE: return vl_stage_blocks_per_cu<real, RS_HLLE>(corr, c->vshape);
+        default: return vl_stage_blocks_per_cu<real, RS_HLLD>(corr, c->vshape);
+    }
+}
+
+// Tiles and z-segments of a fused stage.  Thread t of tile T sits at lo + T w + t (w = tile
+// width - 2); threads 1 .. width-2 update, threads 0 .. width-1 solve faces, threads
+// 0 .. width-2 form corners.  The tile count covers every face, corner and cell of the
+// stage's ranges; a segment's march covers planes ks-1 .. ke.  Segments per tile: with
+// vrounds > 0 the launch has vrounds x (resident workgroups of the GPU) workgroups, split
+// over boxes by work and over a box's tiles as s or s+1 segments, so every round of
+// resident workgroups is full; with vrounds = 0, segments of at most vseg planes.
 int build_tiles(imhd_ctx* c, TilePlanDev& p, bool corr) {
     p.h.clear();
     p.active = 0;
@@ -269,28 +287,44 @@ int build_tiles(imhd_ctx* c, TilePlanDev& p, bool corr) {
     const StageRanges R = stage_ranges(c, corr);
     const int lo = std::min(std::min(R.ft_lo, R.fn_lo), R.u_lo - 1);
     const int fhi = std::max(R.ft_hi, R.fn_hi);
-    const int seg = std::max(1, c->vseg);
-    long long run = 0;
+    const int wx[2] = {vs_tx(c->vshape) - 2, vs_ty(c->vshape) - 2};
+    std::vector<long long> tiles, planes;
+    long long work = 0;
     for (int bx = 0; bx < c->nbox; bx++) {
         const BoxDesc& B = c->hb[bx];
         TileBox tb{};
         tb.x0 = lo;
         tb.y0 = lo;
-        const int wx[2] = {vs_tx(c->vshape) - 2, vs_ty(c->vshape) - 2};
         for (int a = 0; a < 2; a++) {
             const int need = std::max(B.n[a] + fhi - 1, B.n[a] + R.u_hi) - lo;
             tb.nt[a] = (need + wx[a] - 1) / wx[a];
         }
         tb.zlo = lo + 1;
         tb.zhi = B.n[2] + std::max(std::max(R.ft_hi - 1, R.fn_hi), R.u_hi);
-        tb.seg = seg;
-        tb.nseg = (tb.zhi - tb.zlo + 1 + seg - 1) / seg;
-        tb.start = run;
-        run += (long long)tb.nt[0] * tb.nt[1] * tb.nseg;
+        tiles.push_back((long long)tb.nt[0] * tb.nt[1]);
+        planes.push_back(tb.zhi - tb.zlo + 1);
+        work += tiles.back() * planes.back();
         long long cells = 1;
         for (int a = 0; a < 3; a++) cells *= std::max(0, B.n[a] + R.u_hi - R.u_lo + 1);
         p.active += cells;
         p.h.push_back(tb);
+    }
+    long long target = 0;
+    if (c->vrounds > 0) {
+        const int bpc = c->prec == 64 ? stage_blocks_per_cu<double>(c, corr) : stage_blocks_per_cu<float>(c, corr);
+        target = (long long)c->vrounds * bpc * std::max(1, c->ncu);
+    }
+    long long run = 0;
+    for (int bx = 0; bx < c->nbox; bx++) {
+        TileBox& tb = p.h[bx];
+        const long long T = tiles[bx], L = planes[bx];
+        long long n = target > 0 ? std::llround((double)target * (double)(T * L) / (double)work)
+                                 : T * ((L + std::max(1, c->vseg) - 1) / std::max(1, c->vseg));
+        n = std::min(std::max(n, T), T * L);
+        tb.s = (int)(n / T);
+        tb.k1 = (int)(n % T);
+        tb.start = run;
+        run += n;
     }
     p.total = run;
     HIPCK(c, hipMalloc(&p.d, sizeof(TileBox) * p.h.size()));
@@ -1042,7 +1076,8 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     if (const char* e = getenv("IMHD_FUSE3")) c->fuse3 = atoi(e) != 0;      // 0: one launch per direction
     if (const char* e = getenv("IMHD_STRIP")) c->strip = std::max(0, atoi(e));
     if (const char* e = getenv("IMHD_VSTAGE")) c->vstage = atoi(e) != 0;    // 0: three kernels per stage
-    if (const char* e = getenv("IMHD_VSEG")) c->vseg = std::max(1, atoi(e));
+    if (const char* e = getenv("IMHD_VROUNDS")) c->vrounds = std::max(0, atoi(e));
+    if (const char* e = getenv("IMHD_VSEG")) { c->vseg = std::max(1, atoi(e)); c->vrounds = 0; }
     // FP32: 32 x 16 tiles (2 workgroups of 80 KB LDS per CU); FP64: 16 x 16 (a 32 x 16
     // FP64 tile would leave one workgroup per CU)
     c->vshape = g.precision == 32 ? 1 : 0;
@@ -1066,6 +1101,11 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     c->pool = run;
     *out = c;
     HIPCK(c, hipStreamCreate(&c->st));
+    {
+        int dev = 0;
+        HIPCK(c, hipGetDevice(&dev));
+        HIPCK(c, hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
     HIPCK(c, hipMalloc(&c->db, sizeof(BoxDesc) * nbox));
     HIPCK(c, hipMemcpy(c->db, c->hb.data(), sizeof(BoxDesc) * nbox, hipMemcpyHostToDevice));
     const size_t nr = (size_t)run * c->esz, nd = (size_t)run * 8;
@@ -1307,7 +1347,7 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
     // launch shape: z-segment of the column kernels, y-strip of the work enumeration, and
     // tile shape and z-segment of the fused VL stages
     struct Shape {
-        int zseg, strip, vshape, vseg;
+        int zseg, strip, vshape, vseg, vrounds;
     };
     const bool vl = c->tp_pred.d != nullptr;
     auto set_shape = [&](const Shape& sh) -> int {
@@ -1315,6 +1355,7 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
         c->strip = sh.strip;
         c->vshape = sh.vshape;
         c->vseg = sh.vseg;
+        c->vrounds = sh.vrounds;
         int r = build_plan(c, c->pl_pupd, 2, -1, sh.zseg);
         if (!r) r = build_plan(c, c->pl_cupd, 5, -1, sh.zseg);
         if (!r) r = build_plan(c, c->pl_pcorner, 1, -1, sh.zseg);
@@ -1327,7 +1368,7 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
         return r;
     };
     std::string rep;
-    Shape bsh{c->zseg, c->strip, c->vshape, c->vseg};
+    Shape bsh{c->zseg, c->strip, c->vshape, c->vseg, c->vrounds};
     float best = 1e30f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!rc) { HIPCK(c, hipEventCreate(&e0)); HIPCK(c, hipEventCreate(&e1)); }
@@ -1345,7 +1386,8 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
         HIPCK(c, hipEventElapsedTime(&ms, e0, e1));
         ms /= (float)steps;
         char b[128];
-        snprintf(b, sizeof(b), "zseg=%d strip=%d tile=%d vseg=%d: %.4f ms/step; ", sh.zseg, sh.strip, sh.vshape, sh.vseg, ms);
+        snprintf(b, sizeof(b), "zseg=%d strip=%d tile=%d vrounds=%d vseg=%d: %.4f ms/step; ", sh.zseg, sh.strip,
+                 sh.vshape, sh.vrounds, sh.vseg, ms);
         rep += b;
         if (ms < best) { best = ms; bsh = sh; }
         return 0;
@@ -1353,13 +1395,15 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
     // coordinate search, one parameter at a time from the current best
     static const int strips[] = {0, 8, 16, 32, 64};
     static const int zsegs[] = {0, 2, 4, 8, 16};
-    static const int vsegs[] = {8, 12, 16, 20, 24, 32};
+    static const int vsegs[] = {8, 12, 16};
+    static const int vrounds[] = {1, 2, 3, 4, 6, 8};
     const Shape s0 = bsh;
-    for (int s : strips) if (!rc) rc = trial({s0.zseg, s, s0.vshape, s0.vseg});
-    { const Shape b0 = bsh; for (int z : zsegs) if (!rc && z != b0.zseg) rc = trial({z, b0.strip, b0.vshape, b0.vseg}); }
+    for (int s : strips) if (!rc) rc = trial({s0.zseg, s, s0.vshape, s0.vseg, s0.vrounds});
+    { const Shape b0 = bsh; for (int z : zsegs) if (!rc && z != b0.zseg) rc = trial({z, b0.strip, b0.vshape, b0.vseg, b0.vrounds}); }
     if (vl) {
-        { const Shape b0 = bsh; for (int v = 0; v < VS_SHAPES; v++) if (!rc && v != b0.vshape) rc = trial({b0.zseg, b0.strip, v, b0.vseg}); }
-        { const Shape b0 = bsh; for (int v : vsegs) if (!rc && v != b0.vseg) rc = trial({b0.zseg, b0.strip, b0.vshape, v}); }
+        { const Shape b0 = bsh; for (int v = 0; v < VS_SHAPES; v++) if (!rc && v != b0.vshape) rc = trial({b0.zseg, b0.strip, v, b0.vseg, b0.vrounds}); }
+        { const Shape b0 = bsh; for (int r : vrounds) if (!rc && r != b0.vrounds) rc = trial({b0.zseg, b0.strip, b0.vshape, b0.vseg, r}); }
+        { const Shape b0 = bsh; for (int v : vsegs) if (!rc && (b0.vrounds != 0 || v != b0.vseg)) rc = trial({b0.zseg, b0.strip, b0.vshape, v, 0}); }
     }
     if (!rc) rc = set_shape(bsh);
     if (!rc) rc = restore();
@@ -1368,8 +1412,8 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
     for (auto& sv : saved) (void)hipFree(sv.second);
     if (rc) return rc;
     char b[128];
-    snprintf(b, sizeof(b), "chosen zseg=%d strip=%d tile=%d vseg=%d (%.4f ms/step)", bsh.zseg, bsh.strip, bsh.vshape,
-             bsh.vseg, best);
+    snprintf(b, sizeof(b), "chosen zseg=%d strip=%d tile=%d vrounds=%d vseg=%d (%.4f ms/step)", bsh.zseg, bsh.strip,
+             bsh.vshape, bsh.vrounds, bsh.vseg, best);
     rep += b;
     c->prof_report = rep;
     if (report && report_len > 0) {

@@ -89,7 +89,10 @@ struct TileBox {
     int x0, y0;        // cell index of thread (0, 0) of tile (0, 0)
     int zlo, zhi;      // planes marched (updates masked to the update range)
     int nt[2];         // tiles along x, y (tile t starts at x0 + t (TX - 2))
-    int nseg, seg;     // z-segments of seg planes
+    // z-segments: tiles 0 .. k1-1 (x fastest) have s+1 of them, the others s; segment g of
+    // n covers planes zlo + g L / n .. zlo + (g+1) L / n - 1 (L = zhi - zlo + 1).
+    // Workgroups are numbered segment-major: segment g of every tile, then g+1.
+    int s, k1;
     long long start;   // first workgroup of this box
 };
 struct TilePlan {
@@ -120,5 +123,8 @@ struct StageArgs {
 template <class real, int RS>
 void launch_vl_stage(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a,
                      bool corrector, int shape);
+// workgroups of a fused stage resident per CU (occupancy of the kernel as compiled)
+template <class real, int RS>
+int vl_stage_blocks_per_cu(bool corrector, int shape);
 
 }  // namespace imhd

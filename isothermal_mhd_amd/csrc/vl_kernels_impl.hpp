@@ -521,7 +521,8 @@ namespace imhd {
     template void launch_flux_predictor<T, RS>(const Launch&, const SweepArgs<T>&); \
     template void launch_flux_corrector<T, RS>(const Launch&, const SweepArgs<T>&); \
     template void launch_flux3<T, RS>(const Launch&, const Sweep3Args<T>&, int);     \
-    template void launch_vl_stage<T, RS>(hipStream_t, const BoxDesc*, const TilePlan&, const StageArgs<T>&, bool, int);
+    template void launch_vl_stage<T, RS>(hipStream_t, const BoxDesc*, const TilePlan&, const StageArgs<T>&, bool, int); \
+    template int vl_stage_blocks_per_cu<T, RS>(bool, int);
 #define IMHD_INST_VL(T)                                                      \
     IMHD_INST_RS(T, RS_ROE)                                                  \
     IMHD_INST_RS(T, RS_HLLD)                                                 \

@@ -54,14 +54,24 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     }
     const TileBox& tb = plan.boxes[b];
     const long long loc = (long long)blk - tb.start;
-    const int tX = (int)(loc % tb.nt[0]);
-    const long long l1 = loc / tb.nt[0];
-    const int tY = (int)(l1 % tb.nt[1]);
-    const int sg = (int)(l1 / tb.nt[1]);
+    const long long ntile = (long long)tb.nt[0] * tb.nt[1];
+    long long tile;
+    int sg, nsg;
+    if (loc < (long long)tb.s * ntile) {
+        sg = (int)(loc / ntile);
+        tile = loc - (long long)sg * ntile;
+        nsg = tile < tb.k1 ? tb.s + 1 : tb.s;
+    } else {
+        sg = tb.s;
+        tile = loc - (long long)tb.s * ntile;
+        nsg = tb.s + 1;
+    }
+    const int tX = (int)(tile % tb.nt[0]), tY = (int)(tile / tb.nt[0]);
     const BoxDesc& B = boxes[b];
     const int t = threadIdx.x, tx = t % TX, ty = t / TX;
     const int i = tb.x0 + tX * (TX - 2) + tx, j = tb.y0 + tY * (TY - 2) + ty;
-    const int ks = tb.zlo + sg * tb.seg, ke = min(ks + tb.seg, tb.zhi + 1);
+    const long long L = tb.zhi - tb.zlo + 1;
+    const int ks = tb.zlo + (int)(sg * L / nsg), ke = tb.zlo + (int)((sg + 1) * L / nsg);
     const int nb = B.nb, n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
     auto inr = [](int v, int lo, int hi) { return v >= lo && v <= hi; };
@@ -318,6 +328,23 @@ void launch_vl_stage_t(hipStream_t st, const BoxDesc* boxes, const TilePlan& pla
         hipLaunchKernelGGL((k_vl_stage<real, RS, 0, TX, TY>), dim3((unsigned)plan.total), dim3(TX * TY), 0, st, boxes,
                            plan, a);
 }
+template <class real, int RS>
+int vl_stage_blocks_per_cu(bool corrector, int shape) {
+    int n = 0;
+    hipError_t e;
+    if (shape == 1)
+        e = corrector ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_vl_stage<real, RS, 1, vs_tx(1), vs_ty(1)>,
+                                                                       vs_tx(1) * vs_ty(1), 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_vl_stage<real, RS, 0, vs_tx(1), vs_ty(1)>,
+                                                                       vs_tx(1) * vs_ty(1), 0);
+    else
+        e = corrector ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_vl_stage<real, RS, 1, vs_tx(0), vs_ty(0)>,
+                                                                       vs_tx(0) * vs_ty(0), 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_vl_stage<real, RS, 0, vs_tx(0), vs_ty(0)>,
+                                                                       vs_tx(0) * vs_ty(0), 0);
+    return e == hipSuccess && n > 0 ? n : 1;
+}
+
 template <class real, int RS>
 void launch_vl_stage(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a,
                      bool corrector, int shape) {

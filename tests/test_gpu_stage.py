@@ -61,16 +61,19 @@ def test_fused_stage_equals_three_kernel_path_fp64(monkeypatch, case):
 
 
 def test_fused_stage_segments_fp64(monkeypatch):
-    """z-segment length changes only the work split (segment overlap planes are solved
-    twice); results are identical."""
+    """The z-split of the fused stages -- uniform segments (IMHD_VSEG) or a whole number of
+    rounds of resident workgroups with s or s+1 balanced segments per tile (IMHD_VROUNDS) --
+    changes only the work split (segment boundary planes are solved twice); results are
+    identical."""
     case = CASES[0]
     say("[gpu] fused stage z-segments")
     ref = run(monkeypatch, {"IMHD_VSEG": "16"}, *case)
-    for seg in ("1", "5", "64"):
-        got = run(monkeypatch, {"IMHD_VSEG": seg}, *case)
+    for env in ({"IMHD_VSEG": "1"}, {"IMHD_VSEG": "5"}, {"IMHD_VSEG": "64"},
+                {"IMHD_VROUNDS": "1"}, {"IMHD_VROUNDS": "3"}, {"IMHD_VROUNDS": "1000"}):
+        got = run(monkeypatch, env, *case)
         for x, y in zip(ref[0], got[0]):
             if x is not None:
-                assert np.array_equal(x, y), f"state differs (segment {seg})"
+                assert np.array_equal(x, y), f"state differs ({env})"
 
 
 def test_fused_stage_fp32_matches_three_kernel_path(monkeypatch):
