@@ -279,7 +279,10 @@ int stage_blocks_per_cu(const imhd_ctx* c, bool corr) {
 // stage's ranges; a segment's march covers planes ks-1 .. ke.  Segments per tile: with
 // vrounds > 0 the launch has vrounds x (resident workgroups of the GPU) workgroups, split
 // over boxes by work and over a box's tiles as s or s+1 segments, so every round of
-// resident workgroups is full; with vrounds = 0, segments of at most vseg planes.
+// resident workgroups is full (more rounds if segments would exceed VS_MAXSEG planes);
+// with vrounds = 0, segments of at most vseg planes.
+constexpr long long VS_MAXSEG = 32;
+
 int build_tiles(imhd_ctx* c, TilePlanDev& p, bool corr) {
     p.h.clear();
     p.active = 0;
@@ -312,7 +315,12 @@ int build_tiles(imhd_ctx* c, TilePlanDev& p, bool corr) {
     long long target = 0;
     if (c->vrounds > 0) {
         const int bpc = c->prec == 64 ? stage_blocks_per_cu<double>(c, corr) : stage_blocks_per_cu<float>(c, corr);
-        target = (long long)c->vrounds * bpc * std::max(1, c->ncu);
+        const long long slots = (long long)bpc * std::max(1, c->ncu);
+        target = (long long)c->vrounds * slots;
+        // segments of at most VS_MAXSEG planes (a whole number of rounds): many small boxes
+        // would otherwise get one long segment per tile, of unequal lengths across boxes
+        const long long need = (work + VS_MAXSEG - 1) / VS_MAXSEG;
+        if (target < need) target = (need + slots - 1) / slots * slots;
     }
     long long run = 0;
     for (int bx = 0; bx < c->nbox; bx++) {
