@@ -248,8 +248,10 @@ def pmc_summarize(fetch_dir, write_dir):
     return 0
 
 
-# VALU issue peak of MI355X: 256 CUs x 4 SIMDs, a wave64 VALU instruction every 4 cycles per
-# SIMD (16 lanes), 2.4 GHz engine clock
+# Nominal VALU issue rate of MI355X: 256 CUs x 4 SIMDs, a wave64 VALU instruction every 4
+# cycles per SIMD (16 lanes), 2.4 GHz engine clock.  Some simple instructions issue faster
+# (a passive-stage kernel measures above it), so a fraction of it is an approximate
+# utilisation, not a bound.
 VALU_PEAK_GWI = 256 * 4 / 4 * 2.4
 
 
