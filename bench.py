@@ -39,14 +39,16 @@ METRIC = "Mcell-updates/s + HBM GB/s % peak, FP32 vs FP64 path at stated L1 erro
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 
 
-def bytes_per_item(name, esz):
+def bytes_per_item(name, esz, npass=1):
     """Algorithmic (compulsory) HBM bytes per work item of each kernel: every input field
-    read once and every output written once per cell or face."""
+    read once and every output written once per cell or face (npass: passive scalars of
+    the workload, whose velocities the fused stages write)."""
     d = 8
+    pv = 3 * esz if npass > 0 else 0     # passive velocities written by the fused stages
     if name.startswith("stage_pred"):     # fused predictor: 7 state + 3 face B in (t^n), 7 state + 3 face B out (half step)
-        return 14 * esz + 6 * d
+        return 14 * esz + 6 * d + pv
     if name.startswith("stage_corr"):     # fused corrector: half step 7 + 3 face B, t^n rho, m + 3 face B in; 7 + 3 out
-        b = 18 * esz + 9 * d
+        b = 18 * esz + 9 * d + pv
         if esz == 4:
             b += 8 * esz                  # Kahan terms of rho, m: read and written (FP32 path)
         return b
