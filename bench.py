@@ -346,6 +346,40 @@ def run_ensemble(steps, cour, warmup=2):
                         "boxes, whole step in a hipGraph"}
 
 
+def run_io(n, reps=2):
+    """Restart and snapshot files of one n^3 FP32 box with one passive scalar (the 256^3
+    turbulence IC): imhd_write_cube (REAL*4 and REAL*8 xyz cube, packed on the device by
+    k_pack_cube), imhd_write_restart and imhd_read_restart, to files under $TMPDIR.  Rates
+    include PCIe, the host page cache and the file system."""
+    import tempfile
+    from isothermal_mhd_amd import VanLeerSolver, ics
+    q, bf, qp = ics.config_c2(n=n)
+    S = VanLeerSolver((n, n, n), npass=1, riemann="hlld", bc="periodic", precision=32)
+    S.set_state(q, bf, qp)
+    del q, bf, qp
+    out = {"n": n, "dtype": "f32", "reps": reps}
+    with tempfile.TemporaryDirectory() as d:
+        ops = [("cube_f32", lambda p: S.write_cube(p, double_prec=False)),
+               ("cube_f64", lambda p: S.write_cube(p, double_prec=True)),
+               ("write_restart", lambda p: S.write_restart(p, 1.0, 1, (1234, 5678, 9123))),
+               ("read_restart", lambda p: S.read_restart(p))]
+        for name, fn in ops:
+            p = os.path.join(d, "restart.dat" if "restart" in name else name + ".dat")
+            best = None
+            for _ in range(reps):
+                S.sync()
+                t0 = time.perf_counter()
+                fn(p)
+                S.sync()
+                sec = time.perf_counter() - t0
+                best = sec if best is None else min(best, sec)
+            size = os.path.getsize(p)
+            out[name] = {"bytes": size, "s": best, "GB_per_s": size / best / 1e9}
+            print(f"[io] {name}: {size / 1e6:.0f} MB in {best:.3f} s", flush=True)
+    S.close()
+    return out
+
+
 def run_tvd_c1(steps, cour):
     """64^3 periodic, rho = 1, B0 = (0.5,0,0), k^-2 velocity 1<=|k|<=4 at rms Mach 1 (seed 1001),
     K98 Roe-TVD + CTU + CT, FP64, native loop in a hipGraph."""
@@ -430,6 +464,7 @@ def main():
     ap.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--sq-summarize", metavar="SQ_DIR")
     ap.add_argument("--big", metavar="N[,N]", help="only the large-box capacity lines, e.g. 512,1024")
+    ap.add_argument("--io", type=int, metavar="N", help="only the restart/snapshot file timing of an N^3 box")
     args = ap.parse_args()
     if args.pmc_summarize:
         return pmc_summarize(*args.pmc_summarize)
@@ -442,6 +477,9 @@ def main():
         return 2
     if args.pmc_run:
         return pmc_run(args.n)
+    if args.io:
+        print(json.dumps({"io": run_io(args.io)}), flush=True)
+        return 0
     if args.big:
         lines = []
         for nb_ in args.big.split(","):

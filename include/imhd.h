@@ -25,6 +25,7 @@ extern "C" {
 #define IMHD_EHIP (-2)     /* HIP runtime error (allocation, launch, copy) */
 #define IMHD_ESTATE (-3)   /* call out of order (e.g. no state set) */
 #define IMHD_ENOSPEED (-4) /* max speed below 1e-20 (vanleer_mod.f90:3328-3331) */
+#define IMHD_EIO (-5)      /* file: cannot open, short read/write, or records that do not fit the context */
 
 /* integrator: reference mhd_solver_type (constants_mod.f90:58) */
 #define IMHD_TVD 1
@@ -179,6 +180,28 @@ int imhd_autotune(imhd_ctx* ctx, int steps, char* report, int report_len);
 
 /* Synchronise the context's stream. */
 int imhd_sync(imhd_ctx* ctx);
+
+/* Restart file of one box == write_restart / read_restart (io_mod.f90:440-517): Fortran
+ * sequential-unformatted records t_sim, q_3d, bface_3d, qpass_3d (only when npass > 0),
+ * dump_count, s1, s2, s3, framed as gfortran frames them (the reference's compiler,
+ * 3D_MPI/build/Makefile:3): 4-byte length markers around each record, records above
+ * 2147483639 bytes split into subrecords with negated markers.  The state moves between
+ * the device and the file in bounded chunks (no whole-array host copy).  The reader also
+ * accepts single records whose markers hold the length modulo 2^32 (flang), checks every
+ * marker against this context's sizes before it changes any state, and rejects trailing
+ * data; it sets the state as imhd_set_state does and returns t_sim, dump_count and the
+ * seeds without applying them (the driver owns its clock and RNG, driver.f90:311). */
+int imhd_write_restart(imhd_ctx* ctx, int box, const char* path, double t_sim, int dump_count,
+                       const int seeds[3]);
+int imhd_read_restart(imhd_ctx* ctx, int box, const char* path, double* t_sim, int* dump_count,
+                      int seeds[3]);
+/* Snapshot cube of one box, packed on the device: interior cells, i fastest, per cell the 7
+ * cell fields then passive scalars 1..npass_out, REAL*8 (double_prec != 0) or REAL*4
+ * rounded to nearest (SNGL), no record markers.  npass_out = min(npass, 1) is the xyz cube
+ * of writetofile_unformatted (io_mod.f90:249-277); npass_out = npass is the layout of the
+ * parallel dump on one rank (load_parallel_file_buffer + writetofile_unformatted_parallel,
+ * io_mod.f90:302-436). */
+int imhd_write_cube(imhd_ctx* ctx, int box, const char* path, int double_prec, int npass_out);
 
 const char* imhd_last_error(const imhd_ctx* ctx);
 const char* imhd_version(void);

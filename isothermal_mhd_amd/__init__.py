@@ -5,6 +5,7 @@ reference's procedure interface.
 The compute path is libimhd.so (hand-written HIP for gfx950, C ABI in include/imhd.h);
 this package only binds it.
 """
+from ._lib import ImhdError  # noqa: F401
 from .solver import TvdSolver, VanLeerSolver, driver_step, padded_shapes  # noqa: F401
 
-__all__ = ["VanLeerSolver", "TvdSolver", "driver_step", "padded_shapes"]
+__all__ = ["VanLeerSolver", "TvdSolver", "driver_step", "padded_shapes", "ImhdError"]

@@ -11,7 +11,7 @@ IMHD_ROE, IMHD_HLLD, IMHD_HLLE = 1, 2, 3
 IMHD_PLM_PRIM, IMHD_PPM_PRIM, IMHD_PPM_CHAR = 0, 1, 2
 IMHD_BC_OPEN, IMHD_BC_PERIODIC = 1, 2
 
-ERRORS = {-1: "IMHD_EINVAL", -2: "IMHD_EHIP", -3: "IMHD_ESTATE", -4: "IMHD_ENOSPEED"}
+ERRORS = {-1: "IMHD_EINVAL", -2: "IMHD_EHIP", -3: "IMHD_ESTATE", -4: "IMHD_ENOSPEED", -5: "IMHD_EIO"}
 
 # Every symbol include/imhd.h declares.
 EXPORTS = [
@@ -19,6 +19,7 @@ EXPORTS = [
     "imhd_autotune", "imhd_step", "imhd_set_force", "imhd_set_ou_forcing", "imhd_get_ou_modes", "imhd_fill_ghosts", "imhd_max_speed", "imhd_copy_velocity", "imhd_passive_step",
     "imhd_diagnostics", "imhd_floor_count", "imhd_device_bytes", "imhd_run", "imhd_set_time", "imhd_get_time",
     "imhd_set_profiling", "imhd_profile_report", "imhd_sync", "imhd_last_error", "imhd_version",
+    "imhd_write_restart", "imhd_read_restart", "imhd_write_cube",
 ]
 
 
@@ -83,6 +84,9 @@ def lib():
     L.imhd_profile_report.argtypes = [vp]
     L.imhd_profile_report.restype = ctypes.c_char_p
     L.imhd_sync.argtypes = [vp]
+    L.imhd_write_restart.argtypes = [vp, ip, ctypes.c_char_p, dp, ip, vp]
+    L.imhd_read_restart.argtypes = [vp, ip, ctypes.c_char_p, vp, vp, vp]
+    L.imhd_write_cube.argtypes = [vp, ip, ctypes.c_char_p, ip, ip]
     L.imhd_last_error.argtypes = [vp]
     L.imhd_last_error.restype = ctypes.c_char_p
     L.imhd_version.restype = ctypes.c_char_p

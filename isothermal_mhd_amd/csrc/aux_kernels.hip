@@ -417,6 +417,32 @@ __global__ void k_pass_layout(BoxDesc B, const double* hsrc, double* hdst, real*
     else hdst[l] = (double)dev[c];
 }
 
+// Snapshot packing: each workgroup reads 256 consecutive interior cells field by field
+// (coalesced), transposes them through LDS and writes their 256 x (7 + npo) values as one
+// contiguous, coalesced run.  double -> float is the round-to-nearest conversion of SNGL.
+template <class real, class out_t>
+__global__ __launch_bounds__(256) void k_pack_cube(BoxDesc B, CubeArgs<real> a, out_t* __restrict__ out) {
+    __shared__ out_t s[(7 + MAXPASS) * 256];
+    const int V = 7 + a.npo;
+    const long long ncell = (long long)B.n[0] * B.n[1] * a.nk;
+    const long long base = (long long)blockIdx.x * 256;
+    const long long g = base + threadIdx.x;
+    if (g < ncell) {
+        const int i = (int)(g % B.n[0]);
+        const long long t = g / B.n[0];
+        const int j = (int)(t % B.n[1]);
+        const int k = a.k0 + (int)(t / B.n[1]);
+        const long long c = cell_index(B, i + 1, j + 1, k + 1);
+        out_t* row = s + threadIdx.x * V;
+        for (int v = 0; v < 7; v++) row[v] = (out_t)a.q[v][c];
+        for (int p = 0; p < a.npo; p++) row[7 + p] = (out_t)a.qp[p][c];
+    }
+    __syncthreads();
+    const long long left = ncell - base;
+    const int nv = (int)(left < 256 ? left : 256) * V;
+    for (int e = threadIdx.x; e < nv; e += 256) out[base * V + e] = s[e];
+}
+
 // ---------------------------------------------------------------------------------
 #define L256(k, ...) hipLaunchKernelGGL(k, dim3(L.blocks()), dim3(256), 0, L.stream, __VA_ARGS__)
 
@@ -488,6 +514,14 @@ void launch_pass_layout(hipStream_t st, const BoxDesc& B, const double* hsrc, do
                        to_device, g0, m);
 }
 
+template <class real, class out_t>
+void launch_pack_cube(hipStream_t st, const BoxDesc& B, const CubeArgs<real>& a, out_t* out) {
+    const long long ncell = (long long)B.n[0] * B.n[1] * a.nk;
+    if (ncell <= 0) return;
+    hipLaunchKernelGGL((k_pack_cube<real, out_t>), dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st, B, a,
+                       out);
+}
+
 #define INST_AUX(T)                                                                                           \
     template void launch_fill_ghosts<T>(const Launch&, const GhostArgs<T>&);                                  \
     template void launch_fill_ghost_band<T>(const Launch&, const GhostArgs<T>&);                              \
@@ -500,7 +534,9 @@ void launch_pass_layout(hipStream_t st, const BoxDesc& B, const double* hsrc, do
     template void launch_pass_update<T>(const Launch&, const PassUpdArgs<T>&);                                \
     template void launch_convert_in<T>(hipStream_t, const double*, T*, long long);                            \
     template void launch_convert_out<T>(hipStream_t, const T*, double*, long long);                           \
-    template void launch_pass_layout<T>(hipStream_t, const BoxDesc&, const double*, double*, T*, int, long long, long long);
+    template void launch_pass_layout<T>(hipStream_t, const BoxDesc&, const double*, double*, T*, int, long long, long long); \
+    template void launch_pack_cube<T, float>(hipStream_t, const BoxDesc&, const CubeArgs<T>&, float*);         \
+    template void launch_pack_cube<T, double>(hipStream_t, const BoxDesc&, const CubeArgs<T>&, double*);
 INST_AUX(double)
 INST_AUX(float)
 

@@ -80,6 +80,16 @@ struct ShellArgs {
     int lo, hi_off;   // cells with every index in [lo, n + hi_off] are left alone
 };
 
+// Interior planes k0 .. k0 + nk - 1 of one box, interleaved per cell (i fastest): the seven
+// cell fields, then the first npo passive scalars -- the layout of the reference's xyz
+// snapshot cube and parallel dump (io_mod.f90:249-277, 368-436).
+template <class real>
+struct CubeArgs {
+    const real* q[7];
+    const real* qp[MAXPASS];
+    int npo, k0, nk, _pad;
+};
+
 template <class real> void launch_restore_shell(const Launch& L, const ShellArgs<real>& a);
 template <class real> void launch_fill_ghosts(const Launch& L, const GhostArgs<real>& a);
 template <class real> void launch_fill_ghost_band(const Launch& L, const GhostArgs<real>& a);
@@ -95,6 +105,8 @@ void launch_new_dt(hipStream_t st, int nbox, const BoxDesc* boxes, const unsigne
 void launch_reset_u64(hipStream_t st, unsigned long long* p, int n);
 template <class real> void launch_convert_in(hipStream_t st, const double* src, real* dst, long long n);
 template <class real> void launch_convert_out(hipStream_t st, const real* src, double* dst, long long n);
+template <class real, class out_t>
+void launch_pack_cube(hipStream_t st, const BoxDesc& B, const CubeArgs<real>& a, out_t* out);
 template <class real>
 void launch_pass_layout(hipStream_t st, const BoxDesc& B, const double* hsrc, double* hdst, real* dev, int to_device,
                         long long g0, long long m);

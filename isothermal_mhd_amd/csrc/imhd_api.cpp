@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -1050,6 +1051,307 @@ static int ensure_passive_work(imhd_ctx* c) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------------
+// Restart and snapshot files (io_mod.f90).  Fortran sequential-unformatted records as
+// gfortran frames them: a 4-byte length before and after the data; a record longer than
+// MAX_SUBRECORD bytes is split into subrecords whose leading marker is negated when another
+// subrecord follows and whose trailing marker is negated when one precedes.
+
+constexpr long long MAX_SUBRECORD = 2147483639LL;  // gfortran's default and maximum (2^31 - 9)
+constexpr long long IO_CHUNK = 1LL << 22;            // doubles per staged transfer (32 MB)
+
+// IMHD_MAX_SUBRECORD (bytes, 1 .. MAX_SUBRECORD) lowers the split point so that tests can
+// exercise subrecords on small boxes.
+static long long max_subrecord() {
+    const char* e = getenv("IMHD_MAX_SUBRECORD");
+    const long long v = e ? atoll(e) : 0;
+    return (v >= 1 && v <= MAX_SUBRECORD) ? v : MAX_SUBRECORD;
+}
+
+struct SeqWriter {
+    FILE* f;
+    long long rec_left = 0, sub_left = 0, sub_n = 0, max_sub = max_subrecord();
+    bool first = true, ok = true;
+    explicit SeqWriter(FILE* fp) : f(fp) {}
+    void marker(long long m) {
+        const int32_t v = (int32_t)m;
+        ok = ok && fwrite(&v, 4, 1, f) == 1;
+    }
+    void open_sub() {
+        sub_n = std::min(max_sub, rec_left);
+        sub_left = sub_n;
+        marker(sub_n < rec_left ? -sub_n : sub_n);
+    }
+    void close_sub() {
+        marker(first ? sub_n : -sub_n);
+        first = false;
+    }
+    void begin(long long len) {
+        rec_left = len;
+        first = true;
+        open_sub();
+    }
+    void put(const void* p, long long n) {
+        const char* c = (const char*)p;
+        while (n > 0 && ok) {
+            if (sub_left == 0) {
+                close_sub();
+                open_sub();
+            }
+            const long long m = std::min(n, sub_left);
+            ok = ok && fwrite(c, 1, (size_t)m, f) == (size_t)m;
+            c += m;
+            n -= m;
+            sub_left -= m;
+            rec_left -= m;
+        }
+    }
+    void end() {
+        ok = ok && rec_left == 0 && sub_left == 0;
+        close_sub();
+    }
+};
+
+struct SeqReader {
+    FILE* f;
+    long long rec_left = 0, sub_left = 0, sub_n = 0;
+    bool chain = false, more = false, first = true;
+    int32_t lead = 0;
+    std::string err;
+    explicit SeqReader(FILE* fp) : f(fp) {}
+    bool fail(const char* m) {
+        if (err.empty()) err = m;
+        return false;
+    }
+    bool rd(void* p, long long n) { return fread(p, 1, (size_t)n, f) == (size_t)n; }
+    // Opens a record that must hold len bytes: gfortran subrecords, a plain record, or one
+    // record whose markers hold len modulo 2^32.
+    bool begin(long long len) {
+        const long long pos = (long long)ftello(f);
+        if (!rd(&lead, 4)) return fail("truncated file (record marker)");
+        rec_left = len;
+        first = true;
+        const bool wrapped = len > INT32_MAX && (uint32_t)lead == (uint32_t)(len & 0xffffffffLL);
+        if (lead >= 0) {
+            if ((long long)lead != len && !wrapped) return fail("record length does not match the context's sizes");
+            chain = more = false;
+            sub_n = sub_left = len;
+            return true;
+        }
+        const long long n = -(long long)lead;
+        int32_t t = 0;
+        if (fseeko(f, (off_t)(pos + 4 + n), SEEK_SET) == 0 && rd(&t, 4) && t == n) {
+            if (fseeko(f, (off_t)(pos + 4), SEEK_SET) != 0) return fail("seek failed");
+            chain = more = true;
+            sub_n = sub_left = n;
+            return true;
+        }
+        if (fseeko(f, (off_t)(pos + 4), SEEK_SET) != 0) return fail("seek failed");
+        if (!wrapped) return fail("record markers do not match the context's sizes");
+        chain = more = false;
+        sub_n = sub_left = len;
+        return true;
+    }
+    bool next_sub() {
+        if (!chain || !more) return fail("record shorter than the context's sizes");
+        int32_t t = 0, l = 0;
+        if (!rd(&t, 4) || t != (int32_t)(first ? sub_n : -sub_n)) return fail("bad subrecord marker");
+        first = false;
+        if (!rd(&l, 4)) return fail("truncated file (subrecord marker)");
+        more = l < 0;
+        sub_n = sub_left = l < 0 ? -(long long)l : (long long)l;
+        return true;
+    }
+    // dst == nullptr: skip (the validation pass)
+    bool get(void* dst, long long n) {
+        char* c = (char*)dst;
+        while (n > 0) {
+            if (sub_left == 0 && !next_sub()) return false;
+            const long long m = std::min(n, sub_left);
+            if (c) {
+                if (!rd(c, m)) return fail("truncated file (record data)");
+                c += m;
+            } else if (fseeko(f, (off_t)m, SEEK_CUR) != 0) {
+                return fail("seek failed");
+            }
+            n -= m;
+            sub_left -= m;
+            rec_left -= m;
+        }
+        return true;
+    }
+    bool end() {
+        if (rec_left != 0 || sub_left != 0 || (chain && more)) return fail("record longer than the context's sizes");
+        int32_t t = 0;
+        if (!rd(&t, 4)) return fail("truncated file (trailing marker)");
+        if (t != (chain ? (int32_t)(first ? sub_n : -sub_n) : lead)) return fail("bad trailing record marker");
+        return true;
+    }
+    bool record(void* dst, long long len) { return begin(len) && get(dst, len) && end(); }
+};
+
+struct FileCloser {
+    FILE* f;
+    ~FileCloser() {
+        if (f) fclose(f);
+    }
+};
+
+static long long pass_cells(const BoxDesc& B) {
+    return (long long)(B.n[0] + 2 * B.nb - 4) * (B.n[1] + 2 * B.nb - 4) * (B.n[2] + 2 * B.nb - 4);
+}
+
+template <class real>
+static int write_restart_t(imhd_ctx* c, int bx, FILE* f, double t_sim, int dump, const int* seeds) {
+    const BoxDesc& B = c->hb[bx];
+    const long long nc = (long long)B.P[0] * B.P[1] * B.P[2];
+    const int np = c->cfg.npass;
+    std::vector<double> buf((size_t)IO_CHUNK);
+    SeqWriter w(f);
+    int rc;
+    HIPCK(c, hipStreamSynchronize(c->st));
+    w.begin(8);
+    w.put(&t_sim, 8);
+    w.end();
+    w.begin(7 * nc * 8);
+    for (int fl = 0; fl < 7 && w.ok; fl++)
+        for (long long o = 0; o < nc && w.ok; o += IO_CHUNK) {
+            const long long m = std::min(IO_CHUNK, nc - o);
+            if ((rc = download_field<real>(c, R<real>(c->q[c->cur][fl]) + B.off + o, buf.data(), m))) return rc;
+            w.put(buf.data(), m * 8);
+        }
+    w.end();
+    w.begin(3 * nc * 8);
+    for (int d = 0; d < 3 && w.ok; d++)
+        for (long long o = 0; o < nc && w.ok; o += IO_CHUNK) {
+            const long long m = std::min(IO_CHUNK, nc - o);
+            HIPCK(c, hipMemcpy(buf.data(), c->b[c->cur][d] + B.off + o, (size_t)m * 8, hipMemcpyDeviceToHost));
+            w.put(buf.data(), m * 8);
+        }
+    w.end();
+    if (np > 0) {
+        const long long pn = pass_cells(B);
+        if ((rc = ensure_stage(c, (size_t)std::min(pn, IO_CHUNK)))) return rc;
+        w.begin(np * pn * 8);
+        for (int p = 0; p < np && w.ok; p++)
+            for (long long o = 0; o < pn && w.ok; o += IO_CHUNK) {
+                const long long m = std::min(IO_CHUNK, pn - o);
+                launch_pass_layout<real>(c->st, B, nullptr, c->stage, R<real>(c->qp[c->pcur][p]), 0, o, m);
+                HIPCK(c, hipStreamSynchronize(c->st));
+                HIPCK(c, hipMemcpy(buf.data(), c->stage, (size_t)m * 8, hipMemcpyDeviceToHost));
+                w.put(buf.data(), m * 8);
+            }
+        w.end();
+    }
+    const int32_t ints[4] = {dump, seeds[0], seeds[1], seeds[2]};
+    for (int i = 0; i < 4; i++) {
+        w.begin(4);
+        w.put(&ints[i], 4);
+        w.end();
+    }
+    if (!w.ok) FAIL(c, IMHD_EIO, "write_restart: write failed");
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
+template <class real>
+static int read_restart_t(imhd_ctx* c, int bx, FILE* f, double* t_sim, int* dump, int* seeds) {
+    const BoxDesc& B = c->hb[bx];
+    const long long nc = (long long)B.P[0] * B.P[1] * B.P[2];
+    const int np = c->cfg.npass;
+    const long long pn = pass_cells(B);
+    std::vector<long long> len = {8, 7 * nc * 8, 3 * nc * 8};
+    if (np > 0) len.push_back(np * pn * 8);
+    for (int i = 0; i < 4; i++) len.push_back(4);
+    // pass 1: every marker against the context's sizes, before any state changes
+    {
+        SeqReader r(f);
+        for (long long L : len)
+            if (!r.record(nullptr, L)) FAIL(c, IMHD_EIO, "read_restart: %s", r.err.c_str());
+        char extra;
+        if (fread(&extra, 1, 1, f) == 1) FAIL(c, IMHD_EIO, "read_restart: data after the last record");
+        if (fseeko(f, 0, SEEK_SET) != 0) FAIL(c, IMHD_EIO, "read_restart: seek failed");
+    }
+    SeqReader r(f);
+    std::vector<double> buf((size_t)IO_CHUNK);
+    int rc;
+    HIPCK(c, hipStreamSynchronize(c->st));
+#define RD(call) \
+    if (!(call)) FAIL(c, IMHD_EIO, "read_restart: %s", r.err.c_str())
+    RD(r.record(t_sim, 8));
+    RD(r.begin(7 * nc * 8));
+    for (int fl = 0; fl < 7; fl++)
+        for (long long o = 0; o < nc; o += IO_CHUNK) {
+            const long long m = std::min(IO_CHUNK, nc - o);
+            RD(r.get(buf.data(), m * 8));
+            if ((rc = upload_field<real>(c, buf.data(), R<real>(c->q[0][fl]) + B.off + o,
+                                         R<real>(c->q[1][fl]) + B.off + o, m)))
+                return rc;
+        }
+    RD(r.end());
+    RD(r.begin(3 * nc * 8));
+    for (int d = 0; d < 3; d++)
+        for (long long o = 0; o < nc; o += IO_CHUNK) {
+            const long long m = std::min(IO_CHUNK, nc - o);
+            RD(r.get(buf.data(), m * 8));
+            for (int s = 0; s < 2; s++)
+                HIPCK(c, hipMemcpy(c->b[s][d] + B.off + o, buf.data(), (size_t)m * 8, hipMemcpyHostToDevice));
+        }
+    RD(r.end());
+    if (np > 0) {
+        if ((rc = ensure_stage(c, (size_t)std::min(pn, IO_CHUNK)))) return rc;
+        RD(r.begin(np * pn * 8));
+        for (int p = 0; p < np; p++)
+            for (long long o = 0; o < pn; o += IO_CHUNK) {
+                const long long m = std::min(IO_CHUNK, pn - o);
+                RD(r.get(buf.data(), m * 8));
+                HIPCK(c, hipMemcpy(c->stage, buf.data(), (size_t)m * 8, hipMemcpyHostToDevice));
+                for (int s = 0; s < 2; s++)
+                    launch_pass_layout<real>(c->st, B, c->stage, nullptr, R<real>(c->qp[s][p]), 1, o, m);
+                HIPCK(c, hipStreamSynchronize(c->st));
+            }
+        RD(r.end());
+    }
+    int32_t ints[4];
+    for (int i = 0; i < 4; i++) RD(r.record(&ints[i], 4));
+#undef RD
+    *dump = ints[0];
+    for (int i = 0; i < 3; i++) seeds[i] = ints[1 + i];
+    HIPCK(c, hipStreamSynchronize(c->st));
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
+template <class real, class out_t>
+static int write_cube_t(imhd_ctx* c, int bx, FILE* f, int npo) {
+    const BoxDesc& B = c->hb[bx];
+    const int V = 7 + npo;
+    const long long plane = (long long)B.n[0] * B.n[1];
+    // planes per chunk: the packed chunk fits the staging buffer (IO_CHUNK doubles)
+    const long long per = std::max(1LL, (IO_CHUNK * 8) / (plane * V * (long long)sizeof(out_t)));
+    const long long nk = std::min(per, (long long)B.n[2]);
+    const long long chunk_bytes = nk * plane * V * (long long)sizeof(out_t);
+    int rc;
+    if ((rc = ensure_stage(c, (size_t)((chunk_bytes + 7) / 8)))) return rc;
+    std::vector<char> buf((size_t)chunk_bytes);
+    CubeArgs<real> a{};
+    for (int v = 0; v < 7; v++) a.q[v] = R<real>(c->q[c->cur][v]);
+    for (int p = 0; p < npo; p++) a.qp[p] = R<real>(c->qp[c->pcur][p]);
+    a.npo = npo;
+    HIPCK(c, hipStreamSynchronize(c->st));
+    for (int k0 = 0; k0 < B.n[2]; k0 += (int)nk) {
+        a.k0 = k0;
+        a.nk = (int)std::min(nk, (long long)B.n[2] - k0);
+        const size_t bytes = (size_t)(a.nk * plane * V) * sizeof(out_t);
+        launch_pack_cube<real, out_t>(c->st, B, a, (out_t*)c->stage);
+        HIPCK(c, hipGetLastError());
+        HIPCK(c, hipStreamSynchronize(c->st));
+        HIPCK(c, hipMemcpy(buf.data(), c->stage, bytes, hipMemcpyDeviceToHost));
+        if (fwrite(buf.data(), 1, bytes, f) != bytes) FAIL(c, IMHD_EIO, "write_cube: write failed");
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1652,6 +1954,52 @@ int imhd_floor_count(imhd_ctx* c, unsigned int* count) {
 int imhd_device_bytes(imhd_ctx* c, unsigned long long* bytes) {
     if (!c || !bytes) return IMHD_EINVAL;
     *bytes = c->dev_bytes;
+    return IMHD_OK;
+}
+
+int imhd_write_restart(imhd_ctx* c, int box, const char* path, double t_sim, int dump_count, const int seeds[3]) {
+    if (!c || !path || !seeds || box < 0 || box >= c->nbox) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    FileCloser fc{fopen(path, "wb")};
+    if (!fc.f) FAIL(c, IMHD_EIO, "write_restart: cannot open %s", path);
+    int rc = c->prec == 64 ? write_restart_t<double>(c, box, fc.f, t_sim, dump_count, seeds)
+                           : write_restart_t<float>(c, box, fc.f, t_sim, dump_count, seeds);
+    if (rc) return rc;
+    FILE* f = fc.f;
+    fc.f = nullptr;
+    if (fclose(f) != 0) FAIL(c, IMHD_EIO, "write_restart: close failed");
+    return IMHD_OK;
+}
+
+int imhd_read_restart(imhd_ctx* c, int box, const char* path, double* t_sim, int* dump_count, int seeds[3]) {
+    if (!c || !path || !t_sim || !dump_count || !seeds || box < 0 || box >= c->nbox) return IMHD_EINVAL;
+    FileCloser fc{fopen(path, "rb")};
+    if (!fc.f) FAIL(c, IMHD_EIO, "read_restart: cannot open %s", path);
+    int rc = c->prec == 64 ? read_restart_t<double>(c, box, fc.f, t_sim, dump_count, seeds)
+                           : read_restart_t<float>(c, box, fc.f, t_sim, dump_count, seeds);
+    if (rc) return rc;
+    if (c->comp[0]) for (int f = 0; f < 4; f++) HIPCK(c, hipMemset(c->comp[f], 0, (size_t)c->pool * c->esz));
+    c->have_state = true;
+    return IMHD_OK;
+}
+
+int imhd_write_cube(imhd_ctx* c, int box, const char* path, int double_prec, int npass_out) {
+    if (!c || !path || box < 0 || box >= c->nbox) return IMHD_EINVAL;
+    if (npass_out < 0 || npass_out > c->cfg.npass) FAIL(c, IMHD_EINVAL, "npass_out must be 0..npass");
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    FileCloser fc{fopen(path, "wb")};
+    if (!fc.f) FAIL(c, IMHD_EIO, "write_cube: cannot open %s", path);
+    int rc;
+    if (c->prec == 64)
+        rc = double_prec ? write_cube_t<double, double>(c, box, fc.f, npass_out)
+                         : write_cube_t<double, float>(c, box, fc.f, npass_out);
+    else
+        rc = double_prec ? write_cube_t<float, double>(c, box, fc.f, npass_out)
+                         : write_cube_t<float, float>(c, box, fc.f, npass_out);
+    if (rc) return rc;
+    FILE* f = fc.f;
+    fc.f = nullptr;
+    if (fclose(f) != 0) FAIL(c, IMHD_EIO, "write_cube: close failed");
     return IMHD_OK;
 }
 

@@ -5,7 +5,7 @@ MODULE imhd_api
   IMPLICIT NONE
 
   INTEGER(C_INT), PARAMETER :: IMHD_OK = 0, IMHD_EINVAL = -1, IMHD_EHIP = -2, IMHD_ESTATE = -3, &
-                               IMHD_ENOSPEED = -4
+                               IMHD_ENOSPEED = -4, IMHD_EIO = -5
   INTEGER(C_INT), PARAMETER :: IMHD_TVD = 1, IMHD_VL = 2
   INTEGER(C_INT), PARAMETER :: IMHD_ROE = 1, IMHD_HLLD = 2, IMHD_HLLE = 3
   INTEGER(C_INT), PARAMETER :: IMHD_PLM_PRIM = 0, IMHD_PPM_PRIM = 1, IMHD_PPM_CHAR = 2
@@ -127,6 +127,35 @@ MODULE imhd_api
       TYPE(C_PTR), VALUE :: ctx
       REAL(C_DOUBLE), INTENT(OUT) :: t(*), dt(*)
     END FUNCTION imhd_get_time
+
+    ! path: a NUL-terminated C_CHAR array, e.g. TRIM(name)//C_NULL_CHAR
+    INTEGER(C_INT) FUNCTION imhd_write_restart(ctx, box, path, t_sim, dump_count, seeds) &
+        BIND(C, name='imhd_write_restart')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE, C_CHAR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: box, dump_count
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+      REAL(C_DOUBLE), VALUE :: t_sim
+      INTEGER(C_INT), INTENT(IN) :: seeds(3)
+    END FUNCTION imhd_write_restart
+
+    INTEGER(C_INT) FUNCTION imhd_read_restart(ctx, box, path, t_sim, dump_count, seeds) &
+        BIND(C, name='imhd_read_restart')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE, C_CHAR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: box
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: t_sim
+      INTEGER(C_INT), INTENT(OUT) :: dump_count, seeds(3)
+    END FUNCTION imhd_read_restart
+
+    INTEGER(C_INT) FUNCTION imhd_write_cube(ctx, box, path, double_prec, npass_out) &
+        BIND(C, name='imhd_write_cube')
+      IMPORT :: C_INT, C_PTR, C_CHAR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: box, double_prec, npass_out
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+    END FUNCTION imhd_write_cube
 
     INTEGER(C_INT) FUNCTION imhd_sync(ctx) BIND(C, name='imhd_sync')
       IMPORT :: C_INT, C_PTR

@@ -8,6 +8,7 @@ copy_velocity, passive_solve_3d, compute_KE/compute_divB, and the driver's time 
 Arrays are the reference's host layout (FP64, Fortran order, nb ghost layers).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -104,6 +105,33 @@ class VanLeerSolver:
         qp = np.zeros(shp[:3] + (max(self.npass, 1),), order="F")
         self._check(self.lib.imhd_get_state(self._c, box, _p(q), _p(b), _p(qp)))
         return q, b, qp
+
+    # -- restart and snapshot files (io_mod.f90) ---------------------------------------
+    def write_restart(self, path, t_sim, dump_count=0, seeds=(1234, 5678, 9123), box=0):
+        """write_restart (io_mod.f90:440-473) of one box, straight from the device state:
+        records t_sim, q_3d, bface_3d, qpass_3d (npass > 0), dump_count, s1, s2, s3.  The
+        default seeds are the reference's s1_def, s2_def, s3_def (constants_mod.f90:23)."""
+        s = (ctypes.c_int * 3)(*[int(v) for v in seeds])
+        self._check(self.lib.imhd_write_restart(self._c, box, os.fsencode(path), float(t_sim), int(dump_count), s))
+
+    def read_restart(self, path, box=0):
+        """read_restart (io_mod.f90:477-517) into one box's device state; returns
+        (t_sim, dump_count, (s1, s2, s3)).  Raises ImhdError (IMHD_EIO) when the file's
+        records do not match this context's sizes; the state is then unchanged."""
+        t = ctypes.c_double()
+        d = ctypes.c_int()
+        s = (ctypes.c_int * 3)()
+        self._check(self.lib.imhd_read_restart(self._c, box, os.fsencode(path), ctypes.byref(t), ctypes.byref(d), s))
+        return t.value, d.value, tuple(s)
+
+    def write_cube(self, path, double_prec=False, npass_out=None, box=0):
+        """Snapshot cube of the interior (io_mod.f90:249-277): per cell the 7 fields and
+        passives 1..npass_out, REAL*8 or REAL*4.  npass_out defaults to min(npass, 1), the
+        reference's xyz cube; npass_out = npass gives the parallel dump's layout
+        (io_mod.f90:368-436)."""
+        if npass_out is None:
+            npass_out = min(self.npass, 1)
+        self._check(self.lib.imhd_write_cube(self._c, box, os.fsencode(path), int(bool(double_prec)), int(npass_out)))
 
     # -- reference procedures ------------------------------------------------------
     def vanleer_solve_3d(self, dt, drive_next=False):

@@ -168,6 +168,54 @@ CONTAINS
     CALL vanleer_solve_3d(dtl, .TRUE.)
   END SUBROUTINE ref_step_drive
 
+  ! Restart and snapshot files through the reference's own io_mod (io_mod.f90:249-277,
+  ! 440-517): write_restart / read_restart of the module arrays, and
+  ! writetofile_unformatted, whose only enabled output in these builds is the xyz cube.
+  ! The files land under Output/Restart and Output/Snapshots of the working directory.
+  SUBROUTINE ref_write_restart(t_sim, dcount, seeds) BIND(C, name='ref_write_restart')
+    USE io_mod, ONLY: write_restart
+    REAL(C_DOUBLE), VALUE :: t_sim
+    INTEGER(C_INT), VALUE :: dcount
+    INTEGER(C_INT), INTENT(IN) :: seeds(3)
+    dump_count = dcount
+    s1 = seeds(1)
+    s2 = seeds(2)
+    s3 = seeds(3)
+    CALL write_restart(t_sim)
+  END SUBROUTINE ref_write_restart
+
+  SUBROUTINE ref_read_restart(t_sim, dcount, seeds) BIND(C, name='ref_read_restart')
+    USE io_mod, ONLY: read_restart
+    REAL(C_DOUBLE), INTENT(OUT) :: t_sim
+    INTEGER(C_INT), INTENT(OUT) :: dcount
+    INTEGER(C_INT), INTENT(OUT) :: seeds(3)
+    REAL*8 :: t
+    CALL read_restart(t)
+    t_sim = t
+    dcount = dump_count
+    seeds = (/ s1, s2, s3 /)
+  END SUBROUTINE ref_read_restart
+
+  SUBROUTINE ref_write_snapshot(dnum) BIND(C, name='ref_write_snapshot')
+    USE io_mod, ONLY: writetofile_unformatted
+    INTEGER(C_INT), VALUE :: dnum
+    CALL writetofile_unformatted(dnum)
+  END SUBROUTINE ref_write_snapshot
+
+  ! Parallel-I/O buffer fill (io_mod.f90:368-436), read back element by element.  Needs a
+  ! build with parallel_io = .TRUE. (the buffer is allocated by create_grid_arrays).
+  SUBROUTINE ref_parallel_buffer(n, out) BIND(C, name='ref_parallel_buffer')
+    USE io_mod, ONLY: load_parallel_file_buffer
+    INTEGER(C_INT), VALUE :: n
+    REAL(C_DOUBLE), INTENT(OUT) :: out(n)
+    CALL load_parallel_file_buffer(0.d0)
+    IF (double_prec) THEN
+      out = file_buffer_dble(1:n)
+    ELSE
+      out = DBLE(file_buffer_sngl(1:n))
+    END IF
+  END SUBROUTINE ref_parallel_buffer
+
   ! taus88 stream of the driver (turb_driver_mod.f90:95-126)
   SUBROUTINE ref_rand_taus(seeds, n, out) BIND(C, name='ref_rand_taus')
     USE turb_driver_mod, ONLY: set_taus_seed, rand_taus

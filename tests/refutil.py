@@ -266,6 +266,47 @@ class Reference:
     def step_drive(self, dt):
         self.L.ref_step_drive(ctypes.c_double(dt))
 
+    # io_mod.f90 through the harness; files go to Output/ under the working directory
+    def write_restart(self, t_sim, dump_count, seeds):
+        s = (ctypes.c_int * 3)(*seeds)
+        self.L.ref_write_restart(ctypes.c_double(t_sim), ctypes.c_int(dump_count), s)
+
+    def read_restart(self):
+        t = ctypes.c_double()
+        d = ctypes.c_int()
+        s = (ctypes.c_int * 3)()
+        self.L.ref_read_restart(ctypes.byref(t), ctypes.byref(d), s)
+        return t.value, d.value, tuple(s)
+
+    def write_snapshot(self, dump):
+        self.L.ref_write_snapshot(ctypes.c_int(dump))
+
+    def parallel_buffer(self):
+        n = self.nx * self.ny * self.nz * (7 + self.npass)
+        out = np.zeros(n)
+        self.L.ref_parallel_buffer(ctypes.c_int(n), P(out))
+        return out
+
+
+def io_state(nx, ny, nz, nb=6, npass=1, seed=0):
+    """Arbitrary values in every cell, ghosts included, for the file-format tests: signed
+    values over many decades plus a few exact specials (zeros of both signs, values that
+    SNGL rounds up, overflows to infinity or flushes below the float range)."""
+    rng = np.random.default_rng(seed)
+    sh, shp = padded_shapes(nx, ny, nz, nb, npass)
+
+    def arr(shape):
+        a = rng.uniform(-1.0, 1.0, shape) * 10.0 ** rng.integers(-12, 12, shape)
+        flat = a.reshape(-1, order="F")
+        specials = [0.0, -0.0, 1.0 + 2.0 ** -24 + 2.0 ** -40, 3.5e38, -1e-46, 1.0e300, 2.0 ** -149]
+        flat[:len(specials)] = specials
+        return np.asfortranarray(a)
+
+    q = arr(sh + (7,))
+    bf = arr(sh + (3,))
+    qp = arr(shp[:3] + (npass,)) if npass > 0 else None
+    return q, bf, qp
+
 
 def random_state(nx, ny, nz, nb=6, npass=1, seed=0, amp_v=0.5, amp_rho=0.2, b0=(0.5, 0.2, 0.1),
                  amp_b=0.2):
