@@ -102,6 +102,17 @@ int imhd_step(imhd_ctx* ctx, double dt, int drive_next);
  * (constant energy input Edot * dt_turb per kick, turb_driver_mod.f90:428-442).
  * Replaces the delv argument of add_random_force (turb_driver_mod.f90:395). */
 int imhd_set_force(imhd_ctx* ctx, int box, const double* delv_3d, double edot, double dt_turb);
+/* == generate_rand_force (turb_driver_mod.f90:197-391) on the device, for box `box`: the
+ * next force field, generated from the driver's taus88 state seeds = (s1, s2, s3)
+ * (grid_data_mod.f90:24; advanced in place by 6 k_max^3 draws) with modes
+ * 0 <= kx,ky,kz < k_max, pure solenoidal projection, and the 1-D direct inverse DFTs
+ * along x, y, z over the padded box.  The modes the reference leaves uninitialised
+ * (mixed-sign wave vectors, turb_driver_mod.f90:199-200) are zero here.  Takes the place
+ * of imhd_set_force for the next drive step; k_max must satisfy
+ * 1 <= k_max < min(nx,ny,nz)/2 (turb_driver_mod.f90:31-34). */
+int imhd_generate_force(imhd_ctx* ctx, int box, int k_max, double edot, double dt_turb, int seeds[3]);
+/* The current force field of box `box` (layout of imhd_set_force). */
+int imhd_get_force(imhd_ctx* ctx, int box, double* delv_3d);
 
 /* Ornstein-Uhlenbeck turbulence forcing of the native loop (imhd_run); no counterpart in
  * the reference, whose driver kicks impulsively (turb_driver_mod.f90).  Modes: integer

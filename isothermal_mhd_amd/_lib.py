@@ -19,7 +19,7 @@ EXPORTS = [
     "imhd_autotune", "imhd_step", "imhd_set_force", "imhd_set_ou_forcing", "imhd_get_ou_modes", "imhd_fill_ghosts", "imhd_max_speed", "imhd_copy_velocity", "imhd_passive_step",
     "imhd_diagnostics", "imhd_floor_count", "imhd_device_bytes", "imhd_run", "imhd_set_time", "imhd_get_time",
     "imhd_set_profiling", "imhd_profile_report", "imhd_sync", "imhd_last_error", "imhd_version",
-    "imhd_write_restart", "imhd_read_restart", "imhd_write_cube",
+    "imhd_write_restart", "imhd_read_restart", "imhd_write_cube", "imhd_generate_force", "imhd_get_force",
 ]
 
 
@@ -87,6 +87,8 @@ def lib():
     L.imhd_write_restart.argtypes = [vp, ip, ctypes.c_char_p, dp, ip, vp]
     L.imhd_read_restart.argtypes = [vp, ip, ctypes.c_char_p, vp, vp, vp]
     L.imhd_write_cube.argtypes = [vp, ip, ctypes.c_char_p, ip, ip]
+    L.imhd_generate_force.argtypes = [vp, ip, ip, dp, dp, vp]
+    L.imhd_get_force.argtypes = [vp, ip, vp]
     L.imhd_last_error.argtypes = [vp]
     L.imhd_last_error.restype = ctypes.c_char_p
     L.imhd_version.restype = ctypes.c_char_p

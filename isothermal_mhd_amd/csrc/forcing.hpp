@@ -70,6 +70,24 @@ void launch_ou_pass(const Launch& L, const OuApplyArgs<real>& a);
 void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, const PlanBox* plan, int nbox, const double* part,
                         double* mean);
 
+// ---- the reference's random force (generate_rand_force) ----
+// Inverse DFT passes along y and z of the direct transform (turb_driver_mod.f90:461-486):
+// out(i) = sum_{j=-(K-1)}^{K-1} in(j) e^{+i theta0 i j}, summed in that order, twiddles
+// (cos, sin) precomputed per (i, j).  Slots s = k + K, k = -K .. K-1.
+struct RandForceArgs {
+    const double* xr;   // x-pass output [d][kz slot][ky slot][ix]
+    const double* xi;
+    const double* twy;  // [iy][j + K - 1] (cos, sin)
+    const double* twz;  // [iz][j + K - 1]
+    double* yr;         // y-pass output [d][kz slot][iy][ix]
+    double* yi;
+    double* delv[3];    // the box's padded force field
+    int K, P[3];
+    double idft_pre;    // 1 / (nx ny nz)
+};
+void launch_rand_force_y(hipStream_t st, const RandForceArgs& a);
+void launch_rand_force_z(hipStream_t st, const RandForceArgs& a);
+
 template <class real>
 void launch_force_sums(const Launch& L, const ForceSumArgs<real>& a);
 void launch_force_finalize(hipStream_t st, const BoxDesc* boxes, const PlanBox* plan, int nbox, const ForceFinArgs& a);

@@ -282,4 +282,62 @@ template void launch_force_sums<float>(const Launch&, const ForceSumArgs<float>&
 template void launch_force_apply<double>(const Launch&, const ForceApplyArgs<double>&);
 template void launch_force_apply<float>(const Launch&, const ForceApplyArgs<float>&);
 
+// ---------------------------------------------------------------------------------
+// generate_rand_force: y and z passes of the direct inverse DFT (turb_driver_mod.f90:
+// 310-385), per output value the sum over j = -(K-1) .. K-1 in the reference's order and
+// arithmetic form (in_re cos + (-in_im) sin; in_im cos - (-in_re) sin).  Compiled without
+// FMA contraction, with the host's twiddles, so the field is the host restatement's bit
+// for bit.
+__global__ void k_rand_force_y(RandForceArgs a) {
+    const int S = 2 * a.K, J = 2 * a.K - 1;
+    const long long n = 3LL * S * a.P[1] * a.P[0];
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int ix = (int)(g % a.P[0]);
+    long long t = g / a.P[0];
+    const int iy = (int)(t % a.P[1]);
+    const int dz = (int)(t / a.P[1]);   // d * S + kz slot
+    const double sg = -1.0;
+    double re = 0.0, im = 0.0;
+    for (int j = -(a.K - 1); j <= a.K - 1; j++) {
+        const long long xo = ((long long)dz * S + (j + a.K)) * a.P[0] + ix;
+        const double xr = a.xr[xo], xi = a.xi[xo];
+        const double c = a.twy[2 * (iy * J + j + a.K - 1)], s = a.twy[2 * (iy * J + j + a.K - 1) + 1];
+        re = re + (xr * c + sg * xi * s);
+        im = im + (xi * c - sg * xr * s);
+    }
+    a.yr[g] = re;
+    a.yi[g] = im;
+}
+
+__global__ void k_rand_force_z(RandForceArgs a) {
+    const int S = 2 * a.K, J = 2 * a.K - 1;
+    const long long plane = (long long)a.P[0] * a.P[1];
+    const long long n = plane * a.P[2];
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const long long xy = g % plane;
+    const int iz = (int)(g / plane);
+    const double sg = -1.0;
+    for (int d = 0; d < 3; d++) {
+        double re = 0.0;
+        for (int j = -(a.K - 1); j <= a.K - 1; j++) {
+            const long long yo = ((long long)d * S + (j + a.K)) * plane + xy;
+            const double c = a.twz[2 * (iz * J + j + a.K - 1)], s = a.twz[2 * (iz * J + j + a.K - 1) + 1];
+            re = re + (a.yr[yo] * c + sg * a.yi[yo] * s);
+        }
+        a.delv[d][g] = re * a.idft_pre;
+    }
+}
+
+void launch_rand_force_y(hipStream_t st, const RandForceArgs& a) {
+    const long long n = 3LL * 2 * a.K * a.P[1] * a.P[0];
+    hipLaunchKernelGGL(k_rand_force_y, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+}
+
+void launch_rand_force_z(hipStream_t st, const RandForceArgs& a) {
+    const long long n = (long long)a.P[0] * a.P[1] * a.P[2];
+    hipLaunchKernelGGL(k_rand_force_z, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+}
+
 }  // namespace imhd

@@ -1352,6 +1352,159 @@ static int write_cube_t(imhd_ctx* c, int bx, FILE* f, int npo) {
     return 0;
 }
 
+// Force-field buffers of the turbulence driver, allocated on first use.
+static int ensure_force(imhd_ctx* c) {
+    if (c->delv[0]) return 0;
+    for (int d = 0; d < 3; d++) HIPCK(c, hipMalloc(&c->delv[d], (size_t)c->pool * 8));
+    for (int d = 0; d < 3; d++) HIPCK(c, hipMemset(c->delv[d], 0, (size_t)c->pool * 8));
+    HIPCK(c, hipMalloc(&c->force_part, (size_t)(c->pl_int.total / 256 + 1) * FORCE_NSUM * 8));
+    HIPCK(c, hipMalloc(&c->force_par, (size_t)c->nbox * FORCE_NPAR * 8));
+    HIPCK(c, hipMalloc(&c->force_err, 4));
+    HIPCK(c, hipMemset(c->force_err, 0, 4));
+    return 0;
+}
+
+// rand_taus (turb_driver_mod.f90:104-126): taus88 on the driver's signed 32-bit state.
+static double taus88_next(int32_t* st) {
+    uint32_t s1 = (uint32_t)st[0], s2 = (uint32_t)st[1], s3 = (uint32_t)st[2], b;
+    b = ((s1 << 13) ^ s1) >> 19;
+    s1 = ((s1 & 0xFFFFFFFEu) << 12) ^ b;
+    b = ((s2 << 2) ^ s2) >> 25;
+    s2 = ((s2 & 0xFFFFFFF8u) << 4) ^ b;
+    b = ((s3 << 3) ^ s3) >> 11;
+    s3 = ((s3 & 0xFFFFFFF0u) << 17) ^ b;
+    st[0] = (int32_t)s1;
+    st[1] = (int32_t)s2;
+    st[2] = (int32_t)s3;
+    return 0.5 + (double)(int32_t)(s1 ^ s2 ^ s3) * 2.3283064365e-10;
+}
+
+struct DevBuf {
+    double* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// generate_rand_force (turb_driver_mod.f90:197-391) for one box, with the modes the
+// reference never assigns (mixed-sign wave vectors) zero.  Host: the k_max^3 octant modes
+// (six taus88 draws each: Box-Muller amplitude and phase per component, P(k) = k^6
+// exp(-8k/k_pk), projection normal to khat, which is zero on the axis planes), their
+// Hermitian mirrors, the per-axis twiddles and the small x pass.  Device: the y and z
+// passes over the padded box (k_rand_force_y/z), into the force field.  Every COS/SIN
+// pair of one argument is one sincos call, as the reference's compilers emit it (glibc's
+// sincos can differ from separate sin and cos in the last bit).
+static int generate_force(imhd_ctx* c, int bx, int K, int* seeds) {
+    const BoxDesc& B = c->hb[bx];
+    const double twopi = 8.0 * atan(1.0);   // TWOPI (constants_mod.f90)
+    const int W = 2 * K + 1, W3 = W * W * W, S = 2 * K, J = 2 * K - 1;
+    const int* n = B.n;
+    const int* P = B.P;
+    const double L[3] = {(double)n[0] * B.dx, (double)n[1] * B.dx, (double)n[2] * B.dx};
+    const double k0[3] = {twopi / L[0], twopi / L[1], twopi / L[2]};
+    const double k_pk = twopi / std::max(L[0], std::max(L[1], L[2]));
+    const double const_sol = sqrt(1.0), const_comp = sqrt(2.0 * (1.0 - 1.0));   // pure solenoidal driving
+    std::vector<double> re((size_t)3 * W3, 0.0), im((size_t)3 * W3, 0.0);
+    auto mi = [&](int x, int y, int z) { return ((z + K) * W + (y + K)) * W + (x + K); };
+    int32_t st[3] = {seeds[0], seeds[1], seeds[2]};
+    for (int kz = 0; kz < K; kz++)
+        for (int ky = 0; ky < K; ky++)
+            for (int kx = 0; kx < K; kx++) {
+                const int kv[3] = {kx, ky, kz};
+                const double a0 = (double)kx * k0[0], a1 = (double)ky * k0[1], a2 = (double)kz * k0[2];
+                const double k = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+                const double k2 = k * k;
+                const double Pk = (k2 * (k2 * k2)) * exp(-8.0 * k / k_pk);
+                double rn[6], amp[3], ph[3], dre[3], dim[3], kh[3] = {0.0, 0.0, 0.0};
+                for (int r = 0; r < 6; r++) rn[r] = taus88_next(st);
+                for (int d = 0; d < 3; d++) {
+                    amp[d] = sqrt(-2.0 * Pk * log(rn[d]));
+                    ph[d] = twopi * rn[3 + d];
+                }
+                for (int d = 0; d < 3; d++) {
+                    double cs, sn;
+                    sincos(ph[d], &sn, &cs);
+                    dre[d] = amp[d] * cs;
+                    dim[d] = amp[d] * sn;
+                }
+                if (kx >= 1 && ky >= 1 && kz >= 1)
+                    for (int d = 0; d < 3; d++) kh[d] = k0[d] * (double)kv[d] / k;
+                const double dot_re = kh[0] * dre[0] + kh[1] * dre[1] + kh[2] * dre[2];
+                const double dot_im = kh[0] * dim[0] + kh[1] * dim[1] + kh[2] * dim[2];
+                for (int d = 0; d < 3; d++) {
+                    const double cre = kh[d] * dot_re, cim = kh[d] * dot_im;
+                    const double sre = dre[d] - cre, sim = dim[d] - cim;
+                    const int p = d * W3 + mi(kx, ky, kz), q = d * W3 + mi(-kx, -ky, -kz);
+                    re[p] = const_sol * sre + const_comp * cre;
+                    im[p] = const_sol * sim + const_comp * cim;
+                    re[q] = re[p];
+                    im[q] = -im[p];
+                }
+            }
+    // twiddles per axis: theta = theta0 * i * j, i the global cell index 1-nb .. n+nb
+    std::vector<double> tw[3];
+    for (int a = 0; a < 3; a++) {
+        tw[a].resize((size_t)2 * P[a] * J);
+        const double theta0 = twopi / (double)n[a];
+        for (int ii = 0; ii < P[a]; ii++)
+            for (int j = -(K - 1); j <= K - 1; j++) {
+                const double th = theta0 * (double)(ii + 1 - B.nb) * (double)j;
+                sincos(th, &tw[a][2 * (ii * J + j + K - 1) + 1], &tw[a][2 * (ii * J + j + K - 1)]);
+            }
+    }
+    // x pass on the host: X[d][kz slot][ky slot][ix]
+    const size_t nX = (size_t)3 * S * S * P[0];
+    std::vector<double> xr(nX), xi(nX);
+    const double sg = -1.0;
+    for (int d = 0; d < 3; d++)
+        for (int kz = -K; kz < K; kz++)
+            for (int ky = -K; ky < K; ky++)
+                for (int ix = 0; ix < P[0]; ix++) {
+                    double sr = 0.0, si = 0.0;
+                    for (int j = -(K - 1); j <= K - 1; j++) {
+                        const double ar = re[d * W3 + mi(j, ky, kz)], ai = im[d * W3 + mi(j, ky, kz)];
+                        const double cs = tw[0][2 * (ix * J + j + K - 1)], sn = tw[0][2 * (ix * J + j + K - 1) + 1];
+                        sr = sr + (ar * cs + sg * ai * sn);
+                        si = si + (ai * cs - sg * ar * sn);
+                    }
+                    const size_t o = ((size_t)(d * S + kz + K) * S + ky + K) * P[0] + ix;
+                    xr[o] = sr;
+                    xi[o] = si;
+                }
+    int rc;
+    if ((rc = ensure_force(c))) return rc;
+    const size_t nY = (size_t)3 * S * P[1] * P[0];
+    DevBuf dxr, dxi, dyr, dyi, dty, dtz;
+    HIPCK(c, hipMalloc(&dxr.p, nX * 8));
+    HIPCK(c, hipMalloc(&dxi.p, nX * 8));
+    HIPCK(c, hipMalloc(&dyr.p, nY * 8));
+    HIPCK(c, hipMalloc(&dyi.p, nY * 8));
+    HIPCK(c, hipMalloc(&dty.p, tw[1].size() * 8));
+    HIPCK(c, hipMalloc(&dtz.p, tw[2].size() * 8));
+    HIPCK(c, hipStreamSynchronize(c->st));
+    HIPCK(c, hipMemcpy(dxr.p, xr.data(), nX * 8, hipMemcpyHostToDevice));
+    HIPCK(c, hipMemcpy(dxi.p, xi.data(), nX * 8, hipMemcpyHostToDevice));
+    HIPCK(c, hipMemcpy(dty.p, tw[1].data(), tw[1].size() * 8, hipMemcpyHostToDevice));
+    HIPCK(c, hipMemcpy(dtz.p, tw[2].data(), tw[2].size() * 8, hipMemcpyHostToDevice));
+    RandForceArgs a{};
+    a.xr = dxr.p;
+    a.xi = dxi.p;
+    a.twy = dty.p;
+    a.twz = dtz.p;
+    a.yr = dyr.p;
+    a.yi = dyi.p;
+    for (int d = 0; d < 3; d++) a.delv[d] = c->delv[d] + B.off;
+    a.K = K;
+    for (int d = 0; d < 3; d++) a.P[d] = P[d];
+    a.idft_pre = 1.0 / (double)((long long)n[0] * n[1] * n[2]);
+    launch_rand_force_y(c->st, a);
+    launch_rand_force_z(c->st, a);
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipStreamSynchronize(c->st));
+    for (int i = 0; i < 3; i++) seeds[i] = st[i];
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1821,14 +1974,8 @@ int imhd_get_ou_modes(imhd_ctx* c, int box, int* nmode, int* kvec, double* amp) 
 
 int imhd_set_force(imhd_ctx* c, int box, const double* delv, double edot, double dt_turb) {
     if (!c || !delv || box < 0 || box >= c->nbox) return IMHD_EINVAL;
-    if (!c->delv[0]) {
-        for (int d = 0; d < 3; d++) HIPCK(c, hipMalloc(&c->delv[d], (size_t)c->pool * 8));
-        for (int d = 0; d < 3; d++) HIPCK(c, hipMemset(c->delv[d], 0, (size_t)c->pool * 8));
-        HIPCK(c, hipMalloc(&c->force_part, (size_t)(c->pl_int.total / 256 + 1) * FORCE_NSUM * 8));
-        HIPCK(c, hipMalloc(&c->force_par, (size_t)c->nbox * FORCE_NPAR * 8));
-        HIPCK(c, hipMalloc(&c->force_err, 4));
-        HIPCK(c, hipMemset(c->force_err, 0, 4));
-    }
+    int rc;
+    if ((rc = ensure_force(c))) return rc;
     const BoxDesc& B = c->hb[box];
     const long long nc = (long long)B.P[0] * B.P[1] * B.P[2];
     HIPCK(c, hipStreamSynchronize(c->st));
@@ -1837,6 +1984,31 @@ int imhd_set_force(imhd_ctx* c, int box, const double* delv, double edot, double
     c->edot = edot;
     c->dt_turb = dt_turb;
     c->force_set = true;
+    return IMHD_OK;
+}
+
+int imhd_generate_force(imhd_ctx* c, int box, int k_max, double edot, double dt_turb, int seeds[3]) {
+    if (!c || !seeds || box < 0 || box >= c->nbox) return IMHD_EINVAL;
+    const BoxDesc& B = c->hb[box];
+    // init_turb_driver's check (turb_driver_mod.f90:31-34)
+    if (k_max < 1 || k_max >= std::min(B.n[0], std::min(B.n[1], B.n[2])) / 2)
+        FAIL(c, IMHD_EINVAL, "invalid k_max %d (needs 1 <= k_max < min(nx,ny,nz)/2)", k_max);
+    int rc;
+    if ((rc = generate_force(c, box, k_max, seeds))) return rc;
+    c->edot = edot;
+    c->dt_turb = dt_turb;
+    c->force_set = true;
+    return IMHD_OK;
+}
+
+int imhd_get_force(imhd_ctx* c, int box, double* delv) {
+    if (!c || !delv || box < 0 || box >= c->nbox) return IMHD_EINVAL;
+    if (!c->force_set) FAIL(c, IMHD_ESTATE, "no force field (imhd_set_force / imhd_generate_force)");
+    const BoxDesc& B = c->hb[box];
+    const long long nc = (long long)B.P[0] * B.P[1] * B.P[2];
+    HIPCK(c, hipStreamSynchronize(c->st));
+    for (int d = 0; d < 3; d++)
+        HIPCK(c, hipMemcpy(delv + d * nc, c->delv[d] + B.off, (size_t)nc * 8, hipMemcpyDeviceToHost));
     return IMHD_OK;
 }
 

@@ -68,6 +68,22 @@ MODULE imhd_api
       REAL(C_DOUBLE), VALUE :: edot, dt_turb
     END FUNCTION imhd_set_force
 
+    INTEGER(C_INT) FUNCTION imhd_generate_force(ctx, box, k_max, edot, dt_turb, seeds) &
+        BIND(C, name='imhd_generate_force')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: box, k_max
+      REAL(C_DOUBLE), VALUE :: edot, dt_turb
+      INTEGER(C_INT), INTENT(INOUT) :: seeds(3)
+    END FUNCTION imhd_generate_force
+
+    INTEGER(C_INT) FUNCTION imhd_get_force(ctx, box, delv) BIND(C, name='imhd_get_force')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: box
+      REAL(C_DOUBLE), INTENT(OUT) :: delv(*)
+    END FUNCTION imhd_get_force
+
     INTEGER(C_INT) FUNCTION imhd_fill_ghosts(ctx) BIND(C, name='imhd_fill_ghosts')
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ctx

@@ -148,6 +148,21 @@ class VanLeerSolver:
             raise ValueError(f"expected delv {sh + (3,)}")
         self._check(self.lib.imhd_set_force(self._c, box, _p(d), float(edot), float(dt_turb)))
 
+    def generate_rand_force(self, k_max=2, edot=1.0e-3, dt_turb=0.05, seeds=(1234, 5678, 9123), box=0):
+        """generate_rand_force (turb_driver_mod.f90:197-391) on the device, with the modes
+        the reference leaves uninitialised set to zero; the field is applied by the next
+        vanleer_solve_3d(dt, drive_next=True).  seeds is the driver's taus88 state
+        (s1, s2, s3); returns the advanced state."""
+        s = (ctypes.c_int * 3)(*[int(v) for v in seeds])
+        self._check(self.lib.imhd_generate_force(self._c, box, int(k_max), float(edot), float(dt_turb), s))
+        return tuple(s)
+
+    def get_force(self, box=0):
+        sh, _ = self.shapes(box)
+        delv = np.zeros(sh + (3,), order="F")
+        self._check(self.lib.imhd_get_force(self._c, box, _p(delv)))
+        return delv
+
     def mhd_update(self, dt):
         """driver.f90:378-389: the configured integrator's solve on every box."""
         self.vanleer_solve_3d(dt)

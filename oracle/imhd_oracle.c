@@ -12,6 +12,7 @@
  * Parity status per function (how each one is pinned) is listed in the header of
  * tests/test_oracle_vs_reference.py and tests/test_oracle_properties.py.
  */
+#define _GNU_SOURCE   /* sincos */
 #include "imhd_oracle.h"
 
 #include <math.h>
@@ -774,6 +775,8 @@ static void vl_step(orc_ctx *c, double dt) {
 #define orc_ou struct orc_ou_s
 #include "imhd_oracle_ou.inc"
 #undef orc_ou
+
+#include "imhd_oracle_force.inc"
 
 /* ----------------------------------------------------------------------------------- */
 /* forcing application of the reference's turbulence driver                            */

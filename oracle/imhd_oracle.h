@@ -58,6 +58,13 @@ void orc_ou_modes_step(orc_ctx *c, double dt);
 void orc_ou_get(const orc_ctx *c, int *kvec, double *amp);
 void orc_ou_field(const orc_ctx *c, double *out);
 void orc_philox4x32(const unsigned int *ctr, const unsigned int *key, unsigned int *out);
+/* The reference's random force (generate_rand_force, turb_driver_mod.f90:197-391) with its
+ * unassigned modes zeroed: taus88 state seeds[3] in/out, K = k_max; delv: 3 padded fields;
+ * modes (may be NULL): the Fourier coefficients, re then im, [3][2K+1]^3. */
+void orc_taus88(int *seeds, int n, double *out);
+void orc_idft_1d(const double *in_re, const double *in_im, double *out_re, double *out_im, int ilow, int ihi,
+                 int nnx, int K);
+void orc_rand_force(int nx, int ny, int nz, int nb, int K, int *seeds, double *delv, double *modes);
 int orc_step_drive(orc_ctx *c, double dt);
 double orc_max_speed(orc_ctx *c);
 double orc_smax(const orc_ctx *c);

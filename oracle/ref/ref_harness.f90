@@ -168,6 +168,20 @@ CONTAINS
     CALL vanleer_solve_3d(dtl, .TRUE.)
   END SUBROUTINE ref_step_drive
 
+  ! The reference's 1-D inverse DFT by direct summation (turb_driver_mod.f90:461-486) on
+  ! in(-k_max : k_max-1); out holds ihi - ilow + 1 values.  Builds with drive_turbulence
+  ! only (nnmax is set by init_turb_driver).
+  SUBROUTINE ref_idft_1d(in_re, in_im, out_re, out_im, ilow, ihi, nnx) BIND(C, name='ref_idft_1d')
+    USE turb_driver_mod, ONLY: direct_idft_1d, nnmax
+    REAL(C_DOUBLE), INTENT(IN) :: in_re(2*k_max), in_im(2*k_max)
+    REAL(C_DOUBLE), INTENT(OUT) :: out_re(*), out_im(*)
+    INTEGER(C_INT), VALUE :: ilow, ihi, nnx
+    REAL*8 :: ore(0:nnmax-1+2*nb), oim(0:nnmax-1+2*nb)
+    CALL direct_idft_1d(in_re, in_im, ore, oim, ilow, ihi, nnx)
+    out_re(1:ihi-ilow+1) = ore(0:ihi-ilow)
+    out_im(1:ihi-ilow+1) = oim(0:ihi-ilow)
+  END SUBROUTINE ref_idft_1d
+
   ! Restart and snapshot files through the reference's own io_mod (io_mod.f90:249-277,
   ! 440-517): write_restart / read_restart of the module arrays, and
   ! writetofile_unformatted, whose only enabled output in these builds is the xyz cube.

@@ -77,6 +77,9 @@ def oracle_lib():
         L.orc_set_force.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
         L.orc_step_drive.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_step_drive.restype = ctypes.c_int
+        L.orc_taus88.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.orc_idft_1d.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int] * 4
+        L.orc_rand_force.argtypes = [ctypes.c_int] * 5 + [ctypes.c_void_p] * 3
         L.orc_tvd_flux_plane.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p]
@@ -278,6 +281,19 @@ class Reference:
         self.L.ref_read_restart(ctypes.byref(t), ctypes.byref(d), s)
         return t.value, d.value, tuple(s)
 
+    def rand_taus(self, seeds, n):
+        out = np.zeros(n)
+        s = (ctypes.c_int * 3)(*seeds)
+        self.L.ref_rand_taus(s, ctypes.c_int(n), P(out))
+        return out
+
+    def idft_1d(self, in_re, in_im, ilow, ihi, nnx):
+        m = ihi - ilow + 1
+        ore, oim = np.zeros(m), np.zeros(m)
+        a, b = np.ascontiguousarray(in_re, dtype=np.float64), np.ascontiguousarray(in_im, dtype=np.float64)
+        self.L.ref_idft_1d(P(a), P(b), P(ore), P(oim), ctypes.c_int(ilow), ctypes.c_int(ihi), ctypes.c_int(nnx))
+        return ore, oim
+
     def write_snapshot(self, dump):
         self.L.ref_write_snapshot(ctypes.c_int(dump))
 
@@ -286,6 +302,28 @@ class Reference:
         out = np.zeros(n)
         self.L.ref_parallel_buffer(ctypes.c_int(n), P(out))
         return out
+
+
+def oracle_rand_force(nx, ny, nz, K=2, seeds=(1234, 5678, 9123), nb=6):
+    """The oracle's generate_rand_force (modes zeroed first): (delv (Px,Py,Pz,3) Fortran
+    order, modes re/im (3, 2K+1, 2K+1, 2K+1) indexed [d, kz+K, ky+K, kx+K], new seeds)."""
+    L = oracle_lib()
+    sh = (nx + 2 * nb, ny + 2 * nb, nz + 2 * nb)
+    delv = np.zeros(sh + (3,), order="F")
+    W = 2 * K + 1
+    modes = np.zeros(2 * 3 * W ** 3)
+    s = (ctypes.c_int * 3)(*seeds)
+    L.orc_rand_force(nx, ny, nz, nb, K, s, P(delv), P(modes))
+    m = modes.reshape(2, 3, W, W, W)
+    return delv, m[0] + 1j * m[1], tuple(s)
+
+
+def oracle_taus88(seeds, n):
+    L = oracle_lib()
+    out = np.zeros(n)
+    s = (ctypes.c_int * 3)(*seeds)
+    L.orc_taus88(s, n, P(out))
+    return out, tuple(s)
 
 
 def io_state(nx, ny, nz, nb=6, npass=1, seed=0):
