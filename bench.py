@@ -349,8 +349,9 @@ def run_ensemble(steps, cour, warmup=2):
 def run_io(n, reps=2):
     """Restart and snapshot files of one n^3 FP32 box with one passive scalar (the 256^3
     turbulence IC): imhd_write_cube (REAL*4 and REAL*8 xyz cube, packed on the device by
-    k_pack_cube), imhd_write_restart and imhd_read_restart, to files under $TMPDIR.  Rates
-    include PCIe, the host page cache and the file system."""
+    k_pack_cube), imhd_write_restart and imhd_read_restart, to files under $TMPDIR (rates
+    include PCIe, the host page cache and the file system); then the device generation of
+    the reference's random force (imhd_generate_force, k_max = 2)."""
     import tempfile
     from isothermal_mhd_amd import VanLeerSolver, ics
     q, bf, qp = ics.config_c2(n=n)
@@ -376,6 +377,18 @@ def run_io(n, reps=2):
             size = os.path.getsize(p)
             out[name] = {"bytes": size, "s": best, "GB_per_s": size / best / 1e9}
             print(f"[io] {name}: {size / 1e6:.0f} MB in {best:.3f} s", flush=True)
+    # generate_rand_force on the device (k_max = 2): host modes + x pass, device y/z passes
+    seeds = (1234, 5678, 9123)
+    best = None
+    for _ in range(reps):
+        S.sync()
+        t0 = time.perf_counter()
+        seeds = S.generate_rand_force(k_max=2, seeds=seeds)
+        S.sync()
+        sec = time.perf_counter() - t0
+        best = sec if best is None else min(best, sec)
+    out["generate_rand_force"] = {"s": best, "k_max": 2}
+    print(f"[io] generate_rand_force: {best * 1e3:.1f} ms", flush=True)
     S.close()
     return out
 

@@ -26,7 +26,7 @@ MODULE imhd_shim_core
   IMPLICIT NONE
   PRIVATE
   PUBLIC :: shim_create, shim_destroy, shim_upload, shim_download, shim_check, shim_ctx, &
-            imhd_shim_host_modified, imhd_shim_recon, imhd_shim_precision
+            imhd_shim_host_modified, imhd_shim_recon, imhd_shim_precision, shim_force_on_device
 
   TYPE(C_PTR), SAVE :: shim_ctx = C_NULL_PTR
   LOGICAL, SAVE :: host_dirty = .TRUE.
@@ -36,6 +36,9 @@ MODULE imhd_shim_core
   ! 64: FP64 throughout (bitwise with the reference); 32: FP32 bulk with FP64 face B,
   ! corner EMF, CFL and diagnostics, Kahan-compensated rho and rho v
   INTEGER(C_INT), SAVE :: imhd_shim_precision = 64
+  ! set by the turb_driver_mod shim (imhd_shim_turb.f90) once the force field is generated
+  ! on the device; the host delv_re is then not uploaded
+  LOGICAL, SAVE :: shim_force_on_device = .FALSE.
   REAL(C_DOUBLE), ALLOCATABLE, SAVE :: nopass(:)
 
 CONTAINS
@@ -131,8 +134,9 @@ CONTAINS
     CALL shim_destroy()
   END SUBROUTINE destroy_solver_vanleer
 
-  ! vanleer_mod.f90:279-426.  With drive_turbulence and drive_next, the force field the
-  ! driver's generate_rand_force left in delv_re is applied on the GPU as the reference
+  ! vanleer_mod.f90:279-426.  With drive_turbulence and drive_next, the force field (the
+  ! device-generated one when the turb_driver_mod shim is linked, else the one the
+  ! driver's generate_rand_force left in delv_re) is applied on the GPU as the reference
   ! applies it (get_rho_turb before the step, add_random_force after it).
   SUBROUTINE vanleer_solve_3d(dt_in, drive_next)
     REAL*8, INTENT(INOUT) :: dt_in
@@ -141,7 +145,8 @@ CONTAINS
     CALL shim_upload()
     drive = 0
     IF (drive_turbulence .AND. drive_next) THEN
-      CALL shim_check(imhd_set_force(shim_ctx, 0_C_INT, delv_re, Edot, dt_turb), 'imhd_set_force')
+      IF (.NOT. shim_force_on_device) &
+        CALL shim_check(imhd_set_force(shim_ctx, 0_C_INT, delv_re, Edot, dt_turb), 'imhd_set_force')
       drive = 1
     END IF
     CALL shim_check(imhd_step(shim_ctx, dt_in, drive), 'vanleer_solve_3d')
