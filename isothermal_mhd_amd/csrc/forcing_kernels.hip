@@ -288,15 +288,15 @@ template void launch_force_apply<float>(const Launch&, const ForceApplyArgs<floa
 // arithmetic form (in_re cos + (-in_im) sin; in_im cos - (-in_re) sin).  Compiled without
 // FMA contraction, with the host's twiddles, so the field is the host restatement's bit
 // for bit.
-__global__ void k_rand_force_y(RandForceArgs a) {
+// grid: x over the plane (ix fastest), y over the remaining index (d * S + kz slot for
+// the y pass, iz for the z pass): 32-bit index arithmetic, no 64-bit division
+__global__ __launch_bounds__(256) void k_rand_force_y(RandForceArgs a) {
     const int S = 2 * a.K, J = 2 * a.K - 1;
-    const long long n = 3LL * S * a.P[1] * a.P[0];
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    const int ix = (int)(g % a.P[0]);
-    long long t = g / a.P[0];
-    const int iy = (int)(t % a.P[1]);
-    const int dz = (int)(t / a.P[1]);   // d * S + kz slot
+    const int plane = a.P[0] * a.P[1];
+    const int xy = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (xy >= plane) return;
+    const int ix = xy % a.P[0], iy = xy / a.P[0];
+    const int dz = (int)blockIdx.y;   // d * S + kz slot
     const double sg = -1.0;
     double re = 0.0, im = 0.0;
     for (int j = -(a.K - 1); j <= a.K - 1; j++) {
@@ -306,38 +306,78 @@ __global__ void k_rand_force_y(RandForceArgs a) {
         re = re + (xr * c + sg * xi * s);
         im = im + (xi * c - sg * xr * s);
     }
-    a.yr[g] = re;
-    a.yi[g] = im;
+    const long long o = (long long)dz * plane + xy;
+    a.yr[o] = re;
+    a.yi[o] = im;
 }
 
-__global__ void k_rand_force_z(RandForceArgs a) {
+// z pass, z-marching: each thread keeps its column's y-pass values (3 components x
+// (2K-1) wave numbers, complex) in registers and walks RF_ZSEG planes, so the y-pass
+// buffer is read once per segment instead of once per plane.  KC > 0: K <= KC, values
+// cached; KC == 0: any K, values re-read per plane.  The per-value arithmetic and the
+// order of the j sum are those of the plain pass.
+constexpr int RF_ZSEG = 32;
+
+template <int KC>
+__global__ __launch_bounds__(256) void k_rand_force_z(RandForceArgs a) {
     const int S = 2 * a.K, J = 2 * a.K - 1;
-    const long long plane = (long long)a.P[0] * a.P[1];
-    const long long n = plane * a.P[2];
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    const long long xy = g % plane;
-    const int iz = (int)(g / plane);
+    const int plane = a.P[0] * a.P[1];
+    const int xy = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (xy >= plane) return;
+    const int z0 = (int)blockIdx.y * RF_ZSEG;
+    const int z1 = min(z0 + RF_ZSEG, a.P[2]);
     const double sg = -1.0;
-    for (int d = 0; d < 3; d++) {
-        double re = 0.0;
-        for (int j = -(a.K - 1); j <= a.K - 1; j++) {
-            const long long yo = ((long long)d * S + (j + a.K)) * plane + xy;
-            const double c = a.twz[2 * (iz * J + j + a.K - 1)], s = a.twz[2 * (iz * J + j + a.K - 1) + 1];
-            re = re + (a.yr[yo] * c + sg * a.yi[yo] * s);
+    constexpr int JC = KC > 0 ? 2 * KC - 1 : 1;
+    double cr[3][JC], ci[3][JC];
+    if (KC > 0) {
+        for (int d = 0; d < 3; d++)
+            for (int t = 0; t < JC; t++) {
+                cr[d][t] = 0.0;
+                ci[d][t] = 0.0;
+                if (t < J) {
+                    const long long yo = ((long long)d * S + (t + 1)) * plane + xy;
+                    cr[d][t] = a.yr[yo];
+                    ci[d][t] = a.yi[yo];
+                }
+            }
+    }
+    for (int iz = z0; iz < z1; iz++) {
+        const long long g = (long long)iz * plane + xy;
+        for (int d = 0; d < 3; d++) {
+            double re = 0.0;
+            if (KC > 0) {
+#pragma unroll
+                for (int t = 0; t < JC; t++) {
+                    if (t < J) {
+                        const double c = a.twz[2 * (iz * J + t)], s = a.twz[2 * (iz * J + t) + 1];
+                        re = re + (cr[d][t] * c + sg * ci[d][t] * s);
+                    }
+                }
+            } else {
+                for (int t = 0; t < J; t++) {
+                    const long long yo = ((long long)d * S + (t + 1)) * plane + xy;
+                    const double c = a.twz[2 * (iz * J + t)], s = a.twz[2 * (iz * J + t) + 1];
+                    re = re + (a.yr[yo] * c + sg * a.yi[yo] * s);
+                }
+            }
+            a.delv[d][g] = re * a.idft_pre;
         }
-        a.delv[d][g] = re * a.idft_pre;
     }
 }
 
 void launch_rand_force_y(hipStream_t st, const RandForceArgs& a) {
-    const long long n = 3LL * 2 * a.K * a.P[1] * a.P[0];
-    hipLaunchKernelGGL(k_rand_force_y, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+    const int plane = a.P[0] * a.P[1];
+    hipLaunchKernelGGL(k_rand_force_y, dim3((unsigned)((plane + 255) / 256), (unsigned)(3 * 2 * a.K)), dim3(256), 0,
+                       st, a);
 }
 
 void launch_rand_force_z(hipStream_t st, const RandForceArgs& a) {
-    const long long n = (long long)a.P[0] * a.P[1] * a.P[2];
-    hipLaunchKernelGGL(k_rand_force_z, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+    const int plane = a.P[0] * a.P[1];
+    const dim3 grid((unsigned)((plane + 255) / 256), (unsigned)((a.P[2] + RF_ZSEG - 1) / RF_ZSEG));
+    if (a.K <= 4)
+        hipLaunchKernelGGL(k_rand_force_z<4>, grid, dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_rand_force_z<0>, grid, dim3(256), 0, st, a);
 }
 
 }  // namespace imhd

@@ -22,7 +22,8 @@ def say(*a):
 
 
 CASES = [((16, 16, 16), 2, (1234, 5678, 9123)), ((12, 10, 8), 3, (7, 8, 9)),
-         ((20, 12, 16), 4, (-1, 2147483647, 99)), ((32, 24, 40), 2, (5, 5, 5))]
+         ((20, 12, 16), 4, (-1, 2147483647, 99)), ((32, 24, 40), 2, (5, 5, 5)),
+         ((24, 12, 14), 5, (3, 1, 4))]
 
 
 @pytest.mark.parametrize("case", CASES, ids=["x".join(map(str, c[0])) + f"_k{c[1]}" for c in CASES])
