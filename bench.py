@@ -393,6 +393,34 @@ def run_io(n, reps=2):
     return out
 
 
+def run_tvd_big(n, steps, cour, warmup=2):
+    """Forced-turbulence TVD workload at n^3: K98 Roe-TVD + CTU + CT with OU forcing
+    (solenoidal, 1 <= |k| <= 2, t_corr 0.5, sigma 5, seed 3003) and two passive scalars, IC
+    ics.config_c3; FP32 and FP64, whole step in a hipGraph."""
+    from isothermal_mhd_amd import TvdSolver, ics
+    q, bf, qp = ics.config_c3(n=n)
+    out = {"n": n, "steps": steps}
+    for prec in (32, 64):
+        S = TvdSolver((n, n, n), npass=2, riemann="roe", bc="periodic", precision=prec, cour=cour)
+        S.set_state(q, bf, qp)
+        S.set_ou_forcing(kmin=1.0, kmax=2.0, t_corr=0.5, sigma=5.0, seed=3003, solenoidal=True)
+        S.boundary_conditions()
+        S.run(warmup, graph=True)
+        S.sync()
+        t0 = time.perf_counter()
+        S.run(steps, graph=True)
+        S.sync()
+        sec = time.perf_counter() - t0
+        out[f"fp{prec}"] = {"value": n ** 3 * steps / sec / 1e6, "unit": "Mcell-updates/s",
+                            "ms_per_step": sec / steps * 1e3, "device_GB": S.device_bytes() / 1e9}
+        print(f"[tvd] {n}^3 fp{prec}: {out[f'fp{prec}']['value']:.0f} Mcell/s", flush=True)
+        S.close()
+    out["workload"] = (f"{n}^3 periodic, rho=1, B0=(0.2,0,0), v=0, OU solenoidal forcing 1<=|k|<=2 "
+                       "(t_corr 0.5, sigma 5, seed 3003), 2 passives (sin 2 pi x, jittered step); "
+                       "K98 Roe-TVD + CTU + CT, hipGraph native loop")
+    return out
+
+
 def run_tvd_c1(steps, cour):
     """64^3 periodic, rho = 1, B0 = (0.5,0,0), k^-2 velocity 1<=|k|<=4 at rms Mach 1 (seed 1001),
     K98 Roe-TVD + CTU + CT, FP64, native loop in a hipGraph."""
@@ -478,6 +506,7 @@ def main():
     ap.add_argument("--sq-summarize", metavar="SQ_DIR")
     ap.add_argument("--big", metavar="N[,N]", help="only the large-box capacity lines, e.g. 512,1024")
     ap.add_argument("--io", type=int, metavar="N", help="only the restart/snapshot file timing of an N^3 box")
+    ap.add_argument("--tvd", type=int, metavar="N", help="only the forced K98 Roe-TVD workload at N^3 (FP32, FP64)")
     args = ap.parse_args()
     if args.pmc_summarize:
         return pmc_summarize(*args.pmc_summarize)
@@ -490,6 +519,10 @@ def main():
         return 2
     if args.pmc_run:
         return pmc_run(args.n)
+    if args.tvd:
+        print(json.dumps({"tvd": run_tvd_big(args.tvd, args.steps if args.steps != 200 else 20, args.cour)}),
+              flush=True)
+        return 0
     if args.io:
         print(json.dumps({"io": run_io(args.io)}), flush=True)
         return 0

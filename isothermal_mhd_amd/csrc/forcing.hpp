@@ -51,6 +51,9 @@ struct OuDev {
     unsigned int seed_lo, seed_hi;
     int solenoidal;
     const double* dt;        // per box
+    double* col;             // per box, per (j, k) column: sum over the modes with the same
+                             // n_x of a_m e_y e_z, [k-1][j-1][n_x + K][d] (re, im)
+    const long long* coloff; // per box offset into col (doubles)
 };
 
 template <class real>
@@ -65,6 +68,7 @@ struct OuApplyArgs {
 };
 
 void launch_ou_update(hipStream_t st, int nbox, const BoxDesc* boxes, const OuDev& ou, int init);
+void launch_ou_columns(hipStream_t st, int nbox, int maxcols, const BoxDesc* boxes, const OuDev& ou);
 template <class real>
 void launch_ou_pass(const Launch& L, const OuApplyArgs<real>& a);
 void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, const PlanBox* plan, int nbox, const double* part,

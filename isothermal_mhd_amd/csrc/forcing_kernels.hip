@@ -186,22 +186,52 @@ __global__ void k_ou_bump(int nbox, unsigned int* step) {
 }
 
 // a(x) of cell (i, j, k) of box b (interior indices from 1)
-__device__ __forceinline__ void ou_field(const OuDev& ou, const BoxDesc& B, int b, int i, int j, int k, double* a) {
+// Field of the OU modes at interior cell (i, j, k): a(x) = sum_m 2 Re(a_m e^{i k_m.x}),
+// evaluated separably.  k_ou_columns forms, per (j, k) column and per n_x, the sum over
+// the modes with that n_x of a_m e_y(n_y, j) e_z(n_z, k); each cell then sums over the
+// 2K+1 values of n_x only: 2 Re(e_x(n_x, i) D(n_x; j, k)).
+__global__ __launch_bounds__(256) void k_ou_columns(int nbox, const BoxDesc* __restrict__ boxes, OuDev ou) {
+    const int b = (int)blockIdx.y;
+    const BoxDesc& B = boxes[b];
+    const int col = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (col >= B.n[1] * B.n[2]) return;
+    const int j = col % B.n[1] + 1, k = col / B.n[1] + 1;
     const int W = 2 * ou.K + 1;
     const double* tx = ou.tw + ou.twoff[b];
     const double* ty = tx + (size_t)2 * W * B.n[0];
     const double* tz = ty + (size_t)2 * W * B.n[1];
     const double* ar = ou.are + (size_t)b * ou.nmode * 3;
     const double* ai = ou.aim + (size_t)b * ou.nmode * 3;
+    double* D = ou.col + ou.coloff[b] + (size_t)col * W * 6;
+    for (int slot = 0; slot < W; slot++) {
+        double dr[3] = {0.0, 0.0, 0.0}, di[3] = {0.0, 0.0, 0.0};
+        for (int m = 0; m < ou.nmode; m++) {
+            const int* n = ou.n + 3 * m;
+            if (n[0] + ou.K != slot) continue;
+            const double* ey = ty + 2 * ((size_t)(n[1] + ou.K) * B.n[1] + (j - 1));
+            const double* ez = tz + 2 * ((size_t)(n[2] + ou.K) * B.n[2] + (k - 1));
+            const double er = ey[0] * ez[0] - ey[1] * ez[1], ei = ey[0] * ez[1] + ey[1] * ez[0];
+            for (int d = 0; d < 3; d++) {
+                dr[d] = dr[d] + (ar[3 * m + d] * er - ai[3 * m + d] * ei);
+                di[d] = di[d] + (ar[3 * m + d] * ei + ai[3 * m + d] * er);
+            }
+        }
+        for (int d = 0; d < 3; d++) {
+            D[slot * 6 + 2 * d] = dr[d];
+            D[slot * 6 + 2 * d + 1] = di[d];
+        }
+    }
+}
+
+__device__ __forceinline__ void ou_field(const OuDev& ou, const BoxDesc& B, int b, int i, int j, int k, double* a) {
+    const int W = 2 * ou.K + 1;
+    const double* tx = ou.tw + ou.twoff[b];
+    const double* D = ou.col + ou.coloff[b] + ((size_t)(k - 1) * B.n[1] + (j - 1)) * W * 6;
     a[0] = a[1] = a[2] = 0.0;
-    for (int m = 0; m < ou.nmode; m++) {
-        const int* n = ou.n + 3 * m;
-        const double* ex = tx + 2 * ((size_t)(n[0] + ou.K) * B.n[0] + (i - 1));
-        const double* ey = ty + 2 * ((size_t)(n[1] + ou.K) * B.n[1] + (j - 1));
-        const double* ez = tz + 2 * ((size_t)(n[2] + ou.K) * B.n[2] + (k - 1));
-        double pr = ex[0] * ey[0] - ex[1] * ey[1], pi = ex[0] * ey[1] + ex[1] * ey[0];
-        double er = pr * ez[0] - pi * ez[1], ei = pr * ez[1] + pi * ez[0];
-        for (int d = 0; d < 3; d++) a[d] = a[d] + 2.0 * (ar[3 * m + d] * er - ai[3 * m + d] * ei);
+    for (int slot = 0; slot < W; slot++) {
+        const double* ex = tx + 2 * ((size_t)slot * B.n[0] + (i - 1));
+        const double xr = ex[0], xi = ex[1];
+        for (int d = 0; d < 3; d++) a[d] = a[d] + 2.0 * (xr * D[slot * 6 + 2 * d] - xi * D[slot * 6 + 2 * d + 1]);
     }
 }
 
@@ -265,6 +295,10 @@ void launch_ou_update(hipStream_t st, int nbox, const BoxDesc* boxes, const OuDe
     const int n = nbox * ou.nmode;
     hipLaunchKernelGGL(k_ou_update, dim3((n + 255) / 256), dim3(256), 0, st, nbox, boxes, ou, init);
     if (!init) hipLaunchKernelGGL(k_ou_bump, dim3((nbox + 255) / 256), dim3(256), 0, st, nbox, ou.step);
+}
+void launch_ou_columns(hipStream_t st, int nbox, int maxcols, const BoxDesc* boxes, const OuDev& ou) {
+    hipLaunchKernelGGL(k_ou_columns, dim3((unsigned)((maxcols + 255) / 256), (unsigned)nbox), dim3(256), 0, st, nbox,
+                       boxes, ou);
 }
 template <class real>
 void launch_ou_pass(const Launch& L, const OuApplyArgs<real>& a) {

@@ -97,6 +97,9 @@ struct imhd_ctx {
     long long* ou_twoff = nullptr;
     double* ou_part = nullptr;
     double* ou_mean = nullptr;
+    double* ou_col = nullptr;       // per-column mode sums of the separable field evaluation
+    long long* ou_coloff = nullptr;
+    int ou_maxcols = 0;
     int ou_K = 0;
     double* dt = nullptr;
     double* t = nullptr;
@@ -720,6 +723,8 @@ OuDev ou_dev(imhd_ctx* c) {
     o.seed_hi = (unsigned)(c->ou_cfg.seed >> 32);
     o.solenoidal = c->ou_cfg.solenoidal;
     o.dt = c->dt;
+    o.col = c->ou_col;
+    o.coloff = c->ou_coloff;
     return o;
 }
 
@@ -730,6 +735,7 @@ void ou_step(imhd_ctx* c, bool fuse_cmax) {
     ProfScope ps(c, "ou_forcing", c->pl_int.active);
     OuDev o = ou_dev(c);
     launch_ou_update(c->st, c->nbox, c->db, o, 0);
+    launch_ou_columns(c->st, c->nbox, c->ou_maxcols, c->db, o);
     OuApplyArgs<real> a{};
     a.ou = o;
     for (int f = 0; f < 7; f++) a.q[f] = R<real>(c->q[c->cur][f]);
@@ -1691,7 +1697,8 @@ int imhd_destroy(imhd_ctx* c) {
     for (auto p : c->ck) fr(p);
     for (auto p : c->delv) fr(p);
     for (void* p : {(void*)c->ou_n, (void*)c->ou_are, (void*)c->ou_aim, (void*)c->ou_step, (void*)c->ou_tw,
-                    (void*)c->ou_twoff, (void*)c->ou_part, (void*)c->ou_mean})
+                    (void*)c->ou_twoff, (void*)c->ou_part, (void*)c->ou_mean, (void*)c->ou_col,
+                    (void*)c->ou_coloff})
         fr(p);
     for (void* p : {(void*)c->force_part, (void*)c->force_par, (void*)c->force_err, (void*)c->dvmax}) fr(p);
     for (void* p : {(void*)c->smax, (void*)c->cmax, (void*)c->dmax, (void*)c->dt, (void*)c->t, (void*)c->dtprev, (void*)c->cour,
@@ -1912,8 +1919,11 @@ int imhd_set_ou_forcing(imhd_ctx* c, const imhd_ou* ou) {
             if (2 * K >= c->hb[b].n[d]) FAIL(c, IMHD_EINVAL, "OU forcing: kmax must be below n/2 on every axis");
     auto fr = [](void* p) { if (p) (void)hipFree(p); };
     for (void* p : {(void*)c->ou_n, (void*)c->ou_are, (void*)c->ou_aim, (void*)c->ou_step, (void*)c->ou_tw,
-                    (void*)c->ou_twoff, (void*)c->ou_part, (void*)c->ou_mean})
+                    (void*)c->ou_twoff, (void*)c->ou_part, (void*)c->ou_mean, (void*)c->ou_col,
+                    (void*)c->ou_coloff})
         fr(p);
+    c->ou_col = nullptr;
+    c->ou_coloff = nullptr;
     // per-axis twiddles e^{i 2 pi n (i - 1/2) / N}, n in [-K, K], interior i (host libm)
     std::vector<double> tw;
     std::vector<long long> off;
@@ -1942,6 +1952,18 @@ int imhd_set_ou_forcing(imhd_ctx* c, const imhd_ou* ou) {
     HIPCK(c, hipMemcpy(c->ou_twoff, off.data(), 8 * off.size(), hipMemcpyHostToDevice));
     HIPCK(c, hipMalloc(&c->ou_part, 8 * 3 * (size_t)(c->pl_int.total / 256 + 1)));
     HIPCK(c, hipMalloc(&c->ou_mean, 8 * 3 * (size_t)c->nbox));
+    std::vector<long long> coff;
+    long long ncol = 0;
+    c->ou_maxcols = 0;
+    for (int b = 0; b < c->nbox; b++) {
+        coff.push_back(ncol);
+        const int cols = c->hb[b].n[1] * c->hb[b].n[2];
+        ncol += (long long)cols * (2 * K + 1) * 6;
+        c->ou_maxcols = std::max(c->ou_maxcols, cols);
+    }
+    HIPCK(c, hipMalloc(&c->ou_col, 8 * (size_t)ncol));
+    HIPCK(c, hipMalloc(&c->ou_coloff, 8 * coff.size()));
+    HIPCK(c, hipMemcpy(c->ou_coloff, coff.data(), 8 * coff.size(), hipMemcpyHostToDevice));
     c->ou_cfg = *ou;
     c->ou_k = kv;
     c->ou_K = K;

@@ -69,6 +69,19 @@ def config_c2(n=256, seed=2002, nb=6):
     return make_state((n, n, n), nb=nb, npass=1, v=v, b0=(0.0, 0.0, 1.0), passive=[s])
 
 
+def config_c3(n=512, seed=3003, nb=6):
+    """Forced-turbulence TVD configuration: rho = 1, B0 = (0.2,0,0), v = 0 (the OU forcing
+    drives the flow), two passive scalars: sin(2 pi x) and a step at x = 1/2 whose position
+    is jittered per (y, z) column by up to +-2 cells (seeded)."""
+    nx = ny = nz = n
+    x = (np.arange(nx) + 0.5) / nx
+    s1 = np.broadcast_to(np.sin(2.0 * np.pi * x)[:, None, None], (nx, ny, nz))
+    jit = np.random.default_rng(seed).integers(-2, 3, size=(ny, nz))
+    edge = nx // 2 + jit
+    s2 = (np.arange(nx)[:, None, None] >= edge[None, :, :]).astype(np.float64)
+    return make_state((n, n, n), nb=nb, npass=2, v=None, b0=(0.2, 0.0, 0.0), passive=[s1, s2])
+
+
 def config_c1(n=64, seed=1001, nb=6):
     """64^3 periodic, rho = 1, B0 = (0.5,0,0), k^-2 velocity on 1 <= |k| <= 4 at rms Mach 1."""
     v = random_velocity((n, n, n), 1.0, 4.0, 2.0, 1.0, solenoidal=False, seed=seed)
