@@ -116,6 +116,8 @@ struct imhd_ctx {
     PlanDev pl_ghost, pl_pghost, pl_int, pl_vel, pl_ppflux[3], pl_ppupd, pl_pcflux[3], pl_pfupd;
     PlanDev pl_pstage[2];   // fused passive stages (column plans when zseg > 0)
     PlanDev pl_teig_p[3], pl_tflux_p[3], pl_teig_c[3], pl_tflux_c[3];
+    PlanDev pl_tsweep_p[3], pl_tsweep_c[3];   // fused Roe-TVD sweeps
+    bool tvdfuse = true;                     // one launch per Roe-TVD sweep (IMHD_TVDFUSE=0: two)
     int zseg = 4;    // z-segment length of the column (z-marching) kernels
     int strip = 0;   // y-strip height of the work enumeration (0: whole planes)
     bool fuse3 = true;   // the three directional flux sweeps of a stage in one launch
@@ -211,6 +213,10 @@ int build_plan(imhd_ctx* c, PlanDev& p, int which, int dir, int seg = 0) {
                     lo = 3 - nb; hi = normal ? n + nb - 3 : n + nb - 2; break;
                 case 19:  // corrector TVD fluxes and face EMFs
                     lo = normal ? 4 - nb : 3 - nb; hi = normal ? n + nb - 4 : n + nb - 2; break;
+                case 27:  // fused predictor sweep: the faces of case 17
+                    lo = normal ? 2 - nb : 1 - nb; hi = normal ? n + nb - 2 : n + nb; break;
+                case 28:  // fused corrector sweep: the faces of case 19
+                    lo = normal ? 4 - nb : 3 - nb; hi = normal ? n + nb - 4 : n + nb - 2; break;
                 case 20:  // predictor corner EMF (isotvd_mod.f90:1006-1008)
                     lo = 2 - nb; hi = n + nb - 2; break;
                 case 21:  // corrector corner EMF
@@ -242,7 +248,22 @@ int build_plan(imhd_ctx* c, PlanDev& p, int which, int dir, int seg = 0) {
         }
         pb.start = run;
         long long cnt = (long long)pb.r[0] * pb.r[1] * pb.r[2];
-        p.active += seg > 0 ? (long long)pb.r[0] * pb.r[1] * pb.zn : cnt;   // cells
+        if (which == 27 || which == 28) {
+            // fused Roe-TVD sweep: x -- one wave per (row, TVD_XSEG faces);
+            // y, z -- one thread per (x, transverse, segment of seg faces along the sweep)
+            if (dir == 0) {
+                pb.seg = TVD_XSEG;
+                pb.zn = (pb.r[0] + TVD_XSEG - 1) / TVD_XSEG;
+                cnt = (long long)pb.r[1] * pb.r[2] * pb.zn * 64;
+            } else {
+                const int ta = dir == 1 ? 2 : 1;
+                pb.seg = pb.r[dir] >= 128 ? 32 : 8;
+                pb.zn = (pb.r[dir] + pb.seg - 1) / pb.seg;
+                cnt = (long long)pb.r[0] * pb.r[ta] * pb.zn;
+            }
+        }
+        p.active += seg > 0 ? (long long)pb.r[0] * pb.r[1] * pb.zn
+                            : (which == 27 || which == 28) ? (long long)pb.r[0] * pb.r[1] * pb.r[2] : cnt;   // cells
         run += (cnt + 255) / 256 * 256;
         p.h.push_back(pb);
     }
@@ -583,6 +604,14 @@ void tvd_sweep(imhd_ctx* c, int sq, int sb, bool corr, int d) {
     if (c->cfg.riemann == IMHD_HLLE) {
         ProfScope ps(c, nm[corr][2][d], pe.active);
         launch_tvd_hlle<real>(mk(c, pe), a);
+        return;
+    }
+    if (c->tvdfuse) {
+        static const char* nf[2][3] = {{"tvd_sweep_pred_x", "tvd_sweep_pred_y", "tvd_sweep_pred_z"},
+                                       {"tvd_sweep_corr_x", "tvd_sweep_corr_y", "tvd_sweep_corr_z"}};
+        const PlanDev& ps_ = corr ? c->pl_tsweep_c[d] : c->pl_tsweep_p[d];
+        ProfScope ps(c, nf[corr][d], ps_.active);
+        launch_tvd_sweep<real>(mk(c, ps_), a);
         return;
     }
     {
@@ -1545,6 +1574,7 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     if (const char* e = getenv("IMHD_FUSE3")) c->fuse3 = atoi(e) != 0;      // 0: one launch per direction
     if (const char* e = getenv("IMHD_STRIP")) c->strip = std::max(0, atoi(e));
     if (const char* e = getenv("IMHD_VSTAGE")) c->vstage = atoi(e) != 0;    // 0: three kernels per stage
+    if (const char* e = getenv("IMHD_TVDFUSE")) c->tvdfuse = atoi(e) != 0;  // 0: eig + flux kernels
     if (const char* e = getenv("IMHD_VROUNDS")) c->vrounds = std::max(0, atoi(e));
     if (const char* e = getenv("IMHD_VSEG")) { c->vseg = std::max(1, atoi(e)); c->vrounds = 0; }
     // FP32: 32 x 16 tiles (2 workgroups of 80 KB LDS per CU); FP64: 16 x 16 (a 32 x 16
@@ -1606,7 +1636,7 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
             for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->Fc[d][f], nr))) return rc;
             if ((rc = alloc(c, (void**)&c->Ecc[d], nd))) return rc;
         }
-        if (g.riemann == IMHD_ROE)
+        if (g.riemann == IMHD_ROE && !c->tvdfuse)
             for (int w = 0; w < 6; w++) {
                 if ((rc = alloc(c, &c->gt[w], nr))) return rc;
                 if ((rc = alloc(c, &c->ck[w], nr))) return rc;
@@ -1659,6 +1689,8 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
             if ((rc = build_plan(c, c->pl_tflux_p[d], 17, d))) return rc;
             if ((rc = build_plan(c, c->pl_teig_c[d], 18, d))) return rc;
             if ((rc = build_plan(c, c->pl_tflux_c[d], 19, d))) return rc;
+            if ((rc = build_plan(c, c->pl_tsweep_p[d], 27, d))) return rc;
+            if ((rc = build_plan(c, c->pl_tsweep_c[d], 28, d))) return rc;
         }
         if ((rc = build_plan(c, c->pl_tcorner_p, 20, -1))) return rc;
         if ((rc = build_plan(c, c->pl_tcorner_c, 21, -1))) return rc;
@@ -1713,7 +1745,8 @@ int imhd_destroy(imhd_ctx* c) {
                       &c->pl_teig_c[0], &c->pl_teig_c[1], &c->pl_teig_c[2], &c->pl_tflux_c[0],
                       &c->pl_tflux_c[1], &c->pl_tflux_c[2], &c->pl_tcorner_p, &c->pl_tcorner_c,
                       &c->pl_tcenter, &c->pl_thalf, &c->pl_tcfl, &c->pl_pflux3, &c->pl_cflux3,
-                      &c->pl_pstage[0], &c->pl_pstage[1]};
+                      &c->pl_pstage[0], &c->pl_pstage[1], &c->pl_tsweep_p[0], &c->pl_tsweep_p[1],
+                      &c->pl_tsweep_p[2], &c->pl_tsweep_c[0], &c->pl_tsweep_c[1], &c->pl_tsweep_c[2]};
     for (auto p : pls) fr(p->d);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;

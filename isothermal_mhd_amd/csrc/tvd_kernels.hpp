@@ -5,6 +5,10 @@
 
 namespace imhd {
 
+// flux faces per wave of the fused x sweep (64 lanes: 62 faces + one neighbour face on
+// each side)
+constexpr int TVD_XSEG = 62;
+
 // Directional sweep over cell states rotated so that the sweep direction is "x":
 // w[r] = q[perm(d, r)], slot 4 unused (each cell's own upper face field comes from bn).
 template <class real>
@@ -75,6 +79,9 @@ template <class real>
 void launch_tvd_eig(const Launch& L, const TvdSweepArgs<real>& a);
 template <class real>
 void launch_tvd_flux(const Launch& L, const TvdSweepArgs<real>& a);
+// Fused eigen-system + flux sweep (plans: build_plan case 27 for x, 28 for y / z)
+template <class real>
+void launch_tvd_sweep(const Launch& L, const TvdSweepArgs<real>& a);
 template <class real>
 void launch_tvd_hlle(const Launch& L, const TvdSweepArgs<real>& a);
 template <class real>

@@ -224,52 +224,30 @@ __device__ __forceinline__ void tvd_store(const TvdSweepArgs<real>& a, long long
 
 // per face: g~_k and alpha_k (isotvd_mod.f90:1856-2333, 2400-2424)
 template <class real>
-__global__ __launch_bounds__(256) void k_tvd_eig(const BoxDesc* __restrict__ boxes, Plan plan, TvdSweepArgs<real> a) {
-    int b, i, j, k;
-    if (!plan_item(plan, b, i, j, k)) return;
-    const BoxDesc& B = boxes[b];
-    const long long c = cell_index(B, i, j, k), s = tvd_stride(B, a.dir);
-    real uL[7], uR[7];
-    tvd_load(a, c, uL);
-    tvd_load(a, c + s, uR);
-    const real cs = (real)a.cs;
+__device__ __forceinline__ void tvd_eig_face(const real* uL, const real* uR, real cs, real dtdx, real* gt, real* ck) {
     TvdWave<real> W;
     tvd_wave(uL, uR, cs, W);
-    real ck[6];
     tvd_ck(uL, uR, cs, W, ck);
-    const double dtd = a.dt[b] / B.dx;
-    const real dtdx = (real)(a.factor == 1.0 ? dtd : a.factor * dtd);
 #pragma unroll
     for (int w = 0; w < 6; w++) {
         real chi = dtdx * W.ev[w];
         real Qk = tvd_Q(chi, (real)tvd_eps(w));
-        a.gt[w][c] = real(0.5) * (Qk - chi * chi) * ck[w];
-        a.ck[w][c] = ck[w];
+        gt[w] = real(0.5) * (Qk - chi * chi) * ck[w];
     }
 }
 
-// Roe-TVD flux of face c (isotvd_mod.f90:2400-2466)
+// Roe-TVD flux of one face from its states, its alpha_k and the g~_k of the face below
+// (gm), itself (g0) and above (gp) (isotvd_mod.f90:2400-2466); F[4] = 0.
 template <class real>
-__global__ __launch_bounds__(256) void k_tvd_flux(const BoxDesc* __restrict__ boxes, Plan plan, TvdSweepArgs<real> a) {
-    int b, i, j, k;
-    if (!plan_item(plan, b, i, j, k)) return;
-    const BoxDesc& B = boxes[b];
-    const long long c = cell_index(B, i, j, k), s = tvd_stride(B, a.dir);
-    real uL[7], uR[7];
-    tvd_load(a, c, uL);
-    tvd_load(a, c + s, uR);
-    const real cs = (real)a.cs;
+__device__ __forceinline__ void tvd_flux_face(const real* uL, const real* uR, real cs, real dtdx, real dxdt,
+                                              const real* gm, const real* g0, const real* gp, const real* ck, real* F) {
     TvdWave<real> W;
     tvd_wave(uL, uR, cs, W);
-    const double dtd = a.dt[b] / B.dx;
-    const real dtdx = (real)(a.factor == 1.0 ? dtd : a.factor * dtd);
-    const real dxdt = (real)(B.dx / a.dt[b]);
     real FT[6] = {real(0), real(0), real(0), real(0), real(0), real(0)};
 #pragma unroll
     for (int w = 0; w < 6; w++) {
-        const real g0 = a.gt[w][c - s], g1 = a.gt[w][c], g2 = a.gt[w][c + s];
-        const real ckw = a.ck[w][c];
-        const real gk0 = tvd_lim(g0, g1), gk1 = tvd_lim(g1, g2);
+        const real ckw = ck[w];
+        const real gk0 = tvd_lim(gm[w], g0[w]), gk1 = tvd_lim(g0[w], gp[w]);
         real gam = (fabs(ckw) > (real)ZERO) ? fdiv(gk1 - gk0, ckw) : real(0);
         real chi = dtdx * W.ev[w] + gam;
         real Qk = tvd_Q(chi, (real)tvd_eps(w));
@@ -279,7 +257,7 @@ __global__ __launch_bounds__(256) void k_tvd_flux(const BoxDesc* __restrict__ bo
 #pragma unroll
         for (int m = 0; m < 6; m++) FT[m] = FT[m] - betak * r[m];
     }
-    real fa[6], fb[6], F[7];
+    real fa[6], fb[6];
     tvd_cell_flux(uL, cs, fa);
     tvd_cell_flux(uR, cs, fb);
     F[0] = real(0.5) * (dxdt * FT[0] + fa[0] + fb[0]);
@@ -289,7 +267,185 @@ __global__ __launch_bounds__(256) void k_tvd_flux(const BoxDesc* __restrict__ bo
     F[4] = real(0);
     F[5] = real(0.5) * (dxdt * FT[4] + fa[4] + fb[4]);
     F[6] = real(0.5) * (dxdt * FT[5] + fa[5] + fb[5]);
+}
+
+template <class real>
+__device__ __forceinline__ void tvd_dts(const TvdSweepArgs<real>& a, const BoxDesc& B, int b, real& dtdx, real& dxdt) {
+    const double dtd = a.dt[b] / B.dx;
+    dtdx = (real)(a.factor == 1.0 ? dtd : a.factor * dtd);
+    dxdt = (real)(B.dx / a.dt[b]);
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void k_tvd_eig(const BoxDesc* __restrict__ boxes, Plan plan, TvdSweepArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const long long c = cell_index(B, i, j, k), s = tvd_stride(B, a.dir);
+    real uL[7], uR[7], gt[6], ck[6], dtdx, dxdt;
+    tvd_load(a, c, uL);
+    tvd_load(a, c + s, uR);
+    tvd_dts(a, B, b, dtdx, dxdt);
+    tvd_eig_face(uL, uR, (real)a.cs, dtdx, gt, ck);
+#pragma unroll
+    for (int w = 0; w < 6; w++) {
+        a.gt[w][c] = gt[w];
+        a.ck[w][c] = ck[w];
+    }
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void k_tvd_flux(const BoxDesc* __restrict__ boxes, Plan plan, TvdSweepArgs<real> a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const long long c = cell_index(B, i, j, k), s = tvd_stride(B, a.dir);
+    real uL[7], uR[7], gm[6], g0[6], gp[6], ck[6], F[7], dtdx, dxdt;
+    tvd_load(a, c, uL);
+    tvd_load(a, c + s, uR);
+    tvd_dts(a, B, b, dtdx, dxdt);
+#pragma unroll
+    for (int w = 0; w < 6; w++) {
+        gm[w] = a.gt[w][c - s];
+        g0[w] = a.gt[w][c];
+        gp[w] = a.gt[w][c + s];
+        ck[w] = a.ck[w][c];
+    }
+    tvd_flux_face(uL, uR, (real)a.cs, dtdx, dxdt, gm, g0, gp, ck, F);
     tvd_store(a, c, F);
+}
+
+// Fused Roe-TVD sweep, eigen-system and flux in one launch (g~_k and alpha_k never leave
+// the chip).  The sweep's flux faces [lo, lo + r) per axis come from a face plan (build_plan
+// cases 17 / 19); each face's flux needs g~_k of its two neighbours along the sweep.
+//
+// x sweep: one wave per (row, segment of TVD_XSEG = 62 flux faces).  Lane l forms g~_k
+// and alpha_k of face x0 - 1 + l; lanes 1 .. 62 then take their neighbours' g~_k by wave
+// shuffles and form the flux of their own face from the states they already hold.  No
+// LDS and no barrier; every lane of the wave reaches the shuffles.
+template <class real>
+__global__ __launch_bounds__(256) void k_tvd_sweep_x(const BoxDesc* __restrict__ boxes, Plan plan, TvdSweepArgs<real> a) {
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    const long long g = (long long)blk * 256 + threadIdx.x;
+    const long long gw = g - (threadIdx.x & 63);   // first item of this wave
+    if (gw >= plan.total) return;                  // whole wave
+    int b = 0;
+    if (plan.nbox > 1) {
+        int lo = 0, hi = plan.nbox - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (plan.boxes[mid].start <= gw) lo = mid; else hi = mid - 1;
+        }
+        b = lo;
+    }
+    const PlanBox& pb = plan.boxes[b];
+    const long long wseg = (gw - pb.start) >> 6;   // (row, segment) of this wave
+    const int nseg = pb.zn;
+    if (wseg >= (long long)pb.r[1] * pb.r[2] * nseg) return;   // whole wave
+    const int lane = (int)(threadIdx.x & 63);
+    const int sgi = (int)(wseg % nseg);
+    const long long row = wseg / nseg;
+    const int j = pb.lo[1] + (int)(row % pb.r[1]);
+    const int k = pb.lo[2] + (int)(row / pb.r[1]);
+    const int x0 = pb.lo[0] + sgi * TVD_XSEG;
+    const int x1 = min(x0 + TVD_XSEG - 1, pb.lo[0] + pb.r[0] - 1);   // last flux face of the segment
+    const int x = x0 - 1 + lane;                                     // this lane's face
+    const bool eig = x <= x1 + 1;
+    const BoxDesc& B = boxes[b];
+    real uL[7], uR[7], ck[6], gt[6], dtdx, dxdt;
+    tvd_dts(a, B, b, dtdx, dxdt);
+    const long long c = cell_index(B, eig ? x : x0, j, k);
+    if (eig) {
+        tvd_load(a, c, uL);
+        tvd_load(a, c + 1, uR);
+        tvd_eig_face(uL, uR, (real)a.cs, dtdx, gt, ck);
+    } else {
+#pragma unroll
+        for (int w = 0; w < 6; w++) gt[w] = ck[w] = real(0);
+    }
+    real gm[6], gp[6];
+#pragma unroll
+    for (int w = 0; w < 6; w++) {
+        gm[w] = __shfl_up(gt[w], 1, 64);
+        gp[w] = __shfl_down(gt[w], 1, 64);
+    }
+    if (lane >= 1 && lane <= TVD_XSEG && x <= x1) {
+        real F[7];
+        tvd_flux_face(uL, uR, (real)a.cs, dtdx, dxdt, gm, gt, gp, ck, F);
+        tvd_store(a, c, F);
+    }
+}
+
+// y / z sweep: threads along x (coalesced), each marching a segment of pb.seg flux faces
+// along the sweep axis with a three-face window of g~_k in registers.  Items enumerate
+// (x, transverse index, segment), x fastest.
+template <class real>
+__global__ __launch_bounds__(256) void k_tvd_sweep_m(const BoxDesc* __restrict__ boxes, Plan plan, TvdSweepArgs<real> a) {
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    const long long g = (long long)blk * blockDim.x + threadIdx.x;
+    if (g >= plan.total) return;
+    int b = 0;
+    if (plan.nbox > 1) {
+        int lo = 0, hi = plan.nbox - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (plan.boxes[mid].start <= g) lo = mid; else hi = mid - 1;
+        }
+        b = lo;
+    }
+    const PlanBox& pb = plan.boxes[b];
+    const int d = a.dir, ta = d == 1 ? 2 : 1;   // sweep axis, other transverse axis
+    const long long loc = g - pb.start;
+    const long long cols = (long long)pb.r[0] * pb.r[ta];
+    if (loc >= cols * pb.zn) return;
+    const int i = pb.lo[0] + (int)(loc % pb.r[0]);
+    const long long t = loc / pb.r[0];
+    const int it = pb.lo[ta] + (int)(t % pb.r[ta]);
+    const int sgi = (int)(t / pb.r[ta]);
+    const int f0 = pb.lo[d] + sgi * pb.seg;
+    const int f1 = min(f0 + pb.seg - 1, pb.lo[d] + pb.r[d] - 1);
+    const BoxDesc& B = boxes[b];
+    const long long s = tvd_stride(B, d);
+    const long long c0 = d == 1 ? cell_index(B, i, f0, it) : cell_index(B, i, it, f0);
+    const real cs = (real)a.cs;
+    real dtdx, dxdt;
+    tvd_dts(a, B, b, dtdx, dxdt);
+    // One loop body forms every face's g~_k / alpha_k and every flux, so each face gets the
+    // same instruction sequence wherever the segment boundaries fall (FP32 contraction
+    // included): the fluxes stay translation-invariant, as periodic images require.
+    // Iteration f: eigen-system of face f (cells f, f+1); flux of face f-1 once its upper
+    // neighbour's g~_k is known.
+    real uA[7], uB[7], uC[7], gm[6], g0[6], gp[6], ck0[6], ckp[6];
+    tvd_load(a, c0 - s, uB);
+#pragma unroll
+    for (int w = 0; w < 6; w++) {
+        gm[w] = g0[w] = gp[w] = real(0);
+        ck0[w] = ckp[w] = real(0);
+    }
+#pragma unroll
+    for (int r = 0; r < 7; r++) uA[r] = uB[r];
+    long long c = c0 - s;   // face f
+#pragma unroll 1
+    for (int f = f0 - 1; f <= f1 + 1; f++, c += s) {
+        tvd_load(a, c + s, uC);
+        tvd_eig_face(uB, uC, cs, dtdx, gp, ckp);
+        if (f > f0) {
+            real F[7];
+            tvd_flux_face(uA, uB, cs, dtdx, dxdt, gm, g0, gp, ck0, F);
+            tvd_store(a, c - s, F);
+        }
+#pragma unroll
+        for (int w = 0; w < 6; w++) {
+            gm[w] = g0[w];
+            g0[w] = gp[w];
+            ck0[w] = ckp[w];
+        }
+#pragma unroll
+        for (int r = 0; r < 7; r++) {
+            uA[r] = uB[r];
+            uB[r] = uC[r];
+        }
+    }
 }
 
 // HLLE of the TVD module (isotvd_mod.f90:2473-2578): adjacent cell states, each with its
@@ -583,6 +739,14 @@ void launch_tvd_flux(const Launch& L, const TvdSweepArgs<real>& a) {
     if (L.plan.total > 0) hipLaunchKernelGGL((k_tvd_flux<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
 }
 template <class real>
+void launch_tvd_sweep(const Launch& L, const TvdSweepArgs<real>& a) {
+    if (L.plan.total <= 0) return;
+    if (a.dir == 0)
+        hipLaunchKernelGGL((k_tvd_sweep_x<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+    else
+        hipLaunchKernelGGL((k_tvd_sweep_m<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+template <class real>
 void launch_tvd_hlle(const Launch& L, const TvdSweepArgs<real>& a) {
     if (L.plan.total > 0) hipLaunchKernelGGL((k_tvd_hlle<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
 }
@@ -615,6 +779,7 @@ void launch_tvd_max_speed(const Launch& L, const QPtrs<real>& q, double cs, unsi
 #define IMHD_INST_TVD(T)                                                            \
     template void launch_tvd_eig<T>(const Launch&, const TvdSweepArgs<T>&);         \
     template void launch_tvd_flux<T>(const Launch&, const TvdSweepArgs<T>&);        \
+    template void launch_tvd_sweep<T>(const Launch&, const TvdSweepArgs<T>&);       \
     template void launch_tvd_hlle<T>(const Launch&, const TvdSweepArgs<T>&);        \
     template void launch_tvd_corner<T>(const Launch&, const TvdCornerArgs<T>&);     \
     template void launch_tvd_center<T>(const Launch&, const TvdCenterArgs<T>&);     \
