@@ -54,7 +54,9 @@ typedef struct imhd_config {
     int bc;             /* IMHD_BC_OPEN / IMHD_BC_PERIODIC */
     int precision;      /* 64: FP64 throughout; 32: FP32 bulk, FP64 face B, corner EMF, CFL, diagnostics */
     int compensated;    /* precision 32 only: Kahan-compensated accumulation of rho, rho v */
-    int _reserved;
+    int flux_protection; /* VL + Roe + precision 64: the reference's flux_protection
+                            (vanleer_mod.f90:81, off there): cells whose updated density falls
+                            below MIN_DENS take HLLE fluxes and corner EMFs (:2084-2200) */
     double cs;          /* isothermal sound speed (constants_mod.f90:35) */
     double mc_beta;     /* MC limiter beta (vanleer_mod.f90:77), 1.01 in the reference */
     double cour;        /* Courant number for imhd_run (constants_mod.f90:16) */

@@ -28,7 +28,7 @@ class ImhdConfig(ctypes.Structure):
                 ("nb", ctypes.c_int), ("npass", ctypes.c_int), ("integrator", ctypes.c_int),
                 ("riemann", ctypes.c_int), ("recon", ctypes.c_int), ("bc", ctypes.c_int),
                 ("precision", ctypes.c_int), ("compensated", ctypes.c_int),
-                ("_reserved", ctypes.c_int), ("cs", ctypes.c_double),
+                ("flux_protection", ctypes.c_int), ("cs", ctypes.c_double),
                 ("mc_beta", ctypes.c_double), ("cour", ctypes.c_double)]
 
 

@@ -443,6 +443,54 @@ __global__ __launch_bounds__(256) void k_pack_cube(BoxDesc B, CubeArgs<real> a, 
     for (int e = threadIdx.x; e < nv; e += 256) out[base * V + e] = s[e];
 }
 
+// Flux protection (vanleer_mod.f90:2084-2200), whole padded box.  The first-pass update is
+// update_cell's expression without the floor (final_check_unphysical runs after the
+// recompute).
+__global__ __launch_bounds__(256) void k_prot_flag(const BoxDesc* __restrict__ boxes, Plan plan, ProtArgs a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    const long long c = cell_index(B, i, j, k);
+    const int lo = a.lo;
+    const bool in = i >= lo && j >= lo && k >= lo && i <= B.n[0] + a.hi_off && j <= B.n[1] + a.hi_off &&
+                    k <= B.n[2] + a.hi_off;
+    unsigned char f = 0;
+    if (in) {
+        const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+        const double dtdx = a.factor * a.dt[b] / B.dx;
+        const double div = (a.F[0][0][c] - a.F[0][0][c - sx]) + (a.F[1][0][c] - a.F[1][0][c - sy]) +
+                           (a.F[2][0][c] - a.F[2][0][c - sz]);
+        f = (a.src[0][c] - dtdx * div) < a.thr ? 1 : 0;
+    }
+    a.flag[c] = f;
+}
+
+__global__ __launch_bounds__(256) void k_prot_fix(const BoxDesc* __restrict__ boxes, Plan plan, ProtArgs a) {
+    int b, i, j, k;
+    if (!plan_item(plan, b, i, j, k)) return;
+    const BoxDesc& B = boxes[b];
+    if (i >= B.n[0] + B.nb || j >= B.n[1] + B.nb || k >= B.n[2] + B.nb) return;   // m + 1 stays in the box
+    const long long c = cell_index(B, i, j, k);
+    const long long s[3] = {1, B.P[0], (long long)B.P[0] * B.P[1]};
+    const bool f0 = a.flag[c] != 0;
+    bool any = f0;
+    for (int d = 0; d < 3; d++) {
+        const bool fn = a.flag[c + s[d]] != 0;
+        any = any || fn;
+        if (f0 || fn)   // face m + 1/2 along d borders a marked cell
+            for (int q = 0; q < 4; q++) a.F[d][q][c] = a.Fp[d][q][c];
+    }
+    if (any)
+        for (int d = 0; d < 3; d++) a.Ec[d][c] = a.Ecp[d][c];
+}
+
+void launch_prot_flag(const Launch& L, const ProtArgs& a) {
+    if (L.plan.total > 0) hipLaunchKernelGGL(k_prot_flag, dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+void launch_prot_fix(const Launch& L, const ProtArgs& a) {
+    if (L.plan.total > 0) hipLaunchKernelGGL(k_prot_fix, dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+}
+
 // ---------------------------------------------------------------------------------
 #define L256(k, ...) hipLaunchKernelGGL(k, dim3(L.blocks()), dim3(256), 0, L.stream, __VA_ARGS__)
 

@@ -117,6 +117,13 @@ struct imhd_ctx {
     PlanDev pl_pstage[2];   // fused passive stages (column plans when zseg > 0)
     PlanDev pl_teig_p[3], pl_tflux_p[3], pl_teig_c[3], pl_tflux_c[3];
     PlanDev pl_tsweep_p[3], pl_tsweep_c[3];   // fused Roe-TVD sweeps
+    // flux protection of the VL Roe update (vanleer_mod.f90:81, 93-110)
+    bool prot = false;
+    double prot_thr = MIN_DENS;
+    void* Fpr[3][4] = {};
+    void* Epr[6] = {};
+    double* Ecpr[3] = {};
+    unsigned char* pflag = nullptr;
     bool tvdfuse = true;                     // one launch per Roe-TVD sweep (IMHD_TVDFUSE=0: two)
     int zseg = 4;    // z-segment length of the column (z-marching) kernels
     int strip = 0;   // y-strip height of the work enumeration (0: whole planes)
@@ -520,6 +527,56 @@ void passive_stage(imhd_ctx* c, int qold, int qnew, int stage, bool vel);
 // passive step): the passive stages run inline -- the predictor right after the MHD
 // predictor (it needs only t^n data), the corrector after the MHD corrector -- and read
 // the velocities the fused MHD stages write, instead of re-deriving them from rho and m
+// ---- flux protection (vanleer_mod.f90:540, 703, 880, 336, 396, 2084-2200) ----
+// The protection fluxes are the VL module's HLLE on the same interface states (the flux
+// kernels run again with the HLLE solver and write to the protection arrays; smax takes
+// both solvers' speeds, as compute_HLLE_fluxes_1d raises it too).
+template <class real>
+void protection_fluxes(imhd_ctx* c, int s, bool corrector) {
+    ProfScope ps(c, corrector ? "prot_flux_corr" : "prot_flux_pred", 0);
+    for (int d = 0; d < 3; d++)
+        for (int f = 0; f < 4; f++) std::swap(c->F[d][f], c->Fpr[d][f]);
+    for (int e = 0; e < 6; e++) std::swap(c->E[e], c->Epr[e]);
+    sweeps<real, RS_HLLE>(c, s, corrector);
+    for (int d = 0; d < 3; d++)
+        for (int f = 0; f < 4; f++) std::swap(c->F[d][f], c->Fpr[d][f]);
+    for (int e = 0; e < 6; e++) std::swap(c->E[e], c->Epr[e]);
+}
+
+template <class real>
+void protection_corner(imhd_ctx* c, int s, const PlanDev& p) {
+    for (int e = 0; e < 6; e++) std::swap(c->E[e], c->Epr[e]);
+    for (int d = 0; d < 3; d++) std::swap(c->Ec[d], c->Ecpr[d]);
+    corner<real>(c, s, p, "prot_corner");
+    for (int e = 0; e < 6; e++) std::swap(c->E[e], c->Epr[e]);
+    for (int d = 0; d < 3; d++) std::swap(c->Ec[d], c->Ecpr[d]);
+}
+
+// Mark the cells whose first-pass density falls below the threshold and swap in the
+// protection fluxes and corner EMFs around them; the ordinary update then recomputes the
+// range from the stage's base state src.
+template <class real>
+void protection_fix(imhd_ctx* c, int src, bool corrector) {
+    ProfScope ps(c, corrector ? "prot_fix_corr" : "prot_fix_pred", c->pl_ghost.active);
+    ProtArgs a{};
+    for (int f = 0; f < 4; f++) a.src[f] = (const double*)c->q[src][f];
+    for (int d = 0; d < 3; d++)
+        for (int f = 0; f < 4; f++) {
+            a.F[d][f] = (double*)c->F[d][f];
+            a.Fp[d][f] = (const double*)c->Fpr[d][f];
+        }
+    for (int d = 0; d < 3; d++) { a.Ec[d] = c->Ec[d]; a.Ecp[d] = c->Ecpr[d]; }
+    a.flag = c->pflag;
+    a.dt = c->dt;
+    a.factor = corrector ? 1.0 : 0.5;
+    a.thr = c->prot_thr;
+    const int nb = c->hb[0].nb, e = c->cfg.recon == IMHD_PLM_PRIM ? 0 : 1;
+    a.lo = corrector ? 5 - nb + e : 2 - nb;          // build_plan cases 5 and 2
+    a.hi_off = corrector ? nb - 4 - e : nb - 1;
+    launch_prot_flag(mk(c, c->pl_ghost), a);
+    launch_prot_fix(mk(c, c->pl_ghost), a);
+}
+
 template <class real>
 void vl_step(imhd_ctx* c, bool fuse_cmax = false, bool with_passive = false) {
     launch_reset_u64(c->st, c->smax, (int)stripe_slots(c->nbox));
@@ -559,14 +616,20 @@ void vl_step(imhd_ctx* c, bool fuse_cmax = false, bool with_passive = false) {
         case IMHD_HLLE: sweeps<real, RS_HLLE>(c, s0, false); break;
         default: sweeps<real, RS_HLLD>(c, s0, false); break;
     }
+    if (c->prot) protection_fluxes<real>(c, s0, false);
     corner<real>(c, s0, c->pl_pcorner, "corner_pred");
+    if (c->prot) protection_corner<real>(c, s0, c->pl_pcorner);
+    if (c->prot) protection_fix<real>(c, s0, false);
     update<real>(c, s0, s1, 0.5, c->pl_pupd, false, "update_pred");
     switch (c->cfg.riemann) {
         case IMHD_ROE: sweeps<real, RS_ROE>(c, s1, true); break;
         case IMHD_HLLE: sweeps<real, RS_HLLE>(c, s1, true); break;
         default: sweeps<real, RS_HLLD>(c, s1, true); break;
     }
+    if (c->prot) protection_fluxes<real>(c, s1, true);
     corner<real>(c, s1, c->pl_ccorner, "corner_corr");
+    if (c->prot) protection_corner<real>(c, s1, c->pl_ccorner);
+    if (c->prot) protection_fix<real>(c, s0, true);
     update<real>(c, s0, s1, 1.0, c->pl_cupd, true, fuse_cmax ? "update_corr_cfl" : "update_corr", fuse_cmax);
     c->cur = s1;
 }
@@ -1581,6 +1644,18 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     // FP64 tile would leave one workgroup per CU)
     c->vshape = g.precision == 32 ? 1 : 0;
     if (const char* e = getenv("IMHD_VSHAPE")) c->vshape = std::min(VS_SHAPES - 1, std::max(0, atoi(e)));
+    if (g.flux_protection) {
+        // flux_protection of vanleer_mod.f90 (:81): Roe with HLLE protection fluxes, FP64
+        if (g.integrator != IMHD_VL || g.riemann != IMHD_ROE || g.precision != 64) {
+            int r = bad("flux_protection needs the VL integrator, the Roe solver and precision 64");
+            *out = c;
+            return r;
+        }
+        c->prot = true;
+        c->vstage = false;   // the protected update works on the stored flux arrays
+        // test knob: a higher density threshold marks more cells (the reference uses MIN_DENS)
+        if (const char* e = getenv("IMHD_PROT_DENS")) c->prot_thr = atof(e);
+    }
     long long run = 0;
     for (int bx = 0; bx < nbox; bx++) {
         const imhd_config& k = cfg[bx];
@@ -1631,6 +1706,13 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     }
     if (g.precision == 32 && g.compensated)
         for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->comp[f], nr))) return rc;
+    if (c->prot) {
+        for (int d = 0; d < 3; d++)
+            for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->Fpr[d][f], nd))) return rc;
+        for (int e = 0; e < 6; e++) if ((rc = alloc(c, &c->Epr[e], nd))) return rc;
+        for (int d = 0; d < 3; d++) if ((rc = alloc(c, (void**)&c->Ecpr[d], nd))) return rc;
+        if ((rc = alloc(c, (void**)&c->pflag, (size_t)run))) return rc;
+    }
     if (g.integrator == IMHD_TVD) {
         for (int d = 0; d < 3; d++) {
             for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->Fc[d][f], nr))) return rc;
@@ -1727,6 +1809,10 @@ int imhd_destroy(imhd_ctx* c) {
     for (auto p : c->Ecc) fr(p);
     for (auto p : c->gt) fr(p);
     for (auto p : c->ck) fr(p);
+    for (auto& a : c->Fpr) for (auto p : a) fr(p);
+    for (auto p : c->Epr) fr(p);
+    for (auto p : c->Ecpr) fr(p);
+    fr(c->pflag);
     for (auto p : c->delv) fr(p);
     for (void* p : {(void*)c->ou_n, (void*)c->ou_are, (void*)c->ou_aim, (void*)c->ou_step, (void*)c->ou_tw,
                     (void*)c->ou_twoff, (void*)c->ou_part, (void*)c->ou_mean, (void*)c->ou_col,

@@ -12,7 +12,8 @@ MODULE imhd_api
   INTEGER(C_INT), PARAMETER :: IMHD_BC_OPEN = 1, IMHD_BC_PERIODIC = 2
 
   TYPE, BIND(C) :: imhd_config
-    INTEGER(C_INT) :: nx, ny, nz, nb, npass, integrator, riemann, recon, bc, precision, compensated, reserved
+    INTEGER(C_INT) :: nx, ny, nz, nb, npass, integrator, riemann, recon, bc, precision, compensated, &
+                      flux_protection
     REAL(C_DOUBLE) :: cs, mc_beta, cour
   END TYPE imhd_config
 

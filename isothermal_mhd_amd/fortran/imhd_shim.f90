@@ -26,7 +26,8 @@ MODULE imhd_shim_core
   IMPLICIT NONE
   PRIVATE
   PUBLIC :: shim_create, shim_destroy, shim_upload, shim_download, shim_check, shim_ctx, &
-            imhd_shim_host_modified, imhd_shim_recon, imhd_shim_precision, shim_force_on_device
+            imhd_shim_host_modified, imhd_shim_recon, imhd_shim_precision, shim_force_on_device, &
+            imhd_shim_flux_protection
 
   TYPE(C_PTR), SAVE :: shim_ctx = C_NULL_PTR
   LOGICAL, SAVE :: host_dirty = .TRUE.
@@ -36,6 +37,8 @@ MODULE imhd_shim_core
   ! 64: FP64 throughout (bitwise with the reference); 32: FP32 bulk with FP64 face B,
   ! corner EMF, CFL and diagnostics, Kahan-compensated rho and rho v
   INTEGER(C_INT), SAVE :: imhd_shim_precision = 64
+  ! 1: the reference's flux_protection (vanleer_mod.f90:81, a PARAMETER that is .FALSE. there)
+  INTEGER(C_INT), SAVE :: imhd_shim_flux_protection = 0
   ! set by the turb_driver_mod shim (imhd_shim_turb.f90) once the force field is generated
   ! on the device; the host delv_re is then not uploaded
   LOGICAL, SAVE :: shim_force_on_device = .FALSE.
@@ -67,7 +70,7 @@ CONTAINS
     cfg%bc = boundary_type
     cfg%precision = imhd_shim_precision
     cfg%compensated = MERGE(1, 0, imhd_shim_precision .EQ. 32)
-    cfg%reserved = 0
+    cfg%flux_protection = imhd_shim_flux_protection
     cfg%cs = sound_speed
     cfg%mc_beta = 1.01d0
     cfg%cour = COUR_def

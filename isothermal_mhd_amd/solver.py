@@ -40,7 +40,7 @@ class VanLeerSolver:
     integrator = L.IMHD_VL
 
     def __init__(self, dims, nb=6, npass=0, riemann="hlld", recon="plm", bc="periodic",
-                 precision=64, compensated=None, cs=1.0, mc_beta=1.01, cour=0.35):
+                 precision=64, compensated=None, cs=1.0, mc_beta=1.01, cour=0.35, flux_protection=False):
         self.lib = L.lib()
         boxes = [tuple(dims)] if np.ndim(dims) == 1 else [tuple(d) for d in dims]
         if recon not in _RECON:
@@ -53,6 +53,7 @@ class VanLeerSolver:
             c.nx, c.ny, c.nz, c.nb, c.npass = nx, ny, nz, nb, npass
             c.integrator, c.riemann, c.recon = self.integrator, _RIEMANN[riemann], _RECON[recon]
             c.bc, c.precision, c.compensated = _BC[bc], precision, int(bool(compensated))
+            c.flux_protection = int(bool(flux_protection))
             c.cs, c.mc_beta = cs, mc_beta
             c.cour = cour[i] if np.ndim(cour) else cour
         self.boxes, self.nb, self.npass, self.precision = boxes, nb, npass, precision

@@ -33,6 +33,10 @@ struct orc_ctx {
     double *Fs;            /* Fs(.,.,.,7,3) */
     double *emf;           /* emf_3d(.,.,.,3,3) */
     double *Fbuf;          /* TVD: Fs_buff(.,.,.,7,3) */
+    /* flux protection (vanleer_mod.f90:93-110): Fs_prot, emf_3d_prot, emf_corner_prot,
+     * q_3d_prot, protection_flags */
+    double *Fsp, *emfp, *ecp, *qprot;
+    unsigned char *pflag;
     double smax, delv_max, dx, dt;
     int floor_count;
     /* passive (passive_mod.f90:77-84) */
@@ -64,6 +68,9 @@ static inline long cidx(const orc_ctx *c, int i, int j, int k) {
 #define FS(comp, dir, i, j, k) c->Fs[((long)(dir) * 7 + (comp)) * c->NC + cidx(c, i, j, k)]
 #define FB(comp, dir, i, j, k) c->Fbuf[((long)(dir) * 7 + (comp)) * c->NC + cidx(c, i, j, k)]
 #define EMF(comp, slot, i, j, k) c->emf[((long)(slot) * 3 + (comp)) * c->NC + cidx(c, i, j, k)]
+#define FSP(comp, dir, i, j, k) c->Fsp[((long)(dir) * 7 + (comp)) * c->NC + cidx(c, i, j, k)]
+#define EMFP(comp, slot, i, j, k) c->emfp[((long)(slot) * 3 + (comp)) * c->NC + cidx(c, i, j, k)]
+#define ECP(f, i, j, k) c->ecp[(long)(f) * c->NC + cidx(c, i, j, k)]
 
 static inline long pidx(const orc_ctx *c, int i, int j, int k) {
     int nb = c->c.nb;
@@ -615,6 +622,21 @@ static void vl_sweep(orc_ctx *c, int d, int offset, int corrector) {
                 else if (d == 1) { EMF(0, 0, i, j, k) = -FS(6, 1, i, j, k); EMF(2, 1, i, j, k) = FS(4, 1, i, j, k); }
                 else { EMF(1, 1, i, j, k) = -FS(4, 2, i, j, k); EMF(0, 1, i, j, k) = FS(5, 2, i, j, k); }
             }
+            if (c->Fsp) {
+                /* protection fluxes: compute_HLLE_fluxes_1d on the same interface states, after
+                 * the Roe fluxes of the pencil (vanleer_mod.f90:540, 560-573, 703, 731-753, 880,
+                 * 908-930); it raises smax too */
+                for (int n = ilo; n <= ihi; n++) {
+                    double F[7];
+                    hlle_if(qL + n * 7, qR + n * 7, c->c.cs * c->c.cs, F, &c->smax);
+                    int i, j, k;
+                    dir_cell(d, n0 + n, t1, t2, &i, &j, &k);
+                    for (int r = 0; r < 7; r++) FSP(perm[r], d, i, j, k) = F[r];
+                    if (d == 0) { EMFP(2, 0, i, j, k) = -FSP(5, 0, i, j, k); EMFP(1, 0, i, j, k) = FSP(6, 0, i, j, k); }
+                    else if (d == 1) { EMFP(0, 0, i, j, k) = -FSP(6, 1, i, j, k); EMFP(2, 1, i, j, k) = FSP(4, 1, i, j, k); }
+                    else { EMFP(1, 1, i, j, k) = -FSP(4, 2, i, j, k); EMFP(0, 1, i, j, k) = FSP(5, 2, i, j, k); }
+                }
+            }
         }
     }
     free(w); free(wb); free(qL); free(qR); free(delw); free(wLa); free(wRa); free(Rv); free(Lv);
@@ -630,6 +652,8 @@ static void vl_center_emf(orc_ctx *c, int off) {
                 EMF(0, 2, i, j, k) = (Q(5, i, j, k) * Q(3, i, j, k) - Q(6, i, j, k) * Q(2, i, j, k)) * irho;
                 EMF(1, 2, i, j, k) = (Q(6, i, j, k) * Q(1, i, j, k) - Q(4, i, j, k) * Q(3, i, j, k)) * irho;
                 EMF(2, 2, i, j, k) = (Q(4, i, j, k) * Q(2, i, j, k) - Q(5, i, j, k) * Q(1, i, j, k)) * irho;
+                if (c->emfp)   /* vanleer_mod.f90:1848-1853 */
+                    for (int m = 0; m < 3; m++) EMFP(m, 2, i, j, k) = EMF(m, 2, i, j, k);
             }
 }
 
@@ -642,10 +666,12 @@ static inline void upw(double v, double L14, double R14, double L34, double R34,
 
 /* compute_cell_corner_emf_3d (vanleer_mod.f90:1884-2076): Gardiner-Stone upwind corner EMF
  * from face EMFs E(comp, slot) and the state q (used for the upwinding velocities). */
-static void corner_emf(orc_ctx *c, int off, const double *qsrc) {
+static void corner_emf_arr(orc_ctx *c, int off, const double *qsrc, const double *emf_in, double *ec_out) {
     int nb = c->c.nb;
 #define QS(f, i, j, k) qsrc[(long)(f) * c->NC + cidx(c, i, j, k)]
-#define E(comp, slot, i, j, k) EMF(comp, slot, i, j, k)
+#define E(comp, slot, i, j, k) emf_in[((long)(slot) * 3 + (comp)) * c->NC + cidx(c, i, j, k)]
+#undef EC
+#define EC(f, i, j, k) ec_out[(long)(f) * c->NC + cidx(c, i, j, k)]
     for (int k = 1 + off - nb; k <= c->c.nz - off + nb - 1; k++)
         for (int j = 1 + off - nb; j <= c->c.ny - off + nb - 1; j++)
             for (int i = 1 + off - nb; i <= c->c.nx - off + nb - 1; i++) {
@@ -696,6 +722,13 @@ static void corner_emf(orc_ctx *c, int off, const double *qsrc) {
             }
 #undef QS
 #undef E
+#undef EC
+#define EC(f, i, j, k) c->ec[(long)(f) * c->NC + cidx(c, i, j, k)]
+}
+
+static void corner_emf(orc_ctx *c, int off, const double *qsrc) {
+    corner_emf_arr(c, off, qsrc, c->emf, c->ec);
+    if (c->ecp) corner_emf_arr(c, off, qsrc, c->emfp, c->ecp);   /* vanleer_mod.f90:336, 396 */
 }
 
 /* final_check_unphysical (vanleer_mod.f90:3243-3262) */
@@ -706,12 +739,23 @@ static void floor_check(orc_ctx *c, int ilo, int ihi, int jlo, int jhi, int klo,
                 if (Q(0, i, j, k) < MIN_DENS) { Q(0, i, j, k) = MIN_DENS; c->floor_count++; }
 }
 
-/* update_hydro_state_3d (vanleer_mod.f90:2080-2207), flux_protection off */
+/* update_hydro_state_3d (vanleer_mod.f90:2080-2207).  With flux protection: back up the
+ * state, update, flag cells whose density fell below the threshold, give each flagged cell's
+ * six faces the HLLE fluxes and the corner-EMF slots (i,j,k), (i-1,j,k), (i,j-1,k),
+ * (i,j,k-1) of all three components the protection corner EMFs (the reference's index set,
+ * :2155-2169), then redo the update of every flagged cell and face neighbour from the
+ * backup with the modified fluxes (:2177-2198). */
+#define UPD(f, i, j, k) (FS(f, 0, i, j, k) - FS(f, 0, i - 1, j, k)) + (FS(f, 1, i, j, k) - FS(f, 1, i, j - 1, k)) + \
+                        (FS(f, 2, i, j, k) - FS(f, 2, i, j, k - 1))
 static void vl_update_hydro(orc_ctx *c, double dtdx, int off) {
     int nb = c->c.nb;
     int ilo = 2 + off - nb, ihi = c->c.nx - off + nb - 1;
     int jlo = 2 + off - nb, jhi = c->c.ny - off + nb - 1;
     int klo = 2 + off - nb, khi = c->c.nz - off + nb - 1;
+    if (c->pflag) {
+        memset(c->pflag, 0, (size_t)c->NC);
+        memcpy(c->qprot, c->q, sizeof(double) * 7 * c->NC);
+    }
     for (int k = klo; k <= khi; k++)
         for (int j = jlo; j <= jhi; j++)
             for (int i = ilo; i <= ihi; i++)
@@ -719,8 +763,41 @@ static void vl_update_hydro(orc_ctx *c, double dtdx, int off) {
                     Q(f, i, j, k) = Q(f, i, j, k) - dtdx * ((FS(f, 0, i, j, k) - FS(f, 0, i - 1, j, k)) +
                                                             (FS(f, 1, i, j, k) - FS(f, 1, i, j - 1, k)) +
                                                             (FS(f, 2, i, j, k) - FS(f, 2, i, j, k - 1)));
+    if (c->pflag) {
+        const double thr = c->c.prot_dens > 0.0 ? c->c.prot_dens : MIN_DENS;
+        for (int k = klo; k <= khi; k++)
+            for (int j = jlo; j <= jhi; j++)
+                for (int i = ilo; i <= ihi; i++) {
+                    if (!(Q(0, i, j, k) < thr)) continue;
+                    c->pflag[cidx(c, i, j, k)] = 1;
+                    for (int f = 0; f < 7; f++) {
+                        FS(f, 0, i, j, k) = FSP(f, 0, i, j, k);
+                        FS(f, 0, i - 1, j, k) = FSP(f, 0, i - 1, j, k);
+                        FS(f, 1, i, j, k) = FSP(f, 1, i, j, k);
+                        FS(f, 1, i, j - 1, k) = FSP(f, 1, i, j - 1, k);
+                        FS(f, 2, i, j, k) = FSP(f, 2, i, j, k);
+                        FS(f, 2, i, j, k - 1) = FSP(f, 2, i, j, k - 1);
+                    }
+                    for (int m = 0; m < 3; m++) {
+                        EC(m, i, j, k) = ECP(m, i, j, k);
+                        EC(m, i - 1, j, k) = ECP(m, i - 1, j, k);
+                        EC(m, i, j - 1, k) = ECP(m, i, j - 1, k);
+                        EC(m, i, j, k - 1) = ECP(m, i, j, k - 1);
+                    }
+                }
+#define PF(i, j, k) c->pflag[cidx(c, i, j, k)]
+        for (int k = klo; k <= khi; k++)
+            for (int j = jlo; j <= jhi; j++)
+                for (int i = ilo; i <= ihi; i++)
+                    if (PF(i, j, k) || PF(i - 1, j, k) || PF(i + 1, j, k) || PF(i, j - 1, k) || PF(i, j + 1, k) ||
+                        PF(i, j, k - 1) || PF(i, j, k + 1))
+                        for (int f = 0; f < 4; f++)
+                            Q(f, i, j, k) = c->qprot[(long)f * c->NC + cidx(c, i, j, k)] - dtdx * (UPD(f, i, j, k));
+#undef PF
+    }
     floor_check(c, ilo, ihi, jlo, jhi, klo, khi);
 }
+#undef UPD
 
 /* update_CT_3d (vanleer_mod.f90:2211-2259) */
 static void ct_update(orc_ctx *c, double dtdx, int off) {
@@ -748,6 +825,7 @@ static void vl_step(orc_ctx *c, double dt) {
     memcpy(c->bfw, c->bf, n3);
     memset(c->Fs, 0, sizeof(double) * 21 * c->NC);
     memset(c->emf, 0, sizeof(double) * 9 * c->NC);
+    if (c->Fsp) memset(c->Fsp, 0, sizeof(double) * 21 * c->NC);   /* Fs_prot = 0 (vanleer_mod.f90:294) */
     memset(c->ec, 0, n3);
     c->smax = 0.0;
     c->delv_max = 0.0;
@@ -1077,6 +1155,13 @@ orc_ctx *orc_create(const orc_cfg *cfg) {
     c->Fs = calloc((size_t)21 * c->NC, sizeof(double));
     c->emf = calloc((size_t)9 * c->NC, sizeof(double));
     if (cfg->mhd == 1) c->Fbuf = calloc((size_t)21 * c->NC, sizeof(double));
+    if (cfg->mhd == 2 && cfg->riemann == 1 && cfg->protection) {
+        c->Fsp = calloc((size_t)21 * c->NC, sizeof(double));
+        c->emfp = calloc((size_t)9 * c->NC, sizeof(double));
+        c->ecp = calloc((size_t)3 * c->NC, sizeof(double));
+        c->qprot = calloc((size_t)7 * c->NC, sizeof(double));
+        c->pflag = calloc((size_t)c->NC, 1);
+    }
     int nmax = cfg->nx > cfg->ny ? cfg->nx : cfg->ny;
     if (cfg->nz > nmax) nmax = cfg->nz;
     c->dx = 1.0 / (double)nmax;
@@ -1098,6 +1183,7 @@ void orc_destroy(orc_ctx *c) {
     free(c->q); free(c->qw); free(c->bf); free(c->bfw); free(c->ec); free(c->Fs); free(c->emf); free(c->Fbuf);
     free(c->qp); free(c->qpw); free(c->vel); free(c->vtavg); free(c->Fp);
     free(c->delv); free(c->rho0);
+    free(c->Fsp); free(c->emfp); free(c->ecp); free(c->qprot); free(c->pflag);
     if (c->ou) { for (int d = 0; d < 3; d++) free(c->ou->tw[d]); free(c->ou); }
     free(c);
 }

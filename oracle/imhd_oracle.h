@@ -26,6 +26,9 @@ typedef struct orc_cfg {
     int bc;           /* 1 open (zero-gradient), 2 periodic */
     double cs;        /* isothermal sound speed */
     double mc_beta;   /* MC limiter beta (reference 1.01) */
+    int protection;   /* VL + Roe: flux_protection (vanleer_mod.f90:81, off in the reference) */
+    int _pad;
+    double prot_dens; /* density below which a cell is protected (0: MIN_DENS = 1e-20) */
 } orc_cfg;
 
 typedef struct orc_ctx orc_ctx;

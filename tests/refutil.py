@@ -26,7 +26,8 @@ class OrcCfg(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
                 ("nb", ctypes.c_int), ("npass", ctypes.c_int), ("mhd", ctypes.c_int),
                 ("riemann", ctypes.c_int), ("recon", ctypes.c_int), ("bc", ctypes.c_int),
-                ("cs", ctypes.c_double), ("mc_beta", ctypes.c_double)]
+                ("cs", ctypes.c_double), ("mc_beta", ctypes.c_double),
+                ("protection", ctypes.c_int), ("_pad", ctypes.c_int), ("prot_dens", ctypes.c_double)]
 
 
 class OrcDiag(ctypes.Structure):
@@ -97,10 +98,11 @@ class Oracle:
     """The CPU restatement with the reference's array layout."""
 
     def __init__(self, nx, ny, nz, npass=0, mhd=2, riemann=2, recon=0, bc=2, cs=1.0,
-                 nb=6, mc_beta=1.01):
+                 nb=6, mc_beta=1.01, protection=False, prot_dens=0.0):
         self.L = oracle_lib()
         self.dims = (nx, ny, nz, nb, npass)
-        cfg = OrcCfg(nx, ny, nz, nb, npass, mhd, riemann, recon, bc, cs, mc_beta)
+        cfg = OrcCfg(nx, ny, nz, nb, npass, mhd, riemann, recon, bc, cs, mc_beta, int(bool(protection)), 0,
+                     prot_dens)
         self.c = self.L.orc_create(ctypes.byref(cfg))
         self.sh, self.shp = padded_shapes(nx, ny, nz, nb, npass)
         self.npass = npass

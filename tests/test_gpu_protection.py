@@ -1,0 +1,110 @@
+"""Flux protection of the VL Roe update on the GPU (imhd_config.flux_protection; the
+reference's update_hydro_state_3d with flux_protection, vanleer_mod.f90:2080-2207) against
+the oracle's restatement (pinned in tests/test_oracle_protection.py), FP64 bit for bit:
+a rarefaction that the unprotected Roe update drives below the density floor, partial
+marking at a raised threshold (IMHD_PROT_DENS, a test knob), the untriggered case (equal
+to the unprotected step) and the all-marked case (equal to an HLLE step)."""
+import sys
+
+import numpy as np
+import pytest
+
+from refutil import Oracle, random_state
+
+pytestmark = pytest.mark.gpu
+NB = 6
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rarefaction(n, vx0, seed=0):
+    nx, ny, nz = n
+    q, bf, qp = random_state(nx, ny, nz, npass=0, seed=seed, amp_v=0.1)
+    I = (slice(NB, NB + nx), slice(NB, NB + ny), slice(NB, NB + nz))
+    x = (np.arange(nx) + 0.5) / nx
+    q[I + (1,)] = q[I + (0,)] * np.where(x < 0.5, -vx0, vx0)[:, None, None]
+    return q, bf
+
+
+def gpu(monkeypatch, dims, q, bf, steps, dt, riemann="roe", prot=False, thr=None, recon="plm"):
+    from isothermal_mhd_amd import VanLeerSolver
+    if thr is not None:
+        monkeypatch.setenv("IMHD_PROT_DENS", repr(thr))
+    try:
+        S = VanLeerSolver(dims, npass=0, riemann=riemann, recon=recon, bc="periodic", precision=64,
+                          flux_protection=prot)
+    finally:
+        if thr is not None:
+            monkeypatch.delenv("IMHD_PROT_DENS")
+    S.set_state(q, bf, None)
+    S.boundary_conditions()
+    for _ in range(steps):
+        S.vanleer_solve_3d(dt)
+        S.boundary_conditions()
+    out = S.get_state()[:2], S.floor_count(), S.get_max_speed_vanleer()
+    S.close()
+    return out
+
+
+def orc(dims, q, bf, steps, dt, riemann=1, prot=False, thr=0.0, recon=0):
+    O = Oracle(*dims, npass=0, mhd=2, riemann=riemann, recon=recon, bc=2, protection=prot, prot_dens=thr)
+    O.set_state(q, bf, None)
+    O.bc()
+    for _ in range(steps):
+        O.step(dt)
+        O.bc()
+    return O.get_state()[:2], O.floor_count(), O.max_speed()
+
+
+def same(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def test_rarefaction_protected_equals_oracle(monkeypatch):
+    dims = (16, 8, 8)
+    say("[gpu] protection: rarefaction")
+    q, bf = rarefaction(dims, 10.0)
+    dt = 0.3 / 12.0 / 16
+    g = gpu(monkeypatch, dims, q, bf, 3, dt, prot=True)
+    o = orc(dims, q, bf, 3, dt, prot=True)
+    assert same(g[0], o[0])
+    assert g[1] == o[1] == 0
+    assert g[2] == o[2]
+    u = gpu(monkeypatch, dims, q, bf, 3, dt)      # unprotected: density floor hit
+    assert u[1] > 0
+
+
+@pytest.mark.parametrize("recon", ["plm", "ppm_char"])
+def test_partial_marking_equals_oracle(monkeypatch, recon):
+    dims = (14, 12, 10)
+    say(f"[gpu] protection: partial marking ({recon})")
+    q, bf, _ = random_state(*dims, npass=0, seed=9)
+    thr = 0.95          # some cells of the random density field fall below it
+    g = gpu(monkeypatch, dims, q, bf, 2, 2e-3, prot=True, thr=thr, recon=recon)
+    o = orc(dims, q, bf, 2, 2e-3, prot=True, thr=thr, recon=0 if recon == "plm" else 2)
+    assert same(g[0], o[0])
+    assert g[2] == o[2]
+    ref = gpu(monkeypatch, dims, q, bf, 2, 2e-3, recon=recon)
+    assert not same(g[0], ref[0])    # the marking changed the result
+
+
+def test_untriggered_and_all_marked(monkeypatch):
+    dims = (12, 10, 8)
+    say("[gpu] protection: untriggered / all marked")
+    q, bf, _ = random_state(*dims, npass=0, seed=3)
+    plain = gpu(monkeypatch, dims, q, bf, 2, 2e-3)
+    prot = gpu(monkeypatch, dims, q, bf, 2, 2e-3, prot=True)
+    assert same(plain[0], prot[0])
+    allm = gpu(monkeypatch, dims, q, bf, 2, 2e-3, prot=True, thr=1e300)
+    hlle = gpu(monkeypatch, dims, q, bf, 2, 2e-3, riemann="hlle")
+    assert same(allm[0], hlle[0])
+
+
+def test_protection_config_checks():
+    from isothermal_mhd_amd import ImhdError, VanLeerSolver
+    for kw in (dict(riemann="hlld"), dict(riemann="roe", precision=32)):
+        with pytest.raises(ImhdError, match="IMHD_EINVAL"):
+            VanLeerSolver((8, 8, 8), flux_protection=True, **kw)
