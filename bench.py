@@ -351,7 +351,8 @@ def run_io(n, reps=2):
     turbulence IC): imhd_write_cube (REAL*4 and REAL*8 xyz cube, packed on the device by
     k_pack_cube), imhd_write_restart and imhd_read_restart, to files under $TMPDIR (rates
     include PCIe, the host page cache and the file system); then the device generation of
-    the reference's random force (imhd_generate_force, k_max = 2)."""
+    the reference's random force (imhd_generate_force, k_max = 2) and the analysis
+    program's velocity spectrum and density PDF (imhd_spectrum, imhd_density_pdf)."""
     import tempfile
     from isothermal_mhd_amd import VanLeerSolver, ics
     q, bf, qp = ics.config_c2(n=n)
@@ -389,6 +390,17 @@ def run_io(n, reps=2):
         best = sec if best is None else min(best, sec)
     out["generate_rand_force"] = {"s": best, "k_max": 2}
     print(f"[io] generate_rand_force: {best * 1e3:.1f} ms", flush=True)
+    # in-situ diagnostics of the analysis program: velocity spectrum, density PDF
+    for name, fn in (("spectrum_velocity", lambda: S.spectrum("velocity")), ("density_pdf", lambda: S.density_pdf())):
+        best = None
+        for _ in range(reps):
+            S.sync()
+            t0 = time.perf_counter()
+            fn()
+            sec = time.perf_counter() - t0
+            best = sec if best is None else min(best, sec)
+        out[name] = {"s": best}
+        print(f"[io] {name}: {best * 1e3:.1f} ms", flush=True)
     S.close()
     return out
 

@@ -194,6 +194,22 @@ int imhd_autotune(imhd_ctx* ctx, int steps, char* report, int report_len);
 /* Synchronise the context's stream. */
 int imhd_sync(imhd_ctx* ctx);
 
+/* In-situ diagnostics of the reference's analysis program (3D_MPI/Analysis/src/
+ * analysis_main.f90), formed from the device state instead of snapshot files, in the
+ * program's REAL*4 arithmetic:
+ * imhd_spectrum == compute_pk (:543-633) of field 0 density, 1 velocity (rho v / rho),
+ *   2 cell-centred B, 3 velocity parallel to B, 4 velocity perpendicular to B (:105-226):
+ *   unnormalised 3-D DFT of each component (hipFFT), |F|^2 of the octant
+ *   0 <= kx,ky,kz < n/2 summed over components, doubled (zero mode single), binned into
+ *   n shells of width dk = sqrt(3) 2 pi (n/2) / n, times dx^3.  k[b] = b dk, pk[b]: n
+ *   values each.  Cubic boxes with even n; bins accumulate in FP64.
+ * imhd_density_pdf == compute_density_pdf (:637-698): nbins bins on [0, rho_max) (the
+ *   reference: n bins on [0, 4)), normalised to unit integral over the counted cells;
+ *   rho[b] is the bin centre.  Cells at or above rho_max are not counted (the reference
+ *   writes past its array there). */
+int imhd_spectrum(imhd_ctx* ctx, int box, int field, double* k, double* pk);
+int imhd_density_pdf(imhd_ctx* ctx, int box, int nbins, double rho_max, double* rho, double* pdf);
+
 /* Restart file of one box == write_restart / read_restart (io_mod.f90:440-517): Fortran
  * sequential-unformatted records t_sim, q_3d, bface_3d, qpass_3d (only when npass > 0),
  * dump_count, s1, s2, s3, framed as gfortran frames them (the reference's compiler,

@@ -16,6 +16,7 @@ SOURCES = [("vl_kernels_f64.hip", ["-ffp-contract=off"]),
            ("tvd_kernels_f32.hip", ["-ffp-contract=fast"]),
            ("aux_kernels.hip", ["-ffp-contract=off"]),
            ("forcing_kernels.hip", ["-ffp-contract=off"]),
+           ("spectra_kernels.hip", ["-ffp-contract=off"]),
            ("imhd_api.cpp", ["-ffp-contract=off"])]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # FP64 sources: -ffp-contract=off (no FMA contraction), so the FP64 path reproduces the
@@ -46,7 +47,8 @@ def build(verbose=False, jobs=4):
             sys.stderr.write(f"--- {s}\n{txt}\n")
     if failed:
         raise SystemExit("libimhd build failed")
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT] + objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT] + objs + ["-L/opt/rocm/lib", "-lhipfft",
+                                                                            "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise SystemExit("link failed:\n" + r.stdout + r.stderr)

@@ -12,8 +12,11 @@
 #include <string>
 #include <vector>
 
+#include <hipfft/hipfft.h>
+
 #include "../../include/imhd.h"
 #include "aux_kernels.hpp"
+#include "spectra.hpp"
 #include "forcing.hpp"
 #include "kernels.hpp"
 #include "tvd_kernels.hpp"
@@ -1603,6 +1606,84 @@ static int generate_force(imhd_ctx* c, int bx, int K, int* seeds) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------------
+// Power spectra and density PDF of the reference's analysis program (3D_MPI/Analysis/src/
+// analysis_main.f90:543-698), in situ: REAL*4 field components packed from the device
+// state, unnormalised 3-D R2C transforms (hipFFT), octant |F|^2 summed over components in
+// REAL*4, binned on the device.
+
+struct FftPlan {
+    hipfftHandle h = 0;
+    bool ok = false;
+    ~FftPlan() {
+        if (ok) (void)hipfftDestroy(h);
+    }
+};
+
+template <class real>
+static int spectrum_t(imhd_ctx* c, int bx, int which, double* k_out, double* pk_out) {
+    const BoxDesc& B = c->hb[bx];
+    const int n = B.n[0], h = n / 2;
+    const long long nc = (long long)n * n * n, nk = (long long)n * n * (h + 1), no = (long long)h * h * h;
+    DevBuf fin, fout, acc, pk;
+    HIPCK(c, hipMalloc(&fin.p, (size_t)nc * 4));
+    HIPCK(c, hipMalloc(&fout.p, (size_t)nk * 8));
+    HIPCK(c, hipMalloc(&acc.p, (size_t)no * 4));
+    HIPCK(c, hipMalloc(&pk.p, (size_t)n * 8));
+    FftPlan plan;
+    if (hipfftPlan3d(&plan.h, n, n, n, HIPFFT_R2C) != HIPFFT_SUCCESS) FAIL(c, IMHD_EHIP, "hipfftPlan3d failed");
+    plan.ok = true;
+    if (hipfftSetStream(plan.h, c->st) != HIPFFT_SUCCESS) FAIL(c, IMHD_EHIP, "hipfftSetStream failed");
+    SpecPackArgs<real> a{};
+    for (int f = 0; f < 7; f++) a.q[f] = R<real>(c->q[c->cur][f]);
+    a.which = which;
+    a.out = (float*)fin.p;
+    const int ncomp = which == SPEC_DENSITY ? 1 : 3;
+    for (int comp = 0; comp < ncomp; comp++) {
+        a.comp = comp;
+        launch_spec_pack<real>(c->st, B, a);
+        if (hipfftExecR2C(plan.h, (hipfftReal*)fin.p, (hipfftComplex*)fout.p) != HIPFFT_SUCCESS)
+            FAIL(c, IMHD_EHIP, "hipfftExecR2C failed");
+        launch_spec_accum(c->st, (const float*)fout.p, n, (float*)acc.p, comp == 0);
+    }
+    // compute_pk's bin geometry in REAL*4 (analysis_main.f90:573-579, 585-593)
+    const float k0 = (float)(8.0 * atan(1.0));
+    const float kmax = (float)(sqrt(3.0) * (double)k0 * (double)h);
+    const float dk = kmax / (float)n;
+    launch_reset_u64(c->st, (unsigned long long*)pk.p, n);
+    launch_spec_bin(c->st, (const float*)acc.p, n, k0, dk, pk.p);
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipStreamSynchronize(c->st));
+    HIPCK(c, hipMemcpy(pk_out, pk.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    const double dx = 1.0 / (double)n;
+    for (int b = 0; b < n; b++) {
+        pk_out[b] *= dx * dx * dx;
+        k_out[b] = (double)b * (double)dk;
+    }
+    return 0;
+}
+
+template <class real>
+static int density_pdf_t(imhd_ctx* c, int bx, int nbins, double rho_max, double* rho_out, double* pdf_out) {
+    const BoxDesc& B = c->hb[bx];
+    DevBuf cnt;
+    HIPCK(c, hipMalloc(&cnt.p, (size_t)nbins * 8));
+    launch_reset_u64(c->st, (unsigned long long*)cnt.p, nbins);
+    const float drho = (float)rho_max / (float)nbins;
+    launch_density_hist<real>(c->st, B, R<real>(c->q[c->cur][0]), drho, nbins, (unsigned long long*)cnt.p);
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipStreamSynchronize(c->st));
+    std::vector<unsigned long long> h((size_t)nbins);
+    HIPCK(c, hipMemcpy(h.data(), cnt.p, (size_t)nbins * 8, hipMemcpyDeviceToHost));
+    unsigned long long tot = 0;
+    for (auto v : h) tot += v;
+    for (int b = 0; b < nbins; b++) {
+        pdf_out[b] = tot ? (double)h[b] / ((double)drho * (double)tot) : 0.0;
+        rho_out[b] = ((double)b + 0.5) * (double)drho;
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2151,6 +2232,26 @@ int imhd_get_force(imhd_ctx* c, int box, double* delv) {
     for (int d = 0; d < 3; d++)
         HIPCK(c, hipMemcpy(delv + d * nc, c->delv[d] + B.off, (size_t)nc * 8, hipMemcpyDeviceToHost));
     return IMHD_OK;
+}
+
+int imhd_spectrum(imhd_ctx* c, int box, int field, double* k, double* pk) {
+    if (!c || !k || !pk || box < 0 || box >= c->nbox) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    const BoxDesc& B = c->hb[box];
+    if (B.n[0] != B.n[1] || B.n[0] != B.n[2] || (B.n[0] & 1))
+        FAIL(c, IMHD_EINVAL, "spectra need a cubic box with an even number of cells per axis");
+    if (field < 0 || field > 4) FAIL(c, IMHD_EINVAL, "field must be 0 (density) .. 4 (v_perp)");
+    if (B.n[0] > 4096) FAIL(c, IMHD_EINVAL, "spectra: at most 4096 cells per axis");
+    HIPCK(c, hipStreamSynchronize(c->st));
+    return c->prec == 64 ? spectrum_t<double>(c, box, field, k, pk) : spectrum_t<float>(c, box, field, k, pk);
+}
+
+int imhd_density_pdf(imhd_ctx* c, int box, int nbins, double rho_max, double* rho, double* pdf) {
+    if (!c || !rho || !pdf || box < 0 || box >= c->nbox || nbins < 1 || !(rho_max > 0.0)) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    HIPCK(c, hipStreamSynchronize(c->st));
+    return c->prec == 64 ? density_pdf_t<double>(c, box, nbins, rho_max, rho, pdf)
+                         : density_pdf_t<float>(c, box, nbins, rho_max, rho, pdf);
 }
 
 int imhd_fill_ghosts(imhd_ctx* c) {

@@ -164,6 +164,27 @@ class VanLeerSolver:
         self._check(self.lib.imhd_get_force(self._c, box, _p(delv)))
         return delv
 
+    _SPECTRA = {"density": 0, "velocity": 1, "bfield": 2, "vpar": 3, "vperp": 4}
+
+    def spectrum(self, field="velocity", box=0):
+        """compute_pk of the reference's analysis program (analysis_main.f90:543-633) on the
+        device state: field 'density', 'velocity', 'bfield', 'vpar' or 'vperp'; returns
+        (k, Pk), n shells each (cubic boxes)."""
+        n = self.boxes[box][0]
+        k = np.zeros(n)
+        pk = np.zeros(n)
+        self._check(self.lib.imhd_spectrum(self._c, box, self._SPECTRA[field], _p(k), _p(pk)))
+        return k, pk
+
+    def density_pdf(self, nbins=None, rho_max=4.0, box=0):
+        """compute_density_pdf (analysis_main.f90:637-698): (bin centres, PDF); the
+        reference uses nbins = n on [0, 4)."""
+        nbins = int(nbins or min(self.boxes[box]))
+        r = np.zeros(nbins)
+        pdf = np.zeros(nbins)
+        self._check(self.lib.imhd_density_pdf(self._c, box, nbins, float(rho_max), _p(r), _p(pdf)))
+        return r, pdf
+
     def mhd_update(self, dt):
         """driver.f90:378-389: the configured integrator's solve on every box."""
         self.vanleer_solve_3d(dt)

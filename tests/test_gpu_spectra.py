@@ -1,0 +1,83 @@
+"""Device spectra and density PDF (imhd_spectrum, imhd_density_pdf) against the analysis
+program's restatement (oracle/spectra.py, pinned in tests/test_spectra.py).  Tolerance:
+the device transform is hipFFT in FP32 and the oracle's numpy in FP64 (the reference uses
+its own REAL*4 FFT), so each shell agrees to 2e-5 of the largest shell plus 1e-4 of its
+own value; the bin rules are the same REAL*4 arithmetic, checked exactly by single-mode
+fields.  The PDF's counts are exact."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from refutil import random_state
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "oracle"))
+import spectra as SP  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+FIELDS = ["density", "velocity", "bfield", "vpar", "vperp"]
+
+
+def solver(n, precision=64):
+    from isothermal_mhd_amd import VanLeerSolver
+    return VanLeerSolver((n, n, n), npass=0, precision=precision)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_spectra_match_oracle(precision):
+    n = 32
+    print(f"[gpu] spectra n={n} fp{precision}", flush=True)
+    q, bf, _ = random_state(n, n, n, npass=0, seed=17, amp_v=0.6)
+    S = solver(n, precision)
+    S.set_state(q, bf, None)
+    qs = S.get_state()[0]          # the state as the context holds it (FP32 rounded)
+    for which, name in enumerate(FIELDS):
+        k, pk = S.spectrum(name)
+        ko, po = SP.spectrum(qs, (n, n, n), which)
+        assert np.array_equal(k, ko)
+        tol = 2e-5 * np.max(np.abs(po)) + 1e-4 * np.abs(po)
+        assert np.all(np.abs(pk - po) <= tol), name
+    S.close()
+
+
+def test_single_mode_bins_exact():
+    n = 16
+    x = (np.arange(n) + 0.5) / n
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    from test_spectra import cube
+    S = solver(n)
+    for kx0 in (1, 3, 5, 7):
+        q = cube(n, 1.0 + 0.25 * np.cos(2 * np.pi * kx0 * X), b=(0 * X, 0 * X, 1 + 0 * X))
+        bf = np.zeros(q.shape[:3] + (3,), order="F")
+        bf[..., 2] = q[..., 6]
+        S.set_state(q, bf, None)
+        k, pk = S.spectrum("density")
+        ik = int(2 * kx0 / np.sqrt(3.0))
+        assert pk[ik] == pytest.approx(0.25 ** 2 * n ** 3 / 2, rel=1e-4)
+        assert np.max(np.abs(np.delete(pk, [0, ik]))) <= 1e-5 * pk[ik]
+    S.close()
+
+
+def test_density_pdf_matches_oracle():
+    n = 24
+    q, bf, _ = random_state(n, n, n, npass=0, seed=4, amp_rho=0.8)
+    S = solver(n, 32)
+    S.set_state(q, bf, None)
+    qs = S.get_state()[0]
+    for nbins, rmax in ((n, 4.0), (50, 2.0)):
+        r, pdf = S.density_pdf(nbins, rmax)
+        ro, po = SP.density_pdf(qs, (n, n, n), nbins, rmax)
+        assert np.array_equal(r, ro) and np.array_equal(pdf, po)
+    S.close()
+
+
+def test_spectrum_rejects_non_cubic():
+    from isothermal_mhd_amd import ImhdError, VanLeerSolver
+    S = VanLeerSolver((16, 16, 8), npass=0)
+    q, bf, _ = random_state(16, 16, 8, npass=0, seed=1)
+    S.set_state(q, bf, None)
+    with pytest.raises(ImhdError, match="IMHD_EINVAL"):
+        S.spectrum("velocity")
+    S.close()
