@@ -209,6 +209,13 @@ int imhd_sync(imhd_ctx* ctx);
  *   writes past its array there). */
 int imhd_spectrum(imhd_ctx* ctx, int box, int field, double* k, double* pk);
 int imhd_density_pdf(imhd_ctx* ctx, int box, int nbins, double rho_max, double* rho, double* pdf);
+/* == mode_decomposition (:305-540, Cho & Lazarian 2003): per octant mode k != 0 the
+ * velocity transforms are projected onto the Alfven, fast and slow displacement vectors
+ * about the mean field B0; P[m*n + b], m = 0..7: v_Alfven, v_fast, v_slow, b_Alfven,
+ * b_fast, b_slow, rho_fast, rho_slow, shells as imhd_spectrum.  As the analysis program
+ * calls it (:249), the transforms projected are those of v_perp.  Geometry in FP64; slow
+ * amplitudes with a zero slow speed (0/0 in the reference) are not counted. */
+int imhd_mode_spectra(imhd_ctx* ctx, int box, double* k, double* P);
 
 /* Restart file of one box == write_restart / read_restart (io_mod.f90:440-517): Fortran
  * sequential-unformatted records t_sim, q_3d, bface_3d, qpass_3d (only when npass > 0),

@@ -20,7 +20,7 @@ EXPORTS = [
     "imhd_diagnostics", "imhd_floor_count", "imhd_device_bytes", "imhd_run", "imhd_set_time", "imhd_get_time",
     "imhd_set_profiling", "imhd_profile_report", "imhd_sync", "imhd_last_error", "imhd_version",
     "imhd_write_restart", "imhd_read_restart", "imhd_write_cube", "imhd_generate_force", "imhd_get_force",
-    "imhd_spectrum", "imhd_density_pdf",
+    "imhd_spectrum", "imhd_density_pdf", "imhd_mode_spectra",
 ]
 
 
@@ -92,6 +92,7 @@ def lib():
     L.imhd_get_force.argtypes = [vp, ip, vp]
     L.imhd_spectrum.argtypes = [vp, ip, ip, vp, vp]
     L.imhd_density_pdf.argtypes = [vp, ip, ip, dp, vp, vp]
+    L.imhd_mode_spectra.argtypes = [vp, ip, vp, vp]
     L.imhd_last_error.argtypes = [vp]
     L.imhd_last_error.restype = ctypes.c_char_p
     L.imhd_version.restype = ctypes.c_char_p

@@ -1663,6 +1663,48 @@ static int spectrum_t(imhd_ctx* c, int bx, int which, double* k_out, double* pk_
     return 0;
 }
 
+// mode_decomposition (analysis_main.f90:305-540): Alfven / fast / slow spectra of the v_perp
+// transforms (the velocity transforms the program holds when it calls it, :238-249)
+template <class real>
+static int mode_spectra_t(imhd_ctx* c, int bx, double* k_out, double* P_out) {
+    const BoxDesc& B = c->hb[bx];
+    const int n = B.n[0], h = n / 2;
+    const long long nc = (long long)n * n * n, nk = (long long)n * n * (h + 1);
+    DevBuf fin, f[3], P, b0;
+    HIPCK(c, hipMalloc(&fin.p, (size_t)nc * 4));
+    for (int d = 0; d < 3; d++) HIPCK(c, hipMalloc(&f[d].p, (size_t)nk * 8));
+    HIPCK(c, hipMalloc(&P.p, (size_t)8 * n * 8));
+    HIPCK(c, hipMalloc(&b0.p, 3 * 8));
+    FftPlan plan;
+    if (hipfftPlan3d(&plan.h, n, n, n, HIPFFT_R2C) != HIPFFT_SUCCESS) FAIL(c, IMHD_EHIP, "hipfftPlan3d failed");
+    plan.ok = true;
+    if (hipfftSetStream(plan.h, c->st) != HIPFFT_SUCCESS) FAIL(c, IMHD_EHIP, "hipfftSetStream failed");
+    SpecPackArgs<real> a{};
+    for (int q = 0; q < 7; q++) a.q[q] = R<real>(c->q[c->cur][q]);
+    a.which = SPEC_VPERP;
+    a.out = (float*)fin.p;
+    for (int d = 0; d < 3; d++) {
+        a.comp = d;
+        launch_spec_pack<real>(c->st, B, a);
+        if (hipfftExecR2C(plan.h, (hipfftReal*)fin.p, (hipfftComplex*)f[d].p) != HIPFFT_SUCCESS)
+            FAIL(c, IMHD_EHIP, "hipfftExecR2C failed");
+    }
+    launch_reset_u64(c->st, (unsigned long long*)b0.p, 3);
+    launch_mean_b<real>(c->st, B, R<real>(c->q[c->cur][4]), R<real>(c->q[c->cur][5]), R<real>(c->q[c->cur][6]), b0.p);
+    const float k0 = (float)(8.0 * atan(1.0));
+    const float kmax = (float)(sqrt(3.0) * (double)k0 * (double)h);
+    const float dk = kmax / (float)n;
+    launch_reset_u64(c->st, (unsigned long long*)P.p, 8 * n);
+    launch_mode_bin(c->st, (const float*)f[0].p, (const float*)f[1].p, (const float*)f[2].p, n, b0.p, k0, dk, P.p);
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipStreamSynchronize(c->st));
+    HIPCK(c, hipMemcpy(P_out, P.p, (size_t)8 * n * 8, hipMemcpyDeviceToHost));
+    const double dx3 = 1.0 / ((double)n * n * n);
+    for (int m = 0; m < 8 * n; m++) P_out[m] *= dx3;
+    for (int b = 0; b < n; b++) k_out[b] = (double)b * (double)dk;
+    return 0;
+}
+
 template <class real>
 static int density_pdf_t(imhd_ctx* c, int bx, int nbins, double rho_max, double* rho_out, double* pdf_out) {
     const BoxDesc& B = c->hb[bx];
@@ -2244,6 +2286,16 @@ int imhd_spectrum(imhd_ctx* c, int box, int field, double* k, double* pk) {
     if (B.n[0] > 4096) FAIL(c, IMHD_EINVAL, "spectra: at most 4096 cells per axis");
     HIPCK(c, hipStreamSynchronize(c->st));
     return c->prec == 64 ? spectrum_t<double>(c, box, field, k, pk) : spectrum_t<float>(c, box, field, k, pk);
+}
+
+int imhd_mode_spectra(imhd_ctx* c, int box, double* k, double* P) {
+    if (!c || !k || !P || box < 0 || box >= c->nbox) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    const BoxDesc& B = c->hb[box];
+    if (B.n[0] != B.n[1] || B.n[0] != B.n[2] || (B.n[0] & 1))
+        FAIL(c, IMHD_EINVAL, "spectra need a cubic box with an even number of cells per axis");
+    HIPCK(c, hipStreamSynchronize(c->st));
+    return c->prec == 64 ? mode_spectra_t<double>(c, box, k, P) : mode_spectra_t<float>(c, box, k, P);
 }
 
 int imhd_density_pdf(imhd_ctx* c, int box, int nbins, double rho_max, double* rho, double* pdf) {

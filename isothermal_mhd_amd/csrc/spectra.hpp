@@ -24,6 +24,13 @@ void launch_spec_accum(hipStream_t st, const float* F, int n, float* s, int firs
 // bins of compute_pk: doubled sums (zero mode single), k = k0 sqrt(kx^2+ky^2+kz^2) in
 // REAL*4, bin INT(k / dk); pk accumulates in FP64
 void launch_spec_bin(hipStream_t st, const float* s, int n, float k0, float dk, double* pk);
+// Mean cell-centred field of the interior, summed in FP64 into sum[3] (zeroed by the caller)
+template <class real>
+void launch_mean_b(hipStream_t st, const BoxDesc& B, const real* bx, const real* by, const real* bz, double* sum);
+// mode_decomposition: per octant mode k != 0, the v_perp transforms F[3] projected onto the
+// Alfven / fast / slow displacement vectors about B0 (b0sum / n^3); P holds 8 x n shells
+void launch_mode_bin(hipStream_t st, const float* F0, const float* F1, const float* F2, int n, const double* b0sum,
+                     float k0, float dk, double* P);
 template <class real>
 void launch_density_hist(hipStream_t st, const BoxDesc& B, const real* rho, float drho, int nbins,
                          unsigned long long* counts);

@@ -176,6 +176,16 @@ class VanLeerSolver:
         self._check(self.lib.imhd_spectrum(self._c, box, self._SPECTRA[field], _p(k), _p(pk)))
         return k, pk
 
+    def mode_spectra(self, box=0):
+        """mode_decomposition of the analysis program (analysis_main.f90:305-540): returns
+        (k, P), P of shape (8, n): v_Alfven, v_fast, v_slow, b_Alfven, b_fast, b_slow,
+        rho_fast, rho_slow."""
+        n = self.boxes[box][0]
+        k = np.zeros(n)
+        P = np.zeros(8 * n)
+        self._check(self.lib.imhd_mode_spectra(self._c, box, _p(k), _p(P)))
+        return k, P.reshape(8, n)
+
     def density_pdf(self, nbins=None, rho_max=4.0, box=0):
         """compute_density_pdf (analysis_main.f90:637-698): (bin centres, PDF); the
         reference uses nbins = n on [0, 4)."""

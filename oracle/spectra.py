@@ -89,3 +89,83 @@ def density_pdf(q, n, nbins=None, rho_max=4.0, nb=6):
     cnt = np.bincount(ib[ok].ravel(), minlength=nbins)[:nbins].astype(np.float64)
     pdf = cnt / (float(drho) * cnt.sum())
     return (np.arange(nbins) + 0.5) * float(drho), pdf
+
+
+def mode_spectra(q, n, nb=6):
+    """mode_decomposition (analysis_main.f90:305-540): Alfven / fast / slow spectra after
+    Cho & Lazarian (2003).  As the analysis program calls it (:249), the velocity
+    transforms it projects are those of v_perp (the last ones it computed, :238-247).
+    B0 is the mean cell-centred field.  Per octant mode k != 0 (REAL*4, double constants
+    promoted as written): displacement vectors zeta_a = khat_perp x khat_par,
+    zeta_f,s ~ (-1 + alpha +- sqrt D) k_par + (1 + alpha +- sqrt D) k_perp with
+    alpha = 1/|B0|^2, D = max(0, (1+alpha)^2 - 4 alpha |k_par|^2/|k|^2); velocity, field and
+    density amplitudes and their doubled squares binned as in compute_pk.  Returns
+    (k, P) with P of shape (8, n): v_a, v_f, v_s, b_a, b_f, b_s, rho_f, rho_s."""
+    nx, ny, nz = n
+    assert nx == ny == nz
+    nm = nx
+    h = nm // 2
+    comps = field_components(q, n, 4, nb)
+    F = [np.fft.fftn(c.astype(np.float64))[:h, :h, :h] for c in comps]
+    vre = [f.real for f in F]
+    vim = [f.imag for f in F]
+    b = interior_f32(q, n, nb)[4:7]
+    B0 = np.array([float(np.sum(b[d].astype(np.float64))) / nm ** 3 for d in range(3)])
+    bmag = np.sqrt(np.sum(B0 ** 2))
+    alpha = 1.0 / np.sum(B0 ** 2)
+    k0 = float(np.float32(2.0 * np.pi))
+    kk = np.arange(h)
+    KX, KY, KZ = np.meshgrid(kk, kk, kk, indexing="ij")
+    k = k0 * np.sqrt((KX ** 2 + KY ** 2 + KZ ** 2).astype(np.float64))
+    kv = [k0 * KX, k0 * KY, k0 * KZ]
+    ksafe = np.where(k > 0, k, 1.0)
+    khat = [c / ksafe for c in kv]
+    ph = B0 / bmag
+    kdot = kv[0] * ph[0] + kv[1] * ph[1] + kv[2] * ph[2]
+    kpar = [kdot * ph[d] for d in range(3)]
+    kperp = [kv[d] - kpar[d] for d in range(3)]
+    kpn = np.sqrt(kperp[0] ** 2 + kperp[1] ** 2 + kperp[2] ** 2)
+    kph = [np.where(kpn > 0, kperp[d] / np.where(kpn > 0, kpn, 1.0), 0.0) for d in range(3)]
+    D = np.maximum(0.0, (1.0 + alpha) ** 2 - 4.0 * alpha * (kpar[0] ** 2 + kpar[1] ** 2 + kpar[2] ** 2) / ksafe ** 2)
+    sD = np.sqrt(D)
+    za = [kph[1] * ph[2] - kph[2] * ph[1], kph[2] * ph[0] - kph[0] * ph[2], kph[0] * ph[1] - kph[1] * ph[0]]
+
+    def unit(z):
+        nz_ = np.sqrt(z[0] ** 2 + z[1] ** 2 + z[2] ** 2)
+        return [np.where(nz_ > 0, z[d] / np.where(nz_ > 0, nz_, 1.0), z[d]) for d in range(3)]
+
+    zf = unit([(-1.0 + alpha + sD) * kpar[d] + (1.0 + alpha + sD) * kperp[d] for d in range(3)])
+    zs = unit([(-1.0 + alpha - sD) * kpar[d] + (1.0 + alpha - sD) * kperp[d] for d in range(3)])
+
+    def proj(z):
+        return (vre[0] * z[0] + vre[1] * z[1] + vre[2] * z[2], vim[0] * z[0] + vim[1] * z[1] + vim[2] * z[2])
+
+    va, vf, vs = proj(za), proj(zf), proj(zs)
+    cf = np.sqrt(0.5 * bmag ** 2 * (1.0 + alpha + sD))
+    csl = np.sqrt(np.maximum(0.0, 0.5 * bmag ** 2 * (1.0 + alpha - sD)))
+    t1 = khat[0] * zf[0] + khat[1] * zf[1] + khat[2] * zf[2]
+    t2 = khat[0] * zs[0] + khat[1] * zs[1] + khat[2] * zs[2]
+    dvf, dvs = 2.0 * vf[1], 2.0 * vs[1]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rhof = dvf * t1 / cf
+        rhos = dvs * t2 / csl
+
+        def cross_norm(z):
+            c = [B0[1] * z[2] - B0[2] * z[1], B0[2] * z[0] - B0[0] * z[2], B0[0] * z[1] - B0[1] * z[0]]
+            return np.sqrt(c[0] ** 2 + c[1] ** 2 + c[2] ** 2)
+
+        bf_ = dvf * cross_norm(zf) / cf
+        bs_ = dvs * cross_norm(zs) / csl
+    amps = [2 * (va[0] ** 2 + va[1] ** 2), 2 * (vf[0] ** 2 + vf[1] ** 2), 2 * (vs[0] ** 2 + vs[1] ** 2),
+            2 * (va[0] ** 2 + va[1] ** 2), 2 * bf_ ** 2, 2 * bs_ ** 2, 2 * rhof ** 2, 2 * rhos ** 2]
+    kmax = np.float32(np.sqrt(3.0) * k0 * h)
+    dk = float(np.float32(kmax / np.float32(nm)))
+    kf = np.float32(k0) * np.sqrt((KX ** 2 + KY ** 2 + KZ ** 2).astype(np.float32))
+    ik = (kf / np.float32(dk)).astype(np.int64)
+    live = (KX + KY + KZ) > 0
+    P = np.zeros((8, nm))
+    for m, a in enumerate(amps):
+        w = np.where(live, a, 0.0)
+        w = np.where(np.isfinite(w), w, 0.0)
+        P[m] = np.bincount(ik.ravel(), weights=w.ravel(), minlength=nm)[:nm]
+    return np.arange(nm) * dk, P / nm ** 3

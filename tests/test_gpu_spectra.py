@@ -81,3 +81,38 @@ def test_spectrum_rejects_non_cubic():
     with pytest.raises(ImhdError, match="IMHD_EINVAL"):
         S.spectrum("velocity")
     S.close()
+
+
+def test_mode_spectra_match_oracle():
+    n = 32
+    print("[gpu] mode spectra", flush=True)
+    q, bf, _ = random_state(n, n, n, npass=0, seed=21, amp_v=0.6, b0=(0.5, 0.2, 0.1))
+    S = solver(n)
+    S.set_state(q, bf, None)
+    k, P = S.mode_spectra()
+    ko, Po = SP.mode_spectra(S.get_state()[0], (n, n, n))
+    assert np.allclose(k, ko, rtol=0, atol=1e-12)
+    for m in range(8):
+        tol = 2e-5 * np.max(np.abs(Po[m])) + 1e-4 * np.abs(Po[m])
+        assert np.all(np.abs(P[m] - Po[m]) <= tol), m
+    S.close()
+
+
+def test_mode_spectra_alfven_mode():
+    """B0 along z, velocity along y varying in x: a pure Alfven mode (tests/test_spectra.py)."""
+    from test_spectra import cube
+    n, kx0, A = 16, 3, 0.2
+    x = (np.arange(n) + 0.5) / n
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    q = cube(n, 1.0, v=(0 * X, A * np.sin(2 * np.pi * kx0 * X), 0 * X), b=(0 * X, 0 * X, 1.0 + 0 * X))
+    bf = np.zeros(q.shape[:3] + (3,), order="F")
+    bf[..., 2] = q[..., 6]
+    S = solver(n)
+    S.set_state(q, bf, None)
+    k, P = S.mode_spectra()
+    ik = int(2 * kx0 / np.sqrt(3.0))
+    want = A ** 2 * n ** 3 / 2
+    assert P[0, ik] == pytest.approx(want, rel=1e-4) and P[3, ik] == pytest.approx(want, rel=1e-4)
+    for m in (1, 2, 4, 5, 6, 7):
+        assert np.max(np.abs(P[m])) <= 1e-5 * want
+    S.close()

@@ -78,3 +78,34 @@ def test_density_pdf_two_values():
     assert pdf[i1] == pytest.approx(0.5 / drho) and pdf[i2] == pytest.approx(0.5 / drho)
     assert np.count_nonzero(pdf) == 2
     assert r[i1] == pytest.approx((i1 + 0.5) * drho)
+
+
+def test_mode_decomposition_single_modes():
+    """B0 along z, k along x (k_par = 0).  A velocity along y (normal to k and B0) is a pure
+    Alfven mode: its whole power A^2 n^3 / 2 is in the Alfven velocity and field spectra
+    of shell INT(2 kx0 / sqrt 3), nothing in fast or slow.  A velocity along x (along k)
+    is a pure fast mode.  (The v_perp input is v itself here, v being normal to B0.)"""
+    n, kx0, A = 16, 3, 0.2
+    X, Y, Z = grid(n)
+    s = A * np.sin(2 * np.pi * kx0 * X)
+    ik = int(2 * kx0 / np.sqrt(3.0))
+    b = (0 * X, 0 * X, 1.0 + 0 * X)
+    q = cube(n, 1.0, v=(0 * X, s, 0 * X), b=b)
+    k, P = SP.mode_spectra(q, (n, n, n))
+    want = A ** 2 * n ** 3 / 2
+    assert P[0, ik] == pytest.approx(want, rel=1e-5) and P[3, ik] == pytest.approx(want, rel=1e-5)
+    for m in (1, 2, 4, 5, 6, 7):
+        assert np.max(np.abs(P[m])) <= 1e-8 * want
+    # fast mode: a sine on the index grid x = i/n has a purely imaginary coefficient
+    # -i A n^3/2.  At k_par = 0, c_f = |B0| sqrt(1 + alpha) = sqrt(2) and zeta_f = khat, so the
+    # density amplitude 2 Im(v_f) / c_f and the field amplitude 2 Im(v_f) |B0 x zeta_f| / c_f
+    # both have doubled squares of 2 A^2 n^3 / 2 = 2 want.
+    xi = np.arange(n)[:, None, None] / n + 0 * X
+    s0 = A * np.sin(2 * np.pi * kx0 * xi)
+    q = cube(n, 1.0, v=(s0, 0 * X, 0 * X), b=b)
+    k, P = SP.mode_spectra(q, (n, n, n))
+    assert P[1, ik] == pytest.approx(want, rel=1e-5)
+    for m in (0, 2, 3, 5, 7):
+        assert np.max(np.abs(P[m])) <= 1e-8 * want
+    assert P[6, ik] == pytest.approx(2 * want, rel=1e-5)
+    assert P[4, ik] == pytest.approx(2 * want, rel=1e-5)
