@@ -490,9 +490,12 @@ void update(imhd_ctx* c, int src, int dst, double factor, const PlanDev& p, bool
 }
 
 // One whole stage in one launch (k_vl_stage): fluxes from set sflux, update src -> dst.
+// The predictor's update base is its flux state (sflux == src): the kernel reads that
+// state once per cell for both.
 template <class real>
 void vl_stage(imhd_ctx* c, int sflux, int src, int dst, bool corr, bool fuse_cmax, const char* name,
               bool pvel = false) {
+    if (!corr) src = sflux;
     const TilePlanDev& p = corr ? c->tp_corr : c->tp_pred;
     ProfScope ps(c, name, p.active);
     StageArgs<real> a{};

@@ -64,24 +64,25 @@ __global__ __launch_bounds__(256) void k_flux_pred(const BoxDesc* __restrict__ b
     atomic_max_nonneg(a.smax, b, sm);
 }
 
-// Corrector fluxes: primitive PLM reconstruction with the MC-beta slope
-// sign(dC) min(beta|dL|, beta|dR|, |dC|)  -- no zeroing at extrema, as in the
-// reference (vanleer_mod.f90:1160-1215, 1467-1481) -- then the Riemann solver.
+// Primitive cell values of the rotated state r at t: rho, v_n, v_t1, v_t2, B_t1, B_t2.
+template <class real>
+__device__ __forceinline__ void prim_rot(real rho, real m1, real m2, real m3, real bt1, real bt2, real* W) {
+    real irho = fdiv(real(1), rho);
+    W[0] = rho;
+    W[1] = m1 * irho;
+    W[2] = m2 * irho;
+    W[3] = m3 * irho;
+    W[4] = bt1;
+    W[5] = bt2;
+}
+template <class real>
+__device__ __forceinline__ void prim_load(const SweepArgs<real>& a, long long ct, real* W) {
+    prim_rot(a.w[0][ct], a.w[1][ct], a.w[2][ct], a.w[3][ct], a.w[5][ct], a.w[6][ct], W);
+}
+
+// PLM interface states from the primitive values of cells c-1 .. c+2 and the Riemann flux.
 template <class real, int RS>
-__device__ __forceinline__ real flux_corr_calc(const SweepArgs<real>& a, long long c, long long s, real* F) {
-    real W[4][6];
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        long long ct = c + (long long)(t - 1) * s;
-        real rho = a.w[0][ct];
-        real irho = fdiv(real(1), rho);
-        W[t][0] = rho;
-        W[t][1] = a.w[1][ct] * irho;
-        W[t][2] = a.w[2][ct] * irho;
-        W[t][3] = a.w[3][ct] * irho;
-        W[t][4] = a.w[5][ct];
-        W[t][5] = a.w[6][ct];
-    }
+__device__ __forceinline__ real plm_flux(const SweepArgs<real>& a, const real (*W)[6], real wb, real* F) {
     const real beta = (real)a.mc_beta;
     real d1[6], d2[6];
 #pragma unroll
@@ -92,7 +93,6 @@ __device__ __forceinline__ real flux_corr_calc(const SweepArgs<real>& a, long lo
         d2[m] = sgn1(dC) * min3(beta * fabs(dL), beta * fabs(dR), fabs(dC));
     }
     real uL[7], uR[7];
-    real wb = (real)a.bn[c];
     uL[0] = W[1][0] + real(0.5) * d1[0];
     uL[1] = W[1][1] + real(0.5) * d1[1];
     uL[2] = W[1][2] + real(0.5) * d1[2];
@@ -112,6 +112,17 @@ __device__ __forceinline__ real flux_corr_calc(const SweepArgs<real>& a, long lo
     real smax = real(0);
     riemann<RS>(uL, uR, (real)a.cs, F, smax);
     return smax;
+}
+
+// Corrector fluxes: primitive PLM reconstruction with the MC-beta slope
+// sign(dC) min(beta|dL|, beta|dR|, |dC|)  -- no zeroing at extrema, as in the
+// reference (vanleer_mod.f90:1160-1215, 1467-1481) -- then the Riemann solver.
+template <class real, int RS>
+__device__ __forceinline__ real flux_corr_calc(const SweepArgs<real>& a, long long c, long long s, real* F) {
+    real W[4][6];
+#pragma unroll
+    for (int t = 0; t < 4; t++) prim_load(a, c + (long long)(t - 1) * s, W[t]);
+    return plm_flux<real, RS>(a, W, (real)a.bn[c], F);
 }
 template <class real, int RS>
 __device__ __forceinline__ double flux_corr_face(const SweepArgs<real>& a, long long c, long long s) {

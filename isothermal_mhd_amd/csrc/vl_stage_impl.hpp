@@ -25,6 +25,54 @@ template <class real>
 constexpr int vs_waves() { return sizeof(real) == 4 ? 4 : 1; }
 #define VS_WAVES vs_waves<real>()
 
+// x or y face (direction D) of the fused stage.  Predictor: the cell c on the low side is
+// the thread's own, held in registers in the global field order, and only the neighbour is
+// loaded (same arithmetic as flux_pred_calc).  Corrector: flux_corr_calc, which loads its
+// four-cell stencil (holding the own cell across the faces costs the corrector registers
+// it does not have at 4 waves per SIMD).
+template <class real, int RS, int KIND, int D>
+__device__ __forceinline__ real stage_face_own(const SweepArgs<real>& a, const real* U, long long c, long long s,
+                                               real* F) {
+    if constexpr (KIND == 0) {
+        real uL[7], uR[7];
+#pragma unroll
+        for (int r = 0; r < 7; r++) {
+            if (r == 4) continue;
+            uL[r] = U[perm(D, r)];
+            uR[r] = a.w[r][c + s];
+        }
+        uL[4] = (real)a.bn[c];
+        uR[4] = uL[4];
+        real smax = real(0);
+        riemann<RS>(uL, uR, (real)a.cs, F, smax);
+        return smax;
+    } else {
+        return flux_corr_calc<real, RS>(a, c, s, F);
+    }
+}
+// z face k+1/2: predictor between the thread's own cells of planes k (UL) and k+1 (UR),
+// both in registers; corrector as above.
+template <class real, int RS, int KIND>
+__device__ __forceinline__ real stage_face_z(const SweepArgs<real>& a, const real* UL, const real* UR, long long c,
+                                             long long s, real* F) {
+    if constexpr (KIND == 0) {
+        real uL[7], uR[7];
+#pragma unroll
+        for (int r = 0; r < 7; r++) {
+            if (r == 4) continue;
+            uL[r] = UL[perm(2, r)];
+            uR[r] = UR[perm(2, r)];
+        }
+        uL[4] = (real)a.bn[c];
+        uR[4] = uL[4];
+        real smax = real(0);
+        riemann<RS>(uL, uR, (real)a.cs, F, smax);
+        return smax;
+    } else {
+        return flux_corr_calc<real, RS>(a, c, s, F);
+    }
+}
+
 template <class real, int RS, int KIND, int TX, int TY>
 __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* __restrict__ boxes, TilePlan plan,
                                                       StageArgs<real> a) {
@@ -88,15 +136,25 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     const double dtd = a.factor * a.dt[b] / B.dx;
     const real dtdx = (real)dtd;
 
+    // the thread's own cell of plane p of the flux state, global field order: loaded once
+    // per plane for the centre values, the three faces and (predictor) the update
+    auto own = [&](int p, real* U) {
+        if (cell_ij && inr(p, 1 - nb, n2 + nb)) {
+            const long long c = cell_index(B, i, j, p);
+#pragma unroll
+            for (int f = 0; f < 7; f++) U[f] = a.qf[f][c];
+        } else {
+#pragma unroll
+            for (int f = 0; f < 7; f++) U[f] = real(0);
+        }
+    };
     // centre EMFs and velocities of plane p (k_corner's Exc, Eyc, Ezc, m/rho); m_z/rho returned
-    auto cells = [&](int p) -> double {
+    auto cells = [&](int p, const real* U) -> double {
         double vz = 0.0;
         const int q = p & 1;
         if (cell_ij && inr(p, 1 - nb, n2 + nb)) {
-            const long long c = cell_index(B, i, j, p);
-            const double r = (double)a.qf[0][c], m1 = (double)a.qf[1][c], m2 = (double)a.qf[2][c],
-                         m3 = (double)a.qf[3][c], b4 = (double)a.qf[4][c], b5 = (double)a.qf[5][c],
-                         b6 = (double)a.qf[6][c];
+            const double r = (double)U[0], m1 = (double)U[1], m2 = (double)U[2], m3 = (double)U[3],
+                         b4 = (double)U[4], b5 = (double)U[5], b6 = (double)U[6];
             const double ir = 1.0 / r;
             sC[q][0][t] = (b5 * m3 - b6 * m2) * ir;
             sC[q][1][t] = (b6 * m1 - b4 * m3) * ir;
@@ -113,19 +171,15 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         }
         return vz;
     };
-    auto flux = [&](int d, long long c, long long s, real* F) -> real {
-        if (KIND == 0) return flux_pred_calc<real, RS>(a.d[d], c, s, F);
-        return flux_corr_calc<real, RS>(a.d[d], c, s, F);
-    };
     // x- and y-faces of plane p: fluxes and Ez slots to LDS; the Ey (x-face) and Ex (y-face)
     // slots stay with the thread (only its own column reads them)
-    auto xyfaces = [&](int p, real& ey_x, real& ex_y, double& sm) {
+    auto xyfaces = [&](int p, const real* U, real& ey_x, real& ex_y, double& sm) {
         const int q = p & 1, fq = ONE ? 0 : q;
         const long long c = cell_index(B, i, j, p);
         const bool pt = inr(p, a.ft_lo, n2 + a.ft_hi);
         real F[7];
         if (fx_ij && pt) {
-            sm = fmax(sm, (double)flux(0, c, sx, F));
+            sm = fmax(sm, (double)stage_face_own<real, RS, KIND, 0>(a.d[0], U, c, sx, F));
             sF[fq][0][0][t] = F[0]; sF[fq][0][1][t] = F[1]; sF[fq][0][2][t] = F[2]; sF[fq][0][3][t] = F[3];
             sEf[q][0][t] = -F[5];
             ey_x = F[6];
@@ -135,7 +189,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             ey_x = real(0);
         }
         if (fy_ij && pt) {
-            sm = fmax(sm, (double)flux(1, c, sy, F));
+            sm = fmax(sm, (double)stage_face_own<real, RS, KIND, 1>(a.d[1], U, c, sy, F));
             sF[fq][1][0][t] = F[0]; sF[fq][1][2][t] = F[1]; sF[fq][1][3][t] = F[2]; sF[fq][1][1][t] = F[3];
             ex_y = -F[5];
             sEf[q][1][t] = F[6];
@@ -146,10 +200,10 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         }
     };
     // z-face between planes p and p+1
-    auto zface = [&](int p, real* fz, double& sm) {
+    auto zface = [&](int p, const real* UL, const real* UR, real* fz, double& sm) {
         if (fz_ij && inr(p, a.fn_lo, n2 + a.fn_hi)) {
             real F[7];
-            sm = fmax(sm, (double)flux(2, cell_index(B, i, j, p), sz, F));
+            sm = fmax(sm, (double)stage_face_z<real, RS, KIND>(a.d[2], UL, UR, cell_index(B, i, j, p), sz, F));
             fz[0] = F[0]; fz[3] = F[1]; fz[1] = F[2]; fz[2] = F[3];
             sEz3[0][t] = -F[5];
             sEz3[1][t] = F[6];
@@ -162,8 +216,10 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     double sm = 0.0, cfl = 0.0;
     real fz_lo[4], fz_hi[4];
     real eyx_lo, eyx_hi, exy_lo, exy_hi;   // own x-face Ey slot and y-face Ex slot, planes k, k+1
-    double vz_lo = cells(ks - 1), vz_hi;
-    xyfaces(ks - 1, eyx_lo, exy_lo, sm);
+    real U0[7], U1[7];   // own cell of planes k and k+1
+    own(ks - 1, U0);
+    double vz_lo = cells(ks - 1, U0), vz_hi;
+    xyfaces(ks - 1, U0, eyx_lo, exy_lo, sm);
     double bz_m = 0.0;
     bool have_bzm = false;
     // carried from the update phase of plane k-1: corner EMFs of index k-1 (own and the
@@ -174,9 +230,10 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
 #pragma unroll 1
     for (int k = ks - 1; k < ke; k++) {
         // A: plane k+1 and the z-face k+1/2
-        vz_hi = cells(k + 1);
-        xyfaces(k + 1, eyx_hi, exy_hi, sm);
-        zface(k, fz_hi, sm);
+        own(k + 1, U1);
+        vz_hi = cells(k + 1, U1);
+        xyfaces(k + 1, U1, eyx_hi, exy_hi, sm);
+        zface(k, U0, U1, fz_hi, sm);
         __syncthreads();
         // D: corner EMFs of index k, x/y flux differences
         const int q0 = k & 1, q1 = (k + 1) & 1, eq = ONE ? 0 : q0;
@@ -228,7 +285,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 real div = dxy[f] + (fz_hi[f] - fz_lo[f]);
-                real q0v = a.src.q[f][c];
+                real q0v = KIND == 0 ? U0[f] : a.src.q[f][c];   // predictor: the flux state is the base
                 oq[f] = q0v;
                 real v;
                 if (a.comp[0] != nullptr) {
@@ -267,9 +324,9 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 nq[5] = (real)(0.5 * (by + bym));
                 nq[6] = (real)(0.5 * (bz + bz_m));
             } else {
-                nq[4] = a.src.q[4][c];
-                nq[5] = a.src.q[5][c];
-                nq[6] = a.src.q[6][c];
+                nq[4] = KIND == 0 ? U0[4] : a.src.q[4][c];
+                nq[5] = KIND == 0 ? U0[5] : a.src.q[5][c];
+                nq[6] = KIND == 0 ? U0[6] : a.src.q[6][c];
             }
             a.dst.q[4][c] = nq[4];
             a.dst.q[5][c] = nq[5];
@@ -309,6 +366,8 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             fz_lo[f] = fz_hi[f];
             if (ONE) dxy[f] = dxy_n[f];
         }
+#pragma unroll
+        for (int f = 0; f < 7; f++) U0[f] = U1[f];
         eyx_lo = eyx_hi;
         exy_lo = exy_hi;
         vz_lo = vz_hi;
