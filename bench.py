@@ -158,9 +158,10 @@ FAMILY_SYMBOL = {"stage_pred": "k_vl_stage", "stage_corr_cfl": "k_vl_stage",
 
 
 def pmc_run(n=256, steps=3):
-    """Short FP32 run of the workload for counter collection, plus an FP64 set_state whose
-    k_convert_in<double> dispatches are plain copies of known size (calibration).  Writes
-    the family -> (kernel, grid size) map for --pmc-summarize."""
+    """Short FP32 run of the workload for counter collection.  Its set_state converts each
+    FP64 state field to FP32 with a k_convert_in<float> dispatch over the padded box (8 bytes
+    read and 4 written per cell, no reuse): the calibration of the counters.  Writes the
+    family -> (kernel, grid size) map for --pmc-summarize."""
     from isothermal_mhd_amd import VanLeerSolver, ics
     q, bf, qp = ics.config_c2(n=n)
     S = VanLeerSolver((n, n, n), npass=1, riemann="hlld", bc="periodic", precision=32)
@@ -170,10 +171,6 @@ def pmc_run(n=256, steps=3):
     S.run(steps, graph=False)
     prof = S.profile()
     S.close()
-    S64 = VanLeerSolver((n, n, n), npass=1, riemann="hlld", bc="periodic", precision=64)
-    S64.set_state(q, bf, qp)
-    S64.sync()
-    S64.close()
     fam = {}
     for name, (ms, nl, items) in prof.items():
         key = name[:-2] if name[-2:] in ("_x", "_y", "_z") else name
@@ -181,8 +178,8 @@ def pmc_run(n=256, steps=3):
             grids = fam.setdefault(key, {"symbol": FAMILY_SYMBOL[key], "grids": [], "order": len(fam)})
             grids["items"] = items // nl
     nc = (n + 12) ** 3
-    out = {"n": n, "families": fam, "copy": {"symbol": "k_convert_in<double>",
-                                              "bytes_read": nc * 8, "bytes_written": nc * 8}}
+    out = {"n": n, "families": fam, "copy": {"symbol": "k_convert_in<float>", "grid": (nc + 255) // 256 * 256,
+                                              "bytes_read": nc * 8, "bytes_written": nc * 4}}
     os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
     with open(os.path.join(ROOT, "gpurun_out", "pmc_map.json"), "w") as f:
         json.dump(out, f)
@@ -190,16 +187,21 @@ def pmc_run(n=256, steps=3):
     return 0
 
 
-def _read_counter(dirname, counter):
-    """per-dispatch (kernel name, counter value KiB) in dispatch order"""
+def _read_counter(dirname, counter, grid=None):
+    """per-dispatch (kernel name, counter value KiB) in dispatch order (only the dispatches
+    of that grid size when grid is given)"""
     rows = []
     files = glob.glob(os.path.join(dirname, "**", "*counter_collection.csv"), recursive=True)
     if not files:
         raise SystemExit(f"no counter_collection.csv under {dirname}")
+    if len(files) > 1:
+        raise SystemExit(f"{len(files)} counter_collection.csv files under {dirname}: one run per directory")
     for fn in files:
         with open(fn) as f:
             for r in csv.DictReader(f):
                 if r.get("Counter_Name") != counter:
+                    continue
+                if grid is not None and int(r.get("Grid_Size", -1)) != grid:
                     continue
                 rows.append((int(r.get("Dispatch_Id", 0)), r["Kernel_Name"], float(r["Counter_Value"])))
     rows.sort()
@@ -217,9 +219,11 @@ def pmc_summarize(fetch_dir, write_dir):
         mp = json.load(f)
     fr, wr = _read_counter(fetch_dir, "FETCH_SIZE"), _read_counter(write_dir, "WRITE_SIZE")
     cp = mp["copy"]
-    sym = "k_convert_in<double>"
-    cf = [v for n, v in fr if sym in n]
-    cw = [v for n, v in wr if sym in n]
+    sym = cp["symbol"]
+    cf = [v for n, v in _read_counter(fetch_dir, "FETCH_SIZE", cp.get("grid")) if sym in n]
+    cw = [v for n, v in _read_counter(write_dir, "WRITE_SIZE", cp.get("grid")) if sym in n]
+    if not cf or not cw:
+        raise SystemExit(f"no {sym} dispatches (the calibration copy) in the counter files")
     cf, cw = sum(cf) / len(cf), sum(cw) / len(cw)
     f_fac = cf * 1024.0 / cp["bytes_read"]     # counter bytes per true byte (copy kernel)
     w_fac = cw * 1024.0 / cp["bytes_written"]

@@ -47,7 +47,12 @@ __device__ __forceinline__ real stage_face_own(const SweepArgs<real>& a, const r
         riemann<RS>(uL, uR, (real)a.cs, F, smax);
         return smax;
     } else {
-        return flux_corr_calc<real, RS>(a, c, s, F);
+        real W[4][6];
+        prim_load(a, c - s, W[0]);
+        prim_rot(U[0], U[perm(D, 1)], U[perm(D, 2)], U[perm(D, 3)], U[perm(D, 5)], U[perm(D, 6)], W[1]);
+        prim_load(a, c + s, W[2]);
+        prim_load(a, c + 2 * s, W[3]);
+        return plm_flux<real, RS>(a, W, (real)a.bn[c], F);
     }
 }
 // z face k+1/2: predictor between the thread's own cells of planes k (UL) and k+1 (UR),
@@ -200,8 +205,8 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         }
     };
     // z-face between planes p and p+1
-    auto zface = [&](int p, const real* UL, const real* UR, real* fz, double& sm) {
-        if (fz_ij && inr(p, a.fn_lo, n2 + a.fn_hi)) {
+    auto zface = [&](int p, bool live, const real* UL, const real* UR, real* fz, double& sm) {
+        if (live && fz_ij && inr(p, a.fn_lo, n2 + a.fn_hi)) {
             real F[7];
             sm = fmax(sm, (double)stage_face_z<real, RS, KIND>(a.d[2], UL, UR, cell_index(B, i, j, p), sz, F));
             fz[0] = F[0]; fz[3] = F[1]; fz[1] = F[2]; fz[2] = F[3];
@@ -217,9 +222,10 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     real fz_lo[4], fz_hi[4];
     real eyx_lo, eyx_hi, exy_lo, exy_hi;   // own x-face Ey slot and y-face Ex slot, planes k, k+1
     real U0[7], U1[7];   // own cell of planes k and k+1
-    own(ks - 1, U0);
-    double vz_lo = cells(ks - 1, U0), vz_hi;
-    xyfaces(ks - 1, U0, eyx_lo, exy_lo, sm);
+    double vz_lo = 0.0, vz_hi;
+    eyx_lo = exy_lo = real(0);
+#pragma unroll
+    for (int f = 0; f < 4; f++) fz_lo[f] = fz_hi[f] = real(0);
     double bz_m = 0.0;
     bool have_bzm = false;
     // carried from the update phase of plane k-1: corner EMFs of index k-1 (own and the
@@ -227,17 +233,22 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     // while the fluxes of that plane are in LDS)
     double ey_m = 0.0, ex_m = 0.0, eyx_m = 0.0, exy_m = 0.0;
     real dxy[4] = {real(0), real(0), real(0), real(0)}, dxy_n[4];
+    // The first pass (k = ks - 2) only forms plane ks - 1.  Every plane's faces and centre
+    // values come from this one copy of the loop body, so a plane shared by two z-segments
+    // (and the corner EMFs formed from it) gets the same bits in both, also where FMA
+    // contraction is on.
 #pragma unroll 1
-    for (int k = ks - 1; k < ke; k++) {
+    for (int k = ks - 2; k < ke; k++) {
+        const bool live = k >= ks - 1;   // uniform over the workgroup
         // A: plane k+1 and the z-face k+1/2
         own(k + 1, U1);
         vz_hi = cells(k + 1, U1);
         xyfaces(k + 1, U1, eyx_hi, exy_hi, sm);
-        zface(k, U0, U1, fz_hi, sm);
+        zface(k, live, U0, U1, fz_hi, sm);
         __syncthreads();
         // D: corner EMFs of index k, x/y flux differences
         const int q0 = k & 1, q1 = (k + 1) & 1, eq = ONE ? 0 : q0;
-        if (c_ij && inr(k, a.c_lo, n2 + a.c_hi)) {
+        if (live && c_ij && inr(k, a.c_lo, n2 + a.c_hi)) {
             const double vx = 0.5 * (sC[q0][3][t] + sC[q0][3][t + 1]);
             const double vy = 0.5 * (sC[q0][4][t] + sC[q0][4][t + TX]);
             const double vz = 0.5 * (vz_lo + vz_hi);

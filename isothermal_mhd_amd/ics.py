@@ -34,7 +34,7 @@ def random_velocity(n, kmin, kmax, slope, mach, cs=1.0, solenoidal=True, seed=0,
         with np.errstate(invalid="ignore", divide="ignore"):
             f = np.where(k2 > 0, kdotv / k2, 0.0)
         vh = [vh[0] - KX * f, vh[1] - KY * f, vh[2] - KZ * f]
-    v = [np.fft.irfftn(c, s=(nx, ny, nz)) for c in vh]
+    v = [np.fft.irfftn(c, s=(nx, ny, nz), axes=(0, 1, 2)) for c in vh]
     rms = np.sqrt(np.mean(v[0] ** 2 + v[1] ** 2 + v[2] ** 2))
     scale = mach * cs / rms if rms > 0 else 0.0
     return [(c * scale).astype(dtype) for c in v]
