@@ -59,7 +59,8 @@ FORTRAN = os.path.join(HERE, "fortran")
 FBUILD = os.path.join(FORTRAN, "_build")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 # example configurations: (name, constants module)
-F_EXAMPLES = [("vl", "example_constants_vl.f90"), ("tvd", "example_constants_tvd.f90")]
+F_EXAMPLES = [("vl", "example_constants_vl.f90"), ("tvd", "example_constants_tvd.f90"),
+              ("vl_drive", "example_constants_vl_drive.f90")]
 
 
 def fortran_compiler():
@@ -79,6 +80,7 @@ def build_fortran(verbose=False):
         os.makedirs(bdir, exist_ok=True)
         srcs = [os.path.join(FORTRAN, "example", consts), os.path.join(FORTRAN, "example", "example_grid_data.f90"),
                 os.path.join(FORTRAN, "imhd_api.f90"), os.path.join(FORTRAN, "imhd_shim.f90"),
+                os.path.join(FORTRAN, "imhd_shim_turb.f90"),
                 os.path.join(FORTRAN, "example", "imhd_example_driver.f90")]
         objs = []
         for src in srcs:

@@ -11,6 +11,9 @@ MODULE constants_mod
   REAL*8, PARAMETER :: COUR_def = 0.35d0
   LOGICAL, PARAMETER :: drive_turbulence = .FALSE.
   REAL*8, PARAMETER :: Edot = 1.d-3, dt_turb = 0.05d0   ! forcing (unused while not driven)
+  INTEGER, PARAMETER :: k_max = 2
+  INTEGER, PARAMETER :: nranks_x = 1, nranks_y = 1, nranks_z = 1
+  INTEGER, PARAMETER :: s1_def = 1234, s2_def = 5678, s3_def = 9123
   REAL*8 :: sound_speed = 1.d0
   INTEGER :: myrank = 0
 END MODULE constants_mod

@@ -16,7 +16,7 @@ import sys
 import numpy as np
 import pytest
 
-from refutil import Oracle, random_state
+from refutil import Oracle, oracle_rand_force, random_state
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
@@ -39,11 +39,12 @@ def undefined_symbols(path):
     return {l.split()[-1] for l in out.splitlines() if l.strip()}
 
 
-@pytest.mark.parametrize("name", ["vl", "tvd"])
+@pytest.mark.parametrize("name", ["vl", "tvd", "vl_drive"])
 def test_example_driver_links_to_c_abi(name):
     u = undefined_symbols(example(name))
     for sym in ("imhd_create", "imhd_step", "imhd_set_state", "imhd_get_state", "imhd_fill_ghosts",
-                "imhd_max_speed", "imhd_copy_velocity", "imhd_passive_step", "imhd_destroy"):
+                "imhd_max_speed", "imhd_copy_velocity", "imhd_passive_step", "imhd_destroy",
+                "imhd_generate_force"):
         assert sym in u, sym
 
 
@@ -98,8 +99,84 @@ def test_fortran_example_driver_matches_oracle(tmp_path, name, mhd, riemann, rec
     qg = raw[:nq].reshape(q.shape, order="F")
     bg = raw[nq:nq + nb_].reshape(bf.shape, order="F")
     pg = raw[nq + nb_:nq + nb_ + npp].reshape(qp.shape, order="F")
-    tg, dtg = raw[-2], raw[-1]
+    tg, dtg = raw[nq + nb_ + npp], raw[nq + nb_ + npp + 1]
     (qo, bo, po), to, dto = _run_oracle(mhd, riemann, recon, q, bf, qp, nsteps)
     assert np.array_equal(qg, qo) and np.array_equal(bg, bo)
     assert np.array_equal(pg[..., :1], po[..., :1])
     assert tg == to and dtg == dto
+
+
+def _run_oracle_driven(q, bf, qp, nsteps, dt_turb=0.02, edot=1e-3, k_max=2, seeds=(1234, 5678, 9123)):
+    """The reference driver's loop with drive_turbulence (driver.f90:46-160, 336-344) over
+    the oracle: a force field drawn at start-up and after every kick, a kick when t_turb
+    falls in (t_old, t_old + dt] with dt halved for that step, and COUR halved for the dt
+    after it."""
+    O = Oracle(*DIMS, npass=1, mhd=2, riemann=2, recon=2, bc=2)
+    O.set_state(q, bf, qp)
+    O.bc()
+    dt, t = 1e-15, 0.0
+    delv, _, seeds = oracle_rand_force(*DIMS, K=k_max, seeds=seeds)
+    drive_next, t_turb, kicks = False, t + dt_turb, 0
+    for _ in range(nsteps):
+        t_old, cour = t, 0.35
+        O.copy_velocity(0)
+        if t_turb > t_old and t_turb <= t_old + dt:
+            drive_next = True
+            t_turb = max(t_turb + dt_turb, t + 1e-2 * dt)
+            dt = 0.5 * dt
+        if drive_next:
+            O.set_force(delv, edot, dt_turb)
+            O.step_drive(dt)
+        else:
+            O.step(dt)
+        O.copy_velocity(1)
+        O.passive_step(dt)
+        t += dt
+        if drive_next:
+            delv, _, seeds = oracle_rand_force(*DIMS, K=k_max, seeds=seeds)
+            drive_next, cour = False, 0.5 * 0.35
+            kicks += 1
+        dt = cour * (1.0 / max(DIMS)) / O.max_speed()
+        O.bc()
+    return O.get_state()[:3], t, dt, t_turb, seeds, kicks
+
+
+@pytest.mark.gpu
+def test_fortran_driven_example_matches_oracle(tmp_path):
+    """The reference's impulsive forcing schedule through the drop-in boundary: the example
+    driver with drive_turbulence (constants of example_constants_vl_drive.f90: HLLD,
+    characteristic PPM, k_max 2, Edot 1e-3, dt_turb 0.02) calls init_turb_driver,
+    generate_rand_force (imhd_shim_turb.f90: the field drawn on the GPU) and
+    vanleer_solve_3d(dt, drive_next) (force applied on the GPU) on the driver's schedule.
+    The kick count and the taus88 state match the oracle exactly (the fields are drawn bit
+    for bit, tests/test_gpu_force_gen.py); state, clock and next drive time to the forcing
+    application's tolerance (tests/test_gpu_forcing.py: the normalisation sums are reduced
+    in a different order, 1e-12 relative per field)."""
+    exe = example("vl_drive")
+    print("[gpu] fortran driven example", flush=True)
+    q, bf, qp = random_state(*DIMS, npass=1, seed=62)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        for a in (q, bf, qp):
+            f.write(np.asfortranarray(a).tobytes(order="F"))
+    nsteps = 14
+    r = subprocess.run([exe, str(fin), str(fout), str(nsteps)], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-500:], r.stderr[-500:], flush=True)
+    assert r.returncode == 0
+    raw = np.fromfile(fout, dtype=np.float64)
+    nq, nb_, npp = q.size, bf.size, qp.size
+    qg = raw[:nq].reshape(q.shape, order="F")
+    bg = raw[nq:nq + nb_].reshape(bf.shape, order="F")
+    pg = raw[nq + nb_:nq + nb_ + npp].reshape(qp.shape, order="F")
+    tg, dtg, ttg = raw[nq + nb_ + npp:nq + nb_ + npp + 3]
+    sg = tuple(int(v) for v in raw[nq + nb_ + npp + 3:nq + nb_ + npp + 6])
+    (qo, bo, po), to, dto, tto, so, kicks = _run_oracle_driven(q, bf, qp, nsteps)
+    assert kicks >= 3, kicks
+    assert sg == tuple(so)
+    for g, o in ((tg, to), (dtg, dto), (ttg, tto)):
+        assert abs(g - o) <= 1e-12 * abs(o), (g, o)
+    for f in range(7):
+        assert np.max(np.abs(qg[..., f] - qo[..., f])) <= 1e-12 * np.max(np.abs(qo[..., f])), f
+    for d in range(3):
+        assert np.max(np.abs(bg[..., d] - bo[..., d])) <= 1e-12 * np.max(np.abs(bo[..., d])), d
+    assert np.max(np.abs(pg[..., 0] - po[..., 0])) <= 1e-12 * np.max(np.abs(po[..., 0]))
