@@ -175,6 +175,20 @@ int imhd_device_bytes(imhd_ctx* ctx, unsigned long long* bytes);
  * (driver.f90:249) unless imhd_set_time set another.  With use_graph != 0 one step is
  * captured once in a hipGraph and replayed. */
 int imhd_run(imhd_ctx* ctx, int nsteps, double t_end, int use_graph);
+
+/* The reference driver's impulsive turbulence forcing inside imhd_run (driver.f90:57-70,
+ * 102-111, 336-344; replaces the driver's init_turb_driver + generate_rand_force set-up
+ * and its drive_next schedule).  Draws the first field now (as imhd_generate_force, from
+ * the taus88 state seeds = s1, s2, s3) and sets the next drive time to t + dt_turb.  Each
+ * imhd_run step then kicks when that time falls in (t, t + dt]: the step runs with dt
+ * halved and the force applied (get_rho_turb + add_random_force), the next drive time
+ * becomes MAX(t_turb + dt_turb, t + 0.01 dt), the next field is drawn, and the following
+ * dt is taken with COUR halved.  The host reads t and dt once per step.  Single-box VL
+ * contexts, exclusive with OU forcing; k_max <= 0 switches it off.  Errors: IMHD_EINVAL
+ * (k_max >= min(n)/2, TVD, ensemble, OU on, dt_turb <= 0), IMHD_ESTATE (no state). */
+int imhd_set_impulsive_forcing(imhd_ctx* ctx, int k_max, double edot, double dt_turb, const int seeds[3]);
+/* The current taus88 state (write_restart's s1, s2, s3) and next drive time. */
+int imhd_get_impulsive_forcing(imhd_ctx* ctx, int seeds[3], double* t_turb);
 int imhd_set_time(imhd_ctx* ctx, const double* t, const double* dt); /* per box; NULL keeps */
 int imhd_get_time(imhd_ctx* ctx, double* t, double* dt);    /* per box */
 

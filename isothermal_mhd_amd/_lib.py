@@ -16,7 +16,8 @@ ERRORS = {-1: "IMHD_EINVAL", -2: "IMHD_EHIP", -3: "IMHD_ESTATE", -4: "IMHD_ENOSP
 # Every symbol include/imhd.h declares.
 EXPORTS = [
     "imhd_create", "imhd_create_ensemble", "imhd_destroy", "imhd_set_state", "imhd_get_state",
-    "imhd_autotune", "imhd_step", "imhd_set_force", "imhd_set_ou_forcing", "imhd_get_ou_modes", "imhd_fill_ghosts", "imhd_max_speed", "imhd_copy_velocity", "imhd_passive_step",
+    "imhd_autotune", "imhd_step", "imhd_set_force", "imhd_set_ou_forcing", "imhd_get_ou_modes",
+    "imhd_set_impulsive_forcing", "imhd_get_impulsive_forcing", "imhd_fill_ghosts", "imhd_max_speed", "imhd_copy_velocity", "imhd_passive_step",
     "imhd_diagnostics", "imhd_floor_count", "imhd_device_bytes", "imhd_run", "imhd_set_time", "imhd_get_time",
     "imhd_set_profiling", "imhd_profile_report", "imhd_sync", "imhd_last_error", "imhd_version",
     "imhd_write_restart", "imhd_read_restart", "imhd_write_cube", "imhd_generate_force", "imhd_get_force",
@@ -70,6 +71,8 @@ def lib():
     L.imhd_set_force.argtypes = [vp, ip, vp, dp, dp]
     L.imhd_autotune.argtypes = [vp, ip, ctypes.c_char_p, ip]
     L.imhd_set_ou_forcing.argtypes = [vp, ctypes.POINTER(ImhdOU)]
+    L.imhd_set_impulsive_forcing.argtypes = [vp, ip, dp, dp, ctypes.POINTER(ctypes.c_int)]
+    L.imhd_get_impulsive_forcing.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
     L.imhd_get_ou_modes.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_int), vp, vp]
     L.imhd_fill_ghosts.argtypes = [vp]
     L.imhd_max_speed.argtypes = [vp, vp]

@@ -88,6 +88,13 @@ struct imhd_ctx {
     unsigned int* force_err = nullptr;
     bool force_set = false;
     double edot = 0.0, dt_turb = 0.0;
+    // the reference driver's impulsive forcing inside the native loop (driver.f90:57-70,
+    // 102-111): taus88 state, next drive time, 0.5 COUR for the dt after a kick
+    bool imp_on = false;
+    int imp_kmax = 0;
+    int imp_seeds[3] = {0, 0, 0};
+    double imp_t_turb = 0.0;
+    double* cour_half = nullptr;
     // OU forcing of the native loop
     bool ou_on = false;
     imhd_ou ou_cfg{};
@@ -994,6 +1001,22 @@ void native_step(imhd_ctx* c, double t_end) {
     if (c->cfg.npass > 0 && !inl) passive_fused<real>(c, qold);
     fill_ghosts<real>(c);
     launch_new_dt(c->st, c->nbox, c->db, c->cmax, c->smax, c->dvmax, c->cour, t_end, c->t, c->dt, c->dtprev, 1);
+}
+
+// A native VL step with the reference's impulsive kick (vanleer_solve_3d with drive_next:
+// get_rho_turb before the step, add_random_force after it), then the passive step, the
+// ghost fill and the next dt with COUR halved (driver.f90:102-111).  The caller has halved
+// dt (driver.f90:68).  The CFL speed is taken from the kicked state, with delv_max.
+template <class real>
+void native_step_drive(imhd_ctx* c, double t_end) {
+    const int s0 = c->cur;
+    force_sums<real>(c, s0);
+    vl_step<real>(c, false, false);
+    force_apply<real>(c, s0, c->cur);
+    max_speed_launch<real>(c);
+    if (c->cfg.npass > 0) passive_fused<real>(c, s0);
+    fill_ghosts<real>(c);
+    launch_new_dt(c->st, c->nbox, c->db, c->cmax, c->smax, c->dvmax, c->cour_half, t_end, c->t, c->dt, c->dtprev, 1);
 }
 
 __global__ void k_set_dt(double* dt, int n, double v) {
@@ -1944,7 +1967,9 @@ int imhd_destroy(imhd_ctx* c) {
                     (void*)c->ou_twoff, (void*)c->ou_part, (void*)c->ou_mean, (void*)c->ou_col,
                     (void*)c->ou_coloff})
         fr(p);
-    for (void* p : {(void*)c->force_part, (void*)c->force_par, (void*)c->force_err, (void*)c->dvmax}) fr(p);
+    for (void* p : {(void*)c->force_part, (void*)c->force_par, (void*)c->force_err, (void*)c->dvmax,
+                    (void*)c->cour_half})
+        fr(p);
     for (void* p : {(void*)c->smax, (void*)c->cmax, (void*)c->dmax, (void*)c->dt, (void*)c->t, (void*)c->dtprev, (void*)c->cour,
                     (void*)c->diag, (void*)c->floor_count, (void*)c->stage, (void*)c->db})
         fr(p);
@@ -2144,6 +2169,7 @@ int imhd_set_ou_forcing(imhd_ctx* c, const imhd_ou* ou) {
     HIPCK(c, hipStreamSynchronize(c->st));
     for (auto& g : c->gexec) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }   // the step changes shape
     if (!ou || !(ou->sigma > 0.0)) { c->ou_on = false; return IMHD_OK; }
+    if (c->imp_on) FAIL(c, IMHD_EINVAL, "impulsive forcing is on: the native loop takes one kind of forcing");
     if (!(ou->t_corr > 0.0) || !(ou->kmax >= ou->kmin) || !(ou->kmax >= 1.0) || ou->kmax > 16.0)
         FAIL(c, IMHD_EINVAL, "OU forcing needs t_corr > 0 and 1 <= kmax <= 16, kmin <= kmax");
     const int K = (int)floor(ou->kmax + 1e-12);
@@ -2502,11 +2528,9 @@ int imhd_sync(imhd_ctx* c) {
     return IMHD_OK;
 }
 
-int imhd_run(imhd_ctx* c, int nsteps, double t_end, int use_graph) {
-    if (!c || nsteps < 0) return IMHD_EINVAL;
-    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
-    if (nsteps == 0) return IMHD_OK;
-    if (use_graph && c->prof) FAIL(c, IMHD_EINVAL, "profiling needs eager launches (use_graph = 0)");
+// nsteps native steps, eager or replayed from the captured step graph (one per buffer
+// parity), with no host synchronisation.
+static int run_native(imhd_ctx* c, int nsteps, double t_end, int use_graph) {
     auto one = [&](void) {
         if (c->prec == 64) native_step<double>(c, t_end); else native_step<float>(c, t_end);
     };
@@ -2539,6 +2563,93 @@ int imhd_run(imhd_ctx* c, int nsteps, double t_end, int use_graph) {
         c->cur = 1 - c->cur;  // the native step keeps the passive buffer in place
     }
     HIPCK(c, hipGetLastError());
+    return IMHD_OK;
+}
+
+// The native loop with the reference driver's impulsive forcing: per step the host reads
+// t and dt and decides the kick as driver.f90:57-70 does; a kick step runs eagerly
+// (native_step_drive) and is followed by the next field (generate_rand_force,
+// driver.f90:102-111); the other steps run as in run_native.
+static int run_impulsive(imhd_ctx* c, int nsteps, double t_end, int use_graph) {
+    for (int s = 0; s < nsteps; s++) {
+        double td[2];
+        HIPCK(c, hipMemcpyAsync(&td[0], c->t, 8, hipMemcpyDeviceToHost, c->st));
+        HIPCK(c, hipMemcpyAsync(&td[1], c->dt, 8, hipMemcpyDeviceToHost, c->st));
+        HIPCK(c, hipStreamSynchronize(c->st));
+        const double t_old = td[0], dt = td[1];
+        if (!(c->imp_t_turb > t_old && c->imp_t_turb <= t_old + dt)) {
+            int rc = run_native(c, 1, t_end, use_graph);
+            if (rc) return rc;
+            continue;
+        }
+        c->imp_t_turb = std::max(c->imp_t_turb + c->dt_turb, t_old + 1e-2 * dt);
+        set_scalar_dt(c, 0.5 * dt);
+        if (c->prec == 64) native_step_drive<double>(c, t_end); else native_step_drive<float>(c, t_end);
+        HIPCK(c, hipGetLastError());
+        unsigned int err = 0;
+        HIPCK(c, hipMemcpyAsync(&err, c->force_err, 4, hipMemcpyDeviceToHost, c->st));
+        HIPCK(c, hipStreamSynchronize(c->st));
+        if (err) {
+            HIPCK(c, hipMemsetAsync(c->force_err, 0, 4, c->st));
+            FAIL(c, IMHD_EINVAL, "bad value in turbulence driver constant energy normalization "
+                                 "(turb_driver_mod.f90:420-425)");
+        }
+        int rc = generate_force(c, 0, c->imp_kmax, c->imp_seeds);
+        if (rc) return rc;
+        if ((rc = prof_collect(c))) return rc;
+    }
+    return IMHD_OK;
+}
+
+int imhd_run(imhd_ctx* c, int nsteps, double t_end, int use_graph) {
+    if (!c || nsteps < 0) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    if (nsteps == 0) return IMHD_OK;
+    if (use_graph && c->prof) FAIL(c, IMHD_EINVAL, "profiling needs eager launches (use_graph = 0)");
+    if (c->imp_on) return run_impulsive(c, nsteps, t_end, use_graph);
+    return run_native(c, nsteps, t_end, use_graph);
+}
+
+int imhd_set_impulsive_forcing(imhd_ctx* c, int k_max, double edot, double dt_turb, const int* seeds) {
+    if (!c) return IMHD_EINVAL;
+    HIPCK(c, hipStreamSynchronize(c->st));
+    if (k_max <= 0) { c->imp_on = false; return IMHD_OK; }
+    if (!seeds) return IMHD_EINVAL;
+    if (is_tvd(c)) FAIL(c, IMHD_EINVAL, "tvd_solve_3d has no forcing hook (isotvd_mod.f90:226)");
+    if (c->nbox != 1) FAIL(c, IMHD_EINVAL, "impulsive forcing runs on single-box contexts");
+    if (c->ou_on) FAIL(c, IMHD_EINVAL, "OU forcing is on: the native loop takes one kind of forcing");
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    if (!(dt_turb > 0.0)) FAIL(c, IMHD_EINVAL, "dt_turb must be positive");
+    const BoxDesc& B = c->hb[0];
+    if (k_max >= std::min(B.n[0], std::min(B.n[1], B.n[2])) / 2)   // turb_driver_mod.f90:31-34
+        FAIL(c, IMHD_EINVAL, "invalid k_max %d (needs 1 <= k_max < min(nx,ny,nz)/2)", k_max);
+    if (!c->cour_half) {
+        HIPCK(c, hipMalloc(&c->cour_half, 8));
+    }
+    double cour = 0.0;
+    HIPCK(c, hipMemcpy(&cour, c->cour, 8, hipMemcpyDeviceToHost));
+    const double half = 0.5 * cour;
+    HIPCK(c, hipMemcpy(c->cour_half, &half, 8, hipMemcpyHostToDevice));
+    int sd[3] = {seeds[0], seeds[1], seeds[2]};
+    int rc = generate_force(c, 0, k_max, sd);   // driver.f90:336-339
+    if (rc) return rc;
+    double t = 0.0;
+    HIPCK(c, hipMemcpy(&t, c->t, 8, hipMemcpyDeviceToHost));
+    c->edot = edot;
+    c->dt_turb = dt_turb;
+    c->force_set = true;
+    c->imp_kmax = k_max;
+    for (int i = 0; i < 3; i++) c->imp_seeds[i] = sd[i];
+    c->imp_t_turb = t + dt_turb;                  // driver.f90:344
+    c->imp_on = true;
+    return IMHD_OK;
+}
+
+int imhd_get_impulsive_forcing(imhd_ctx* c, int* seeds, double* t_turb) {
+    if (!c) return IMHD_EINVAL;
+    if (!c->imp_on) FAIL(c, IMHD_ESTATE, "impulsive forcing is off");
+    if (seeds) for (int i = 0; i < 3; i++) seeds[i] = c->imp_seeds[i];
+    if (t_turb) *t_turb = c->imp_t_turb;
     return IMHD_OK;
 }
 

@@ -241,6 +241,21 @@ class VanLeerSolver:
         o = L.ImhdOU(kmin, kmax, t_corr, sigma, seed, int(bool(solenoidal)), 0)
         self._check(self.lib.imhd_set_ou_forcing(self._c, ctypes.byref(o)))
 
+    def set_impulsive_forcing(self, k_max=2, edot=1e-3, dt_turb=0.05, seeds=(1234, 5678, 9123)):
+        """The reference driver's impulsive forcing inside run() (driver.f90:57-70, 102-111,
+        336-344): draws the first field from the taus88 state seeds and kicks every dt_turb
+        with that step's dt halved and the next one's COUR halved (see imhd.h); k_max <= 0
+        switches it off."""
+        sd = (ctypes.c_int * 3)(*[int(np.int32(v)) for v in seeds])
+        self._check(self.lib.imhd_set_impulsive_forcing(self._c, int(k_max), float(edot), float(dt_turb), sd))
+
+    def impulsive_forcing_state(self):
+        """(taus88 state s1, s2, s3, next drive time)"""
+        sd = (ctypes.c_int * 3)()
+        tt = ctypes.c_double()
+        self._check(self.lib.imhd_get_impulsive_forcing(self._c, sd, ctypes.byref(tt)))
+        return tuple(sd), tt.value
+
     def ou_modes(self, box=0):
         """(integer wave vectors (nmode, 3), complex amplitudes (nmode, 3))"""
         n = ctypes.c_int()

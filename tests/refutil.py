@@ -398,3 +398,42 @@ def random_state(nx, ny, nz, nb=6, npass=1, seed=0, amp_v=0.5, amp_rho=0.2, b0=(
     for p in range(npass):
         qp[Ip + (p,)] = 0.5 + 0.4 * smooth()
     return q, bf, qp
+
+
+def oracle_driven(dims, q, bf, qp, nsteps, riemann=2, recon=2, npass=1, dt_turb=0.02, edot=1e-3, k_max=2,
+                  seeds=(1234, 5678, 9123), cour=0.35):
+    """The reference driver's loop with drive_turbulence (driver.f90:46-160, 336-344) over
+    the oracle: a force field drawn at start-up and after every kick, a kick when t_turb
+    falls in (t_old, t_old + dt] with dt halved for that step, and COUR halved for the dt
+    after it.  Returns ((q, bf, qp), t, dt, t_turb, seeds, kicks)."""
+    O = Oracle(*dims, npass=npass, mhd=2, riemann=riemann, recon=recon, bc=2)
+    O.set_state(q, bf, qp)
+    O.bc()
+    dt, t = 1e-15, 0.0
+    dx = 1.0 / max(dims)
+    delv, _, seeds = oracle_rand_force(*dims, K=k_max, seeds=seeds)
+    drive_next, t_turb, kicks = False, t + dt_turb, 0
+    for _ in range(nsteps):
+        t_old, c = t, cour
+        if npass:
+            O.copy_velocity(0)
+        if t_turb > t_old and t_turb <= t_old + dt:
+            drive_next = True
+            t_turb = max(t_turb + dt_turb, t + 1e-2 * dt)
+            dt = 0.5 * dt
+        if drive_next:
+            O.set_force(delv, edot, dt_turb)
+            O.step_drive(dt)
+        else:
+            O.step(dt)
+        if npass:
+            O.copy_velocity(1)
+            O.passive_step(dt)
+        t += dt
+        if drive_next:
+            delv, _, seeds = oracle_rand_force(*dims, K=k_max, seeds=seeds)
+            drive_next, c = False, 0.5 * cour
+            kicks += 1
+        dt = c * dx / O.max_speed()
+        O.bc()
+    return O.get_state()[:3], t, dt, t_turb, tuple(seeds), kicks

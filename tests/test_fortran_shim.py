@@ -16,7 +16,7 @@ import sys
 import numpy as np
 import pytest
 
-from refutil import Oracle, oracle_rand_force, random_state
+from refutil import Oracle, oracle_driven, random_state
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
@@ -106,41 +106,6 @@ def test_fortran_example_driver_matches_oracle(tmp_path, name, mhd, riemann, rec
     assert tg == to and dtg == dto
 
 
-def _run_oracle_driven(q, bf, qp, nsteps, dt_turb=0.02, edot=1e-3, k_max=2, seeds=(1234, 5678, 9123)):
-    """The reference driver's loop with drive_turbulence (driver.f90:46-160, 336-344) over
-    the oracle: a force field drawn at start-up and after every kick, a kick when t_turb
-    falls in (t_old, t_old + dt] with dt halved for that step, and COUR halved for the dt
-    after it."""
-    O = Oracle(*DIMS, npass=1, mhd=2, riemann=2, recon=2, bc=2)
-    O.set_state(q, bf, qp)
-    O.bc()
-    dt, t = 1e-15, 0.0
-    delv, _, seeds = oracle_rand_force(*DIMS, K=k_max, seeds=seeds)
-    drive_next, t_turb, kicks = False, t + dt_turb, 0
-    for _ in range(nsteps):
-        t_old, cour = t, 0.35
-        O.copy_velocity(0)
-        if t_turb > t_old and t_turb <= t_old + dt:
-            drive_next = True
-            t_turb = max(t_turb + dt_turb, t + 1e-2 * dt)
-            dt = 0.5 * dt
-        if drive_next:
-            O.set_force(delv, edot, dt_turb)
-            O.step_drive(dt)
-        else:
-            O.step(dt)
-        O.copy_velocity(1)
-        O.passive_step(dt)
-        t += dt
-        if drive_next:
-            delv, _, seeds = oracle_rand_force(*DIMS, K=k_max, seeds=seeds)
-            drive_next, cour = False, 0.5 * 0.35
-            kicks += 1
-        dt = cour * (1.0 / max(DIMS)) / O.max_speed()
-        O.bc()
-    return O.get_state()[:3], t, dt, t_turb, seeds, kicks
-
-
 @pytest.mark.gpu
 def test_fortran_driven_example_matches_oracle(tmp_path):
     """The reference's impulsive forcing schedule through the drop-in boundary: the example
@@ -170,7 +135,7 @@ def test_fortran_driven_example_matches_oracle(tmp_path):
     pg = raw[nq + nb_:nq + nb_ + npp].reshape(qp.shape, order="F")
     tg, dtg, ttg = raw[nq + nb_ + npp:nq + nb_ + npp + 3]
     sg = tuple(int(v) for v in raw[nq + nb_ + npp + 3:nq + nb_ + npp + 6])
-    (qo, bo, po), to, dto, tto, so, kicks = _run_oracle_driven(q, bf, qp, nsteps)
+    (qo, bo, po), to, dto, tto, so, kicks = oracle_driven(DIMS, q, bf, qp, nsteps, riemann=2, recon=2)
     assert kicks >= 3, kicks
     assert sg == tuple(so)
     for g, o in ((tg, to), (dtg, dto), (ttg, tto)):
