@@ -336,26 +336,81 @@ __global__ __launch_bounds__(256) void k_pass_stage_z(const BoxDesc* __restrict_
     const long long sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
     const real dtdx = (real)(a.factor * a.dt[b] / B.dx);
     const long long c0 = cell_index(B, i, j, kb);
-    for (int p = 0; p < a.npass; p++) {
-        const real* w = a.rec[p];
-        long long c = c0;
-        // carried: v_z at k, the scalar at k-1, k, k+1 (k, k+1 for the predictor), and the
-        // value at the face between k-1 and k
-        real vz1 = V(2, c);
-        real flo, wa, wb, wc;
-        {
-            const real vz0 = V(2, c - sz);
-            const real va = real(0.5) * (vz0 + vz1);
-            if (STAGE) {
-                const real wmm = w[c - 2 * sz];
-                wa = w[c - sz]; wb = w[c]; wc = w[c + sz];
-                flo = pass_face_v<real, 1>(wmm, wa, wb, wc, va);
-            } else {
-                wa = w[c - sz]; wb = w[c]; wc = wb;
-                flo = pass_face_v<real, 0>(wa, wa, wb, wb, va);
+    if constexpr (VARR) {
+        // velocities are loads: one passive at a time keeps the fewest live registers
+        for (int p = 0; p < a.npass; p++) {
+            const real* w = a.rec[p];
+            long long c = c0;
+            // carried: v_z at k, the scalar at k-1, k, k+1 (k, k+1 for the predictor), and the
+            // value at the face between k-1 and k
+            real vz1 = V(2, c);
+            real flo, wa, wb, wc;
+            {
+                const real vz0 = V(2, c - sz);
+                const real va = real(0.5) * (vz0 + vz1);
+                if (STAGE) {
+                    const real wmm = w[c - 2 * sz];
+                    wa = w[c - sz]; wb = w[c]; wc = w[c + sz];
+                    flo = pass_face_v<real, 1>(wmm, wa, wb, wc, va);
+                } else {
+                    wa = w[c - sz]; wb = w[c]; wc = wb;
+                    flo = pass_face_v<real, 0>(wa, wa, wb, wb, va);
+                }
+            }
+    #pragma unroll 1
+            for (int k = kb; k < ke; k++, c += sz) {
+                // one live cell index instead of a strength-reduced pointer per array and offset
+                __asm__ volatile("" : "+v"(c));
+                const real vx = V(0, c), vy = V(1, c);
+                const real vz2 = V(2, c + sz);
+                const real vxh = real(0.5) * (vx + V(0, c + 1));
+                const real vxl = real(0.5) * (V(0, c - 1) + vx);
+                const real vyh = real(0.5) * (vy + V(1, c + sy));
+                const real vyl = real(0.5) * (V(1, c - sy) + vy);
+                const real vzh = real(0.5) * (vz1 + vz2);
+                real fhi;
+                if (STAGE) {
+                    const real wd = w[c + 2 * sz];
+                    fhi = pass_face_v<real, 1>(wa, wb, wc, wd, vzh);
+                    wa = wb; wb = wc; wc = wd;
+                } else {
+                    const real wn = w[c + sz];
+                    fhi = pass_face_v<real, 0>(wb, wb, wn, wn, vzh);
+                    wb = wn;
+                }
+                const real dF0 = pass_face(w, c, (long long)1, vxh, STAGE) - pass_face(w, c - 1, (long long)1, vxl, STAGE);
+                const real dF1 = pass_face(w, c, sy, vyh, STAGE) - pass_face(w, c - sy, sy, vyl, STAGE);
+                const real dF2 = fhi - flo;
+                a.dst[p][c] = a.src[p][c] - dtdx * (vx * dF0 + vy * dF1 + vz1 * dF2);
+                flo = fhi;
+                vz1 = vz2;
             }
         }
-#pragma unroll 1
+    } else {
+        // The velocities of a cell are formed once for all passives.  Carried per passive: the
+        // scalar at k-1, k, k+1 (k, k+1 for the predictor) and the value at the face between
+        // k-1 and k; v_z at k.
+        real vz1 = V(2, c0);
+        real flo[MAXPASS], wa[MAXPASS], wb[MAXPASS], wc[MAXPASS];
+        {
+            const real vz0 = V(2, c0 - sz);
+            const real va = real(0.5) * (vz0 + vz1);
+    #pragma unroll
+            for (int p = 0; p < MAXPASS; p++) {
+                if (p >= a.npass) break;
+                const real* w = a.rec[p];
+                if (STAGE) {
+                    const real wmm = w[c0 - 2 * sz];
+                    wa[p] = w[c0 - sz]; wb[p] = w[c0]; wc[p] = w[c0 + sz];
+                    flo[p] = pass_face_v<real, 1>(wmm, wa[p], wb[p], wc[p], va);
+                } else {
+                    wa[p] = w[c0 - sz]; wb[p] = w[c0]; wc[p] = wb[p];
+                    flo[p] = pass_face_v<real, 0>(wa[p], wa[p], wb[p], wb[p], va);
+                }
+            }
+        }
+        long long c = c0;
+    #pragma unroll 1
         for (int k = kb; k < ke; k++, c += sz) {
             // one live cell index instead of a strength-reduced pointer per array and offset
             __asm__ volatile("" : "+v"(c));
@@ -366,21 +421,26 @@ __global__ __launch_bounds__(256) void k_pass_stage_z(const BoxDesc* __restrict_
             const real vyh = real(0.5) * (vy + V(1, c + sy));
             const real vyl = real(0.5) * (V(1, c - sy) + vy);
             const real vzh = real(0.5) * (vz1 + vz2);
-            real fhi;
-            if (STAGE) {
-                const real wd = w[c + 2 * sz];
-                fhi = pass_face_v<real, 1>(wa, wb, wc, wd, vzh);
-                wa = wb; wb = wc; wc = wd;
-            } else {
-                const real wn = w[c + sz];
-                fhi = pass_face_v<real, 0>(wb, wb, wn, wn, vzh);
-                wb = wn;
+    #pragma unroll
+            for (int p = 0; p < MAXPASS; p++) {
+                if (p >= a.npass) break;
+                const real* w = a.rec[p];
+                real fhi;
+                if (STAGE) {
+                    const real wd = w[c + 2 * sz];
+                    fhi = pass_face_v<real, 1>(wa[p], wb[p], wc[p], wd, vzh);
+                    wa[p] = wb[p]; wb[p] = wc[p]; wc[p] = wd;
+                } else {
+                    const real wn = w[c + sz];
+                    fhi = pass_face_v<real, 0>(wb[p], wb[p], wn, wn, vzh);
+                    wb[p] = wn;
+                }
+                const real dF0 = pass_face(w, c, (long long)1, vxh, STAGE) - pass_face(w, c - 1, (long long)1, vxl, STAGE);
+                const real dF1 = pass_face(w, c, sy, vyh, STAGE) - pass_face(w, c - sy, sy, vyl, STAGE);
+                const real dF2 = fhi - flo[p];
+                a.dst[p][c] = a.src[p][c] - dtdx * (vx * dF0 + vy * dF1 + vz1 * dF2);
+                flo[p] = fhi;
             }
-            const real dF0 = pass_face(w, c, (long long)1, vxh, STAGE) - pass_face(w, c - 1, (long long)1, vxl, STAGE);
-            const real dF1 = pass_face(w, c, sy, vyh, STAGE) - pass_face(w, c - sy, sy, vyl, STAGE);
-            const real dF2 = fhi - flo;
-            a.dst[p][c] = a.src[p][c] - dtdx * (vx * dF0 + vy * dF1 + vz1 * dF2);
-            flo = fhi;
             vz1 = vz2;
         }
     }
