@@ -59,7 +59,8 @@ struct imhd_ctx {
     hipStream_t st = nullptr;
     // state buffers (two) and work arrays
     void* pvel[3] = {};    // passive velocities written by the fused VL stages
-    void* q[3][7] = {};    // set 2: half-step state of the fused VL stages
+    void* q[4][7] = {};    // set 2: half-step state of the fused VL stages; TVD: sets 2, 3 hold
+                           // the y and z half states of k_tvd_half3
     double* b[3][3] = {};
     void* qp[2][MAXPASS] = {};
     void* F[3][4] = {};
@@ -135,6 +136,7 @@ struct imhd_ctx {
     double* Ecpr[3] = {};
     unsigned char* pflag = nullptr;
     bool tvdfuse = true;                     // one launch per Roe-TVD sweep (IMHD_TVDFUSE=0: two)
+    bool tvdhalf3 = true;                    // the three TVD half updates in one pass (IMHD_TVDHALF3=0: three)
     int zseg = 4;    // z-segment length of the column (z-marching) kernels
     int strip = 0;   // y-strip height of the work enumeration (0: whole planes)
     bool fuse3 = true;   // the three directional flux sweeps of a stage in one launch
@@ -739,6 +741,27 @@ void tvd_step(imhd_ctx* c) {
         a.dt = c->dt;
         launch_tvd_center<real>(mk(c, c->pl_tcenter), a);
     }
+    if (c->q[3][0] != nullptr) {   // the three half states in one pass, then the sweeps
+        {
+            ProfScope ps(c, "tvd_half3", c->pl_thalf.active);
+            TvdHalfArgs<real> a{};
+            TvdHalfDst<real> o{};
+            const int hs[3] = {s1, 2, 3};
+            for (int f = 0; f < 7; f++) a.q[f] = R<real>(c->q[s0][f]);
+            for (int e = 0; e < 3; e++) {
+                a.b0[e] = c->b[s0][e];
+                a.Ec[e] = c->Ec[e];
+                for (int f = 0; f < 4; f++) a.Fb[e][f] = R<real>(c->F[e][f]);
+                for (int f = 0; f < 7; f++) o.dst[e][f] = R<real>(c->q[hs[e]][f]);
+            }
+            a.dt = c->dt;
+            a.floor_count = c->floor_count;
+            launch_tvd_half3<real>(mk(c, c->pl_thalf), a, o);
+        }
+        tvd_sweep<real>(c, s1, s1, true, 0);
+        tvd_sweep<real>(c, 2, s1, true, 1);
+        tvd_sweep<real>(c, 3, s1, true, 2);
+    } else {
     static const char* hn[3] = {"tvd_half_x", "tvd_half_y", "tvd_half_z"};
     for (int d = 0; d < 3; d++) {
         {
@@ -756,6 +779,7 @@ void tvd_step(imhd_ctx* c) {
             launch_tvd_half<real>(mk(c, c->pl_thalf), a);
         }
         tvd_sweep<real>(c, s1, s1, true, d);
+    }
     }
     tvd_corner<real>(c, s0, true);
     {
@@ -1787,6 +1811,7 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     if (const char* e = getenv("IMHD_STRIP")) c->strip = std::max(0, atoi(e));
     if (const char* e = getenv("IMHD_VSTAGE")) c->vstage = atoi(e) != 0;    // 0: three kernels per stage
     if (const char* e = getenv("IMHD_TVDFUSE")) c->tvdfuse = atoi(e) != 0;  // 0: eig + flux kernels
+    if (const char* e = getenv("IMHD_TVDHALF3")) c->tvdhalf3 = atoi(e) != 0;
     if (const char* e = getenv("IMHD_VROUNDS")) c->vrounds = std::max(0, atoi(e));
     if (const char* e = getenv("IMHD_VSEG")) { c->vseg = std::max(1, atoi(e)); c->vrounds = 0; }
     // FP32: 32 x 16 tiles (2 workgroups of 80 KB LDS per CU); FP64: 16 x 16 (a 32 x 16
@@ -1848,6 +1873,9 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
         if (g.npass > 0)
             for (int d = 0; d < 3; d++) if ((rc = alloc(c, &c->pvel[d], nr))) return rc;
     } else {
+        if (g.integrator == IMHD_TVD && c->tvdhalf3)   // per-direction half states (k_tvd_half3)
+            for (int s = 2; s < 4; s++)
+                for (int f = 0; f < 7; f++) if ((rc = alloc(c, &c->q[s][f], nr))) return rc;
         for (int d = 0; d < 3; d++)
             for (int f = 0; f < 4; f++) if ((rc = alloc(c, &c->F[d][f], nr))) return rc;
         for (int e = 0; e < 6; e++) if ((rc = alloc(c, &c->E[e], nr))) return rc;
@@ -1943,6 +1971,7 @@ int imhd_destroy(imhd_ctx* c) {
         for (auto p : c->qp[s]) fr(p);
     }
     for (auto p : c->q[2]) fr(p);
+    for (auto p : c->q[3]) fr(p);
     for (auto p : c->pvel) fr(p);
     for (auto p : c->b[2]) fr(p);
     fr(c->tp_pred.d);

@@ -57,6 +57,11 @@ struct TvdHalfArgs {
 };
 
 template <class real>
+struct TvdHalfDst {        // the three half states of k_tvd_half3, one set per sweep direction
+    real* dst[3][7];
+};
+
+template <class real>
 struct TvdCtArgs {         // predictor CT: whole box, half step inside [3-nb, n+nb-3]
     const double* b0[3];
     double* b1[3];
@@ -90,6 +95,8 @@ template <class real>
 void launch_tvd_center(const Launch& L, const TvdCenterArgs<real>& a);
 template <class real>
 void launch_tvd_half(const Launch& L, const TvdHalfArgs<real>& a);
+template <class real>
+void launch_tvd_half3(const Launch& L, const TvdHalfArgs<real>& a, const TvdHalfDst<real>& o);
 template <class real>
 void launch_tvd_ct_pred(const Launch& L, const TvdCtArgs<real>& a);
 template <class real>
