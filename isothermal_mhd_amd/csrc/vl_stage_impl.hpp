@@ -56,7 +56,8 @@ __device__ __forceinline__ real stage_face_own(const SweepArgs<real>& a, const r
     }
 }
 // z face k+1/2: predictor between the thread's own cells of planes k (UL) and k+1 (UR),
-// both in registers; corrector as above.
+// both in registers; FP64 corrector with those two of its four stencil cells from
+// registers; FP32 corrector as above (holding them spills at 128 VGPRs).
 template <class real, int RS, int KIND>
 __device__ __forceinline__ real stage_face_z(const SweepArgs<real>& a, const real* UL, const real* UR, long long c,
                                              long long s, real* F) {
@@ -73,6 +74,14 @@ __device__ __forceinline__ real stage_face_z(const SweepArgs<real>& a, const rea
         real smax = real(0);
         riemann<RS>(uL, uR, (real)a.cs, F, smax);
         return smax;
+    } else if constexpr (sizeof(real) == 8) {
+        // FP64 (2 waves per SIMD, registers to spare): planes k and k+1 from registers
+        real W[4][6];
+        prim_load(a, c - s, W[0]);
+        prim_rot(UL[0], UL[perm(2, 1)], UL[perm(2, 2)], UL[perm(2, 3)], UL[perm(2, 5)], UL[perm(2, 6)], W[1]);
+        prim_rot(UR[0], UR[perm(2, 1)], UR[perm(2, 2)], UR[perm(2, 3)], UR[perm(2, 5)], UR[perm(2, 6)], W[2]);
+        prim_load(a, c + 2 * s, W[3]);
+        return plm_flux<real, RS>(a, W, (real)a.bn[c], F);
     } else {
         return flux_corr_calc<real, RS>(a, c, s, F);
     }
