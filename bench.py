@@ -478,6 +478,141 @@ def l1_errors(sa, sb, nb=6):
     return out
 
 
+def l1_fields(pairs, nb=6, npass=0):
+    """Relative L1 of the FP32 run against the FP64 run, sum|a - b| / sum|b| over the
+    interiors of all boxes, per conserved field, cell-centred B and passive scalar; pairs is
+    a list of ((q, bf, qp) FP32, (q, bf, qp) FP64) per box."""
+    import numpy as np
+    names = ["rho", "mx", "my", "mz", "bx", "by", "bz"] + [f"passive{p}" for p in range(npass)]
+    num = dict.fromkeys(names, 0.0)
+    den = dict.fromkeys(names, 0.0)
+    I = (slice(nb, -nb),) * 3
+    Ip = (slice(nb - 2, -(nb - 2)),) * 3
+    for (qa, _, pa), (qb, _, pb) in pairs:
+        for f, nm in enumerate(names[:7]):
+            a, b = qa[I + (f,)], qb[I + (f,)]
+            num[nm] += float(np.abs(a - b).sum())
+            den[nm] += float(np.abs(b).sum())
+        for p in range(npass):
+            a, b = pa[Ip + (p,)], pb[Ip + (p,)]
+            num[f"passive{p}"] += float(np.abs(a - b).sum())
+            den[f"passive{p}"] += float(np.abs(b).sum())
+    return {nm: (num[nm] / den[nm] if den[nm] > 0 else None) for nm in names}   # None: field identically 0
+
+
+def run_l1_configs(cour):
+    """FP32 against FP64 from the same IC, per configuration: the forced TVD workload at
+    512^3 (500 steps), the 48-box ensemble (200 steps) and the 1024^3 physics (lognormal
+    density, k^-5/3 velocity at Mach 3, HLLD + PLM + OU) at 512^3 (200 steps; an FP64 run
+    of 1024^3 does not fit on one GPU).  Reports the relative L1 per field and max|div B|.
+    For the forced workload it also measures how far two FP64 runs drift apart when one
+    starts from rho perturbed by 1e-12, for the TVD and the VL integrator at 256^3: the
+    FP32 error is only meaningful next to that sensitivity of the method itself."""
+    import numpy as np
+    from isothermal_mhd_amd import TvdSolver, VanLeerSolver, ics
+    res = {}
+
+    def advance(S, steps, tag):
+        done = 0
+        while done < steps:
+            k = min(100, steps - done)
+            S.run(k, graph=True)
+            S.sync()
+            done += k
+            print(f"[l1] {tag}: {done}/{steps} steps", flush=True)
+
+    n = 512
+    q, bf, qp = ics.config_c3(n=n)
+    runs = {}
+    for prec in (32, 64):
+        S = TvdSolver((n, n, n), npass=2, riemann="roe", bc="periodic", precision=prec, cour=cour)
+        S.set_state(q, bf, qp)
+        S.set_ou_forcing(kmin=1.0, kmax=2.0, t_corr=0.5, sigma=5.0, seed=3003, solenoidal=True)
+        S.boundary_conditions()
+        advance(S, 500, f"c3 fp{prec}")
+        runs[prec] = (S.get_state(), S.diagnostics()["divb_max"], float(S.get_time()[0][0]))
+        S.close()
+    del q, bf, qp
+    res["c3_forced_tvd_512"] = {
+        "workload": "512^3, K98 Roe-TVD + CTU + CT, OU forcing (1<=|k|<=2, t_corr 0.5, sigma 5, seed 3003), "
+                    "2 passives, 500 steps", "l1": l1_fields([(runs[32][0], runs[64][0])], npass=2),
+        "max_divb_fp32": runs[32][1], "max_divb_fp64": runs[64][1], "t_fp32": runs[32][2], "t_fp64": runs[64][2]}
+    del runs
+
+    dims, states, _ = ics.ensemble_c4()
+    runs = {}
+    for prec in (32, 64):
+        S = VanLeerSolver(dims, npass=0, riemann="hlle", precision=prec, cour=cour)
+        for b, (q, bf, qp) in enumerate(states):
+            S.set_state(q, bf, None, box=b)
+        S.boundary_conditions()
+        advance(S, 200, f"c4 fp{prec}")
+        diag = S.diagnostics()
+        runs[prec] = ([S.get_state(box=b) for b in range(len(dims))], max(d["divb_max"] for d in diag))
+        S.close()
+    del states
+    res["c4_ensemble"] = {
+        "workload": "48 boxes U{32..192}^3 (seed 4004), HLLE + VL2 + CT, PLM, 200 steps",
+        "l1": l1_fields(list(zip(runs[32][0], runs[64][0]))),
+        "max_divb_fp32": runs[32][1], "max_divb_fp64": runs[64][1]}
+    del runs
+
+    # sensitivity of the forced workload to its initial state, per integrator: FP64 against
+    # FP64 started from rho perturbed by 1e-12 (relative, seeded), next to FP32 against FP64
+    n = 256
+    q, bf, qp = ics.config_c3(n=n)
+    qpert = q.copy(order="F")
+    I = (slice(6, -6),) * 3
+    qpert[I + (0,)] *= 1.0 + 1e-12 * np.random.default_rng(7).standard_normal(qpert[I + (0,)].shape)
+    sens = {}
+    for integ in ("tvd", "vl"):
+        st = {}
+        for tag, prec, qq in (("fp64", 64, q), ("fp64_perturbed", 64, qpert), ("fp32", 32, q)):
+            if integ == "tvd":
+                S = TvdSolver((n, n, n), npass=2, riemann="roe", bc="periodic", precision=prec, cour=cour)
+            else:
+                S = VanLeerSolver((n, n, n), npass=2, riemann="hlld", recon="plm", bc="periodic", precision=prec,
+                                  cour=cour)
+            S.set_state(qq, bf, qp)
+            S.set_ou_forcing(kmin=1.0, kmax=2.0, t_corr=0.5, sigma=5.0, seed=3003, solenoidal=True)
+            S.boundary_conditions()
+            st[tag] = []
+            for k in (10, 90, 400):
+                advance(S, k, f"sensitivity {integ} {tag}")
+                st[tag].append(S.get_state())
+            S.close()
+        sens[integ] = {f"after_{s_}_steps": {
+            "fp64_perturbed_vs_fp64": l1_fields([(st["fp64_perturbed"][m], st["fp64"][m])], npass=2),
+            "fp32_vs_fp64": l1_fields([(st["fp32"][m], st["fp64"][m])], npass=2)}
+            for m, s_ in enumerate((10, 100, 500))}
+        del st
+    del q, bf, qp, qpert
+    res["c3_sensitivity_256"] = {
+        "workload": "256^3 forced workload of c3 (OU 1<=|k|<=2, t_corr 0.5, sigma 5, 2 passives), K98 Roe-TVD "
+                    "and, for comparison, HLLD + VL2 + PLM; FP64 from rho * (1 + 1e-12 g) against FP64",
+        "l1": sens}
+
+    n = 512
+    q, bf, qp = ics.config_c5(n=n)
+    runs = {}
+    for prec in (32, 64):
+        S = VanLeerSolver((n, n, n), npass=0, riemann="hlld", recon="plm", bc="periodic", precision=prec,
+                          cour=cour)
+        S.set_state(q, bf, None)
+        S.set_ou_forcing(kmin=1.0, kmax=2.0, t_corr=0.5, sigma=5.0, seed=5005, solenoidal=True)
+        S.boundary_conditions()
+        advance(S, 200, f"c5 physics at 512^3 fp{prec}")
+        runs[prec] = (S.get_state(), S.diagnostics()["divb_max"], float(S.get_time()[0][0]))
+        S.close()
+    del q, bf, qp
+    res["c5_physics_512"] = {
+        "workload": "512^3, rho = 1 +-10% lognormal, B0=(0.05,0.05,0), k^-5/3 velocity 1<=|k|<=16 rms Mach 3 "
+                    "(seed 5005), HLLD + VL2 + CT, PLM, OU forcing (1<=|k|<=2, t_corr 0.5, sigma 5), 200 steps",
+        "l1": l1_fields([(runs[32][0], runs[64][0])]),
+        "max_divb_fp32": runs[32][1], "max_divb_fp64": runs[64][1], "t_fp32": runs[32][2], "t_fp64": runs[64][2]}
+    return res
+
+
 def cpu_baseline(n_cpu, steps, cour):
     """The CPU restatement (oracle/imhd_oracle.c, FP64, one thread) on a bounded sample of
     the same workload: the same IC generator at n_cpu^3, `steps` driver steps."""
@@ -517,6 +652,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--pmc-run", action="store_true", help="short run for rocprofv3 counter collection")
     ap.add_argument("--no-extra", action="store_true", help="skip the ensemble and TVD side measurements")
+    ap.add_argument("--l1", action="store_true",
+                    help="only the FP32-vs-FP64 L1 errors of the forced TVD, ensemble and 1024^3-physics configurations")
     ap.add_argument("--extra-steps", type=int, default=50)
     ap.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--sq-summarize", metavar="SQ_DIR")
@@ -541,6 +678,9 @@ def main():
         return 0
     if args.io:
         print(json.dumps({"io": run_io(args.io)}), flush=True)
+        return 0
+    if args.l1:
+        print(json.dumps({"l1": run_l1_configs(args.cour)}), flush=True)
         return 0
     if args.big:
         lines = []

@@ -106,6 +106,17 @@ def ensemble_c4(nbox=48, seed=4004, lo=32, hi=192, nb=6):
     return dims, states, machs
 
 
+def config_c5(n=1024, seed=5005, nb=6):
+    """rho = 1 with seeded +-10 % lognormal noise (rho = exp(0.1 g - 0.005), g standard
+    normal per cell), B0 = (0.05, 0.05, 0), k^-5/3 velocity on 1 <= |k| <= 16 at rms Mach 3
+    (the 1024^3 configuration; smaller n where an FP64 run must fit beside it)."""
+    g = np.random.default_rng(seed + 7).standard_normal((n, n, n))
+    rho = np.exp(0.1 * g - 0.005)
+    del g
+    v = random_velocity((n, n, n), 1.0, 16.0, 5.0 / 3.0, 3.0, solenoidal=False, seed=seed)
+    return make_state((n, n, n), nb=nb, npass=0, rho=rho, v=v, b0=(0.05, 0.05, 0.0))
+
+
 def separable_large(n, nb=6, nmodes=6, kmax=16, mach=2.0, b0=(0.05, 0.05, 0.0), seed=5005):
     """Large-box throughput IC built without 3-D FFTs: rho = 1, uniform B0, and a velocity
     whose x-component depends on (y, z) only, y on (z, x) and z on (x, y) -- divergence-free
