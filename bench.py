@@ -639,6 +639,53 @@ def cpu_baseline(n_cpu, steps, cour):
                       f"oracle/imhd_oracle.c FP64 single-threaded, {sec:.1f} s"}
 
 
+def _cpu_reference_run(n_cpu, steps, cour):
+    """Body of cpu_reference, run in a child process (the reference's modules PRINT to
+    standard output)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from isothermal_mhd_amd import ics
+    from refutil import Reference
+    q, bf, qp = ics.config_c2(n=n_cpu)
+    R = Reference(f"vl_hlld_p{n_cpu}")
+    R.set_state(q, bf, qp)
+    R.bc()
+    dt = 1.0e-15
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        R.copy_velocity(0)
+        R.step(dt)
+        R.copy_velocity(1)
+        R.passive_step(dt)
+        R.bc()
+        dt = cour * (1.0 / n_cpu) / R.max_speed()
+    return time.perf_counter() - t0
+
+
+def cpu_reference(n_cpu, steps, cour):
+    """The reference's own solver on the host, beside the port: its vanleer_mod /
+    passive_mod / boundary_conditions_mod compiled as they lie with flang -O3 for an
+    n_cpu^3 periodic HLLD box with one passive (oracle/ref/build_ref.py, config
+    vl_hlld_p128; present only where that build was made), one thread, the driver's
+    step order on the same IC generator.  As compiled the reference reconstructs with
+    characteristic PPM, its default; the PLM branch of the benchmark line is unreachable
+    in its build.  Runs in a child process; returns None without the build."""
+    import subprocess
+    if not os.path.exists(os.path.join(ROOT, "oracle", "_ref", f"libimhd_ref_vl_hlld_p{n_cpu}.so")):
+        return None
+    code = (f"import sys; sys.path.insert(0, {ROOT!r}); import bench; "
+            f"print('SECONDS', bench._cpu_reference_run({n_cpu}, {steps}, {cour}))")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT, timeout=600)
+    lines = [l for l in r.stdout.splitlines() if l.startswith("SECONDS ")]
+    if r.returncode != 0 or not lines:
+        return {"error": (r.stderr or r.stdout)[-300:]}
+    sec = float(lines[-1].split()[1])
+    return {"value": n_cpu ** 3 * steps / sec / 1e6, "unit": "Mcell-updates/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"{steps} driver steps at {n_cpu}^3 (same IC generator) of the reference's solver modules "
+                      f"built with flang -O3 (vanleer_solve_3d with HLLD and its default characteristic PPM, "
+                      f"passive_solve_3d, boundary_conditions), FP64 single-threaded, {sec:.1f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -741,6 +788,9 @@ def main():
                 res[key] = {"error": repr(e)}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps, args.cour)
+        ref = cpu_reference(args.cpu_n, args.cpu_steps, args.cour)
+        if ref is not None:
+            res["cpu_reference"] = ref
     print(json.dumps(res))
     return 0
 
