@@ -356,7 +356,8 @@ def run_io(n, reps=2):
     k_pack_cube), imhd_write_restart and imhd_read_restart, to files under $TMPDIR (rates
     include PCIe, the host page cache and the file system); then the device generation of
     the reference's random force (imhd_generate_force, k_max = 2) and the analysis
-    program's velocity spectrum and density PDF (imhd_spectrum, imhd_density_pdf)."""
+    program's velocity spectrum, density PDF and mode decomposition (imhd_spectrum,
+    imhd_density_pdf, imhd_mode_spectra)."""
     import tempfile
     from isothermal_mhd_amd import VanLeerSolver, ics
     q, bf, qp = ics.config_c2(n=n)
@@ -395,7 +396,8 @@ def run_io(n, reps=2):
     out["generate_rand_force"] = {"s": best, "k_max": 2}
     print(f"[io] generate_rand_force: {best * 1e3:.1f} ms", flush=True)
     # in-situ diagnostics of the analysis program: velocity spectrum, density PDF
-    for name, fn in (("spectrum_velocity", lambda: S.spectrum("velocity")), ("density_pdf", lambda: S.density_pdf())):
+    for name, fn in (("spectrum_velocity", lambda: S.spectrum("velocity")), ("density_pdf", lambda: S.density_pdf()),
+                     ("mode_spectra", lambda: S.mode_spectra())):
         best = None
         for _ in range(reps):
             S.sync()

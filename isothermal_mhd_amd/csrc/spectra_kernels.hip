@@ -2,6 +2,7 @@
 // (3D_MPI/Analysis/src/analysis_main.f90:105-226, 543-698) on the device.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include "spectra.hpp"
 
 namespace imhd {
@@ -78,20 +79,58 @@ __global__ __launch_bounds__(256) void k_spec_bin(const float* __restrict__ s, i
         if (hist[b] != 0.0) atomicAdd(&pk[b], hist[b]);
 }
 
-template <class real>
+// Density histogram.  A narrow distribution (a nearly uniform density puts every cell in
+// one bin) must not serialise on one counter: each wave adds once per distinct bin among
+// its lanes (leader's bin by ballot, count of lanes sharing it), into a per-workgroup LDS
+// histogram (nbins <= HIST_LDS_BINS) that is flushed once per block, or straight into the
+// global counts for more bins.  Counts are exact integers, so the order does not matter.
+constexpr int HIST_LDS_BINS = 4096;
+template <class real, bool LDS>
 __global__ __launch_bounds__(256) void k_density_hist(BoxDesc B, const real* __restrict__ rho, float drho, int nbins,
                                                       unsigned long long* __restrict__ counts) {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ unsigned int sh[LDS ? HIST_LDS_BINS : 1];
+    if (LDS) {
+        for (int b = threadIdx.x; b < nbins; b += blockDim.x) sh[b] = 0u;
+        __syncthreads();
+    }
     const long long n = (long long)B.n[0] * B.n[1] * B.n[2];
-    if (g >= n) return;
-    const int i = (int)(g % B.n[0]);
-    const long long t = g / B.n[0];
-    const int j = (int)(t % B.n[1]), k = (int)(t / B.n[1]);
-    const float r = (float)rho[cell_index(B, i + 1, j + 1, k + 1)];
-    const float x = r / drho;
-    if (!(x >= 0.0f)) return;
-    const int ib = (int)x;
-    if (ib < nbins) atomicAdd(&counts[ib], 1ull);
+    const int lane = threadIdx.x & 63;
+    // the trip count depends on the block only, so every lane of a wave takes part in
+    // every ballot
+    for (long long g0 = (long long)blockIdx.x * blockDim.x; g0 < n; g0 += (long long)gridDim.x * blockDim.x) {
+        const long long g = g0 + threadIdx.x;
+        int ib = -1;
+        if (g < n) {
+            const int i = (int)(g % B.n[0]);
+            const long long t = g / B.n[0];
+            const int j = (int)(t % B.n[1]), k = (int)(t / B.n[1]);
+            const float r = (float)rho[cell_index(B, i + 1, j + 1, k + 1)];
+            const float x = r / drho;
+            if (x >= 0.0f) {
+                const int v = (int)x;
+                if (v < nbins) ib = v;
+            }
+        }
+        bool pending = ib >= 0;
+        while (true) {
+            const unsigned long long m = __ballot(pending);
+            if (m == 0ull) break;
+            const int leader = __ffsll((long long)m) - 1;
+            const int bin = __shfl(ib, leader);
+            const bool mine = pending && ib == bin;
+            const unsigned long long same = __ballot(mine);
+            if (lane == leader) {
+                if (LDS) atomicAdd(&sh[bin], (unsigned int)__popcll(same));
+                else atomicAdd(&counts[bin], (unsigned long long)__popcll(same));
+            }
+            if (mine) pending = false;
+        }
+    }
+    if (LDS) {
+        __syncthreads();
+        for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+            if (sh[b]) atomicAdd(&counts[b], (unsigned long long)sh[b]);
+    }
 }
 
 template <class real>
@@ -248,8 +287,11 @@ template <class real>
 void launch_density_hist(hipStream_t st, const BoxDesc& B, const real* rho, float drho, int nbins,
                          unsigned long long* counts) {
     const long long n = (long long)B.n[0] * B.n[1] * B.n[2];
-    hipLaunchKernelGGL((k_density_hist<real>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, B, rho, drho,
-                       nbins, counts);
+    const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 4096);
+    if (nbins <= HIST_LDS_BINS)
+        hipLaunchKernelGGL((k_density_hist<real, true>), dim3(blocks), dim3(256), 0, st, B, rho, drho, nbins, counts);
+    else
+        hipLaunchKernelGGL((k_density_hist<real, false>), dim3(blocks), dim3(256), 0, st, B, rho, drho, nbins, counts);
 }
 
 template void launch_spec_pack<double>(hipStream_t, const BoxDesc&, const SpecPackArgs<double>&);

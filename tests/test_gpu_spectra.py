@@ -66,10 +66,28 @@ def test_density_pdf_matches_oracle():
     S = solver(n, 32)
     S.set_state(q, bf, None)
     qs = S.get_state()[0]
-    for nbins, rmax in ((n, 4.0), (50, 2.0)):
+    # 5000 bins: more than the per-workgroup LDS histogram holds (global counters)
+    for nbins, rmax in ((n, 4.0), (50, 2.0), (5000, 3.0)):
         r, pdf = S.density_pdf(nbins, rmax)
         ro, po = SP.density_pdf(qs, (n, n, n), nbins, rmax)
         assert np.array_equal(r, ro) and np.array_equal(pdf, po)
+    S.close()
+
+
+def test_density_pdf_uniform_density():
+    """Every cell in one bin (the uniform density of a fresh IC): the wave-aggregated
+    counting must still count each cell once."""
+    n = 48
+    q, bf, _ = random_state(n, n, n, npass=0, seed=5)
+    q[..., 0] = 1.0
+    S = solver(n, 32)
+    S.set_state(q, bf, None)
+    for nbins, rmax in ((n, 4.0), (5000, 4.0)):
+        r, pdf = S.density_pdf(nbins, rmax)
+        ro, po = SP.density_pdf(S.get_state()[0], (n, n, n), nbins, rmax)
+        assert np.array_equal(pdf, po)
+        ib = int(1.0 / (rmax / nbins))
+        assert np.count_nonzero(pdf) == 1 and pdf[ib] > 0
     S.close()
 
 
