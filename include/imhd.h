@@ -216,7 +216,9 @@ int imhd_sync(imhd_ctx* ctx);
  *   unnormalised 3-D DFT of each component (hipFFT), |F|^2 of the octant
  *   0 <= kx,ky,kz < n/2 summed over components, doubled (zero mode single), binned into
  *   n shells of width dk = sqrt(3) 2 pi (n/2) / n, times dx^3.  k[b] = b dk, pk[b]: n
- *   values each.  Cubic boxes with even n; bins accumulate in FP64.
+ *   values each.  Cubic boxes with even n; bins accumulate in FP64.  The first call on
+ *   a cube edge n creates the hipFFT plan and keeps its work buffers until imhd_destroy:
+ *   4 n^3 + 8 n^2 (n/2+1) bytes, and 16 n^2 (n/2+1) more once imhd_mode_spectra ran.
  * imhd_density_pdf == compute_density_pdf (:637-698): nbins bins on [0, rho_max) (the
  *   reference: n bins on [0, 4)), normalised to unit integral over the counted cells;
  *   rho[b] is the bin centre.  Cells at or above rho_max are not counted (the reference

@@ -96,6 +96,14 @@ struct imhd_ctx {
     int imp_seeds[3] = {0, 0, 0};
     double imp_t_turb = 0.0;
     double* cour_half = nullptr;
+    // in-situ spectra: the hipFFT R2C plan and its work buffers, kept between calls for
+    // the cube edge fft_n (created on the first diagnostics call, freed by imhd_destroy)
+    hipfftHandle fft_plan = 0;
+    int fft_n = 0;
+    void* fft_in = nullptr;         // n^3 floats (packed field)
+    void* fft_out[3] = {};          // n^2 (n/2+1) complex each (mode decomposition uses 3)
+    void* fft_acc = nullptr;        // (n/2)^3 floats (octant power)
+    void* fft_bins = nullptr;       // 8 n doubles (bins) + 3 (mean field)
     // OU forcing of the native loop
     bool ou_on = false;
     imhd_ou ou_cfg{};
@@ -1662,28 +1670,45 @@ static int generate_force(imhd_ctx* c, int bx, int K, int* seeds) {
 // state, unnormalised 3-D R2C transforms (hipFFT), octant |F|^2 summed over components in
 // REAL*4, binned on the device.
 
-struct FftPlan {
-    hipfftHandle h = 0;
-    bool ok = false;
-    ~FftPlan() {
-        if (ok) (void)hipfftDestroy(h);
+static void fft_release(imhd_ctx* c) {
+    if (c->fft_n) (void)hipfftDestroy(c->fft_plan);
+    for (void* p : {c->fft_in, c->fft_out[0], c->fft_out[1], c->fft_out[2], c->fft_acc, c->fft_bins})
+        if (p) (void)hipFree(p);
+    c->fft_in = c->fft_acc = c->fft_bins = nullptr;
+    c->fft_out[0] = c->fft_out[1] = c->fft_out[2] = nullptr;
+    c->fft_n = 0;
+}
+
+// The plan and buffers for cube edge n, with nout transform outputs.
+static int fft_setup(imhd_ctx* c, int n, int nout) {
+    if (c->fft_n != n) {
+        HIPCK(c, hipStreamSynchronize(c->st));
+        fft_release(c);
+        const int h = n / 2;
+        const long long nc = (long long)n * n * n, no = (long long)h * h * h;
+        if (hipfftPlan3d(&c->fft_plan, n, n, n, HIPFFT_R2C) != HIPFFT_SUCCESS) FAIL(c, IMHD_EHIP, "hipfftPlan3d failed");
+        c->fft_n = n;
+        if (hipfftSetStream(c->fft_plan, c->st) != HIPFFT_SUCCESS) FAIL(c, IMHD_EHIP, "hipfftSetStream failed");
+        HIPCK(c, hipMalloc(&c->fft_in, (size_t)nc * 4));
+        HIPCK(c, hipMalloc(&c->fft_acc, (size_t)no * 4));
+        HIPCK(c, hipMalloc(&c->fft_bins, (size_t)(8 * n + 3) * 8));
     }
-};
+    const long long nk = (long long)n * n * (n / 2 + 1);
+    for (int d = 0; d < nout; d++)
+        if (!c->fft_out[d]) HIPCK(c, hipMalloc(&c->fft_out[d], (size_t)nk * 8));
+    return IMHD_OK;
+}
 
 template <class real>
 static int spectrum_t(imhd_ctx* c, int bx, int which, double* k_out, double* pk_out) {
     const BoxDesc& B = c->hb[bx];
     const int n = B.n[0], h = n / 2;
-    const long long nc = (long long)n * n * n, nk = (long long)n * n * (h + 1), no = (long long)h * h * h;
-    DevBuf fin, fout, acc, pk;
-    HIPCK(c, hipMalloc(&fin.p, (size_t)nc * 4));
-    HIPCK(c, hipMalloc(&fout.p, (size_t)nk * 8));
-    HIPCK(c, hipMalloc(&acc.p, (size_t)no * 4));
-    HIPCK(c, hipMalloc(&pk.p, (size_t)n * 8));
-    FftPlan plan;
-    if (hipfftPlan3d(&plan.h, n, n, n, HIPFFT_R2C) != HIPFFT_SUCCESS) FAIL(c, IMHD_EHIP, "hipfftPlan3d failed");
-    plan.ok = true;
-    if (hipfftSetStream(plan.h, c->st) != HIPFFT_SUCCESS) FAIL(c, IMHD_EHIP, "hipfftSetStream failed");
+    int rc = fft_setup(c, n, 1);
+    if (rc) return rc;
+    struct Raw { double* p; };   // the cached buffers, in the form the launch calls take
+    const Raw fin{(double*)c->fft_in}, fout{(double*)c->fft_out[0]}, acc{(double*)c->fft_acc},
+        pk{(double*)c->fft_bins};
+    const hipfftHandle plan = c->fft_plan;
     SpecPackArgs<real> a{};
     for (int f = 0; f < 7; f++) a.q[f] = R<real>(c->q[c->cur][f]);
     a.which = which;
@@ -1692,7 +1717,7 @@ static int spectrum_t(imhd_ctx* c, int bx, int which, double* k_out, double* pk_
     for (int comp = 0; comp < ncomp; comp++) {
         a.comp = comp;
         launch_spec_pack<real>(c->st, B, a);
-        if (hipfftExecR2C(plan.h, (hipfftReal*)fin.p, (hipfftComplex*)fout.p) != HIPFFT_SUCCESS)
+        if (hipfftExecR2C(plan, (hipfftReal*)fin.p, (hipfftComplex*)fout.p) != HIPFFT_SUCCESS)
             FAIL(c, IMHD_EHIP, "hipfftExecR2C failed");
         launch_spec_accum(c->st, (const float*)fout.p, n, (float*)acc.p, comp == 0);
     }
@@ -1719,16 +1744,13 @@ template <class real>
 static int mode_spectra_t(imhd_ctx* c, int bx, double* k_out, double* P_out) {
     const BoxDesc& B = c->hb[bx];
     const int n = B.n[0], h = n / 2;
-    const long long nc = (long long)n * n * n, nk = (long long)n * n * (h + 1);
-    DevBuf fin, f[3], P, b0;
-    HIPCK(c, hipMalloc(&fin.p, (size_t)nc * 4));
-    for (int d = 0; d < 3; d++) HIPCK(c, hipMalloc(&f[d].p, (size_t)nk * 8));
-    HIPCK(c, hipMalloc(&P.p, (size_t)8 * n * 8));
-    HIPCK(c, hipMalloc(&b0.p, 3 * 8));
-    FftPlan plan;
-    if (hipfftPlan3d(&plan.h, n, n, n, HIPFFT_R2C) != HIPFFT_SUCCESS) FAIL(c, IMHD_EHIP, "hipfftPlan3d failed");
-    plan.ok = true;
-    if (hipfftSetStream(plan.h, c->st) != HIPFFT_SUCCESS) FAIL(c, IMHD_EHIP, "hipfftSetStream failed");
+    int rc = fft_setup(c, n, 3);
+    if (rc) return rc;
+    struct Raw { double* p; };
+    const Raw fin{(double*)c->fft_in},
+        f[3] = {{(double*)c->fft_out[0]}, {(double*)c->fft_out[1]}, {(double*)c->fft_out[2]}},
+        P{(double*)c->fft_bins}, b0{(double*)c->fft_bins + 8 * n};
+    const hipfftHandle plan = c->fft_plan;
     SpecPackArgs<real> a{};
     for (int q = 0; q < 7; q++) a.q[q] = R<real>(c->q[c->cur][q]);
     a.which = SPEC_VPERP;
@@ -1736,7 +1758,7 @@ static int mode_spectra_t(imhd_ctx* c, int bx, double* k_out, double* P_out) {
     for (int d = 0; d < 3; d++) {
         a.comp = d;
         launch_spec_pack<real>(c->st, B, a);
-        if (hipfftExecR2C(plan.h, (hipfftReal*)fin.p, (hipfftComplex*)f[d].p) != HIPFFT_SUCCESS)
+        if (hipfftExecR2C(plan, (hipfftReal*)fin.p, (hipfftComplex*)f[d].p) != HIPFFT_SUCCESS)
             FAIL(c, IMHD_EHIP, "hipfftExecR2C failed");
     }
     launch_reset_u64(c->st, (unsigned long long*)b0.p, 3);
@@ -1999,6 +2021,7 @@ int imhd_destroy(imhd_ctx* c) {
     for (void* p : {(void*)c->force_part, (void*)c->force_par, (void*)c->force_err, (void*)c->dvmax,
                     (void*)c->cour_half})
         fr(p);
+    fft_release(c);
     for (void* p : {(void*)c->smax, (void*)c->cmax, (void*)c->dmax, (void*)c->dt, (void*)c->t, (void*)c->dtprev, (void*)c->cour,
                     (void*)c->diag, (void*)c->floor_count, (void*)c->stage, (void*)c->db})
         fr(p);
