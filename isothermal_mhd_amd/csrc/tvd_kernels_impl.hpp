@@ -379,9 +379,9 @@ __global__ __launch_bounds__(256) void k_tvd_sweep_x(const BoxDesc* __restrict__
 // y / z sweep: threads along x (coalesced), each marching a segment of pb.seg flux faces
 // along the sweep axis with a three-face window of g~_k in registers.  Items enumerate
 // (x, transverse index, segment), x fastest.
-template <class real>
 // FP64: at most 256 registers, two waves per SIMD (the unbounded allocation took 268 and
-// left one: 1371 us per call at 256^3; bounded, 6 registers spill)
+// left one: 1371 us per call at 256^3; bounded, 6 registers spill).
+template <class real>
 __global__ __launch_bounds__(256, sizeof(real) == 8 ? 2 : 1) void k_tvd_sweep_m(const BoxDesc* __restrict__ boxes, Plan plan,
                                                                               TvdSweepArgs<real> a) {
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
