@@ -155,7 +155,10 @@ int imhd_max_speed(imhd_ctx* ctx, double* cmax);
 int imhd_copy_velocity(imhd_ctx* ctx, int stage);
 int imhd_passive_step(imhd_ctx* ctx, double dt);
 
-/* compute_KE + compute_divB (driver.f90:492-553) and max|div B|; out[box] */
+/* compute_KE + compute_divB (driver.f90:492-553) and max|div B|; out[box].  Sums are
+ * reduced in a fixed order (per-workgroup partial rows, then one fixed-order pass per box),
+ * so repeated calls on the same state return the same bits.  The first call allocates the
+ * partial rows: 1024 x 8 doubles per box (64 KiB), counted in imhd_device_bytes. */
 int imhd_diagnostics(imhd_ctx* ctx, imhd_diag* out);
 
 /* Number of cells the density floor reset since creation (final_check_unphysical,
@@ -201,7 +204,8 @@ const char* imhd_profile_report(imhd_ctx* ctx);
 /* Launch-shape autotuner (no counterpart in the reference): times `steps` native-loop steps
  * (imhd_run without a graph) from the current state for each candidate z-segment length of
  * the column update kernels and y-strip height of the work enumeration, keeps the fastest,
- * and restores state, time and forcing state exactly.  `report` (may be NULL) receives the
+ * and restores state, time and forcing state exactly -- also when a trial fails, in which
+ * case the launch shape in use before the call is kept and the first error returned.  `report` (may be NULL) receives the
  * per-candidate timings and the choice. */
 int imhd_autotune(imhd_ctx* ctx, int steps, char* report, int report_len);
 
@@ -216,9 +220,14 @@ int imhd_sync(imhd_ctx* ctx);
  *   unnormalised 3-D DFT of each component (hipFFT), |F|^2 of the octant
  *   0 <= kx,ky,kz < n/2 summed over components, doubled (zero mode single), binned into
  *   n shells of width dk = sqrt(3) 2 pi (n/2) / n, times dx^3.  k[b] = b dk, pk[b]: n
- *   values each.  Cubic boxes with even n; bins accumulate in FP64.  The first call on
- *   a cube edge n creates the hipFFT plan and keeps its work buffers until imhd_destroy:
- *   4 n^3 + 8 n^2 (n/2+1) bytes, and 16 n^2 (n/2+1) more once imhd_mode_spectra ran.
+ *   values each.  Cubic boxes with even n; bins accumulate in FP64 with device atomics, so
+ *   the order of the additions varies from call to call: a bin of m modes may differ
+ *   between two calls on the same state by at most m 2^-53 of its value (in practice a few
+ *   ulp).  The first call on a cube edge n creates the hipFFT plan and keeps its buffers
+ *   until imhd_destroy: 4 n^3 (packed field) + n^3 / 2 (octant power) + 8 n^2 (n/2+1)
+ *   (transform) + 64 n + 24 (bins) bytes plus hipFFT's work area, and 16 n^2 (n/2+1) more
+ *   once imhd_mode_spectra ran; all of it counts in imhd_device_bytes.  A failed set-up
+ *   releases what it made, so the next call plans again.
  * imhd_density_pdf == compute_density_pdf (:637-698): nbins bins on [0, rho_max) (the
  *   reference: n bins on [0, 4)), normalised to unit integral over the counted cells;
  *   rho[b] is the bin centre.  Cells at or above rho_max are not counted (the reference

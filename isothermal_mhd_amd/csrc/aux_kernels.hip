@@ -138,42 +138,78 @@ __global__ void k_reset_u64(unsigned long long* p, int n) {
 
 // ---------------------------------------------------------------------------------
 // Diagnostics (driver.f90:492-553) in FP64 plus total mass and max|div B|.
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+//
+// Deterministic two-pass reduction: DIAG_WG workgroups per box each walk a fixed set of
+// the box's interior cells (cell loc = w*256 + t + s*DIAG_WG*256, x fastest), sum serially
+// per thread, then in a fixed tree per workgroup, and write one partial row of DIAG_STRIDE
+// doubles (7 sums + the maximum); one workgroup per box then adds its DIAG_WG rows in a
+// fixed order.  The result bits do not depend on scheduling, so repeated calls agree
+// exactly; no memory has to start at zero (every partial and result is written).
+__device__ __forceinline__ void block_reduce8(double (*sh)[256], double* v) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int s = 0; s < DIAG_STRIDE; s++) sh[s][t] = v[s];
     __syncthreads();
-    if (l == 0) sh[w] = v;
-    __syncthreads();
-    double s = 0.0;
-    if (threadIdx.x == 0) for (int x = 0; x < (int)(blockDim.x >> 6); x++) s += sh[x];
-    return s;
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+#pragma unroll
+            for (int s = 0; s < DIAG_STRIDE - 1; s++) sh[s][t] = sh[s][t] + sh[s][t + w];
+            sh[DIAG_STRIDE - 1][t] = fmax(sh[DIAG_STRIDE - 1][t], sh[DIAG_STRIDE - 1][t + w]);
+        }
+        __syncthreads();
+    }
 }
 
 template <class real>
-__global__ __launch_bounds__(256) void k_diag(const BoxDesc* __restrict__ boxes, Plan plan, DiagArgs<real> a) {
-    __shared__ double sh[4];
-    int b = 0, i, j, k;
-    double v[7] = {0, 0, 0, 0, 0, 0, 0};
-    double dmax = 0.0;
-    bool act = plan_item(plan, b, i, j, k);
-    if (act) {
-        const BoxDesc& B = boxes[b];
-        long long c = cell_index(B, i, j, k);
-        const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
-        double r0 = a.q[0][c];
-        double rho = fmin(fmax(r0, 1e-15), 1e10);
-        double m1 = a.q[1][c], m2 = a.q[2][c], m3 = a.q[3][c];
-        double vsqr = fmin((m1 * m1 + m2 * m2 + m3 * m3) / (rho * rho), 1e5);
-        double db = (a.b[0][c] - a.b[0][c - sx]) + (a.b[1][c] - a.b[1][c - sy]) + (a.b[2][c] - a.b[2][c - sz]);
-        v[0] = rho * vsqr; v[1] = vsqr; v[2] = m1; v[3] = m2; v[4] = m3; v[5] = db; v[6] = r0;
-        dmax = fabs(db);
+__global__ __launch_bounds__(256) void k_diag_part(const BoxDesc* __restrict__ boxes, DiagArgs<real> a) {
+    __shared__ double sh[DIAG_STRIDE][256];
+    const int b = blockIdx.y;
+    const BoxDesc& B = boxes[b];
+    const unsigned long long nx = B.n[0], nxy = nx * (unsigned long long)B.n[1], ncell = nxy * B.n[2];
+    const long long sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
+    double v[DIAG_STRIDE] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (unsigned long long loc = (unsigned long long)blockIdx.x * 256 + threadIdx.x; loc < ncell;
+         loc += (unsigned long long)DIAG_WG * 256) {
+        unsigned long long k, r, j, i;
+        if (ncell <= 0xffffffffull) {   // 32-bit index arithmetic whenever the box allows it
+            const unsigned l32 = (unsigned)loc, k32 = l32 / (unsigned)nxy, r32 = l32 - k32 * (unsigned)nxy;
+            const unsigned j32 = r32 / (unsigned)nx;
+            k = k32; j = j32; i = r32 - j32 * (unsigned)nx;
+        } else {
+            k = loc / nxy; r = loc - k * nxy; j = r / nx; i = r - j * nx;
+        }
+        const long long c = cell_index(B, (int)i + 1, (int)j + 1, (int)k + 1);
+        const double r0 = a.q[0][c];
+        const double rho = fmin(fmax(r0, 1e-15), 1e10);
+        const double m1 = a.q[1][c], m2 = a.q[2][c], m3 = a.q[3][c];
+        const double vsqr = fmin((m1 * m1 + m2 * m2 + m3 * m3) / (rho * rho), 1e5);
+        const double db = (a.b[0][c] - a.b[0][c - 1]) + (a.b[1][c] - a.b[1][c - sy]) + (a.b[2][c] - a.b[2][c - sz]);
+        v[0] = v[0] + rho * vsqr;
+        v[1] = v[1] + vsqr;
+        v[2] = v[2] + m1;
+        v[3] = v[3] + m2;
+        v[4] = v[4] + m3;
+        v[5] = v[5] + db;
+        v[6] = v[6] + r0;
+        v[7] = fmax(v[7], fabs(db));
     }
-    // one box per block for diagnostics plans (256-aligned starts)
-    for (int f = 0; f < 7; f++) {
-        double s = block_sum(v[f], sh);
-        if (threadIdx.x == 0) atomicAdd(&a.out[b * DIAG_STRIDE + f], s);
+    block_reduce8(sh, v);
+    if (threadIdx.x < DIAG_STRIDE) a.part[((size_t)b * DIAG_WG + blockIdx.x) * DIAG_STRIDE + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// one workgroup per box: its DIAG_WG partial rows in a fixed order -> out[b * DIAG_STRIDE + s]
+__global__ __launch_bounds__(256) void k_diag_final(const double* __restrict__ part, double* __restrict__ out) {
+    __shared__ double sh[DIAG_STRIDE][256];
+    const int b = blockIdx.x;
+    double v[DIAG_STRIDE] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int w = threadIdx.x; w < DIAG_WG; w += 256) {
+        const double* p = part + ((size_t)b * DIAG_WG + w) * DIAG_STRIDE;
+#pragma unroll
+        for (int s = 0; s < DIAG_STRIDE - 1; s++) v[s] = v[s] + p[s];
+        v[DIAG_STRIDE - 1] = fmax(v[DIAG_STRIDE - 1], p[DIAG_STRIDE - 1]);
     }
-    atomic_max_nonneg(a.divb_max, b, dmax);
+    block_reduce8(sh, v);
+    if (threadIdx.x < DIAG_STRIDE) out[(size_t)b * DIAG_STRIDE + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // ---------------------------------------------------------------------------------
@@ -582,8 +618,9 @@ void launch_max_speed(const Launch& L, const QPtrs<real>& q, double cs, unsigned
     if (L.plan.total > 0) L256((k_max_speed<real>), L.boxes, L.plan, q, cs, cmax);
 }
 template <class real>
-void launch_diag(const Launch& L, const DiagArgs<real>& a) {
-    if (L.plan.total > 0) L256((k_diag<real>), L.boxes, L.plan, a);
+void launch_diag(hipStream_t st, const BoxDesc* boxes, int nbox, const DiagArgs<real>& a) {
+    hipLaunchKernelGGL((k_diag_part<real>), dim3(DIAG_WG, nbox), dim3(256), 0, st, boxes, a);
+    hipLaunchKernelGGL(k_diag_final, dim3(nbox), dim3(256), 0, st, (const double*)a.part, a.out);
 }
 template <class real>
 void launch_copy_velocity(const Launch& L, const VelArgs<real>& a) {
@@ -636,7 +673,7 @@ void launch_pack_cube(hipStream_t st, const BoxDesc& B, const CubeArgs<real>& a,
     template void launch_restore_shell<T>(const Launch&, const ShellArgs<T>&);                                \
     template void launch_pass_stage<T>(const Launch&, const PassStageArgs<T>&);                               \
     template void launch_max_speed<T>(const Launch&, const QPtrs<T>&, double, unsigned long long*);                 \
-    template void launch_diag<T>(const Launch&, const DiagArgs<T>&);                                          \
+    template void launch_diag<T>(hipStream_t, const BoxDesc*, int, const DiagArgs<T>&);                      \
     template void launch_copy_velocity<T>(const Launch&, const VelArgs<T>&);                                  \
     template void launch_pass_flux<T>(const Launch&, const PassFluxArgs<T>&);                                 \
     template void launch_pass_update<T>(const Launch&, const PassUpdArgs<T>&);                                \

@@ -6,6 +6,7 @@
 namespace imhd {
 
 constexpr int DIAG_STRIDE = 8;  // per box: rho v^2, v^2, mx, my, mz, sum divB, mass, max|divB|
+constexpr int DIAG_WG = 1024;   // partial-sum workgroups (and partial rows) per box
 
 template <class real>
 struct GhostArgs {
@@ -23,8 +24,8 @@ template <class real>
 struct DiagArgs {
     const real* q[4];
     const double* b[3];
+    double* part;                  // DIAG_WG partial rows of DIAG_STRIDE doubles per box
     double* out;                   // DIAG_STRIDE doubles per box
-    unsigned long long* divb_max;  // striped per-box maxima
 };
 
 template <class real>
@@ -116,7 +117,7 @@ template <class real> void launch_fill_ghosts(const Launch& L, const GhostArgs<r
 template <class real> void launch_fill_ghost_band(const Launch& L, const GhostArgs<real>& a);
 template <class real> void launch_pass_stage(const Launch& L, const PassStageArgs<real>& a);
 template <class real> void launch_max_speed(const Launch& L, const QPtrs<real>& q, double cs, unsigned long long* cmax);
-template <class real> void launch_diag(const Launch& L, const DiagArgs<real>& a);
+template <class real> void launch_diag(hipStream_t st, const BoxDesc* boxes, int nbox, const DiagArgs<real>& a);
 template <class real> void launch_copy_velocity(const Launch& L, const VelArgs<real>& a);
 template <class real> void launch_pass_flux(const Launch& L, const PassFluxArgs<real>& a);
 template <class real> void launch_pass_update(const Launch& L, const PassUpdArgs<real>& a);

@@ -80,7 +80,7 @@ struct imhd_ctx {
     // per-box device scalars
     unsigned long long* smax = nullptr;
     unsigned long long* cmax = nullptr;
-    unsigned long long* dmax = nullptr;  // max|div B| stripes
+    double* diag_part = nullptr;         // imhd_diagnostics: DIAG_WG partial rows per box (first call)
     unsigned long long* dvmax = nullptr; // forcing: max |delta m| / rho0 of the last step (delv_max)
     // forcing application (driver-supplied field, turb_driver_mod.f90)
     double* delv[3] = {};
@@ -99,7 +99,9 @@ struct imhd_ctx {
     // in-situ spectra: the hipFFT R2C plan and its work buffers, kept between calls for
     // the cube edge fft_n (created on the first diagnostics call, freed by imhd_destroy)
     hipfftHandle fft_plan = 0;
+    bool fft_have_plan = false;
     int fft_n = 0;
+    unsigned long long fft_bytes = 0;   // plan work area and buffers (in dev_bytes too)
     void* fft_in = nullptr;         // n^3 floats (packed field)
     void* fft_out[3] = {};          // n^2 (n/2+1) complex each (mode decomposition uses 3)
     void* fft_acc = nullptr;        // (n/2)^3 floats (octant power)
@@ -1671,31 +1673,62 @@ static int generate_force(imhd_ctx* c, int bx, int K, int* seeds) {
 // REAL*4, binned on the device.
 
 static void fft_release(imhd_ctx* c) {
-    if (c->fft_n) (void)hipfftDestroy(c->fft_plan);
+    if (c->fft_have_plan) (void)hipfftDestroy(c->fft_plan);
+    c->fft_have_plan = false;
     for (void* p : {c->fft_in, c->fft_out[0], c->fft_out[1], c->fft_out[2], c->fft_acc, c->fft_bins})
         if (p) (void)hipFree(p);
     c->fft_in = c->fft_acc = c->fft_bins = nullptr;
     c->fft_out[0] = c->fft_out[1] = c->fft_out[2] = nullptr;
     c->fft_n = 0;
+    c->dev_bytes -= c->fft_bytes;
+    c->fft_bytes = 0;
 }
 
-// The plan and buffers for cube edge n, with nout transform outputs.
+static int fft_malloc(imhd_ctx* c, void** p, size_t bytes) {
+    HIPCK(c, hipMalloc(p, bytes));
+    c->fft_bytes += bytes;
+    c->dev_bytes += bytes;
+    return IMHD_OK;
+}
+
+// The plan and buffers for cube edge n, with nout transform outputs.  All or nothing: on
+// any failure everything made so far is released (fft_n stays 0), so the next call plans
+// again instead of launching on missing buffers.  The bytes (plan work area included)
+// count in imhd_device_bytes.
 static int fft_setup(imhd_ctx* c, int n, int nout) {
+    int rc = IMHD_OK;
     if (c->fft_n != n) {
         HIPCK(c, hipStreamSynchronize(c->st));
         fft_release(c);
         const int h = n / 2;
         const long long nc = (long long)n * n * n, no = (long long)h * h * h;
+        size_t work = 0;
         if (hipfftPlan3d(&c->fft_plan, n, n, n, HIPFFT_R2C) != HIPFFT_SUCCESS) FAIL(c, IMHD_EHIP, "hipfftPlan3d failed");
+        c->fft_have_plan = true;
+        if (hipfftSetStream(c->fft_plan, c->st) != HIPFFT_SUCCESS || hipfftGetSize(c->fft_plan, &work) != HIPFFT_SUCCESS) {
+            fft_release(c);
+            FAIL(c, IMHD_EHIP, "hipfftSetStream / hipfftGetSize failed");
+        }
+        c->fft_bytes += work;
+        c->dev_bytes += work;
+        if ((rc = fft_malloc(c, &c->fft_in, (size_t)nc * 4)) || (rc = fft_malloc(c, &c->fft_acc, (size_t)no * 4)) ||
+            (rc = fft_malloc(c, &c->fft_bins, (size_t)(8 * n + 3) * 8))) {
+            fft_release(c);
+            return rc;
+        }
+        if (getenv("IMHD_TEST_FFT_SETUP_FAIL")) {   // tests: a set-up that fails after allocating
+            fft_release(c);
+            FAIL(c, IMHD_EHIP, "spectrum set-up failed (IMHD_TEST_FFT_SETUP_FAIL)");
+        }
         c->fft_n = n;
-        if (hipfftSetStream(c->fft_plan, c->st) != HIPFFT_SUCCESS) FAIL(c, IMHD_EHIP, "hipfftSetStream failed");
-        HIPCK(c, hipMalloc(&c->fft_in, (size_t)nc * 4));
-        HIPCK(c, hipMalloc(&c->fft_acc, (size_t)no * 4));
-        HIPCK(c, hipMalloc(&c->fft_bins, (size_t)(8 * n + 3) * 8));
     }
+    if (!c->fft_in || !c->fft_acc || !c->fft_bins) FAIL(c, IMHD_ESTATE, "spectrum buffers missing");
     const long long nk = (long long)n * n * (n / 2 + 1);
     for (int d = 0; d < nout; d++)
-        if (!c->fft_out[d]) HIPCK(c, hipMalloc(&c->fft_out[d], (size_t)nk * 8));
+        if (!c->fft_out[d] && (rc = fft_malloc(c, &c->fft_out[d], (size_t)nk * 8))) {
+            c->fft_out[d] = nullptr;   // the next call allocates it again
+            return rc;
+        }
     return IMHD_OK;
 }
 
@@ -1925,7 +1958,6 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     }
     if ((rc = alloc(c, (void**)&c->smax, 8 * stripe_slots(nbox)))) return rc;
     if ((rc = alloc(c, (void**)&c->cmax, 8 * stripe_slots(nbox)))) return rc;
-    if ((rc = alloc(c, (void**)&c->dmax, 8 * stripe_slots(nbox)))) return rc;
     if ((rc = alloc(c, (void**)&c->dvmax, 8 * stripe_slots(nbox)))) return rc;
     if ((rc = alloc(c, (void**)&c->dt, 8 * nbox))) return rc;
     if ((rc = alloc(c, (void**)&c->t, 8 * nbox))) return rc;
@@ -2022,7 +2054,7 @@ int imhd_destroy(imhd_ctx* c) {
                     (void*)c->cour_half})
         fr(p);
     fft_release(c);
-    for (void* p : {(void*)c->smax, (void*)c->cmax, (void*)c->dmax, (void*)c->dt, (void*)c->t, (void*)c->dtprev, (void*)c->cour,
+    for (void* p : {(void*)c->smax, (void*)c->cmax, (void*)c->diag_part, (void*)c->dt, (void*)c->t, (void*)c->dtprev, (void*)c->cour,
                     (void*)c->diag, (void*)c->floor_count, (void*)c->stage, (void*)c->db})
         fr(p);
     PlanDev* pls[] = {&c->pl_pflux[0], &c->pl_pflux[1], &c->pl_pflux[2], &c->pl_pcorner, &c->pl_pupd,
@@ -2161,6 +2193,7 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
     };
     std::string rep;
     Shape bsh{c->zseg, c->strip, c->vshape, c->vseg, c->vrounds};
+    const Shape orig = bsh;
     float best = 1e30f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!rc) { HIPCK(c, hipEventCreate(&e0)); HIPCK(c, hipEventCreate(&e1)); }
@@ -2199,6 +2232,16 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
     }
     if (!rc) rc = set_shape(bsh);
     if (!rc) rc = restore();
+    if (rc) {
+        // a failed trial (or snapshot) leaves the launch shape and state as they were before
+        // the call: the original shape and every snapshot taken are put back; the first
+        // error is the one returned
+        const std::string first = c->err;
+        (void)set_shape(orig);
+        (void)hipStreamSynchronize(c->st);
+        (void)restore();
+        c->err = first;
+    }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     for (auto& sv : saved) (void)hipFree(sv.second);
@@ -2448,32 +2491,7 @@ int imhd_max_speed(imhd_ctx* c, double* cmax) {
     return IMHD_OK;
 }
 
-int imhd_diagnostics(imhd_ctx* c, imhd_diag* out) {
-    if (!c || !out) return IMHD_EINVAL;
-    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
-    HIPCK(c, hipMemsetAsync(c->diag, 0, 8 * DIAG_STRIDE * c->nbox, c->st));
-    launch_reset_u64(c->st, c->dmax, (int)stripe_slots(c->nbox));
-    if (c->prec == 64) {
-        DiagArgs<double> a{};
-        for (int f = 0; f < 4; f++) a.q[f] = R<double>(c->q[c->cur][f]);
-        for (int d = 0; d < 3; d++) a.b[d] = c->b[c->cur][d];
-        a.out = c->diag;
-        a.divb_max = c->dmax;
-        launch_diag<double>(mk(c, c->pl_int), a);
-    } else {
-        DiagArgs<float> a{};
-        for (int f = 0; f < 4; f++) a.q[f] = R<float>(c->q[c->cur][f]);
-        for (int d = 0; d < 3; d++) a.b[d] = c->b[c->cur][d];
-        a.out = c->diag;
-        a.divb_max = c->dmax;
-        launch_diag<float>(mk(c, c->pl_int), a);
-    }
-    HIPCK(c, hipGetLastError());
-    std::vector<double> h(DIAG_STRIDE * c->nbox);
-    std::vector<unsigned long long> dm(stripe_slots(c->nbox));
-    HIPCK(c, hipStreamSynchronize(c->st));
-    HIPCK(c, hipMemcpy(h.data(), c->diag, 8 * h.size(), hipMemcpyDeviceToHost));
-    HIPCK(c, hipMemcpy(dm.data(), c->dmax, 8 * dm.size(), hipMemcpyDeviceToHost));
+static void diag_unpack(const imhd_ctx* c, const double* h, imhd_diag* out) {
     for (int b = 0; b < c->nbox; b++) {
         double dx = c->hb[b].dx, dx3 = dx * dx * dx;
         const double* v = &h[b * DIAG_STRIDE];
@@ -2484,11 +2502,44 @@ int imhd_diagnostics(imhd_ctx* c, imhd_diag* out) {
         out[b].momz = v[4] * dx3;
         out[b].divb_sum = v[5];
         out[b].mass = v[6] * dx3;
-        unsigned long long m = 0;
-        for (int st = 0; st < NSTRIPE; st++) m = std::max(m, dm[((size_t)b * NSTRIPE + st) * STRIPE_STRIDE]);
-        memcpy(&out[b].divb_max, &m, 8);
+        out[b].divb_max = v[7];
     }
+}
+
+// partial rows into `part` (DIAG_WG * DIAG_STRIDE doubles per box), results into `res`
+// (DIAG_STRIDE per box), both on the device, then the results to the host
+static int diagnostics_into(imhd_ctx* c, double* part, double* res, imhd_diag* out) {
+    if (c->prec == 64) {
+        DiagArgs<double> a{};
+        for (int f = 0; f < 4; f++) a.q[f] = R<double>(c->q[c->cur][f]);
+        for (int d = 0; d < 3; d++) a.b[d] = c->b[c->cur][d];
+        a.part = part;
+        a.out = res;
+        launch_diag<double>(c->st, c->db, c->nbox, a);
+    } else {
+        DiagArgs<float> a{};
+        for (int f = 0; f < 4; f++) a.q[f] = R<float>(c->q[c->cur][f]);
+        for (int d = 0; d < 3; d++) a.b[d] = c->b[c->cur][d];
+        a.part = part;
+        a.out = res;
+        launch_diag<float>(c->st, c->db, c->nbox, a);
+    }
+    HIPCK(c, hipGetLastError());
+    std::vector<double> h(DIAG_STRIDE * c->nbox);
+    HIPCK(c, hipMemcpyAsync(h.data(), res, 8 * h.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCK(c, hipStreamSynchronize(c->st));
+    diag_unpack(c, h.data(), out);
     return IMHD_OK;
+}
+
+int imhd_diagnostics(imhd_ctx* c, imhd_diag* out) {
+    if (!c || !out) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    if (!c->diag_part) {
+        int rc = alloc(c, (void**)&c->diag_part, sizeof(double) * DIAG_WG * DIAG_STRIDE * (size_t)c->nbox);
+        if (rc) { c->diag_part = nullptr; return rc; }
+    }
+    return diagnostics_into(c, c->diag_part, c->diag, out);
 }
 
 int imhd_floor_count(imhd_ctx* c, unsigned int* count) {
