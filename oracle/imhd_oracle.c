@@ -848,6 +848,23 @@ static void vl_step(orc_ctx *c, double dt) {
     ct_update(c, dtdx, offset);
 }
 
+/* One update_hydro_state_3d + update_CT_3d (vanleer_mod.f90:2080-2259) on the context's
+ * current state with caller-given arrays: Fs and Fs_prot (21 padded fields each,
+ * [dir][comp]), emf_corner and emf_corner_prot (3 each); the stage's active region is
+ * shrunk by `off` as in vanleer_solve_3d.  Lets a test check the protection's flag, face,
+ * corner and redo sets on arbitrary fluxes.  Needs a context created with protection. */
+int orc_update_test(orc_ctx *c, double dtdx, int off, const double *Fs, const double *Fsp, const double *ec,
+                    const double *ecp) {
+    if (!c->Fsp) return -1;
+    memcpy(c->Fs, Fs, sizeof(double) * 21 * c->NC);
+    memcpy(c->Fsp, Fsp, sizeof(double) * 21 * c->NC);
+    memcpy(c->ec, ec, sizeof(double) * 3 * c->NC);
+    memcpy(c->ecp, ecp, sizeof(double) * 3 * c->NC);
+    vl_update_hydro(c, dtdx, off);
+    ct_update(c, dtdx, off);
+    return 0;
+}
+
 #include "imhd_oracle_tvd.inc"
 
 #define orc_ou struct orc_ou_s

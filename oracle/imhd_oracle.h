@@ -75,6 +75,10 @@ void orc_copy_velocity(orc_ctx *c, int stage);
 void orc_passive_step(orc_ctx *c, double dt);
 void orc_diagnostics(const orc_ctx *c, orc_diag *out);
 int orc_floor_count(const orc_ctx *c);
+/* one hydro + CT update of the current state with given Fs, Fs_prot, emf_corner and
+ * emf_corner_prot (protection contexts only; -1 otherwise) */
+int orc_update_test(orc_ctx *c, double dtdx, int off, const double *Fs, const double *Fsp, const double *ec,
+                    const double *ecp);
 
 /* Per-interface pins: ni x nj planes (Fortran order, component slowest). */
 void orc_riemann_plane(int kind, int ni, int nj, double cs, const double *qL,

@@ -62,6 +62,8 @@ def oracle_lib():
         L.orc_floor_count.argtypes = [ctypes.c_void_p]
         L.orc_floor_count.restype = ctypes.c_int
         L.orc_diagnostics.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcDiag)]
+        L.orc_update_test.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int] + [ctypes.c_void_p] * 4
+        L.orc_update_test.restype = ctypes.c_int
         L.orc_riemann_plane.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _dp]
         L.orc_recon_plane.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

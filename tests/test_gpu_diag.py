@@ -65,7 +65,7 @@ def test_diagnostics_ensemble_equals_single_boxes():
         S.set_state(states[i][0], states[i][1])
         ds = S.diagnostics()
         for k in KEYS:
-            assert ds[k] == de[k][i], (i, k)
+            assert ds[k] == de[i][k], (i, k)
         S.close()
     E.close()
 
