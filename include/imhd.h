@@ -87,6 +87,41 @@ int imhd_destroy(imhd_ctx* ctx);
  * imhd_fill_ghosts() first for valid ghost cells. */
 int imhd_set_state(imhd_ctx* ctx, int box, const double* q_3d, const double* bface_3d, const double* qpass_3d);
 int imhd_get_state(imhd_ctx* ctx, int box, double* q_3d, double* bface_3d, double* qpass_3d);
+/* One field of box `box` into a host array: field 0..6 = q_3d(…,field+1), 7..9 =
+ * bface_3d(…,field-6), both on the padded grid; 10 + p = qpass_3d(…,p+1) on its own bounds
+ * (no counterpart in the reference, which reads its module arrays directly; lets a native
+ * caller fetch a 1024^3 box field by field instead of as one 89 GB host state). */
+int imhd_get_field(imhd_ctx* ctx, int box, int field, double* out);
+
+/* Seeded random-field start of box `box`, generated on the device (no counterpart in the
+ * reference, whose initial conditions are fixed test problems, init_domain_mod.f90; this
+ * is the benchmark workloads' start, built where the state lives so that a 1024^3 box needs
+ * neither host FFTs nor an 89 GB host upload):
+ *   rho   = rho0 exp(sigma_ln g - sigma_ln^2 / 2), g standard normal per interior cell
+ *           (Philox4x32-10, key = seed, counter = (cell, 0, 2, 0) with cell = i + nx (j +
+ *           ny k) from 0; Box-Muller cosine branch), uniform rho0 when sigma_ln = 0;
+ *   v     = sum over integer wave vectors n in one half space (n_z > 0, or n_z = 0 and
+ *           n_y > 0, or n_z = n_y = 0 and n_x > 0) with kmin <= |n| <= kmax of
+ *           2 Re(v_n e^{2 pi i (n_x i / nx + n_y j / ny + n_z k / nz)}), cell (i, j, k)
+ *           from 0.  Mode m (its place in that enumeration: n_z, then n_y, then n_x
+ *           ascending) draws v_n,d = |n|^{-(slope+2)/2} (g1 + i g2) from Philox counter
+ *           (m, d, 1, 0); solenoidal != 0 projects each mode normal to n.  The amplitudes
+ *           are scaled so that the mean of |v|^2 over the interior is exactly (mach cs)^2
+ *           (Parseval; needs floor(kmax) < n_d / 2 on every axis);
+ *   rho v = rho * v (FP64, rounded once in FP32 contexts); B = b0 in every cell and face;
+ *   passive scalars (passive != 0) U[0,1) per cell from counter (cell, 0, 3 + p, 0), else 0.
+ * Ghost cells are filled (imhd_fill_ghosts); the state replaces any earlier one as
+ * imhd_set_state does.  A temporary device buffer of 48 (floor(kmax)+1) nx ny bytes is
+ * allocated and freed during the call. */
+typedef struct imhd_turb_ic {
+    double rho0, sigma_ln;
+    double b0[3];
+    double kmin, kmax, slope, mach;
+    unsigned long long seed;
+    int solenoidal;
+    int passive;
+} imhd_turb_ic;
+int imhd_init_turbulence(imhd_ctx* ctx, int box, const imhd_turb_ic* ic);
 
 /* One integrator step of every box with the given dt:
  * vanleer_solve_3d(dt_in, drive_next) (vanleer_mod.f90:279-426) or tvd_solve_3d(dt_in)

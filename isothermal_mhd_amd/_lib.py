@@ -15,7 +15,8 @@ ERRORS = {-1: "IMHD_EINVAL", -2: "IMHD_EHIP", -3: "IMHD_ESTATE", -4: "IMHD_ENOSP
 
 # Every symbol include/imhd.h declares.
 EXPORTS = [
-    "imhd_create", "imhd_create_ensemble", "imhd_destroy", "imhd_set_state", "imhd_get_state",
+    "imhd_create", "imhd_create_ensemble", "imhd_destroy", "imhd_set_state", "imhd_get_state", "imhd_get_field",
+    "imhd_init_turbulence",
     "imhd_autotune", "imhd_step", "imhd_set_force", "imhd_set_ou_forcing", "imhd_get_ou_modes",
     "imhd_set_impulsive_forcing", "imhd_get_impulsive_forcing", "imhd_fill_ghosts", "imhd_max_speed", "imhd_copy_velocity", "imhd_passive_step",
     "imhd_diagnostics", "imhd_floor_count", "imhd_device_bytes", "imhd_run", "imhd_set_time", "imhd_get_time",
@@ -45,6 +46,13 @@ class ImhdOU(ctypes.Structure):
                 ("_reserved", ctypes.c_int)]
 
 
+class ImhdTurbIC(ctypes.Structure):
+    _fields_ = [("rho0", ctypes.c_double), ("sigma_ln", ctypes.c_double), ("b0", ctypes.c_double * 3),
+                ("kmin", ctypes.c_double), ("kmax", ctypes.c_double), ("slope", ctypes.c_double),
+                ("mach", ctypes.c_double), ("seed", ctypes.c_ulonglong), ("solenoidal", ctypes.c_int),
+                ("passive", ctypes.c_int)]
+
+
 class ImhdError(RuntimeError):
     pass
 
@@ -67,6 +75,8 @@ def lib():
     L.imhd_destroy.argtypes = [vp]
     L.imhd_set_state.argtypes = [vp, ip, vp, vp, vp]
     L.imhd_get_state.argtypes = [vp, ip, vp, vp, vp]
+    L.imhd_get_field.argtypes = [vp, ip, ip, vp]
+    L.imhd_init_turbulence.argtypes = [vp, ip, ctypes.POINTER(ImhdTurbIC)]
     L.imhd_step.argtypes = [vp, dp, ip]
     L.imhd_set_force.argtypes = [vp, ip, vp, dp, dp]
     L.imhd_autotune.argtypes = [vp, ip, ctypes.c_char_p, ip]

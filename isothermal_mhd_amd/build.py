@@ -17,6 +17,7 @@ SOURCES = [("vl_kernels_f64.hip", ["-ffp-contract=off"]),
            ("aux_kernels.hip", ["-ffp-contract=off"]),
            ("forcing_kernels.hip", ["-ffp-contract=off"]),
            ("spectra_kernels.hip", ["-ffp-contract=off"]),
+           ("ic_kernels.hip", ["-ffp-contract=off"]),
            ("imhd_api.cpp", ["-ffp-contract=off"])]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # FP64 sources: -ffp-contract=off (no FMA contraction), so the FP64 path reproduces the

@@ -60,19 +60,20 @@ template <class real>
 struct OuApplyArgs {
     OuDev ou;
     real* q[7];              // new state (rho, m read; m written; all read for the CFL)
-    double* part;            // 3 sums per 256-item block
+    double* part;            // OU_WG rows of 3 sums per box
     double* mean;            // 3 per box
     unsigned long long* cmax;// non-null: fused CFL speed of the forced cells
     double cs;
-    int pass;                // 0: sums of rho a; 1: apply
 };
+constexpr int OU_WG = 1024;  // partial-sum workgroups (and rows) per box of the <rho a> pass
 
 void launch_ou_update(hipStream_t st, int nbox, const BoxDesc* boxes, const OuDev& ou, int init);
 void launch_ou_columns(hipStream_t st, int nbox, int maxcols, const BoxDesc* boxes, const OuDev& ou);
 template <class real>
-void launch_ou_pass(const Launch& L, const OuApplyArgs<real>& a);
-void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, const PlanBox* plan, int nbox, const double* part,
-                        double* mean);
+void launch_ou_sums(hipStream_t st, int nbox, const BoxDesc* boxes, const OuApplyArgs<real>& a);
+template <class real>
+void launch_ou_kick(hipStream_t st, int nbox, long long maxcells, const BoxDesc* boxes, const OuApplyArgs<real>& a);
+void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, int nbox, const double* part, double* mean);
 
 // ---- the reference's random force (generate_rand_force) ----
 // Inverse DFT passes along y and z of the direct transform (turb_driver_mod.f90:461-486):

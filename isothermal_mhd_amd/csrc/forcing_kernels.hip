@@ -114,30 +114,6 @@ void launch_force_apply(const Launch& L, const ForceApplyArgs<real>& a) {
 // ---------------------------------------------------------------------------------
 // Ornstein-Uhlenbeck forcing
 
-// Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11): 10 rounds of two 32x32->64
-// multiplies with the Weyl key schedule.
-__device__ __forceinline__ void philox4x32(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0,
-                                           unsigned k1, unsigned* o) {
-#pragma unroll
-    for (int r = 0; r < 10; r++) {
-        unsigned long long p0 = (unsigned long long)0xD2511F53u * c0;
-        unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c2;
-        unsigned y0 = (unsigned)(p1 >> 32) ^ c1 ^ k0, y1 = (unsigned)p1;
-        unsigned y2 = (unsigned)(p0 >> 32) ^ c3 ^ k1, y3 = (unsigned)p0;
-        c0 = y0; c1 = y1; c2 = y2; c3 = y3;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    o[0] = c0; o[1] = c1; o[2] = c2; o[3] = c3;
-}
-
-// Box-Muller (cosine branch) from two 32-bit words mapped to (0, 1)
-__device__ __forceinline__ double box_muller(unsigned a, unsigned b) {
-    const double s = 2.3283064365386963e-10;   // 2^-32
-    double u1 = ((double)a + 0.5) * s, u2 = ((double)b + 0.5) * s;
-    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
-}
-
 // one thread per (box, mode): OU update of the 3 complex components (init: draw from
 // the stationary distribution, counter word 3 = 1).  The box's step counter is advanced
 // afterwards by k_ou_bump.
@@ -235,60 +211,95 @@ __device__ __forceinline__ void ou_field(const OuDev& ou, const BoxDesc& B, int 
     }
 }
 
+// interior cell (i, j, k) (from 1) of a box from its linear index, i fastest; 32-bit
+// arithmetic whenever the box has fewer than 2^32 cells
+__device__ __forceinline__ void interior_ijk(unsigned long long loc, unsigned long long nx, unsigned long long nxy,
+                                             bool small, int& i, int& j, int& k) {
+    if (small) {
+        const unsigned l32 = (unsigned)loc, k32 = l32 / (unsigned)nxy, r32 = l32 - k32 * (unsigned)nxy;
+        const unsigned j32 = r32 / (unsigned)nx;
+        i = (int)(r32 - j32 * (unsigned)nx) + 1; j = (int)j32 + 1; k = (int)k32 + 1;
+    } else {
+        const unsigned long long kk = loc / nxy, r = loc - kk * nxy, jj = r / nx;
+        i = (int)(r - jj * nx) + 1; j = (int)jj + 1; k = (int)kk + 1;
+    }
+}
+
+// <rho a> partial sums: OU_WG workgroups per box stride over its interior cells; partial
+// row (box, workgroup) of 3 sums, reduced in a fixed order by k_ou_finalize
 template <class real>
-__global__ __launch_bounds__(256) void k_ou_pass(const BoxDesc* __restrict__ boxes, Plan plan, OuApplyArgs<real> a) {
+__global__ __launch_bounds__(256) void k_ou_sums(const BoxDesc* __restrict__ boxes, OuApplyArgs<real> a) {
     __shared__ double sh[FORCE_NSUM][256];
-    int b, i, j, k;
-    const bool act = plan_item_box(plan, b, i, j, k);
+    const int b = blockIdx.y;
+    const BoxDesc& B = boxes[b];
+    const unsigned long long nx = B.n[0], nxy = nx * (unsigned long long)B.n[1], ncell = nxy * B.n[2];
+    const bool small = ncell <= 0xffffffffull;
     double v[FORCE_NSUM] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (unsigned long long loc = (unsigned long long)blockIdx.x * 256 + threadIdx.x; loc < ncell;
+         loc += (unsigned long long)OU_WG * 256) {
+        int i, j, k;
+        interior_ijk(loc, nx, nxy, small, i, j, k);
+        double f[3];
+        ou_field(a.ou, B, b, i, j, k, f);
+        const double rho = (double)a.q[0][cell_index(B, i, j, k)];
+        v[0] = v[0] + rho * f[0];
+        v[1] = v[1] + rho * f[1];
+        v[2] = v[2] + rho * f[2];
+    }
+    block_sum5(sh, v);
+    if (threadIdx.x < 3) a.part[((size_t)b * OU_WG + blockIdx.x) * 3 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// kick: m += dt (rho a - <rho a>) on every interior cell, and (cmax non-null) the CFL
+// speed of the kicked cell: FP64 from the FP64 state, FP32 (hardware rcp/rsqrt/sqrt) from
+// an FP32 state, maximum in FP64
+template <class real>
+__global__ __launch_bounds__(256) void k_ou_kick(const BoxDesc* __restrict__ boxes, OuApplyArgs<real> a) {
+    const int b = blockIdx.y;
+    const BoxDesc& B = boxes[b];
+    const unsigned long long nx = B.n[0], nxy = nx * (unsigned long long)B.n[1], ncell = nxy * B.n[2];
+    const unsigned long long loc = (unsigned long long)blockIdx.x * 256 + threadIdx.x;
     double cfl = 0.0;
-    if (act) {
-        const BoxDesc& B = boxes[b];
+    if (loc < ncell) {
+        int i, j, k;
+        interior_ijk(loc, nx, nxy, ncell <= 0xffffffffull, i, j, k);
         const long long c = cell_index(B, i, j, k);
         double f[3];
         ou_field(a.ou, B, b, i, j, k, f);
         const double rho = (double)a.q[0][c];
-        if (a.pass == 0) {
-            v[0] = rho * f[0];
-            v[1] = rho * f[1];
-            v[2] = rho * f[2];
-        } else {
-            const double dt = a.ou.dt[b];
-            const double* mean = a.mean + 3 * b;
-            for (int d = 0; d < 3; d++) a.q[1 + d][c] = (real)((double)a.q[1 + d][c] + dt * (rho * f[d] - mean[d]));
-            if (a.cmax != nullptr) {
-                double q7[7];
-                for (int g = 0; g < 7; g++) q7[g] = (double)a.q[g][c];
+        const double dt = a.ou.dt[b];
+        const double* mean = a.mean + 3 * b;
+        real q7[7];
+        q7[0] = a.q[0][c];
+        for (int d = 0; d < 3; d++) {
+            q7[1 + d] = (real)((double)a.q[1 + d][c] + dt * (rho * f[d] - mean[d]));
+            a.q[1 + d][c] = q7[1 + d];
+        }
+        if (a.cmax != nullptr) {
+            for (int g = 4; g < 7; g++) q7[g] = a.q[g][c];
+            if constexpr (sizeof(real) == 4) {
+                cfl = (double)cell_fast_speed_f(q7, (float)a.cs);
+            } else {
                 cfl = cell_fast_speed(q7, a.cs);
             }
         }
     }
-    if (a.pass == 0) {
-        block_sum5(sh, v);
-        if (threadIdx.x == 0) {
-            const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-            for (int s = 0; s < 3; s++) a.part[(size_t)blk * 3 + s] = sh[s][0];
-        }
-    } else if (a.cmax != nullptr) {
-        atomic_max_nonneg(a.cmax, b, cfl);
-    }
+    if (a.cmax != nullptr) atomic_max_nonneg(a.cmax, b, cfl);
 }
 
-// per box: <rho a> = sum / cells, fixed summation order
-__global__ __launch_bounds__(256) void k_ou_finalize(const BoxDesc* __restrict__ boxes, const PlanBox* __restrict__ plan,
-                                                     const double* part, double* mean) {
+// per box: <rho a> = sum / cells, its OU_WG partial rows in a fixed order
+__global__ __launch_bounds__(256) void k_ou_finalize(const BoxDesc* __restrict__ boxes, const double* part,
+                                                     double* mean) {
     __shared__ double sh[FORCE_NSUM][256];
     const int b = blockIdx.x;
-    const PlanBox& pb = plan[b];
-    const long long first = pb.start / 256;
-    const long long ncell = (long long)pb.r[0] * pb.r[1] * pb.r[2];
-    const long long nblk = (ncell + 255) / 256;
+    const BoxDesc& B = boxes[b];
+    const double ncell = (double)B.n[0] * (double)B.n[1] * (double)B.n[2];
     double v[FORCE_NSUM] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (long long q = threadIdx.x; q < nblk; q += 256)
-        for (int s = 0; s < 3; s++) v[s] = v[s] + part[(size_t)(first + q) * 3 + s];
+    for (int w = threadIdx.x; w < OU_WG; w += 256)
+        for (int s = 0; s < 3; s++) v[s] = v[s] + part[((size_t)b * OU_WG + w) * 3 + s];
     block_sum5(sh, v);
     if (threadIdx.x == 0)
-        for (int s = 0; s < 3; s++) mean[3 * b + s] = sh[s][0] / (double)ncell;
+        for (int s = 0; s < 3; s++) mean[3 * b + s] = sh[s][0] / ncell;
 }
 
 void launch_ou_update(hipStream_t st, int nbox, const BoxDesc* boxes, const OuDev& ou, int init) {
@@ -301,15 +312,21 @@ void launch_ou_columns(hipStream_t st, int nbox, int maxcols, const BoxDesc* box
                        boxes, ou);
 }
 template <class real>
-void launch_ou_pass(const Launch& L, const OuApplyArgs<real>& a) {
-    if (L.plan.total > 0) hipLaunchKernelGGL((k_ou_pass<real>), dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+void launch_ou_sums(hipStream_t st, int nbox, const BoxDesc* boxes, const OuApplyArgs<real>& a) {
+    hipLaunchKernelGGL((k_ou_sums<real>), dim3(OU_WG, (unsigned)nbox), dim3(256), 0, st, boxes, a);
 }
-void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, const PlanBox* plan, int nbox, const double* part,
-                        double* mean) {
-    hipLaunchKernelGGL(k_ou_finalize, dim3(nbox), dim3(256), 0, st, boxes, plan, part, mean);
+template <class real>
+void launch_ou_kick(hipStream_t st, int nbox, long long maxcells, const BoxDesc* boxes, const OuApplyArgs<real>& a) {
+    hipLaunchKernelGGL((k_ou_kick<real>), dim3((unsigned)((maxcells + 255) / 256), (unsigned)nbox), dim3(256), 0, st,
+                       boxes, a);
 }
-template void launch_ou_pass<double>(const Launch&, const OuApplyArgs<double>&);
-template void launch_ou_pass<float>(const Launch&, const OuApplyArgs<float>&);
+void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, int nbox, const double* part, double* mean) {
+    hipLaunchKernelGGL(k_ou_finalize, dim3(nbox), dim3(256), 0, st, boxes, part, mean);
+}
+template void launch_ou_sums<double>(hipStream_t, int, const BoxDesc*, const OuApplyArgs<double>&);
+template void launch_ou_sums<float>(hipStream_t, int, const BoxDesc*, const OuApplyArgs<float>&);
+template void launch_ou_kick<double>(hipStream_t, int, long long, const BoxDesc*, const OuApplyArgs<double>&);
+template void launch_ou_kick<float>(hipStream_t, int, long long, const BoxDesc*, const OuApplyArgs<float>&);
 
 template void launch_force_sums<double>(const Launch&, const ForceSumArgs<double>&);
 template void launch_force_sums<float>(const Launch&, const ForceSumArgs<float>&);

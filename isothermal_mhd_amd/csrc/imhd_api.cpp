@@ -18,6 +18,7 @@
 #include "aux_kernels.hpp"
 #include "spectra.hpp"
 #include "forcing.hpp"
+#include "ic_kernels.hpp"
 #include "kernels.hpp"
 #include "tvd_kernels.hpp"
 
@@ -321,9 +322,9 @@ StageRanges stage_ranges(const imhd_ctx* c, bool corr) {
 template <class real>
 int stage_blocks_per_cu(const imhd_ctx* c, bool corr) {
     switch (c->cfg.riemann) {
-        case IMHD_ROE: return vl_stage_blocks_per_cu<real, RS_ROE>(corr, c->vshape);
-        case IMHD_HLLE: return vl_stage_blocks_per_cu<real, RS_HLLE>(corr, c->vshape);
-        default: return vl_stage_blocks_per_cu<real, RS_HLLD>(corr, c->vshape);
+        case IMHD_ROE: return vl_stage_blocks_per_cu<real, RS_ROE>(vs_kind(corr, c->cfg.recon), c->vshape);
+        case IMHD_HLLE: return vl_stage_blocks_per_cu<real, RS_HLLE>(vs_kind(corr, c->cfg.recon), c->vshape);
+        default: return vl_stage_blocks_per_cu<real, RS_HLLD>(vs_kind(corr, c->cfg.recon), c->vshape);
     }
 }
 
@@ -539,9 +540,9 @@ void vl_stage(imhd_ctx* c, int sflux, int src, int dst, bool corr, bool fuse_cma
     for (int d = 0; d < 3; d++) a.pvel[d] = pvel ? R<real>(c->pvel[d]) : nullptr;
     const TilePlan tp{p.d, c->nbox, p.total};
     switch (c->cfg.riemann) {
-        case IMHD_ROE: launch_vl_stage<real, RS_ROE>(c->st, c->db, tp, a, corr, c->vshape); break;
-        case IMHD_HLLE: launch_vl_stage<real, RS_HLLE>(c->st, c->db, tp, a, corr, c->vshape); break;
-        default: launch_vl_stage<real, RS_HLLD>(c->st, c->db, tp, a, corr, c->vshape); break;
+        case IMHD_ROE: launch_vl_stage<real, RS_ROE>(c->st, c->db, tp, a, vs_kind(corr, c->cfg.recon), c->vshape); break;
+        case IMHD_HLLE: launch_vl_stage<real, RS_HLLE>(c->st, c->db, tp, a, vs_kind(corr, c->cfg.recon), c->vshape); break;
+        default: launch_vl_stage<real, RS_HLLD>(c->st, c->db, tp, a, vs_kind(corr, c->cfg.recon), c->vshape); break;
     }
 }
 
@@ -618,7 +619,7 @@ void vl_step(imhd_ctx* c, bool fuse_cmax = false, bool with_passive = false) {
     if (c->vstage) {
         // Fused stages.  The corrector's fluxes read the half step while its update writes
         // the new state, so the half step goes to its own set (2) when the corrector is
-        // fused too (PLM); the PPM corrector keeps the three-kernel path on set s1.
+        // fused too; with IMHD_VPPM=0 a PPM corrector keeps the three-kernel path on set s1.
         const bool fcorr = c->q[2][0] != nullptr;
         const int sh = fcorr ? 2 : s1;
         vl_stage<real>(c, s0, s0, sh, false, false, "stage_pred");
@@ -881,12 +882,12 @@ void ou_step(imhd_ctx* c, bool fuse_cmax) {
     a.part = c->ou_part;
     a.mean = c->ou_mean;
     a.cs = c->cfg.cs;
-    a.pass = 0;
-    launch_ou_pass<real>(mk(c, c->pl_int), a);
-    launch_ou_finalize(c->st, c->db, c->pl_int.d, c->nbox, c->ou_part, c->ou_mean);
-    a.pass = 1;
+    launch_ou_sums<real>(c->st, c->nbox, c->db, a);
+    launch_ou_finalize(c->st, c->db, c->nbox, c->ou_part, c->ou_mean);
     a.cmax = fuse_cmax ? c->cmax : nullptr;
-    launch_ou_pass<real>(mk(c, c->pl_int), a);
+    long long maxcells = 0;
+    for (const BoxDesc& B : c->hb) maxcells = std::max(maxcells, (long long)B.n[0] * B.n[1] * B.n[2]);
+    launch_ou_kick<real>(c->st, c->nbox, maxcells, c->db, a);
 }
 
 // Drop-in step only: cells outside the VL final-update range take their t^n values, as
@@ -1547,6 +1548,169 @@ struct DevBuf {
     }
 };
 
+// ---------------------------------------------------------------------------------
+// Seeded random-field start (imhd_init_turbulence).  Host: the modes of the half space
+// inside the shell, their Philox draws, the exact rms scaling (Parseval) and the sum over
+// n_x for every (n_y, n_z) pair; device: the sums over n_y (k_ic_ypass) and n_z, rho and
+// the stores (k_ic_cells), then the ghost fill and a copy of the new state into the other
+// state set, as imhd_set_state leaves both sets.
+
+// (cos, sin) of 2 pi r / n for r = 0 .. n-1
+static std::vector<double> twiddle_table(int n) {
+    std::vector<double> t(2 * (size_t)n);
+    for (int r = 0; r < n; r++) {
+        const double a = 6.283185307179586 * (double)r / (double)n;
+        t[2 * r] = cos(a);
+        t[2 * r + 1] = sin(a);
+    }
+    return t;
+}
+
+template <class real>
+static int init_turbulence_t(imhd_ctx* c, int bx, const imhd_turb_ic* ic) {
+    const BoxDesc& B = c->hb[bx];
+    const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+    const int K = ic->kmax >= 1.0 ? (int)floor(ic->kmax) : 0;
+    if (K > 0 && (2 * K >= nx || 2 * K >= ny || 2 * K >= nz))
+        FAIL(c, IMHD_EINVAL, "init_turbulence: floor(kmax) = %d needs fewer than n/2 modes per axis (box %d x %d x %d)",
+             K, nx, ny, nz);
+    const unsigned slo = (unsigned)ic->seed, shi = (unsigned)(ic->seed >> 32);
+    // modes of the half space inside the shell, in the documented order, and their draws
+    struct Mode { int n[3]; double re[3], im[3]; };
+    std::vector<Mode> modes;
+    double S = 0.0;
+    for (int kz = 0; kz <= K; kz++)
+        for (int ky = -K; ky <= K; ky++)
+            for (int kx = -K; kx <= K; kx++) {
+                if (!(kz > 0 || (kz == 0 && ky > 0) || (kz == 0 && ky == 0 && kx > 0))) continue;
+                const double n2 = (double)kx * kx + (double)ky * ky + (double)kz * kz, kn = sqrt(n2);
+                if (kn < ic->kmin || kn > ic->kmax) continue;
+                Mode md{{kx, ky, kz}, {}, {}};
+                const double amp = pow(kn, -0.5 * (ic->slope + 2.0));
+                const unsigned m = (unsigned)modes.size();
+                for (int d = 0; d < 3; d++) {
+                    unsigned o[4];
+                    philox4x32(m, (unsigned)d, 1u, 0u, slo, shi, o);
+                    md.re[d] = amp * box_muller(o[0], o[1]);
+                    md.im[d] = amp * box_muller(o[2], o[3]);
+                }
+                if (ic->solenoidal) {   // normal to the physical wave vector k_d = n_d / n_cells,d
+                    const double k[3] = {(double)kx / nx, (double)ky / ny, (double)kz / nz};
+                    const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+                    const double dr = k[0] * md.re[0] + k[1] * md.re[1] + k[2] * md.re[2];
+                    const double di = k[0] * md.im[0] + k[1] * md.im[1] + k[2] * md.im[2];
+                    for (int d = 0; d < 3; d++) {
+                        md.re[d] = md.re[d] - k[d] * dr / k2;
+                        md.im[d] = md.im[d] - k[d] * di / k2;
+                    }
+                }
+                for (int d = 0; d < 3; d++) S = S + 2.0 * (md.re[d] * md.re[d] + md.im[d] * md.im[d]);
+                modes.push_back(md);
+            }
+    const bool vel = S > 0.0 && ic->mach != 0.0;
+    const double scale = vel ? ic->mach * c->cfg.cs / sqrt(S) : 0.0;
+    const int W = 2 * K + 1;
+    const size_t nA = vel ? (size_t)3 * (K + 1) * W * nx * 2 : 0;
+    const size_t nB = vel ? (size_t)3 * (K + 1) * ny * nx * 2 : 0;
+    const size_t nty = vel ? 2 * (size_t)ny : 0, ntz = vel ? (size_t)2 * nz * (K + 1) : 0;
+    std::vector<double> host(nA + nty + ntz, 0.0);
+    if (vel) {
+        const std::vector<double> tx = twiddle_table(nx), ty = twiddle_table(ny), tz = twiddle_table(nz);
+        double* A = host.data();
+        for (const Mode& md : modes) {
+            const int iy = md.n[1] + K, kz = md.n[2];
+            for (int d = 0; d < 3; d++) {
+                const double vr = scale * md.re[d], vi = scale * md.im[d];
+                double* a = A + (((size_t)d * (K + 1) + kz) * W + iy) * nx * 2;
+                for (int i = 0; i < nx; i++) {
+                    long long r = ((long long)md.n[0] * i) % nx;
+                    if (r < 0) r += nx;
+                    const double cr = tx[2 * r], sr = tx[2 * r + 1];
+                    a[2 * i] = a[2 * i] + (vr * cr - vi * sr);
+                    a[2 * i + 1] = a[2 * i + 1] + (vr * sr + vi * cr);
+                }
+            }
+        }
+        std::copy(ty.begin(), ty.end(), host.begin() + nA);
+        double* twz = host.data() + nA + nty;
+        for (int k = 0; k < nz; k++)
+            for (int kz = 0; kz <= K; kz++) {
+                const long long r = ((long long)kz * k) % nz;
+                twz[((size_t)k * (K + 1) + kz) * 2] = tz[2 * r];
+                twz[((size_t)k * (K + 1) + kz) * 2 + 1] = tz[2 * r + 1];
+            }
+    }
+    double* dbuf = nullptr;
+    if (vel) {
+        HIPCK(c, hipMalloc(&dbuf, (host.size() + nB) * sizeof(double)));
+        hipError_t e = hipMemcpy(dbuf, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            hipFree(dbuf);
+            FAIL(c, IMHD_EHIP, "init_turbulence: upload failed: %s", hipGetErrorString(e));
+        }
+        launch_ic_ypass(c->st, nx, ny, K, dbuf, dbuf + nA, dbuf + host.size());
+    }
+    IcArgs<real> a{};
+    for (int f = 0; f < 7; f++) a.q[f] = R<real>(c->q[c->cur][f]);
+    for (int d = 0; d < 3; d++) a.b[d] = c->b[c->cur][d];
+    for (int p = 0; p < c->cfg.npass; p++) a.qp[p] = R<real>(c->qp[c->pcur][p]);
+    a.Bv = vel ? dbuf + host.size() : nullptr;
+    a.twz = vel ? dbuf + nA + nty : nullptr;
+    a.rho0 = ic->rho0;
+    a.sigma = ic->sigma_ln;
+    for (int d = 0; d < 3; d++) a.b0[d] = ic->b0[d];
+    a.seed_lo = slo;
+    a.seed_hi = shi;
+    a.K = K;
+    a.npass = c->cfg.npass;
+    a.passive = ic->passive;
+    launch_ic_cells<real>(c->st, B, a);
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (dbuf) hipFree(dbuf);
+    if (e != hipSuccess) FAIL(c, IMHD_EHIP, "init_turbulence: %s", hipGetErrorString(e));
+    HIPCK(c, hipGetLastError());
+    if (c->comp[0])
+        for (int f = 0; f < 4; f++) HIPCK(c, hipMemsetAsync(c->comp[f], 0, (size_t)c->pool * c->esz, c->st));
+    fill_ghosts<real>(c);
+    const long long nc = (long long)B.P[0] * B.P[1] * B.P[2];
+    for (int f = 0; f < 7; f++)
+        HIPCK(c, hipMemcpyAsync(R<real>(c->q[1 - c->cur][f]) + B.off, R<real>(c->q[c->cur][f]) + B.off,
+                                (size_t)nc * sizeof(real), hipMemcpyDeviceToDevice, c->st));
+    for (int d = 0; d < 3; d++)
+        HIPCK(c, hipMemcpyAsync(c->b[1 - c->cur][d] + B.off, c->b[c->cur][d] + B.off, (size_t)nc * 8,
+                                hipMemcpyDeviceToDevice, c->st));
+    for (int p = 0; p < c->cfg.npass; p++)
+        HIPCK(c, hipMemcpyAsync(R<real>(c->qp[1 - c->pcur][p]) + B.off, R<real>(c->qp[c->pcur][p]) + B.off,
+                                (size_t)nc * sizeof(real), hipMemcpyDeviceToDevice, c->st));
+    HIPCK(c, hipStreamSynchronize(c->st));
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
+template <class real>
+static int get_field_t(imhd_ctx* c, int bx, int field, double* out) {
+    const BoxDesc& B = c->hb[bx];
+    const long long nc = (long long)B.P[0] * B.P[1] * B.P[2];
+    HIPCK(c, hipStreamSynchronize(c->st));
+    if (field < 7) return download_field<real>(c, R<real>(c->q[c->cur][field]) + B.off, out, nc);
+    if (field < 10) {
+        HIPCK(c, hipMemcpy(out, c->b[c->cur][field - 7] + B.off, (size_t)nc * 8, hipMemcpyDeviceToHost));
+        return 0;
+    }
+    const int p = field - 10;
+    const long long pn = (long long)(B.n[0] + 2 * B.nb - 4) * (B.n[1] + 2 * B.nb - 4) * (B.n[2] + 2 * B.nb - 4);
+    int rc;
+    if ((rc = ensure_stage(c, (size_t)std::min(pn, STAGE_CHUNK)))) return rc;
+    for (long long o = 0; o < pn; o += STAGE_CHUNK) {
+        const long long m = std::min(STAGE_CHUNK, pn - o);
+        launch_pass_layout<real>(c->st, B, nullptr, c->stage, R<real>(c->qp[c->pcur][p]), 0, o, m);
+        HIPCK(c, hipStreamSynchronize(c->st));
+        HIPCK(c, hipMemcpy(out + o, c->stage, (size_t)m * 8, hipMemcpyDeviceToHost));
+    }
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
 // generate_rand_force (turb_driver_mod.f90:197-391) for one box, with the modes the
 // reference never assigns (mixed-sign wave vectors) zero.  Host: the k_max^3 octant modes
 // (six taus88 draws each: Box-Muller amplitude and phase per component, P(k) = k^6
@@ -1872,7 +2036,10 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     // FP32: 32 x 16 tiles (2 workgroups of 80 KB LDS per CU); FP64: 16 x 16 (a 32 x 16
     // FP64 tile would leave one workgroup per CU)
     c->vshape = g.precision == 32 ? 1 : 0;
-    if (const char* e = getenv("IMHD_VSHAPE")) c->vshape = std::min(VS_SHAPES - 1, std::max(0, atoi(e)));
+    if (const char* e = getenv("IMHD_VSHAPE")) {
+        const int v = std::min(VS_SHAPES - 1, std::max(0, atoi(e)));
+        if (vs_shape_ok(g.precision, v)) c->vshape = v;
+    }
     if (g.flux_protection) {
         // flux_protection of vanleer_mod.f90 (:81): Roe with HLLE protection fluxes, FP64
         if (g.integrator != IMHD_VL || g.riemann != IMHD_ROE || g.precision != 64) {
@@ -1918,10 +2085,13 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
         for (int d = 0; d < 3; d++) if ((rc = alloc(c, (void**)&c->b[s][d], nd))) return rc;
         for (int p = 0; p < g.npass; p++) if ((rc = alloc(c, &c->qp[s][p], nr))) return rc;
     }
-    // Both VL stages fused (PLM): a half-step set, and no flux / face-EMF / corner-EMF work
+    // Both VL stages fused: a half-step set, and no flux / face-EMF / corner-EMF work
     // arrays (18 real + 3 FP64 per cell, the largest share of a 1024^3 box).  Otherwise
-    // (TVD, PPM corrector, IMHD_VSTAGE=0) the three-kernel stages need the work arrays.
-    const bool fused = g.integrator == IMHD_VL && c->vstage && g.recon == IMHD_PLM_PRIM;
+    // (TVD, IMHD_VSTAGE=0, a PPM corrector with IMHD_VPPM=0, flux protection) the
+    // three-kernel stages need the work arrays.
+    bool vppm = true;
+    if (const char* e = getenv("IMHD_VPPM")) vppm = atoi(e) != 0;   // 0: PPM corrector on three kernels
+    const bool fused = g.integrator == IMHD_VL && c->vstage && (g.recon == IMHD_PLM_PRIM || vppm);
     if (fused) {
         for (int f = 0; f < 7; f++) if ((rc = alloc(c, &c->q[2][f], nr))) return rc;
         for (int d = 0; d < 3; d++) if ((rc = alloc(c, (void**)&c->b[2][d], nd))) return rc;
@@ -2090,6 +2260,22 @@ int imhd_get_state(imhd_ctx* c, int box, double* q, double* bf, double* qp) {
     return c->prec == 64 ? get_state_t<double>(c, box, q, bf, qp) : get_state_t<float>(c, box, q, bf, qp);
 }
 
+int imhd_get_field(imhd_ctx* c, int box, int field, double* out) {
+    if (!c || !out || box < 0 || box >= c->nbox || field < 0 || field >= 10 + c->cfg.npass) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    return c->prec == 64 ? get_field_t<double>(c, box, field, out) : get_field_t<float>(c, box, field, out);
+}
+
+int imhd_init_turbulence(imhd_ctx* c, int box, const imhd_turb_ic* ic) {
+    if (!c || !ic || box < 0 || box >= c->nbox) return IMHD_EINVAL;
+    if (!(ic->rho0 > 0.0) || !(ic->sigma_ln >= 0.0) || !(ic->kmax <= 1024.0) || !(ic->mach >= 0.0))
+        FAIL(c, IMHD_EINVAL, "init_turbulence: needs rho0 > 0, sigma_ln >= 0, mach >= 0, kmax <= 1024");
+    int rc = c->prec == 64 ? init_turbulence_t<double>(c, box, ic) : init_turbulence_t<float>(c, box, ic);
+    if (rc) return rc;
+    c->have_state = true;
+    return IMHD_OK;
+}
+
 int imhd_step(imhd_ctx* c, double dt, int drive_next) {
     if (!c) return IMHD_EINVAL;
     if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
@@ -2226,7 +2412,7 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
     for (int s : strips) if (!rc) rc = trial({s0.zseg, s, s0.vshape, s0.vseg, s0.vrounds});
     { const Shape b0 = bsh; for (int z : zsegs) if (!rc && z != b0.zseg) rc = trial({z, b0.strip, b0.vshape, b0.vseg, b0.vrounds}); }
     if (vl) {
-        { const Shape b0 = bsh; for (int v = 0; v < VS_SHAPES; v++) if (!rc && v != b0.vshape) rc = trial({b0.zseg, b0.strip, v, b0.vseg, b0.vrounds}); }
+        { const Shape b0 = bsh; for (int v = 0; v < VS_SHAPES; v++) if (!rc && v != b0.vshape && vs_shape_ok(c->prec, v)) rc = trial({b0.zseg, b0.strip, v, b0.vseg, b0.vrounds}); }
         { const Shape b0 = bsh; for (int r : vrounds) if (!rc && r != b0.vrounds) rc = trial({b0.zseg, b0.strip, b0.vshape, b0.vseg, r}); }
         { const Shape b0 = bsh; for (int v : vsegs) if (!rc && (b0.vrounds != 0 || v != b0.vseg)) rc = trial({b0.zseg, b0.strip, b0.vshape, v, 0}); }
     }
@@ -2316,7 +2502,7 @@ int imhd_set_ou_forcing(imhd_ctx* c, const imhd_ou* ou) {
     HIPCK(c, hipMemcpy(c->ou_tw, tw.data(), 8 * tw.size(), hipMemcpyHostToDevice));
     HIPCK(c, hipMalloc(&c->ou_twoff, 8 * off.size()));
     HIPCK(c, hipMemcpy(c->ou_twoff, off.data(), 8 * off.size(), hipMemcpyHostToDevice));
-    HIPCK(c, hipMalloc(&c->ou_part, 8 * 3 * (size_t)(c->pl_int.total / 256 + 1)));
+    HIPCK(c, hipMalloc(&c->ou_part, 8 * 3 * (size_t)OU_WG * c->nbox));
     HIPCK(c, hipMalloc(&c->ou_mean, 8 * 3 * (size_t)c->nbox));
     std::vector<long long> coff;
     long long ncol = 0;

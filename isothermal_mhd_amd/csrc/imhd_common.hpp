@@ -217,6 +217,31 @@ __device__ __forceinline__ double striped_max(const unsigned long long* base, in
     return __longlong_as_double((long long)m);
 }
 
+// Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11): 10 rounds of two 32x32->64
+// multiplies with the Weyl key schedule.  Host and device run the same code, so the OU
+// noise and the seeded starts are identical wherever they are drawn.
+__host__ __device__ __forceinline__ void philox4x32(unsigned c0, unsigned c1, unsigned c2, unsigned c3,
+                                                    unsigned k0, unsigned k1, unsigned* o) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        unsigned long long p0 = (unsigned long long)0xD2511F53u * c0;
+        unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c2;
+        unsigned y0 = (unsigned)(p1 >> 32) ^ c1 ^ k0, y1 = (unsigned)p1;
+        unsigned y2 = (unsigned)(p0 >> 32) ^ c3 ^ k1, y3 = (unsigned)p0;
+        c0 = y0; c1 = y1; c2 = y2; c3 = y3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    o[0] = c0; o[1] = c1; o[2] = c2; o[3] = c3;
+}
+
+// Box-Muller (cosine branch) from two 32-bit words mapped to (0, 1)
+__host__ __device__ __forceinline__ double box_muller(unsigned a, unsigned b) {
+    const double s = 2.3283064365386963e-10;   // 2^-32
+    double u1 = ((double)a + 0.5) * s, u2 = ((double)b + 0.5) * s;
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
 // rotated-component permutation of a directional sweep (vanleer_mod.f90:449-464,
 // 610-625, 788-803): rotated slot r <- global field perm(d, r)
 __host__ __device__ constexpr int perm(int d, int r) {

@@ -81,10 +81,13 @@ void launch_update(const Launch& L, const UpdateArgs<real>& a);
 // One whole VL stage (fluxes, corner EMFs, update) in one launch: k_vl_stage.  A workgroup
 // owns a tile of TX x TY (i, j) columns, of which the inner (TX-2) x (TY-2) are updated,
 // and marches a z-segment; fluxes and EMFs stay in LDS and registers.  Tile shapes:
-// 0 = 16 x 16 (256 threads), 1 = 32 x 16 (512 threads).
-constexpr int VS_SHAPES = 2;
-constexpr int vs_tx(int shape) { return shape == 1 ? 32 : 16; }
-constexpr int vs_ty(int shape) { return 16; }
+// 0 = 16 x 16 (256 threads), 1 = 32 x 16 (512 threads), 2 = 32 x 32 (1024 threads, FP32
+// only: its 160 KB of LDS fill a CU, and a 30 x 30 update per tile solves 3.4 faces per
+// updated cell where 30 x 14 solves 3.66).
+constexpr int VS_SHAPES = 3;
+constexpr int vs_tx(int shape) { return shape >= 1 ? 32 : 16; }
+constexpr int vs_ty(int shape) { return shape == 2 ? 32 : 16; }
+constexpr bool vs_shape_ok(int precision, int shape) { return shape < 2 || precision == 32; }
 struct TileBox {
     int x0, y0;        // cell index of thread (0, 0) of tile (0, 0)
     int zlo, zhi;      // planes marched (updates masked to the update range)
@@ -120,11 +123,14 @@ struct StageArgs {
     // and the new state
     real* pvel[3];
 };
+// kind: 0 predictor, 1 PLM corrector, 2 PPM-primitive corrector, 3 PPM-characteristic
+// corrector (vs_kind)
 template <class real, int RS>
 void launch_vl_stage(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a,
-                     bool corrector, int shape);
+                     int kind, int shape);
 // workgroups of a fused stage resident per CU (occupancy of the kernel as compiled)
 template <class real, int RS>
-int vl_stage_blocks_per_cu(bool corrector, int shape);
+int vl_stage_blocks_per_cu(int kind, int shape);
+constexpr int vs_kind(bool corrector, int recon) { return corrector ? 1 + recon : 0; }
 
 }  // namespace imhd

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_flux_corr(const BoxDesc* __restrict__ b
 // Corrector fluxes with PPM reconstruction (characteristic: the reference default;
 // primitive: the compiled branch), six cells c-2 .. c+3 per interface.
 template <class real, int RS, bool CHAR>
-__device__ __forceinline__ double flux_corr_ppm_face(const SweepArgs<real>& a, long long c, long long s) {
+__device__ __forceinline__ real ppm_flux_calc(const SweepArgs<real>& a, long long c, long long s, real* F) {
     real W[6][6], wbc[6];
 #pragma unroll
     for (int t = 0; t < 6; t++) {
@@ -162,10 +162,16 @@ __device__ __forceinline__ double flux_corr_ppm_face(const SweepArgs<real>& a, l
         W[t][5] = a.w[6][ct];
         wbc[t] = (real)a.bn[ct];
     }
-    real uL[7], uR[7], F[7];
+    real uL[7], uR[7];
     ppm_states<real, CHAR>(W, wbc, (real)a.cs, (real)a.mc_beta, uL, uR);
     real smax = real(0);
     riemann<RS>(uL, uR, (real)a.cs, F, smax);
+    return smax;
+}
+template <class real, int RS, bool CHAR>
+__device__ __forceinline__ double flux_corr_ppm_face(const SweepArgs<real>& a, long long c, long long s) {
+    real F[7];
+    const real smax = ppm_flux_calc<real, RS, CHAR>(a, c, s, F);
     store_flux(a, c, F);
     return (double)smax;
 }
@@ -532,8 +538,8 @@ namespace imhd {
     template void launch_flux_predictor<T, RS>(const Launch&, const SweepArgs<T>&); \
     template void launch_flux_corrector<T, RS>(const Launch&, const SweepArgs<T>&); \
     template void launch_flux3<T, RS>(const Launch&, const Sweep3Args<T>&, int);     \
-    template void launch_vl_stage<T, RS>(hipStream_t, const BoxDesc*, const TilePlan&, const StageArgs<T>&, bool, int); \
-    template int vl_stage_blocks_per_cu<T, RS>(bool, int);
+    template void launch_vl_stage<T, RS>(hipStream_t, const BoxDesc*, const TilePlan&, const StageArgs<T>&, int, int); \
+    template int vl_stage_blocks_per_cu<T, RS>(int, int);
 #define IMHD_INST_VL(T)                                                      \
     IMHD_INST_RS(T, RS_ROE)                                                  \
     IMHD_INST_RS(T, RS_HLLD)                                                 \

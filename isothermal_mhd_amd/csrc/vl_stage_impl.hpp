@@ -20,16 +20,19 @@ namespace imhd {
 
 // waves per SIMD the register allocation must allow: in FP32 the LDS of a 16 x 16 tile
 // (40 KB) fits 4 workgroups per CU and that of a 32 x 16 tile (80 KB) 2, 16 waves per CU
-// either way; FP64 needs more of both and gets 2 waves per SIMD
-template <class real>
-constexpr int vs_waves() { return sizeof(real) == 4 ? 4 : 1; }
-#define VS_WAVES vs_waves<real>()
+// either way; FP64 needs more of both and gets 2 waves per SIMD; the PPM correctors
+// (KIND 2, 3: six-cell stencils, the characteristic one with 6 x 6 eigenvector matrices)
+// get 2 in FP32 too, rather than spill at 128 registers
+template <class real, int KIND>
+constexpr int vs_waves() { return sizeof(real) == 4 ? (KIND >= 2 ? 2 : 4) : 1; }
+#define VS_WAVES (vs_waves<real, KIND>())
 
 // x or y face (direction D) of the fused stage.  Predictor: the cell c on the low side is
 // the thread's own, held in registers in the global field order, and only the neighbour is
-// loaded (same arithmetic as flux_pred_calc).  Corrector: flux_corr_calc, which loads its
-// four-cell stencil (holding the own cell across the faces costs the corrector registers
-// it does not have at 4 waves per SIMD).
+// loaded (same arithmetic as flux_pred_calc).  PLM corrector (KIND 1): flux_corr_calc's
+// arithmetic with the own cell from registers and the other three of its four-cell
+// stencil loaded.  PPM correctors (KIND 2 primitive, 3 characteristic): ppm_flux_calc,
+// which loads its six-cell stencil.
 template <class real, int RS, int KIND, int D>
 __device__ __forceinline__ real stage_face_own(const SweepArgs<real>& a, const real* U, long long c, long long s,
                                                real* F) {
@@ -46,13 +49,15 @@ __device__ __forceinline__ real stage_face_own(const SweepArgs<real>& a, const r
         real smax = real(0);
         riemann<RS>(uL, uR, (real)a.cs, F, smax);
         return smax;
-    } else {
+    } else if constexpr (KIND == 1) {
         real W[4][6];
         prim_load(a, c - s, W[0]);
         prim_rot(U[0], U[perm(D, 1)], U[perm(D, 2)], U[perm(D, 3)], U[perm(D, 5)], U[perm(D, 6)], W[1]);
         prim_load(a, c + s, W[2]);
         prim_load(a, c + 2 * s, W[3]);
         return plm_flux<real, RS>(a, W, (real)a.bn[c], F);
+    } else {
+        return ppm_flux_calc<real, RS, KIND == 3>(a, c, s, F);
     }
 }
 // z face k+1/2: predictor between the thread's own cells of planes k (UL) and k+1 (UR),
@@ -74,6 +79,8 @@ __device__ __forceinline__ real stage_face_z(const SweepArgs<real>& a, const rea
         real smax = real(0);
         riemann<RS>(uL, uR, (real)a.cs, F, smax);
         return smax;
+    } else if constexpr (KIND >= 2) {
+        return ppm_flux_calc<real, RS, KIND == 3>(a, c, s, F);
     } else if constexpr (sizeof(real) == 8) {
         // FP64 (2 waves per SIMD, registers to spare): planes k and k+1 from registers
         real W[4][6];
@@ -396,40 +403,64 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     if (a.cmax != nullptr) atomic_max_nonneg(a.cmax, b, cfl);
 }
 
-template <class real, int RS, int SHAPE>
-void launch_vl_stage_t(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a,
-                       bool corrector) {
-    constexpr int TX = vs_tx(SHAPE), TY = vs_ty(SHAPE);
-    if (corrector)
-        hipLaunchKernelGGL((k_vl_stage<real, RS, 1, TX, TY>), dim3((unsigned)plan.total), dim3(TX * TY), 0, st, boxes,
-                           plan, a);
+// kind: 0 predictor, 1 PLM corrector, 2 PPM-primitive corrector, 3 PPM-characteristic
+// corrector; shape 2 exists in FP32 only
+template <class real, int RS, int KIND>
+void launch_vl_stage_k(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a,
+                       int shape) {
+    if constexpr (sizeof(real) == 4) {
+        if (shape == 2) {
+            hipLaunchKernelGGL((k_vl_stage<real, RS, KIND, vs_tx(2), vs_ty(2)>), dim3((unsigned)plan.total),
+                               dim3(vs_tx(2) * vs_ty(2)), 0, st, boxes, plan, a);
+            return;
+        }
+    }
+    if (shape == 1)
+        hipLaunchKernelGGL((k_vl_stage<real, RS, KIND, vs_tx(1), vs_ty(1)>), dim3((unsigned)plan.total),
+                           dim3(vs_tx(1) * vs_ty(1)), 0, st, boxes, plan, a);
     else
-        hipLaunchKernelGGL((k_vl_stage<real, RS, 0, TX, TY>), dim3((unsigned)plan.total), dim3(TX * TY), 0, st, boxes,
-                           plan, a);
+        hipLaunchKernelGGL((k_vl_stage<real, RS, KIND, vs_tx(0), vs_ty(0)>), dim3((unsigned)plan.total),
+                           dim3(vs_tx(0) * vs_ty(0)), 0, st, boxes, plan, a);
 }
-template <class real, int RS>
-int vl_stage_blocks_per_cu(bool corrector, int shape) {
+template <class real, int RS, int KIND>
+int vl_stage_blocks_per_cu_k(int shape) {
     int n = 0;
     hipError_t e;
+    if constexpr (sizeof(real) == 4) {
+        if (shape == 2) {
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_vl_stage<real, RS, KIND, vs_tx(2), vs_ty(2)>,
+                                                             vs_tx(2) * vs_ty(2), 0);
+            return e == hipSuccess && n > 0 ? n : 1;
+        }
+    }
     if (shape == 1)
-        e = corrector ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_vl_stage<real, RS, 1, vs_tx(1), vs_ty(1)>,
-                                                                       vs_tx(1) * vs_ty(1), 0)
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_vl_stage<real, RS, 0, vs_tx(1), vs_ty(1)>,
-                                                                       vs_tx(1) * vs_ty(1), 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_vl_stage<real, RS, KIND, vs_tx(1), vs_ty(1)>,
+                                                         vs_tx(1) * vs_ty(1), 0);
     else
-        e = corrector ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_vl_stage<real, RS, 1, vs_tx(0), vs_ty(0)>,
-                                                                       vs_tx(0) * vs_ty(0), 0)
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_vl_stage<real, RS, 0, vs_tx(0), vs_ty(0)>,
-                                                                       vs_tx(0) * vs_ty(0), 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_vl_stage<real, RS, KIND, vs_tx(0), vs_ty(0)>,
+                                                         vs_tx(0) * vs_ty(0), 0);
     return e == hipSuccess && n > 0 ? n : 1;
+}
+template <class real, int RS>
+int vl_stage_blocks_per_cu(int kind, int shape) {
+    switch (kind) {
+        case 0: return vl_stage_blocks_per_cu_k<real, RS, 0>(shape);
+        case 1: return vl_stage_blocks_per_cu_k<real, RS, 1>(shape);
+        case 2: return vl_stage_blocks_per_cu_k<real, RS, 2>(shape);
+        default: return vl_stage_blocks_per_cu_k<real, RS, 3>(shape);
+    }
 }
 
 template <class real, int RS>
-void launch_vl_stage(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a,
-                     bool corrector, int shape) {
+void launch_vl_stage(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a, int kind,
+                     int shape) {
     if (plan.total <= 0) return;
-    if (shape == 1) launch_vl_stage_t<real, RS, 1>(st, boxes, plan, a, corrector);
-    else launch_vl_stage_t<real, RS, 0>(st, boxes, plan, a, corrector);
+    switch (kind) {
+        case 0: launch_vl_stage_k<real, RS, 0>(st, boxes, plan, a, shape); break;
+        case 1: launch_vl_stage_k<real, RS, 1>(st, boxes, plan, a, shape); break;
+        case 2: launch_vl_stage_k<real, RS, 2>(st, boxes, plan, a, shape); break;
+        default: launch_vl_stage_k<real, RS, 3>(st, boxes, plan, a, shape); break;
+    }
 }
 
 }  // namespace imhd

@@ -107,6 +107,30 @@ class VanLeerSolver:
         self._check(self.lib.imhd_get_state(self._c, box, _p(q), _p(b), _p(qp)))
         return q, b, qp
 
+    def get_field(self, field, box=0, out=None):
+        """One field of the device state on its padded grid: 0..6 q_3d(...,1..7), 7..9
+        bface_3d(...,1..3), 10 + p qpass_3d(...,p+1).  `out` (FP64, Fortran order, that
+        shape) is filled and returned instead of a new array."""
+        sh, shp = self.shapes(box)
+        shape = shp[:3] if field >= 10 else sh
+        if out is None:
+            out = np.zeros(shape, order="F")
+        elif out.shape != shape or out.dtype != np.float64 or not out.flags.f_contiguous:
+            raise ValueError(f"out must be a Fortran-ordered float64 array of shape {shape}")
+        self._check(self.lib.imhd_get_field(self._c, box, int(field), _p(out)))
+        return out
+
+    def init_turbulence(self, rho0=1.0, sigma_ln=0.0, b0=(0.0, 0.0, 1.0), kmin=1.0, kmax=8.0, slope=2.0,
+                        mach=1.0, seed=0, solenoidal=True, passive=False, box=0):
+        """Seeded random-field start generated on the device (see imhd.h,
+        imhd_init_turbulence): lognormal density of width sigma_ln about rho0, Gaussian
+        Fourier-mode velocity with shell spectrum k^-slope on kmin <= |n| <= kmax scaled to
+        rms Mach `mach`, uniform field b0, passive scalars U[0,1) when `passive`."""
+        ic = L.ImhdTurbIC(float(rho0), float(sigma_ln), (ctypes.c_double * 3)(*[float(x) for x in b0]),
+                          float(kmin), float(kmax), float(slope), float(mach), int(seed),
+                          int(bool(solenoidal)), int(bool(passive)))
+        self._check(self.lib.imhd_init_turbulence(self._c, box, ctypes.byref(ic)))
+
     # -- restart and snapshot files (io_mod.f90) ---------------------------------------
     def write_restart(self, path, t_sim, dump_count=0, seeds=(1234, 5678, 9123), box=0):
         """write_restart (io_mod.f90:440-473) of one box, straight from the device state:

@@ -1,6 +1,6 @@
 """The fused VL stage kernel (one launch per stage: directional fluxes, corner EMFs and the
 update) against the three-kernel path it replaces (k_flux3 -> k_corner -> k_update_z), and
-its two tile shapes against each other.  FP64 must agree bit for bit over native-loop runs
+its tile shapes against each other.  FP64 must agree bit for bit over native-loop runs
 (ghost fills, passive scalars, the CFL dt sequence and the density-floor count included);
 FP32 within rounding.  The path and shape are chosen per context from IMHD_VSTAGE and
 IMHD_VSHAPE, which imhd_create reads."""
@@ -100,7 +100,7 @@ def test_fused_stage_fp32_div_b_every_z_split(monkeypatch):
     n = 128
     q, bf, qp = ics.config_c2(n=n)
     for env in ({}, {"IMHD_VSEG": "5"}, {"IMHD_VSEG": "16"}, {"IMHD_VROUNDS": "3"},
-                {"IMHD_VSHAPE": "0", "IMHD_VSEG": "7"}):
+                {"IMHD_VSHAPE": "0", "IMHD_VSEG": "7"}, {"IMHD_VSHAPE": "2"}, {"IMHD_VSHAPE": "2", "IMHD_VSEG": "5"}):
         say(f"[gpu] fp32 fused stage div B {n}^3 {env}")
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -119,3 +119,24 @@ def test_fused_stage_fp32_div_b_every_z_split(monkeypatch):
         S.close()
         assert d0 <= 1e-12, d0
         assert d1 <= 1e-12, f"{env}: max|div B| {d1}"
+
+
+def test_fused_stage_fp32_32x32_tiles(monkeypatch):
+    """The 32 x 32 tile shape (FP32 only; its LDS fills a CU) forms every face, corner and
+    cell with the same per-value code as the 32 x 16 shape, over a different tiling:
+    agreement to rounding (contraction may differ between the instantiations), on
+    multi-tile, thin and open boxes with every solver; in an FP64 context the shape is not
+    available and the default stays."""
+    for case in CASES[:3]:
+        say(f"[gpu] fp32 32x32 tiles vs 32x16 {case}")
+        ref = run(monkeypatch, {"IMHD_VSHAPE": "1"}, *case, precision=32)
+        got = run(monkeypatch, {"IMHD_VSHAPE": "2"}, *case, precision=32)
+        for x, y in zip(ref[0][:2], got[0][:2]):
+            assert np.max(np.abs(x - y)) <= 1e-5 * max(1.0, np.max(np.abs(x)))
+        assert abs(ref[1][1][0] - got[1][1][0]) <= 1e-5 * ref[1][1][0]
+        assert ref[2] == got[2]
+    ref = run(monkeypatch, {}, *CASES[0])
+    got = run(monkeypatch, {"IMHD_VSHAPE": "2"}, *CASES[0])
+    for x, y in zip(ref[0], got[0]):
+        if x is not None:
+            assert np.array_equal(x, y)
