@@ -196,6 +196,16 @@ int imhd_passive_step(imhd_ctx* ctx, double dt);
  * partial rows: 1024 x 8 doubles per box (64 KiB), counted in imhd_device_bytes. */
 int imhd_diagnostics(imhd_ctx* ctx, imhd_diag* out);
 
+/* imhd_diagnostics without any allocation: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc), where imhd_diagnostics's first call would allocate its
+ * partial rows.  Same kernels and fixed reduction order, so the same bits.  The partial
+ * rows live in a fixed 4.2 MB device array per GPU, kept until the process ends, whose
+ * address imhd_create looks up: contexts of at most 64 boxes (IMHD_EINVAL above that).
+ * Every call on a GPU uses that GPU's one array: calls that overlap on one GPU (other
+ * threads, other contexts' streams) race and must be serialised by the caller. */
+int imhd_diagnostics_static(imhd_ctx* ctx, imhd_diag* out);
+
 /* Number of cells the density floor reset since creation (final_check_unphysical,
  * vanleer_mod.f90:3243-3262, which prints instead). */
 int imhd_floor_count(imhd_ctx* ctx, unsigned int* count);

@@ -19,7 +19,7 @@ EXPORTS = [
     "imhd_init_turbulence",
     "imhd_autotune", "imhd_step", "imhd_set_force", "imhd_set_ou_forcing", "imhd_get_ou_modes",
     "imhd_set_impulsive_forcing", "imhd_get_impulsive_forcing", "imhd_fill_ghosts", "imhd_max_speed", "imhd_copy_velocity", "imhd_passive_step",
-    "imhd_diagnostics", "imhd_floor_count", "imhd_device_bytes", "imhd_run", "imhd_set_time", "imhd_get_time",
+    "imhd_diagnostics", "imhd_diagnostics_static", "imhd_floor_count", "imhd_device_bytes", "imhd_run", "imhd_set_time", "imhd_get_time",
     "imhd_set_profiling", "imhd_profile_report", "imhd_sync", "imhd_last_error", "imhd_version",
     "imhd_write_restart", "imhd_read_restart", "imhd_write_cube", "imhd_generate_force", "imhd_get_force",
     "imhd_spectrum", "imhd_density_pdf", "imhd_mode_spectra",
@@ -89,6 +89,7 @@ def lib():
     L.imhd_copy_velocity.argtypes = [vp, ip]
     L.imhd_passive_step.argtypes = [vp, dp]
     L.imhd_diagnostics.argtypes = [vp, ctypes.POINTER(ImhdDiag)]
+    L.imhd_diagnostics_static.argtypes = [vp, ctypes.POINTER(ImhdDiag)]
     L.imhd_floor_count.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
     L.imhd_device_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong)]
     L.imhd_run.argtypes = [vp, ip, dp, ip]

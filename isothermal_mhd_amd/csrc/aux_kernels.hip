@@ -2,6 +2,8 @@
 // conversion kernels.
 #include "aux_kernels.hpp"
 
+#include <mutex>
+
 namespace imhd {
 
 // ---------------------------------------------------------------------------------
@@ -622,6 +624,33 @@ void launch_diag(hipStream_t st, const BoxDesc* boxes, int nbox, const DiagArgs<
     hipLaunchKernelGGL((k_diag_part<real>), dim3(DIAG_WG, nbox), dim3(256), 0, st, boxes, a);
     hipLaunchKernelGGL(k_diag_final, dim3(nbox), dim3(256), 0, st, (const double*)a.part, a.out);
 }
+// Device scratch of imhd_diagnostics_static, the entry point that allocates nothing: the
+// partial rows (DIAG_WG x DIAG_STRIDE doubles per box) and the results (DIAG_STRIDE per
+// box) of up to DIAG_STATIC_MAXBOX boxes, 4.2 MB.  Every device holds its own copy, for
+// the life of the process; calls on one device share it.  Host code never passes the
+// array's name to a kernel: it is a host-side stand-in, not a device address.
+static __device__ __align__(256) double scratch[(size_t)DIAG_STATIC_MAXBOX * (DIAG_WG + 1) * DIAG_STRIDE];
+
+// The device address of the current device's copy of `scratch`, looked up once per
+// device (with that device current) and kept.
+hipError_t diag_static_scratch(double** out) {
+    static std::mutex mu;
+    static double* addr[DIAG_STATIC_MAXDEV] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= DIAG_STATIC_MAXDEV) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = static_cast<double*>(p);
+    }
+    *out = addr[dev];
+    return hipSuccess;
+}
+
 template <class real>
 void launch_copy_velocity(const Launch& L, const VelArgs<real>& a) {
     if (L.plan.total > 0) L256((k_copy_velocity<real>), L.boxes, L.plan, a);

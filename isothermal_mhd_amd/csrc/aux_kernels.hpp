@@ -7,6 +7,11 @@ namespace imhd {
 
 constexpr int DIAG_STRIDE = 8;  // per box: rho v^2, v^2, mx, my, mz, sum divB, mass, max|divB|
 constexpr int DIAG_WG = 1024;   // partial-sum workgroups (and partial rows) per box
+constexpr int DIAG_STATIC_MAXBOX = 64;   // boxes imhd_diagnostics_static serves
+constexpr int DIAG_STATIC_MAXDEV = 64;   // devices whose scratch address is kept
+// device address of this device's fixed scratch for imhd_diagnostics_static (partial rows
+// at the start, results after DIAG_STATIC_MAXBOX x DIAG_WG rows)
+hipError_t diag_static_scratch(double** out);
 
 template <class real>
 struct GhostArgs {

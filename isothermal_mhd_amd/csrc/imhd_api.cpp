@@ -82,6 +82,8 @@ struct imhd_ctx {
     unsigned long long* smax = nullptr;
     unsigned long long* cmax = nullptr;
     double* diag_part = nullptr;         // imhd_diagnostics: DIAG_WG partial rows per box (first call)
+    double* diag_static = nullptr;       // imhd_diagnostics_static: this device's fixed scratch
+    std::string diag_static_err;         // why the scratch address could not be had
     unsigned long long* dvmax = nullptr; // forcing: max |delta m| / rho0 of the last step (delv_max)
     // forcing application (driver-supplied field, turb_driver_mod.f90)
     double* delv[3] = {};
@@ -2075,6 +2077,13 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
         int dev = 0;
         HIPCK(c, hipGetDevice(&dev));
         HIPCK(c, hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        // set-up of the allocation-free diagnostics: this device's scratch address (a
+        // failure only disables imhd_diagnostics_static)
+        const hipError_t e = diag_static_scratch(&c->diag_static);
+        if (e != hipSuccess) {
+            c->diag_static = nullptr;
+            c->diag_static_err = hipGetErrorString(e);
+        }
     }
     HIPCK(c, hipMalloc(&c->db, sizeof(BoxDesc) * nbox));
     HIPCK(c, hipMemcpy(c->db, c->hb.data(), sizeof(BoxDesc) * nbox, hipMemcpyHostToDevice));
@@ -2711,10 +2720,10 @@ static int diagnostics_into(imhd_ctx* c, double* part, double* res, imhd_diag* o
         launch_diag<float>(c->st, c->db, c->nbox, a);
     }
     HIPCK(c, hipGetLastError());
-    std::vector<double> h(DIAG_STRIDE * c->nbox);
-    HIPCK(c, hipMemcpyAsync(h.data(), res, 8 * h.size(), hipMemcpyDeviceToHost, c->st));
+    double h[DIAG_STRIDE * MAXBOX];
+    HIPCK(c, hipMemcpyAsync(h, res, 8 * (size_t)DIAG_STRIDE * c->nbox, hipMemcpyDeviceToHost, c->st));
     HIPCK(c, hipStreamSynchronize(c->st));
-    diag_unpack(c, h.data(), out);
+    diag_unpack(c, h, out);
     return IMHD_OK;
 }
 
@@ -2726,6 +2735,18 @@ int imhd_diagnostics(imhd_ctx* c, imhd_diag* out) {
         if (rc) { c->diag_part = nullptr; return rc; }
     }
     return diagnostics_into(c, c->diag_part, c->diag, out);
+}
+
+int imhd_diagnostics_static(imhd_ctx* c, imhd_diag* out) {
+    if (!c || !out) return IMHD_EINVAL;
+    if (!c->have_state) FAIL(c, IMHD_ESTATE, "no state set");
+    if (c->nbox > DIAG_STATIC_MAXBOX)
+        FAIL(c, IMHD_EINVAL, "imhd_diagnostics_static serves at most %d boxes (this context has %d): use imhd_diagnostics",
+             DIAG_STATIC_MAXBOX, c->nbox);
+    if (!c->diag_static)
+        FAIL(c, IMHD_EHIP, "imhd_diagnostics_static: no device scratch on this device (%s)", c->diag_static_err.c_str());
+    double* res = c->diag_static + (size_t)DIAG_STATIC_MAXBOX * DIAG_WG * DIAG_STRIDE;
+    return diagnostics_into(c, c->diag_static, res, out);
 }
 
 int imhd_floor_count(imhd_ctx* c, unsigned int* count) {
