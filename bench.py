@@ -319,12 +319,11 @@ FAMILY_SYMBOL = {"stage_pred": "k_vl_stage", "stage_corr_cfl": "k_vl_stage",
                  "corner_corr": "k_corner", "update_pred": "k_update_z", "update_corr_cfl": "k_update_z",
                  "pass_stage_pred": "k_pass_stage_z", "pass_stage_corr": "k_pass_stage_z",
                  "fill_ghosts": "k_fill_ghost_band", "fill_ghosts_pass": "k_fill_ghost_band",
-                 "stage_corr": "k_vl_stage", "ou_forcing": "k_ou_pass"}
+                 "stage_corr": "k_vl_stage", "ou_forcing": "k_ou_kick"}
 
 
-# kernel dispatches of a family's symbol per launch of the family (the OU step runs its
-# k_ou_pass twice: the <rho a> sums, then the kick)
-FAMILY_DISPATCHES = {"ou_forcing": 2}
+# kernel dispatches of a family's symbol per launch of the family, where not one
+FAMILY_DISPATCHES = {}
 
 
 def pmc_run(n=1024, steps=3, n_cal=256):

@@ -60,20 +60,21 @@ template <class real>
 struct OuApplyArgs {
     OuDev ou;
     real* q[7];              // new state (rho, m read; m written; all read for the CFL)
-    double* part;            // OU_WG rows of 3 sums per box
+    double* part;            // ou_part_rows partial rows of 3 sums
     double* mean;            // 3 per box
     unsigned long long* cmax;// non-null: fused CFL speed of the forced cells
     double cs;
 };
-constexpr int OU_WG = 1024;  // partial-sum workgroups (and rows) per box of the <rho a> pass
+constexpr int OU_RY = 1024;  // row groups per x-block of the OU passes (and partial rows of <rho a>)
 
 void launch_ou_update(hipStream_t st, int nbox, const BoxDesc* boxes, const OuDev& ou, int init);
 void launch_ou_columns(hipStream_t st, int nbox, int maxcols, const BoxDesc* boxes, const OuDev& ou);
 template <class real>
-void launch_ou_sums(hipStream_t st, int nbox, const BoxDesc* boxes, const OuApplyArgs<real>& a);
+void launch_ou_sums(hipStream_t st, int nbox, int maxnx, const BoxDesc* boxes, const OuApplyArgs<real>& a);
 template <class real>
-void launch_ou_kick(hipStream_t st, int nbox, long long maxcells, const BoxDesc* boxes, const OuApplyArgs<real>& a);
-void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, int nbox, const double* part, double* mean);
+void launch_ou_kick(hipStream_t st, int nbox, int maxnx, const BoxDesc* boxes, const OuApplyArgs<real>& a);
+void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, int nbox, int maxnx, const double* part, double* mean);
+size_t ou_part_rows(int nbox, int maxnx);   // partial rows of <rho a> (3 doubles each)
 
 // ---- the reference's random force (generate_rand_force) ----
 // Inverse DFT passes along y and z of the direct transform (turb_driver_mod.f90:461-486):

@@ -199,104 +199,132 @@ __global__ __launch_bounds__(256) void k_ou_columns(int nbox, const BoxDesc* __r
     }
 }
 
-__device__ __forceinline__ void ou_field(const OuDev& ou, const BoxDesc& B, int b, int i, int j, int k, double* a) {
-    const int W = 2 * ou.K + 1;
-    const double* tx = ou.tw + ou.twoff[b];
-    const double* D = ou.col + ou.coloff[b] + ((size_t)(k - 1) * B.n[1] + (j - 1)) * W * 6;
-    a[0] = a[1] = a[2] = 0.0;
-    for (int slot = 0; slot < W; slot++) {
-        const double* ex = tx + 2 * ((size_t)slot * B.n[0] + (i - 1));
-        const double xr = ex[0], xi = ex[1];
-        for (int d = 0; d < 3; d++) a[d] = a[d] + 2.0 * (xr * D[slot * 6 + 2 * d] - xi * D[slot * 6 + 2 * d + 1]);
+// The OU passes walk interior rows: a workgroup holds 256 consecutive x-cells (x-block xb)
+// and takes the rows (j, k) r = ry, ry + OU_RY, ...; the x-twiddles of its cell stay in
+// registers (KC = K <= 4; KC = 0: any K, loaded per row), and the row's mode sums D are
+// uniform over the workgroup.  Per cell the field a(x) = sum_slot 2 Re(e_x D) is summed in
+// slot order, as ou_field does.
+template <int KC>
+struct OuRowField {
+    static constexpr int WC = KC > 0 ? 2 * KC + 1 : 1;
+    double xr[WC], xi[WC];
+    const double* tx;
+    int W, nx, i;
+    __device__ __forceinline__ OuRowField(const OuDev& ou, const BoxDesc& B, int b, int i_) : i(i_) {
+        W = 2 * ou.K + 1;
+        nx = B.n[0];
+        tx = ou.tw + ou.twoff[b];
+        if constexpr (KC > 0) {
+#pragma unroll
+            for (int s = 0; s < WC; s++) {
+                const double* ex = tx + 2 * ((size_t)s * nx + (i - 1));
+                xr[s] = ex[0];
+                xi[s] = ex[1];
+            }
+        }
     }
-}
-
-// interior cell (i, j, k) (from 1) of a box from its linear index, i fastest; 32-bit
-// arithmetic whenever the box has fewer than 2^32 cells
-__device__ __forceinline__ void interior_ijk(unsigned long long loc, unsigned long long nx, unsigned long long nxy,
-                                             bool small, int& i, int& j, int& k) {
-    if (small) {
-        const unsigned l32 = (unsigned)loc, k32 = l32 / (unsigned)nxy, r32 = l32 - k32 * (unsigned)nxy;
-        const unsigned j32 = r32 / (unsigned)nx;
-        i = (int)(r32 - j32 * (unsigned)nx) + 1; j = (int)j32 + 1; k = (int)k32 + 1;
-    } else {
-        const unsigned long long kk = loc / nxy, r = loc - kk * nxy, jj = r / nx;
-        i = (int)(r - jj * nx) + 1; j = (int)jj + 1; k = (int)kk + 1;
+    __device__ __forceinline__ void eval(const double* D, double* a) const {
+        a[0] = a[1] = a[2] = 0.0;
+        if constexpr (KC > 0) {
+#pragma unroll
+            for (int s = 0; s < WC; s++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) a[d] = a[d] + 2.0 * (xr[s] * D[s * 6 + 2 * d] - xi[s] * D[s * 6 + 2 * d + 1]);
+        } else {
+            for (int s = 0; s < W; s++) {
+                const double* ex = tx + 2 * ((size_t)s * nx + (i - 1));
+                const double er = ex[0], ei = ex[1];
+                for (int d = 0; d < 3; d++) a[d] = a[d] + 2.0 * (er * D[s * 6 + 2 * d] - ei * D[s * 6 + 2 * d + 1]);
+            }
+        }
     }
-}
+};
 
-// <rho a> partial sums: OU_WG workgroups per box stride over its interior cells; partial
-// row (box, workgroup) of 3 sums, reduced in a fixed order by k_ou_finalize
-template <class real>
-__global__ __launch_bounds__(256) void k_ou_sums(const BoxDesc* __restrict__ boxes, OuApplyArgs<real> a) {
+// <rho a> partial sums: one partial row of 3 sums per workgroup (box, x-block, row group),
+// reduced in a fixed order by k_ou_finalize
+template <class real, int KC>
+__global__ __launch_bounds__(256) void k_ou_sums(const BoxDesc* __restrict__ boxes, OuApplyArgs<real> a, int nxb) {
     __shared__ double sh[FORCE_NSUM][256];
-    const int b = blockIdx.y;
+    const int b = blockIdx.z;
     const BoxDesc& B = boxes[b];
-    const unsigned long long nx = B.n[0], nxy = nx * (unsigned long long)B.n[1], ncell = nxy * B.n[2];
-    const bool small = ncell <= 0xffffffffull;
+    const int i = (int)blockIdx.x * 256 + (int)threadIdx.x + 1;
+    const int ny = B.n[1], nrow = B.n[1] * B.n[2];
+    const int W = 2 * a.ou.K + 1;
+    const double* col = a.ou.col + a.ou.coloff[b];
     double v[FORCE_NSUM] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (unsigned long long loc = (unsigned long long)blockIdx.x * 256 + threadIdx.x; loc < ncell;
-         loc += (unsigned long long)OU_WG * 256) {
-        int i, j, k;
-        interior_ijk(loc, nx, nxy, small, i, j, k);
-        double f[3];
-        ou_field(a.ou, B, b, i, j, k, f);
-        const double rho = (double)a.q[0][cell_index(B, i, j, k)];
-        v[0] = v[0] + rho * f[0];
-        v[1] = v[1] + rho * f[1];
-        v[2] = v[2] + rho * f[2];
+    if (i <= B.n[0]) {
+        const OuRowField<KC> F(a.ou, B, b, i);
+#pragma unroll 4
+        for (int r = (int)blockIdx.y; r < nrow; r += OU_RY) {
+            const int k = r / ny, j = r - k * ny;
+            double f[3];
+            F.eval(col + (size_t)r * W * 6, f);
+            const double rho = (double)a.q[0][cell_index(B, i, j + 1, k + 1)];
+            v[0] = v[0] + rho * f[0];
+            v[1] = v[1] + rho * f[1];
+            v[2] = v[2] + rho * f[2];
+        }
     }
     block_sum5(sh, v);
-    if (threadIdx.x < 3) a.part[((size_t)b * OU_WG + blockIdx.x) * 3 + threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x < 3)
+        a.part[(((size_t)b * nxb + blockIdx.x) * OU_RY + blockIdx.y) * 3 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // kick: m += dt (rho a - <rho a>) on every interior cell, and (cmax non-null) the CFL
 // speed of the kicked cell: FP64 from the FP64 state, FP32 (hardware rcp/rsqrt/sqrt) from
 // an FP32 state, maximum in FP64
-template <class real>
+template <class real, int KC>
 __global__ __launch_bounds__(256) void k_ou_kick(const BoxDesc* __restrict__ boxes, OuApplyArgs<real> a) {
-    const int b = blockIdx.y;
+    const int b = blockIdx.z;
     const BoxDesc& B = boxes[b];
-    const unsigned long long nx = B.n[0], nxy = nx * (unsigned long long)B.n[1], ncell = nxy * B.n[2];
-    const unsigned long long loc = (unsigned long long)blockIdx.x * 256 + threadIdx.x;
+    const int i = (int)blockIdx.x * 256 + (int)threadIdx.x + 1;
+    const int ny = B.n[1], nrow = B.n[1] * B.n[2];
+    const int W = 2 * a.ou.K + 1;
+    const double* col = a.ou.col + a.ou.coloff[b];
+    const double dt = a.ou.dt[b];
+    const double* mean = a.mean + 3 * b;
     double cfl = 0.0;
-    if (loc < ncell) {
-        int i, j, k;
-        interior_ijk(loc, nx, nxy, ncell <= 0xffffffffull, i, j, k);
-        const long long c = cell_index(B, i, j, k);
-        double f[3];
-        ou_field(a.ou, B, b, i, j, k, f);
-        const double rho = (double)a.q[0][c];
-        const double dt = a.ou.dt[b];
-        const double* mean = a.mean + 3 * b;
-        real q7[7];
-        q7[0] = a.q[0][c];
-        for (int d = 0; d < 3; d++) {
-            q7[1 + d] = (real)((double)a.q[1 + d][c] + dt * (rho * f[d] - mean[d]));
-            a.q[1 + d][c] = q7[1 + d];
-        }
-        if (a.cmax != nullptr) {
-            for (int g = 4; g < 7; g++) q7[g] = a.q[g][c];
-            if constexpr (sizeof(real) == 4) {
-                cfl = (double)cell_fast_speed_f(q7, (float)a.cs);
-            } else {
-                cfl = cell_fast_speed(q7, a.cs);
+    if (i <= B.n[0]) {
+        const OuRowField<KC> F(a.ou, B, b, i);
+        const double m0 = mean[0], m1 = mean[1], m2 = mean[2];
+#pragma unroll 2
+        for (int r = (int)blockIdx.y; r < nrow; r += OU_RY) {
+            const int k = r / ny, j = r - k * ny;
+            const long long c = cell_index(B, i, j + 1, k + 1);
+            double f[3];
+            F.eval(col + (size_t)r * W * 6, f);
+            real q7[7];
+            q7[0] = a.q[0][c];
+            const double rho = (double)q7[0];
+            q7[1] = (real)((double)a.q[1][c] + dt * (rho * f[0] - m0));
+            q7[2] = (real)((double)a.q[2][c] + dt * (rho * f[1] - m1));
+            q7[3] = (real)((double)a.q[3][c] + dt * (rho * f[2] - m2));
+            a.q[1][c] = q7[1];
+            a.q[2][c] = q7[2];
+            a.q[3][c] = q7[3];
+            if (a.cmax != nullptr) {
+                q7[4] = a.q[4][c];
+                q7[5] = a.q[5][c];
+                q7[6] = a.q[6][c];
+                if constexpr (sizeof(real) == 4) cfl = fmax(cfl, (double)cell_fast_speed_f(q7, (float)a.cs));
+                else cfl = fmax(cfl, cell_fast_speed(q7, a.cs));
             }
         }
     }
     if (a.cmax != nullptr) atomic_max_nonneg(a.cmax, b, cfl);
 }
 
-// per box: <rho a> = sum / cells, its OU_WG partial rows in a fixed order
+// per box: <rho a> = sum / cells, its nxb x OU_RY partial rows in a fixed order
 __global__ __launch_bounds__(256) void k_ou_finalize(const BoxDesc* __restrict__ boxes, const double* part,
-                                                     double* mean) {
+                                                     double* mean, int nxb) {
     __shared__ double sh[FORCE_NSUM][256];
     const int b = blockIdx.x;
     const BoxDesc& B = boxes[b];
     const double ncell = (double)B.n[0] * (double)B.n[1] * (double)B.n[2];
+    const int nrows = nxb * OU_RY;
     double v[FORCE_NSUM] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int w = threadIdx.x; w < OU_WG; w += 256)
-        for (int s = 0; s < 3; s++) v[s] = v[s] + part[((size_t)b * OU_WG + w) * 3 + s];
+    for (int w = threadIdx.x; w < nrows; w += 256)
+        for (int s = 0; s < 3; s++) v[s] = v[s] + part[((size_t)b * nrows + w) * 3 + s];
     block_sum5(sh, v);
     if (threadIdx.x == 0)
         for (int s = 0; s < 3; s++) mean[3 * b + s] = sh[s][0] / ncell;
@@ -311,22 +339,40 @@ void launch_ou_columns(hipStream_t st, int nbox, int maxcols, const BoxDesc* box
     hipLaunchKernelGGL(k_ou_columns, dim3((unsigned)((maxcols + 255) / 256), (unsigned)nbox), dim3(256), 0, st, nbox,
                        boxes, ou);
 }
+// x-blocks of 256 cells covering the widest box
+static int ou_nxb(int maxnx) { return (maxnx + 255) / 256; }
+
 template <class real>
-void launch_ou_sums(hipStream_t st, int nbox, const BoxDesc* boxes, const OuApplyArgs<real>& a) {
-    hipLaunchKernelGGL((k_ou_sums<real>), dim3(OU_WG, (unsigned)nbox), dim3(256), 0, st, boxes, a);
+void launch_ou_sums(hipStream_t st, int nbox, int maxnx, const BoxDesc* boxes, const OuApplyArgs<real>& a) {
+    const int nxb = ou_nxb(maxnx);
+    const dim3 g((unsigned)nxb, OU_RY, (unsigned)nbox);
+    switch (a.ou.K) {
+        case 1: hipLaunchKernelGGL((k_ou_sums<real, 1>), g, dim3(256), 0, st, boxes, a, nxb); break;
+        case 2: hipLaunchKernelGGL((k_ou_sums<real, 2>), g, dim3(256), 0, st, boxes, a, nxb); break;
+        case 3: hipLaunchKernelGGL((k_ou_sums<real, 3>), g, dim3(256), 0, st, boxes, a, nxb); break;
+        case 4: hipLaunchKernelGGL((k_ou_sums<real, 4>), g, dim3(256), 0, st, boxes, a, nxb); break;
+        default: hipLaunchKernelGGL((k_ou_sums<real, 0>), g, dim3(256), 0, st, boxes, a, nxb); break;
+    }
 }
 template <class real>
-void launch_ou_kick(hipStream_t st, int nbox, long long maxcells, const BoxDesc* boxes, const OuApplyArgs<real>& a) {
-    hipLaunchKernelGGL((k_ou_kick<real>), dim3((unsigned)((maxcells + 255) / 256), (unsigned)nbox), dim3(256), 0, st,
-                       boxes, a);
+void launch_ou_kick(hipStream_t st, int nbox, int maxnx, const BoxDesc* boxes, const OuApplyArgs<real>& a) {
+    const dim3 g((unsigned)ou_nxb(maxnx), OU_RY, (unsigned)nbox);
+    switch (a.ou.K) {
+        case 1: hipLaunchKernelGGL((k_ou_kick<real, 1>), g, dim3(256), 0, st, boxes, a); break;
+        case 2: hipLaunchKernelGGL((k_ou_kick<real, 2>), g, dim3(256), 0, st, boxes, a); break;
+        case 3: hipLaunchKernelGGL((k_ou_kick<real, 3>), g, dim3(256), 0, st, boxes, a); break;
+        case 4: hipLaunchKernelGGL((k_ou_kick<real, 4>), g, dim3(256), 0, st, boxes, a); break;
+        default: hipLaunchKernelGGL((k_ou_kick<real, 0>), g, dim3(256), 0, st, boxes, a); break;
+    }
 }
-void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, int nbox, const double* part, double* mean) {
-    hipLaunchKernelGGL(k_ou_finalize, dim3(nbox), dim3(256), 0, st, boxes, part, mean);
+void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, int nbox, int maxnx, const double* part, double* mean) {
+    hipLaunchKernelGGL(k_ou_finalize, dim3(nbox), dim3(256), 0, st, boxes, part, mean, ou_nxb(maxnx));
 }
-template void launch_ou_sums<double>(hipStream_t, int, const BoxDesc*, const OuApplyArgs<double>&);
-template void launch_ou_sums<float>(hipStream_t, int, const BoxDesc*, const OuApplyArgs<float>&);
-template void launch_ou_kick<double>(hipStream_t, int, long long, const BoxDesc*, const OuApplyArgs<double>&);
-template void launch_ou_kick<float>(hipStream_t, int, long long, const BoxDesc*, const OuApplyArgs<float>&);
+size_t ou_part_rows(int nbox, int maxnx) { return (size_t)nbox * ou_nxb(maxnx) * OU_RY; }
+template void launch_ou_sums<double>(hipStream_t, int, int, const BoxDesc*, const OuApplyArgs<double>&);
+template void launch_ou_sums<float>(hipStream_t, int, int, const BoxDesc*, const OuApplyArgs<float>&);
+template void launch_ou_kick<double>(hipStream_t, int, int, const BoxDesc*, const OuApplyArgs<double>&);
+template void launch_ou_kick<float>(hipStream_t, int, int, const BoxDesc*, const OuApplyArgs<float>&);
 
 template void launch_force_sums<double>(const Launch&, const ForceSumArgs<double>&);
 template void launch_force_sums<float>(const Launch&, const ForceSumArgs<float>&);

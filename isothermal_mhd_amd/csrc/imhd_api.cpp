@@ -884,12 +884,12 @@ void ou_step(imhd_ctx* c, bool fuse_cmax) {
     a.part = c->ou_part;
     a.mean = c->ou_mean;
     a.cs = c->cfg.cs;
-    launch_ou_sums<real>(c->st, c->nbox, c->db, a);
-    launch_ou_finalize(c->st, c->db, c->nbox, c->ou_part, c->ou_mean);
+    int maxnx = 0;
+    for (const BoxDesc& B : c->hb) maxnx = std::max(maxnx, B.n[0]);
+    launch_ou_sums<real>(c->st, c->nbox, maxnx, c->db, a);
+    launch_ou_finalize(c->st, c->db, c->nbox, maxnx, c->ou_part, c->ou_mean);
     a.cmax = fuse_cmax ? c->cmax : nullptr;
-    long long maxcells = 0;
-    for (const BoxDesc& B : c->hb) maxcells = std::max(maxcells, (long long)B.n[0] * B.n[1] * B.n[2]);
-    launch_ou_kick<real>(c->st, c->nbox, maxcells, c->db, a);
+    launch_ou_kick<real>(c->st, c->nbox, maxnx, c->db, a);
 }
 
 // Drop-in step only: cells outside the VL final-update range take their t^n values, as
@@ -2511,7 +2511,11 @@ int imhd_set_ou_forcing(imhd_ctx* c, const imhd_ou* ou) {
     HIPCK(c, hipMemcpy(c->ou_tw, tw.data(), 8 * tw.size(), hipMemcpyHostToDevice));
     HIPCK(c, hipMalloc(&c->ou_twoff, 8 * off.size()));
     HIPCK(c, hipMemcpy(c->ou_twoff, off.data(), 8 * off.size(), hipMemcpyHostToDevice));
-    HIPCK(c, hipMalloc(&c->ou_part, 8 * 3 * (size_t)OU_WG * c->nbox));
+    {
+        int maxnx = 0;
+        for (const BoxDesc& B : c->hb) maxnx = std::max(maxnx, B.n[0]);
+        HIPCK(c, hipMalloc(&c->ou_part, 8 * 3 * ou_part_rows(c->nbox, maxnx)));
+    }
     HIPCK(c, hipMalloc(&c->ou_mean, 8 * 3 * (size_t)c->nbox));
     std::vector<long long> coff;
     long long ncol = 0;
