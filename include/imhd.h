@@ -105,7 +105,8 @@ int imhd_get_field(imhd_ctx* ctx, int box, int field, double* out);
  *           2 Re(v_n e^{2 pi i (n_x i / nx + n_y j / ny + n_z k / nz)}), cell (i, j, k)
  *           from 0.  Mode m (its place in that enumeration: n_z, then n_y, then n_x
  *           ascending) draws v_n,d = |n|^{-(slope+2)/2} (g1 + i g2) from Philox counter
- *           (m, d, 1, 0); solenoidal != 0 projects each mode normal to n.  The amplitudes
+ *           (m, d, 1, 0); solenoidal != 0 projects each mode normal to its physical
+ *           wave vector (n_x / nx, n_y / ny, n_z / nz; normal to n in a cube).  The amplitudes
  *           are scaled so that the mean of |v|^2 over the interior is exactly (mach cs)^2
  *           (Parseval; needs floor(kmax) < n_d / 2 on every axis);
  *   rho v = rho * v (FP64, rounded once in FP32 contexts); B = b0 in every cell and face;

@@ -63,6 +63,7 @@ struct imhd_ctx {
     void* q[4][7] = {};    // set 2: half-step state of the fused VL stages; TVD: sets 2, 3 hold
                            // the y and z half states of k_tvd_half3
     double* b[3][3] = {};
+    void* bh[3] = {};      // FP32 fused VL: the half-step face field in FP32 (set 2's faces)
     void* qp[2][MAXPASS] = {};
     void* F[3][4] = {};
     void* E[6] = {};
@@ -528,6 +529,12 @@ void vl_stage(imhd_ctx* c, int sflux, int src, int dst, bool corr, bool fuse_cma
         a.dst.q[f] = R<real>(c->q[dst][f]);
     }
     for (int d = 0; d < 3; d++) { a.src.b[d] = c->b[src][d]; a.dst.b[d] = c->b[dst][d]; }
+    if (c->bh[0] != nullptr) {   // FP32: the half-step faces live in bh
+        for (int d = 0; d < 3; d++) {
+            if (dst == 2) a.bhw[d] = R<real>(c->bh[d]);
+            if (sflux == 2) a.d[d].bnr = R<real>(c->bh[d]);
+        }
+    }
     a.dt = c->dt;
     a.factor = corr ? 1.0 : 0.5;
     a.ccb_lo_shift = c->ccb_shift;
@@ -2103,7 +2110,10 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
     const bool fused = g.integrator == IMHD_VL && c->vstage && (g.recon == IMHD_PLM_PRIM || vppm);
     if (fused) {
         for (int f = 0; f < 7; f++) if ((rc = alloc(c, &c->q[2][f], nr))) return rc;
-        for (int d = 0; d < 3; d++) if ((rc = alloc(c, (void**)&c->b[2][d], nd))) return rc;
+        // the half-step face field: FP64 in FP64 contexts; FP32 in FP32 contexts, whose
+        // fused corrector converts it to FP32 interface states and never reads it otherwise
+        for (int d = 0; d < 3; d++)
+            if ((rc = g.precision == 64 ? alloc(c, (void**)&c->b[2][d], nd) : alloc(c, &c->bh[d], nr))) return rc;
         if (g.npass > 0)
             for (int d = 0; d < 3; d++) if ((rc = alloc(c, &c->pvel[d], nr))) return rc;
     } else {
@@ -2207,6 +2217,7 @@ int imhd_destroy(imhd_ctx* c) {
     for (auto p : c->q[3]) fr(p);
     for (auto p : c->pvel) fr(p);
     for (auto p : c->b[2]) fr(p);
+    for (auto p : c->bh) fr(p);
     fr(c->tp_pred.d);
     fr(c->tp_corr.d);
     for (auto& a : c->F) for (auto p : a) fr(p);

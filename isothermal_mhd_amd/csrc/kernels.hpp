@@ -10,6 +10,8 @@ template <class real>
 struct SweepArgs {
     const real* w[7];
     const double* bn;
+    const real* bnr;  // non-null: the normal face field in the working precision (the fused
+                      // FP32 corrector's half-step faces), read instead of bn
     real* F[4];       // rotated hydro fluxes: rho, m_n, m_t1, m_t2
     real* Et1;        // face EMF = -flux(B_t1)
     real* Et2;        // face EMF = +flux(B_t2)
@@ -19,6 +21,12 @@ struct SweepArgs {
     double mc_beta;
     unsigned long long* smax;  // per box, bit pattern of a non-negative double
 };
+
+// normal face field of a sweep at cell c, in the working precision
+template <class real>
+__device__ __forceinline__ real bn_at(const SweepArgs<real>& a, long long c) {
+    return a.bnr != nullptr ? a.bnr[c] : (real)a.bn[c];
+}
 
 template <class real>
 struct StateP {
@@ -122,6 +130,9 @@ struct StageArgs {
     // copy_velocity -- the predictor writes m/rho at t^n, the corrector the average of t^n
     // and the new state
     real* pvel[3];
+    // non-null (FP32 predictor): the half-step face field goes here in the working precision
+    // instead of to dst.b -- the fused corrector reads it only as FP32 interface states
+    real* bhw[3];
 };
 // kind: 0 predictor, 1 PLM corrector, 2 PPM-primitive corrector, 3 PPM-characteristic
 // corrector (vs_kind)

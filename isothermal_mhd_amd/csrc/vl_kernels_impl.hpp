@@ -38,7 +38,7 @@ __device__ __forceinline__ real flux_pred_calc(const SweepArgs<real>& a, long lo
         uL[r] = a.w[r][c];
         uR[r] = a.w[r][c + s];
     }
-    uL[4] = (real)a.bn[c];
+    uL[4] = bn_at(a, c);
     uR[4] = uL[4];
     real smax = real(0);
     riemann<RS>(uL, uR, (real)a.cs, F, smax);
@@ -122,7 +122,7 @@ __device__ __forceinline__ real flux_corr_calc(const SweepArgs<real>& a, long lo
     real W[4][6];
 #pragma unroll
     for (int t = 0; t < 4; t++) prim_load(a, c + (long long)(t - 1) * s, W[t]);
-    return plm_flux<real, RS>(a, W, (real)a.bn[c], F);
+    return plm_flux<real, RS>(a, W, bn_at(a, c), F);
 }
 template <class real, int RS>
 __device__ __forceinline__ double flux_corr_face(const SweepArgs<real>& a, long long c, long long s) {
@@ -160,7 +160,7 @@ __device__ __forceinline__ real ppm_flux_calc(const SweepArgs<real>& a, long lon
         W[t][3] = a.w[3][ct] * irho;
         W[t][4] = a.w[5][ct];
         W[t][5] = a.w[6][ct];
-        wbc[t] = (real)a.bn[ct];
+        wbc[t] = bn_at(a, ct);
     }
     real uL[7], uR[7];
     ppm_states<real, CHAR>(W, wbc, (real)a.cs, (real)a.mc_beta, uL, uR);

@@ -44,7 +44,7 @@ __device__ __forceinline__ real stage_face_own(const SweepArgs<real>& a, const r
             uL[r] = U[perm(D, r)];
             uR[r] = a.w[r][c + s];
         }
-        uL[4] = (real)a.bn[c];
+        uL[4] = bn_at(a, c);
         uR[4] = uL[4];
         real smax = real(0);
         riemann<RS>(uL, uR, (real)a.cs, F, smax);
@@ -55,7 +55,7 @@ __device__ __forceinline__ real stage_face_own(const SweepArgs<real>& a, const r
         prim_rot(U[0], U[perm(D, 1)], U[perm(D, 2)], U[perm(D, 3)], U[perm(D, 5)], U[perm(D, 6)], W[1]);
         prim_load(a, c + s, W[2]);
         prim_load(a, c + 2 * s, W[3]);
-        return plm_flux<real, RS>(a, W, (real)a.bn[c], F);
+        return plm_flux<real, RS>(a, W, bn_at(a, c), F);
     } else {
         return ppm_flux_calc<real, RS, KIND == 3>(a, c, s, F);
     }
@@ -74,7 +74,7 @@ __device__ __forceinline__ real stage_face_z(const SweepArgs<real>& a, const rea
             uL[r] = UL[perm(2, r)];
             uR[r] = UR[perm(2, r)];
         }
-        uL[4] = (real)a.bn[c];
+        uL[4] = bn_at(a, c);
         uR[4] = uL[4];
         real smax = real(0);
         riemann<RS>(uL, uR, (real)a.cs, F, smax);
@@ -88,7 +88,7 @@ __device__ __forceinline__ real stage_face_z(const SweepArgs<real>& a, const rea
         prim_rot(UL[0], UL[perm(2, 1)], UL[perm(2, 2)], UL[perm(2, 3)], UL[perm(2, 5)], UL[perm(2, 6)], W[1]);
         prim_rot(UR[0], UR[perm(2, 1)], UR[perm(2, 2)], UR[perm(2, 3)], UR[perm(2, 5)], UR[perm(2, 6)], W[2]);
         prim_load(a, c + 2 * s, W[3]);
-        return plm_flux<real, RS>(a, W, (real)a.bn[c], F);
+        return plm_flux<real, RS>(a, W, bn_at(a, c), F);
     } else {
         return flux_corr_calc<real, RS>(a, c, s, F);
     }
@@ -339,9 +339,15 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             double bx = a.src.b[0][c] - dtd * ((ez - ez_y) - (ey - ey_m));
             double by = a.src.b[1][c] - dtd * ((ex - ex_m) - (ez - ez_x));
             double bz = a.src.b[2][c] - dtd * ((ey - ey_x) - (ex - ex_y));
-            a.dst.b[0][c] = bx;
-            a.dst.b[1][c] = by;
-            a.dst.b[2][c] = bz;
+            if (a.bhw[0] != nullptr) {   // FP32 half step: the corrector reads these as FP32
+                a.bhw[0][c] = (real)bx;
+                a.bhw[1][c] = (real)by;
+                a.bhw[2][c] = (real)bz;
+            } else {
+                a.dst.b[0][c] = bx;
+                a.dst.b[1][c] = by;
+                a.dst.b[2][c] = bz;
+            }
             if (ccb_ij && k >= a.u_lo + sh) {
                 const double ez_xy = sEz[t - 1 - TX];
                 double bxm = a.src.b[0][c - sx] - dtd * ((ez_x - ez_xy) - (ey_x - eyx_m));

@@ -88,6 +88,19 @@ def test_fused_stage_fp32_matches_three_kernel_path(monkeypatch):
     assert abs(ref[1][1][0] - got[1][1][0]) <= 1e-5 * ref[1][1][0]
 
 
+def test_fused_ppm_corrector_fp32_matches_three_kernel_path(monkeypatch):
+    """FP32 fused PPM correctors (characteristic and primitive, two waves per SIMD) against
+    the three-kernel path (IMHD_VPPM=0): to rounding, on a multi-tile periodic box and an
+    open one."""
+    for case in ((( 40, 36, 24), "hlld", "periodic", "ppm_char", 1), ((22, 17, 13), "roe", "open", "ppm_prim", 0)):
+        say(f"[gpu] fp32 fused PPM corrector vs three-kernel {case}")
+        ref = run(monkeypatch, {"IMHD_VPPM": "0"}, *case, precision=32)
+        got = run(monkeypatch, {}, *case, precision=32)
+        for x, y in zip(ref[0][:2], got[0][:2]):
+            assert np.max(np.abs(x - y)) <= 1e-5 * max(1.0, np.max(np.abs(x)))
+        assert abs(ref[1][1][0] - got[1][1][0]) <= 1e-5 * ref[1][1][0]
+
+
 def test_fused_stage_fp32_div_b_every_z_split(monkeypatch):
     """FP32 bulk with FP64 face B and corner EMFs: constrained transport keeps div B at FP64
     rounding only if every corner EMF that two z-segments (or two tiles) both form has the
