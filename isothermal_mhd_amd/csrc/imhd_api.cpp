@@ -160,6 +160,8 @@ struct imhd_ctx {
     int vrounds = 4;     // > 0: fused-stage workgroups = vrounds x resident workgroups of the GPU
     int ncu = 0;         // compute units of the device
     int vshape = 0;      // tile shape of the fused stages (0: 16 x 16, 1: 32 x 16; FP32 default 1)
+    int vshape_c = 0;    // tile shape of the fused corrector (the autotuner may choose it apart)
+    bool vsh = true;     // FP32 PLM corrector from LDS-held primitives with shared slopes (kind 4)
     TilePlanDev tp_pred, tp_corr;
     PlanDev pl_tcorner_p, pl_tcorner_c, pl_tcenter, pl_thalf, pl_tcfl;
     int ccb_shift = 1;
@@ -322,12 +324,20 @@ StageRanges stage_ranges(const imhd_ctx* c, bool corr) {
 // width - 2); threads 1 .. width-2 update, threads 0 .. width-1 solve faces, threads
 // 0 .. width-2 form corners.  The tile count covers every face, corner and cell of the
 // stage's ranges; the march covers planes zlo-1 .. zhi+1.
+// kind of a fused stage's kernel: vs_kind, with the FP32 PLM corrector on its shared-slope
+// form unless IMHD_VSH=0
+int stage_kind(const imhd_ctx* c, bool corr) {
+    const int k = vs_kind(corr, c->cfg.recon);
+    return k == 1 && c->prec == 32 && c->vsh ? 4 : k;
+}
+int stage_shape(const imhd_ctx* c, bool corr) { return corr ? c->vshape_c : c->vshape; }
+
 template <class real>
 int stage_blocks_per_cu(const imhd_ctx* c, bool corr) {
     switch (c->cfg.riemann) {
-        case IMHD_ROE: return vl_stage_blocks_per_cu<real, RS_ROE>(vs_kind(corr, c->cfg.recon), c->vshape);
-        case IMHD_HLLE: return vl_stage_blocks_per_cu<real, RS_HLLE>(vs_kind(corr, c->cfg.recon), c->vshape);
-        default: return vl_stage_blocks_per_cu<real, RS_HLLD>(vs_kind(corr, c->cfg.recon), c->vshape);
+        case IMHD_ROE: return vl_stage_blocks_per_cu<real, RS_ROE>(stage_kind(c, corr), stage_shape(c, corr));
+        case IMHD_HLLE: return vl_stage_blocks_per_cu<real, RS_HLLE>(stage_kind(c, corr), stage_shape(c, corr));
+        default: return vl_stage_blocks_per_cu<real, RS_HLLD>(stage_kind(c, corr), stage_shape(c, corr));
     }
 }
 
@@ -348,7 +358,7 @@ int build_tiles(imhd_ctx* c, TilePlanDev& p, bool corr) {
     const StageRanges R = stage_ranges(c, corr);
     const int lo = std::min(std::min(R.ft_lo, R.fn_lo), R.u_lo - 1);
     const int fhi = std::max(R.ft_hi, R.fn_hi);
-    const int wx[2] = {vs_tx(c->vshape) - 2, vs_ty(c->vshape) - 2};
+    const int wx[2] = {vs_tx(stage_shape(c, corr)) - 2, vs_ty(stage_shape(c, corr)) - 2};
     std::vector<long long> tiles, planes;
     long long work = 0;
     for (int bx = 0; bx < c->nbox; bx++) {
@@ -522,19 +532,23 @@ void vl_stage(imhd_ctx* c, int sflux, int src, int dst, bool corr, bool fuse_cma
     const TilePlanDev& p = corr ? c->tp_corr : c->tp_pred;
     ProfScope ps(c, name, p.active);
     StageArgs<real> a{};
-    for (int d = 0; d < 3; d++) a.d[d] = sweep_args<real>(c, sflux, d);
     for (int f = 0; f < 7; f++) {
         a.qf[f] = R<real>(c->q[sflux][f]);
         a.src.q[f] = R<real>(c->q[src][f]);
         a.dst.q[f] = R<real>(c->q[dst][f]);
     }
-    for (int d = 0; d < 3; d++) { a.src.b[d] = c->b[src][d]; a.dst.b[d] = c->b[dst][d]; }
+    for (int d = 0; d < 3; d++) {
+        a.bf[d] = c->b[sflux][d];
+        a.src.b[d] = c->b[src][d];
+        a.dst.b[d] = c->b[dst][d];
+    }
     if (c->bh[0] != nullptr) {   // FP32: the half-step faces live in bh
         for (int d = 0; d < 3; d++) {
             if (dst == 2) a.bhw[d] = R<real>(c->bh[d]);
-            if (sflux == 2) a.d[d].bnr = R<real>(c->bh[d]);
+            if (sflux == 2) a.bfr[d] = R<real>(c->bh[d]);
         }
     }
+    a.mc_beta = c->cfg.mc_beta;
     a.dt = c->dt;
     a.factor = corr ? 1.0 : 0.5;
     a.ccb_lo_shift = c->ccb_shift;
@@ -549,9 +563,9 @@ void vl_stage(imhd_ctx* c, int sflux, int src, int dst, bool corr, bool fuse_cma
     for (int d = 0; d < 3; d++) a.pvel[d] = pvel ? R<real>(c->pvel[d]) : nullptr;
     const TilePlan tp{p.d, c->nbox, p.total};
     switch (c->cfg.riemann) {
-        case IMHD_ROE: launch_vl_stage<real, RS_ROE>(c->st, c->db, tp, a, vs_kind(corr, c->cfg.recon), c->vshape); break;
-        case IMHD_HLLE: launch_vl_stage<real, RS_HLLE>(c->st, c->db, tp, a, vs_kind(corr, c->cfg.recon), c->vshape); break;
-        default: launch_vl_stage<real, RS_HLLD>(c->st, c->db, tp, a, vs_kind(corr, c->cfg.recon), c->vshape); break;
+        case IMHD_ROE: launch_vl_stage<real, RS_ROE>(c->st, c->db, tp, a, stage_kind(c, corr), stage_shape(c, corr)); break;
+        case IMHD_HLLE: launch_vl_stage<real, RS_HLLE>(c->st, c->db, tp, a, stage_kind(c, corr), stage_shape(c, corr)); break;
+        default: launch_vl_stage<real, RS_HLLD>(c->st, c->db, tp, a, stage_kind(c, corr), stage_shape(c, corr)); break;
     }
 }
 
@@ -2049,6 +2063,14 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
         const int v = std::min(VS_SHAPES - 1, std::max(0, atoi(e)));
         if (vs_shape_ok(g.precision, v)) c->vshape = v;
     }
+    if (const char* e = getenv("IMHD_VSH")) c->vsh = atoi(e) != 0;
+    // the FP32 shared-slope PLM corrector runs 16 x 16 tiles at three workgroups per CU
+    // (168 registers); its 32 x 16 form holds one workgroup per CU
+    c->vshape_c = g.precision == 32 && g.recon == IMHD_PLM_PRIM && c->vsh && !getenv("IMHD_VSHAPE") ? 0 : c->vshape;
+    if (const char* e = getenv("IMHD_VSHAPE_C")) {   // the corrector's alone
+        const int v = std::min(VS_SHAPES - 1, std::max(0, atoi(e)));
+        if (vs_shape_ok(g.precision, v)) c->vshape_c = v;
+    }
     if (g.flux_protection) {
         // flux_protection of vanleer_mod.f90 (:81): Roe with HLLE protection fluxes, FP64
         if (g.integrator != IMHD_VL || g.riemann != IMHD_ROE || g.precision != 64) {
@@ -2377,13 +2399,14 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
     // launch shape: z-segment of the column kernels, y-strip of the work enumeration, and
     // tile shape and z-segment of the fused VL stages
     struct Shape {
-        int zseg, strip, vshape, vseg, vrounds;
+        int zseg, strip, vshape, vseg, vrounds, vshape_c;
     };
     const bool vl = c->tp_pred.d != nullptr;
     auto set_shape = [&](const Shape& sh) -> int {
         c->zseg = sh.zseg;
         c->strip = sh.strip;
         c->vshape = sh.vshape;
+        c->vshape_c = sh.vshape_c;
         c->vseg = sh.vseg;
         c->vrounds = sh.vrounds;
         int r = build_plan(c, c->pl_pupd, 2, -1, sh.zseg);
@@ -2398,7 +2421,7 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
         return r;
     };
     std::string rep;
-    Shape bsh{c->zseg, c->strip, c->vshape, c->vseg, c->vrounds};
+    Shape bsh{c->zseg, c->strip, c->vshape, c->vseg, c->vrounds, c->vshape_c};
     const Shape orig = bsh;
     float best = 1e30f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2416,9 +2439,9 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
         float ms = 0.f;
         HIPCK(c, hipEventElapsedTime(&ms, e0, e1));
         ms /= (float)steps;
-        char b[128];
-        snprintf(b, sizeof(b), "zseg=%d strip=%d tile=%d vrounds=%d vseg=%d: %.4f ms/step; ", sh.zseg, sh.strip,
-                 sh.vshape, sh.vrounds, sh.vseg, ms);
+        char b[160];
+        snprintf(b, sizeof(b), "zseg=%d strip=%d tile=%d ctile=%d vrounds=%d vseg=%d: %.4f ms/step; ", sh.zseg,
+                 sh.strip, sh.vshape, sh.vshape_c, sh.vrounds, sh.vseg, ms);
         rep += b;
         if (ms < best) { best = ms; bsh = sh; }
         return 0;
@@ -2429,12 +2452,14 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
     static const int vsegs[] = {8, 12, 16};
     static const int vrounds[] = {1, 2, 3, 4, 6, 8};
     const Shape s0 = bsh;
-    for (int s : strips) if (!rc) rc = trial({s0.zseg, s, s0.vshape, s0.vseg, s0.vrounds});
-    { const Shape b0 = bsh; for (int z : zsegs) if (!rc && z != b0.zseg) rc = trial({z, b0.strip, b0.vshape, b0.vseg, b0.vrounds}); }
+    for (int s : strips) if (!rc) rc = trial({s0.zseg, s, s0.vshape, s0.vseg, s0.vrounds, s0.vshape_c});
+    { const Shape b0 = bsh; for (int z : zsegs) if (!rc && z != b0.zseg) rc = trial({z, b0.strip, b0.vshape, b0.vseg, b0.vrounds, b0.vshape_c}); }
     if (vl) {
-        { const Shape b0 = bsh; for (int v = 0; v < VS_SHAPES; v++) if (!rc && v != b0.vshape && vs_shape_ok(c->prec, v)) rc = trial({b0.zseg, b0.strip, v, b0.vseg, b0.vrounds}); }
-        { const Shape b0 = bsh; for (int r : vrounds) if (!rc && r != b0.vrounds) rc = trial({b0.zseg, b0.strip, b0.vshape, b0.vseg, r}); }
-        { const Shape b0 = bsh; for (int v : vsegs) if (!rc && (b0.vrounds != 0 || v != b0.vseg)) rc = trial({b0.zseg, b0.strip, b0.vshape, v, 0}); }
+        // both stages' tile shape, then the corrector's alone
+        { const Shape b0 = bsh; for (int v = 0; v < VS_SHAPES; v++) if (!rc && (v != b0.vshape || v != b0.vshape_c) && vs_shape_ok(c->prec, v)) rc = trial({b0.zseg, b0.strip, v, b0.vseg, b0.vrounds, v}); }
+        { const Shape b0 = bsh; for (int v = 0; v < VS_SHAPES; v++) if (!rc && v != b0.vshape_c && vs_shape_ok(c->prec, v)) rc = trial({b0.zseg, b0.strip, b0.vshape, b0.vseg, b0.vrounds, v}); }
+        { const Shape b0 = bsh; for (int r : vrounds) if (!rc && r != b0.vrounds) rc = trial({b0.zseg, b0.strip, b0.vshape, b0.vseg, r, b0.vshape_c}); }
+        { const Shape b0 = bsh; for (int v : vsegs) if (!rc && (b0.vrounds != 0 || v != b0.vseg)) rc = trial({b0.zseg, b0.strip, b0.vshape, v, 0, b0.vshape_c}); }
     }
     if (!rc) rc = set_shape(bsh);
     if (!rc) rc = restore();
@@ -2452,9 +2477,9 @@ int imhd_autotune(imhd_ctx* c, int steps, char* report, int report_len) {
     if (e1) (void)hipEventDestroy(e1);
     for (auto& sv : saved) (void)hipFree(sv.second);
     if (rc) return rc;
-    char b[128];
-    snprintf(b, sizeof(b), "chosen zseg=%d strip=%d tile=%d vrounds=%d vseg=%d (%.4f ms/step)", bsh.zseg, bsh.strip,
-             bsh.vshape, bsh.vrounds, bsh.vseg, best);
+    char b[160];
+    snprintf(b, sizeof(b), "chosen zseg=%d strip=%d tile=%d ctile=%d vrounds=%d vseg=%d (%.4f ms/step)", bsh.zseg,
+             bsh.strip, bsh.vshape, bsh.vshape_c, bsh.vrounds, bsh.vseg, best);
     rep += b;
     c->prof_report = rep;
     if (report && report_len > 0) {

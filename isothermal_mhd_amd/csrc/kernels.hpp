@@ -113,8 +113,12 @@ struct TilePlan {
 };
 template <class real>
 struct StageArgs {
-    SweepArgs<real> d[3];   // flux state per direction (rotated fields, normal face B); F, Et unused
-    const real* qf[7];      // flux state, global order (centre EMFs, velocities)
+    const real* qf[7];      // flux state, global order; direction d's rotated slot r is
+                            // qf[perm(d, r)] (the kernel forms its sweep views from these)
+    const double* bf[3];    // face field of the flux state
+    const real* bfr[3];     // non-null: the flux state's face field in the working precision
+                            // (FP32 half-step faces), read instead of bf
+    double mc_beta;
     StateP<real> src, dst;  // update base and output
     const double* dt;
     double factor;          // 0.5 predictor, 1 corrector
@@ -135,7 +139,8 @@ struct StageArgs {
     real* bhw[3];
 };
 // kind: 0 predictor, 1 PLM corrector, 2 PPM-primitive corrector, 3 PPM-characteristic
-// corrector (vs_kind)
+// corrector (vs_kind), 4 FP32 PLM corrector reconstructing from LDS-held primitives with
+// shared slopes
 template <class real, int RS>
 void launch_vl_stage(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a,
                      int kind, int shape);

@@ -13,7 +13,15 @@
 // Two barriers per plane; FP32 LDS: 40 KB per 16 x 16 tile, 80 KB per 32 x 16 tile.
 // Every face, corner and cell is masked to the reference's own index range for that
 // quantity, so the reads and the signal-speed maximum are those of the separate kernels.
+//
+// The FP32 PLM corrector (SH below) reconstructs from primitive values kept in LDS: each
+// plane's primitives are formed once per cell (from the cell's own load of plane k+2, one
+// march step ahead) into a plane-parity buffer; the x-slopes are formed once per cell and
+// passed to the x-neighbour by a DPP lane shift, the y-faces read their four-cell stencil
+// from the buffer, and the z-slope of a plane is formed once and carried to the next
+// plane's face.  Only a tile's edge lanes load neighbour cells from HBM.
 #pragma once
+#include <type_traits>
 #include "kernels.hpp"
 
 namespace imhd {
@@ -23,9 +31,13 @@ namespace imhd {
 // either way; FP64 needs more of both and gets 2 waves per SIMD; the PPM correctors
 // (KIND 2, 3: six-cell stencils, the characteristic one with 6 x 6 eigenvector matrices)
 // get 2 in FP32 too, rather than spill at 128 registers
-template <class real, int KIND>
-constexpr int vs_waves() { return sizeof(real) == 4 ? (KIND >= 2 ? 2 : 4) : 1; }
-#define VS_WAVES (vs_waves<real, KIND>())
+template <class real, int KIND, int N>
+constexpr int vs_waves() {
+    if constexpr (sizeof(real) == 8) return 1;
+    if constexpr (KIND == 4) return N == 256 ? 3 : (N == 512 ? 2 : 4);
+    return KIND == 2 || KIND == 3 ? 2 : 4;
+}
+#define VS_WAVES (vs_waves<real, KIND, TX * TY>())
 
 // x or y face (direction D) of the fused stage.  Predictor: the cell c on the low side is
 // the thread's own, held in registers in the global field order, and only the neighbour is
@@ -94,6 +106,60 @@ __device__ __forceinline__ real stage_face_z(const SweepArgs<real>& a, const rea
     }
 }
 
+// Values of the lanes l - 1 and l + 1 of the wavefront (DPP wave_shr:1 / wave_shl:1, all
+// lanes active).  A tile row is TX lanes (16 or 32, dividing 64), so within a row these are
+// the x-neighbours; at a row's first and last lane they are not, and those lanes use loaded
+// values instead.
+__device__ __forceinline__ float lane_prev(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_next(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+}
+
+// MC-beta limited slope sign(dC) min(beta|dL|, beta|dR|, |dC|) of one primitive from three
+// cell values (plm_flux's arithmetic)
+template <class real>
+__device__ __forceinline__ real mc_slope(real wm, real w0, real wp, real beta) {
+    const real dL = w0 - wm, dR = wp - w0, dC = real(0.5) * (dL + dR);
+    return sgn1(dC) * min3(beta * fabs(dL), beta * fabs(dR), fabs(dC));
+}
+// Primitive slot r (rho, v_n, v_t1, v_t2, B_t1, B_t2) of direction D in a primitive vector
+// in global order (rho, v_x, v_y, v_z, B_x, B_y, B_z)
+template <int D>
+__host__ __device__ constexpr int rslot(int r) { return perm(D, r < 4 ? r : r + 1); }
+// Riemann flux from PLM interface primitives (rotated, 6 each) and the face's normal field:
+// plm_flux's interface states and call
+template <class real, int RS>
+__device__ __forceinline__ real plm_solve(const real* sl, const real* sr, real wb, real cs, real* F) {
+    real uL[7], uR[7];
+    uL[0] = sl[0]; uL[4] = wb; uL[5] = sl[4]; uL[6] = sl[5];
+    uR[0] = sr[0]; uR[4] = wb; uR[5] = sr[4]; uR[6] = sr[5];
+    uL[1] = sl[1] * uL[0]; uL[2] = sl[2] * uL[0]; uL[3] = sl[3] * uL[0];
+    uR[1] = sr[1] * uR[0]; uR[2] = sr[2] * uR[0]; uR[3] = sr[3] * uR[0];
+    real smax = real(0);
+    riemann<RS>(uL, uR, cs, F, smax);
+    return smax;
+}
+
+// Direction d's sweep view of the flux state: the rotated slots are the global fields
+// permuted at compile time, so the three views share the seven field pointers (and their
+// scalar registers) instead of passing 21 copies.
+// The predictor's flux state is the state at t^n, whose faces are never the FP32 half-step
+// ones, so its view reads the FP64 faces without testing.
+template <class real, int D, int KIND>
+__device__ __forceinline__ SweepArgs<real> stage_sweep(const StageArgs<real>& a) {
+    SweepArgs<real> s{};
+#pragma unroll
+    for (int r = 0; r < 7; r++) s.w[r] = r == 4 ? nullptr : a.qf[perm(D, r)];
+    s.bn = a.bf[D];
+    s.bnr = KIND == 0 ? nullptr : a.bfr[D];
+    s.dir = D;
+    s.cs = a.cs;
+    s.mc_beta = a.mc_beta;
+    return s;
+}
+
 template <class real, int RS, int KIND, int TX, int TY>
 __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* __restrict__ boxes, TilePlan plan,
                                                       StageArgs<real> a) {
@@ -104,12 +170,20 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     // two (by plane / index parity) and fewer live registers.
     constexpr bool ONE = sizeof(real) == 4;
     constexpr int NB = ONE ? 1 : 2;
-    __shared__ real sF[NB][2][4][N];   // [buffer][x, y face][flux, global order]
+    // FP32 PLM corrector: primitives of two planes in LDS (28 KB per 32 x 16 tile), x-flux
+    // differences by lane shift (no x slot in sF), centre values in FP32: 80 KB in all.  Its
+    // flux state is the half step, whose normal faces are the FP32 ones in a.bfr.
+    constexpr bool SH = ONE && KIND == 4;
+    constexpr int FK = KIND == 4 ? 1 : KIND;   // per-face arithmetic of the other paths
+    constexpr int FY = SH ? 0 : 1;   // sF slot of the y face
+    using ctype = typename std::conditional<SH, float, double>::type;
+    __shared__ real sF[NB][SH ? 1 : 2][4][N];   // [buffer][x, y face][flux, global order]
     __shared__ real sEf[2][2][N];      // [plane parity][x face: E(2,0); y face: E(2,1)]
     __shared__ real sEz3[2][N];        // z face k+1/2: E(1,1), E(0,1)
-    __shared__ double sC[2][5][N];     // [plane parity][centre Ex, Ey, Ez, m_x/rho, m_y/rho]
+    __shared__ ctype sC[2][5][N];      // [plane parity][centre Ex, Ey, Ez, m_x/rho, m_y/rho]
     __shared__ double sEz[N];          // corner Ez of plane k
     __shared__ double sEc[NB][2][N];   // [buffer][Ex, Ey] corner EMFs of index k (at k+1/2)
+    __shared__ real sW[SH ? 2 : 1][SH ? 7 : 1][SH ? N : 1];   // SH: [plane parity][primitive, global order]
 
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
     int b = 0;
@@ -156,6 +230,10 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     const bool int_ij = i >= 1 && j >= 1 && i <= n0 && j <= n1;
     const double dtd = a.factor * a.dt[b] / B.dx;
     const real dtdx = (real)dtd;
+    const SweepArgs<real> sw0 = stage_sweep<real, 0, KIND>(a), sw1 = stage_sweep<real, 1, KIND>(a),
+                          sw2 = stage_sweep<real, 2, KIND>(a);
+    // update base face field: the predictor's base is its flux state
+    const double* const sb[3] = {a.src.b[0], a.src.b[1], a.src.b[2]};
 
     // the thread's own cell of plane p of the flux state, global field order: loaded once
     // per plane for the centre values, the three faces and (predictor) the update
@@ -194,13 +272,13 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     };
     // x- and y-faces of plane p: fluxes and Ez slots to LDS; the Ey (x-face) and Ex (y-face)
     // slots stay with the thread (only its own column reads them)
-    auto xyfaces = [&](int p, const real* U, real& ey_x, real& ex_y, double& sm) {
+    auto xyfaces = [&](int p, const real* U, real& ey_x, real& ex_y, real& sm) {
         const int q = p & 1, fq = ONE ? 0 : q;
         const long long c = cell_index(B, i, j, p);
         const bool pt = inr(p, a.ft_lo, n2 + a.ft_hi);
         real F[7];
         if (fx_ij && pt) {
-            sm = fmax(sm, (double)stage_face_own<real, RS, KIND, 0>(a.d[0], U, c, sx, F));
+            sm = fmax(sm, stage_face_own<real, RS, FK, 0>(sw0, U, c, sx, F));
             sF[fq][0][0][t] = F[0]; sF[fq][0][1][t] = F[1]; sF[fq][0][2][t] = F[2]; sF[fq][0][3][t] = F[3];
             sEf[q][0][t] = -F[5];
             ey_x = F[6];
@@ -210,21 +288,21 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             ey_x = real(0);
         }
         if (fy_ij && pt) {
-            sm = fmax(sm, (double)stage_face_own<real, RS, KIND, 1>(a.d[1], U, c, sy, F));
-            sF[fq][1][0][t] = F[0]; sF[fq][1][2][t] = F[1]; sF[fq][1][3][t] = F[2]; sF[fq][1][1][t] = F[3];
+            sm = fmax(sm, stage_face_own<real, RS, FK, 1>(sw1, U, c, sy, F));
+            sF[fq][FY][0][t] = F[0]; sF[fq][FY][2][t] = F[1]; sF[fq][FY][3][t] = F[2]; sF[fq][FY][1][t] = F[3];
             ex_y = -F[5];
             sEf[q][1][t] = F[6];
         } else {
-            sF[fq][1][0][t] = sF[fq][1][1][t] = sF[fq][1][2][t] = sF[fq][1][3][t] = real(0);
+            sF[fq][FY][0][t] = sF[fq][FY][1][t] = sF[fq][FY][2][t] = sF[fq][FY][3][t] = real(0);
             sEf[q][1][t] = real(0);
             ex_y = real(0);
         }
     };
     // z-face between planes p and p+1
-    auto zface = [&](int p, bool live, const real* UL, const real* UR, real* fz, double& sm) {
+    auto zface = [&](int p, bool live, const real* UL, const real* UR, real* fz, real& sm) {
         if (live && fz_ij && inr(p, a.fn_lo, n2 + a.fn_hi)) {
             real F[7];
-            sm = fmax(sm, (double)stage_face_z<real, RS, KIND>(a.d[2], UL, UR, cell_index(B, i, j, p), sz, F));
+            sm = fmax(sm, stage_face_z<real, RS, FK>(sw2, UL, UR, cell_index(B, i, j, p), sz, F));
             fz[0] = F[0]; fz[3] = F[1]; fz[1] = F[2]; fz[2] = F[3];
             sEz3[0][t] = -F[5];
             sEz3[1][t] = F[6];
@@ -234,11 +312,47 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         }
     };
 
-    double sm = 0.0, cfl = 0.0;
+    // SH: conserved values of cell (ii, jj) of plane p of the flux state, zero outside the
+    // box's cells: all seven in global order (load_cell), or the six of direction D's
+    // rotated primitive slots (load_rot: rho, m_n, m_t1, m_t2, B_t1, B_t2); to_prims turns
+    // either into primitives with prim_rot's arithmetic (zero stays zero)
+    auto cell_ok = [&](int ii, int jj, int p) {
+        return inr(ii, 1 - nb, n0 + nb) && inr(jj, 1 - nb, n1 + nb) && inr(p, 1 - nb, n2 + nb);
+    };
+    auto load_cell = [&](int ii, int jj, int p, real* u) {
+        if (cell_ok(ii, jj, p)) {
+            const long long c = cell_index(B, ii, jj, p);
+#pragma unroll
+            for (int f = 0; f < 7; f++) u[f] = a.qf[f][c];
+        } else {
+#pragma unroll
+            for (int f = 0; f < 7; f++) u[f] = real(0);
+        }
+    };
+    auto load_rot = [&](auto dir, int ii, int jj, int p, real* g) {
+        constexpr int D = decltype(dir)::value;
+        if (cell_ok(ii, jj, p)) {
+            const long long c = cell_index(B, ii, jj, p);
+            g[0] = a.qf[0][c];
+            g[1] = a.qf[perm(D, 1)][c];
+            g[2] = a.qf[perm(D, 2)][c];
+            g[3] = a.qf[perm(D, 3)][c];
+            g[4] = a.qf[perm(D, 5)][c];
+            g[5] = a.qf[perm(D, 6)][c];
+        }
+    };
+    auto to_prims = [&](real* u) {
+        const real ir = u[0] != real(0) ? fdiv(real(1), u[0]) : real(0);
+        u[1] = u[1] * ir;
+        u[2] = u[2] * ir;
+        u[3] = u[3] * ir;
+    };
+
+    real sm = real(0), cfl = real(0);   // running maxima (the working precision holds them exactly)
     real fz_lo[4], fz_hi[4];
     real eyx_lo, eyx_hi, exy_lo, exy_hi;   // own x-face Ey slot and y-face Ex slot, planes k, k+1
     real U0[7], U1[7];   // own cell of planes k and k+1
-    double vz_lo = 0.0, vz_hi;
+    ctype vz_lo = ctype(0), vz_hi;
     eyx_lo = exy_lo = real(0);
 #pragma unroll
     for (int f = 0; f < 4; f++) fz_lo[f] = fz_hi[f] = real(0);
@@ -249,6 +363,23 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     // while the fluxes of that plane are in LDS)
     double ey_m = 0.0, ex_m = 0.0, eyx_m = 0.0, exy_m = 0.0;
     real dxy[4] = {real(0), real(0), real(0), real(0)}, dxy_n[4];
+    // SH: x-flux differences of plane k+1 (by lane shift), and the upper PLM state of plane k
+    // at the z-face k+1/2 (rotated primitives), formed with plane k's slope one step earlier
+    real dxn[4] = {real(0), real(0), real(0), real(0)}, uLz[6];
+    if constexpr (SH) {
+        real W[7];
+        load_cell(i, j, ks - 2, W);
+        to_prims(W);
+#pragma unroll
+        for (int f = 0; f < 7; f++) sW[(ks - 2) & 1][f][t] = W[f];
+        load_cell(i, j, ks - 1, W);
+        to_prims(W);
+#pragma unroll
+        for (int f = 0; f < 7; f++) sW[(ks - 1) & 1][f][t] = W[f];
+#pragma unroll
+        for (int r = 0; r < 6; r++) uLz[r] = real(0);
+        __syncthreads();
+    }
     // The first pass (k = ks - 2) only forms plane ks - 1.  Every plane's faces and centre
     // values come from this one copy of the loop body, so a plane shared by two z-segments
     // (and the corner EMFs formed from it) gets the same bits in both, also where FMA
@@ -257,17 +388,150 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     for (int k = ks - 2; k < ke; k++) {
         const bool live = k >= ks - 1;   // uniform over the workgroup
         // A: plane k+1 and the z-face k+1/2
-        own(k + 1, U1);
-        vz_hi = cells(k + 1, U1);
-        xyfaces(k + 1, U1, eyx_hi, exy_hi, sm);
-        zface(k, live, U0, U1, fz_hi, sm);
+        if constexpr (SH) {
+            const int q1 = (k + 1) & 1, q2 = k & 1;   // buffers of planes k+1 and k (refilled with k+2)
+            const int p = k + 1;
+            const bool pt = inr(p, a.ft_lo, n2 + a.ft_hi);
+            const real beta = (real)a.mc_beta, cs = (real)a.cs;
+            const long long c = cell_index(B, i, j, p);
+            const bool zlive = live && fz_ij && inr(k, a.fn_lo, n2 + a.fn_hi);
+            const bool ylive = fy_ij && pt && ty < TY - 1;   // row TY-1's y-faces are formed by the next tile row
+            const bool xlive = fx_ij && pt && tx < TX - 1;   // column TX-1's x-faces by the next tile
+            // the phase's HBM loads first, all in flight together: the own cell of plane k+2,
+            // the edge lanes' outer neighbours and the faces' normal field (index 0 for idle lanes)
+            real u2[7], gy[6], gx[6];
+            load_cell(i, j, k + 2, u2);
+            gy[0] = gy[1] = gy[2] = gy[3] = gy[4] = gy[5] = real(0);
+            gx[0] = gx[1] = gx[2] = gx[3] = gx[4] = gx[5] = real(0);
+            if (ty == 0 || ty == TY - 2) load_rot(std::integral_constant<int, 1>{}, i, ty == 0 ? j - 1 : j + 2, p, gy);
+            if (tx == 0 || tx == TX - 1) load_rot(std::integral_constant<int, 0>{}, tx == 0 ? i - 1 : i + 1, j, p, gx);
+            const real wbz = a.bfr[2][zlive ? cell_index(B, i, j, k) : 0];
+            const real wby = a.bfr[1][ylive ? c : 0];
+            const real wbx = a.bfr[0][xlive ? c : 0];
+            real F[7];
+            {   // z: plane k+1's slope from planes k, k+1, k+2 (own cells); face k+1/2
+                real sr[6], uLn[6];
+                to_prims(u2);
+#pragma unroll
+                for (int r = 0; r < 6; r++) {
+                    const int f = rslot<2>(r);
+                    const real w1 = sW[q1][f][t];
+                    const real d = mc_slope(sW[q2][f][t], w1, u2[f], beta);
+                    sr[r] = w1 - real(0.5) * d;
+                    uLn[r] = w1 + real(0.5) * d;
+                }
+#pragma unroll
+                for (int f = 0; f < 7; f++) sW[q2][f][t] = u2[f];
+                if (zlive) {
+                    sm = fmax(sm, plm_solve<real, RS>(uLz, sr, wbz, cs, F));
+                    fz_hi[0] = F[0]; fz_hi[3] = F[1]; fz_hi[1] = F[2]; fz_hi[2] = F[3];
+                    sEz3[0][t] = -F[5];
+                    sEz3[1][t] = F[6];
+                } else {
+                    fz_hi[0] = fz_hi[1] = fz_hi[2] = fz_hi[3] = real(0);
+                    sEz3[0][t] = sEz3[1][t] = real(0);
+                }
+#pragma unroll
+                for (int r = 0; r < 6; r++) uLz[r] = uLn[r];
+            }
+            vz_hi = ctype(0);
+            if (cell_ij && inr(p, 1 - nb, n2 + nb)) {   // centre EMFs and velocities
+                real W[7];   // own primitives of plane k+1
+#pragma unroll
+                for (int f = 0; f < 7; f++) W[f] = sW[q1][f][t];
+                sC[q1][0][t] = W[5] * W[3] - W[6] * W[2];
+                sC[q1][1][t] = W[6] * W[1] - W[4] * W[3];
+                sC[q1][2][t] = W[4] * W[2] - W[5] * W[1];
+                sC[q1][3][t] = W[1];
+                sC[q1][4][t] = W[2];
+                vz_hi = W[3];
+            }
+            {   // y-face j+1/2: four-cell stencil from the buffer (edge rows load the cell beyond)
+                const int tm = ty > 0 ? t - TX : t, tp1 = ty < TY - 1 ? t + TX : t, tp2 = ty < TY - 2 ? t + 2 * TX : t;
+                real sl[6], sr[6];
+                to_prims(gy);
+#pragma unroll
+                for (int r = 0; r < 6; r++) {
+                    const int f = rslot<1>(r);
+                    const real w0 = sW[q1][f][t], w1 = sW[q1][f][tp1], lm = sW[q1][f][tm], l2 = sW[q1][f][tp2];
+                    const real wm = ty == 0 ? gy[r] : lm;
+                    const real w2 = ty == TY - 2 ? gy[r] : l2;
+                    sl[r] = w0 + real(0.5) * mc_slope(wm, w0, w1, beta);
+                    sr[r] = w1 - real(0.5) * mc_slope(w0, w1, w2, beta);
+                }
+                if (ylive) {
+                    sm = fmax(sm, plm_solve<real, RS>(sl, sr, wby, cs, F));
+                    sF[0][FY][0][t] = F[0]; sF[0][FY][2][t] = F[1]; sF[0][FY][3][t] = F[2]; sF[0][FY][1][t] = F[3];
+                    exy_hi = -F[5];
+                    sEf[q1][1][t] = F[6];
+                } else {
+                    sF[0][FY][0][t] = sF[0][FY][1][t] = sF[0][FY][2][t] = sF[0][FY][3][t] = real(0);
+                    sEf[q1][1][t] = real(0);
+                    exy_hi = real(0);
+                }
+            }
+            {   // x-face i+1/2: each lane forms its cell's slope; the right state comes from lane + 1
+                real sl[6], rr[6], sr[6], fx[4];
+                to_prims(gx);
+#pragma unroll
+                for (int r = 0; r < 6; r++) {
+                    const int f = rslot<0>(r);
+                    const real w0 = sW[q1][f][t], lp = lane_prev(w0), ln = lane_next(w0);
+                    const real wm = tx == 0 ? gx[r] : lp;
+                    const real wp = tx == TX - 1 ? gx[r] : ln;
+                    const real d = mc_slope(wm, w0, wp, beta);
+                    sl[r] = w0 + real(0.5) * d;
+                    rr[r] = w0 - real(0.5) * d;
+                }
+#pragma unroll
+                for (int r = 0; r < 6; r++) sr[r] = lane_next(rr[r]);
+                if (xlive) {
+                    sm = fmax(sm, plm_solve<real, RS>(sl, sr, wbx, cs, F));
+                    fx[0] = F[0]; fx[1] = F[1]; fx[2] = F[2]; fx[3] = F[3];
+                    sEf[q1][0][t] = -F[5];
+                    eyx_hi = F[6];
+                } else {
+                    fx[0] = fx[1] = fx[2] = fx[3] = real(0);
+                    sEf[q1][0][t] = real(0);
+                    eyx_hi = real(0);
+                }
+#pragma unroll
+                for (int f = 0; f < 4; f++) dxn[f] = fx[f] - lane_prev(fx[f]);
+            }
+        } else {
+            own(k + 1, U1);
+            vz_hi = cells(k + 1, U1);
+            xyfaces(k + 1, U1, eyx_hi, exy_hi, sm);
+            zface(k, live, U0, U1, fz_hi, sm);
+        }
         __syncthreads();
-        // D: corner EMFs of index k, x/y flux differences
+        // D: corner EMFs of index k, x/y flux differences.  The update's inputs at t^n are
+        // loaded first, so that their latency overlaps the corner work and the barrier.
         const int q0 = k & 1, q1 = (k + 1) & 1, eq = ONE ? 0 : q0;
+        const bool upd = k >= ks && u_ij && inr(k, a.u_lo, n2 + a.u_hi);
+        const bool ccb = ccb_ij && k >= a.u_lo + sh;
+        real pq[4] = {real(0), real(0), real(0), real(0)}, pc[4] = {real(0), real(0), real(0), real(0)};
+        double pb[3] = {0.0, 0.0, 0.0}, pbm[3] = {0.0, 0.0, 0.0};
+        if (upd) {
+            const long long c = cell_index(B, i, j, k);
+#pragma unroll
+            for (int f = 0; f < 4; f++) pq[f] = KIND == 0 ? U0[f] : a.src.q[f][c];   // predictor: the flux state is the base
+            if (KIND != 0 && a.comp[0] != nullptr) {
+#pragma unroll
+                for (int f = 0; f < 4; f++) pc[f] = a.comp[f][c];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) pb[d] = sb[d][c];
+            if (ccb) {
+                pbm[0] = sb[0][c - sx];
+                pbm[1] = sb[1][c - sy];
+                if (!have_bzm) pbm[2] = sb[2][c - sz];
+            }
+        }
         if (live && c_ij && inr(k, a.c_lo, n2 + a.c_hi)) {
-            const double vx = 0.5 * (sC[q0][3][t] + sC[q0][3][t + 1]);
-            const double vy = 0.5 * (sC[q0][4][t] + sC[q0][4][t + TX]);
-            const double vz = 0.5 * (vz_lo + vz_hi);
+            const double vx = 0.5 * ((double)sC[q0][3][t] + (double)sC[q0][3][t + 1]);
+            const double vy = 0.5 * ((double)sC[q0][4][t] + (double)sC[q0][4][t + TX]);
+            const double vz = 0.5 * ((double)vz_lo + (double)vz_hi);
             double a14, a34, b14, b34;
             {   // Ez at (i+1/2, j+1/2, k)
                 double e00 = sC[q0][2][t], e10 = sC[q0][2][t + 1], e01 = sC[q0][2][t + TX], e11 = sC[q0][2][t + TX + 1];
@@ -291,17 +555,18 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 sEc[eq][0][t] = 0.25 * (y0 + y1 + z0 + z1) + 0.125 * ((a14 - a34) + (b14 - b34));
             }
         }
-        const bool upd = k >= ks && u_ij && inr(k, a.u_lo, n2 + a.u_hi);
         if (ONE) {   // plane k+1, while its fluxes are in the single buffer
             if (u_ij) {
 #pragma unroll
-                for (int f = 0; f < 4; f++)
-                    dxy_n[f] = (sF[0][0][f][t] - sF[0][0][f][t - 1]) + (sF[0][1][f][t] - sF[0][1][f][t - TX]);
+                for (int f = 0; f < 4; f++) {
+                    const real dx = SH ? dxn[f] : sF[0][0][f][t] - sF[0][0][f][t - 1];
+                    dxy_n[f] = dx + (sF[0][FY][f][t] - sF[0][FY][f][t - TX]);
+                }
             }
         } else if (upd) {   // plane k, from its parity buffer
 #pragma unroll
             for (int f = 0; f < 4; f++)
-                dxy[f] = (sF[q0][0][f][t] - sF[q0][0][f][t - 1]) + (sF[q0][1][f][t] - sF[q0][1][f][t - TX]);
+                dxy[f] = (sF[q0][0][f][t] - sF[q0][0][f][t - 1]) + (sF[q0][FY][f][t] - sF[q0][FY][f][t - TX]);
         }
         __syncthreads();
         // G: update of plane k (k_update_z's arithmetic)
@@ -312,11 +577,11 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 real div = dxy[f] + (fz_hi[f] - fz_lo[f]);
-                real q0v = KIND == 0 ? U0[f] : a.src.q[f][c];   // predictor: the flux state is the base
+                const real q0v = pq[f];
                 oq[f] = q0v;
                 real v;
-                if (a.comp[0] != nullptr) {
-                    real y = -(dtdx * div) - a.comp[f][c];
+                if (KIND != 0 && a.comp[0] != nullptr) {   // Kahan terms: corrector only
+                    real y = -(dtdx * div) - pc[f];
                     v = q0v + y;
                     a.comp[f][c] = (v - q0v) - y;
                 } else {
@@ -324,7 +589,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 }
                 if (f == 0 && v < (real)MIN_DENS) {
                     v = (real)MIN_DENS;
-                    if (a.comp[0] != nullptr) a.comp[0][c] = real(0);
+                    if (KIND != 0 && a.comp[0] != nullptr) a.comp[0][c] = real(0);
                     atomicAdd(a.floor_count, 1u);
                 }
                 a.dst.q[f][c] = v;
@@ -336,10 +601,10 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 const int qm = (k - 1) & 1;
                 ey_m = sEc[qm][1][t]; ex_m = sEc[qm][0][t]; eyx_m = sEc[qm][1][t - 1]; exy_m = sEc[qm][0][t - TX];
             }
-            double bx = a.src.b[0][c] - dtd * ((ez - ez_y) - (ey - ey_m));
-            double by = a.src.b[1][c] - dtd * ((ex - ex_m) - (ez - ez_x));
-            double bz = a.src.b[2][c] - dtd * ((ey - ey_x) - (ex - ex_y));
-            if (a.bhw[0] != nullptr) {   // FP32 half step: the corrector reads these as FP32
+            double bx = pb[0] - dtd * ((ez - ez_y) - (ey - ey_m));
+            double by = pb[1] - dtd * ((ex - ex_m) - (ez - ez_x));
+            double bz = pb[2] - dtd * ((ey - ey_x) - (ex - ex_y));
+            if (KIND == 0 && a.bhw[0] != nullptr) {   // FP32 half step: the corrector reads these as FP32
                 a.bhw[0][c] = (real)bx;
                 a.bhw[1][c] = (real)by;
                 a.bhw[2][c] = (real)bz;
@@ -348,11 +613,11 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 a.dst.b[1][c] = by;
                 a.dst.b[2][c] = bz;
             }
-            if (ccb_ij && k >= a.u_lo + sh) {
+            if (ccb) {
                 const double ez_xy = sEz[t - 1 - TX];
-                double bxm = a.src.b[0][c - sx] - dtd * ((ez_x - ez_xy) - (ey_x - eyx_m));
-                double bym = a.src.b[1][c - sy] - dtd * ((ex_y - exy_m) - (ez_y - ez_xy));
-                if (!have_bzm) bz_m = a.src.b[2][c - sz] - dtd * ((ey_m - eyx_m) - (ex_m - exy_m));
+                double bxm = pbm[0] - dtd * ((ez_x - ez_xy) - (ey_x - eyx_m));
+                double bym = pbm[1] - dtd * ((ex_y - exy_m) - (ez_y - ez_xy));
+                if (!have_bzm) bz_m = pbm[2] - dtd * ((ey_m - eyx_m) - (ex_m - exy_m));
                 nq[4] = (real)(0.5 * (bx + bxm));
                 nq[5] = (real)(0.5 * (by + bym));
                 nq[6] = (real)(0.5 * (bz + bz_m));
@@ -381,9 +646,9 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                     }
                 }
             }
-            if (a.cmax != nullptr && int_ij && k >= 1 && k <= n2) {
+            if (KIND != 0 && a.cmax != nullptr && int_ij && k >= 1 && k <= n2) {   // corrector only
                 if constexpr (ONE) {
-                    cfl = fmax(cfl, (double)cell_fast_speed_f(nq, (float)a.cs));
+                    cfl = fmax(cfl, cell_fast_speed_f(nq, (float)a.cs));
                 } else {
                     double q7[7];
                     for (int f = 0; f < 7; f++) q7[f] = (double)nq[f];
@@ -405,8 +670,8 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         exy_lo = exy_hi;
         vz_lo = vz_hi;
     }
-    atomic_max_nonneg(a.smax, b, sm);
-    if (a.cmax != nullptr) atomic_max_nonneg(a.cmax, b, cfl);
+    atomic_max_nonneg(a.smax, b, (double)sm);
+    if (KIND != 0 && a.cmax != nullptr) atomic_max_nonneg(a.cmax, b, (double)cfl);
 }
 
 // kind: 0 predictor, 1 PLM corrector, 2 PPM-primitive corrector, 3 PPM-characteristic
@@ -453,6 +718,9 @@ int vl_stage_blocks_per_cu(int kind, int shape) {
         case 0: return vl_stage_blocks_per_cu_k<real, RS, 0>(shape);
         case 1: return vl_stage_blocks_per_cu_k<real, RS, 1>(shape);
         case 2: return vl_stage_blocks_per_cu_k<real, RS, 2>(shape);
+        case 4:
+            if constexpr (sizeof(real) == 4) return vl_stage_blocks_per_cu_k<real, RS, 4>(shape);
+            return vl_stage_blocks_per_cu_k<real, RS, 1>(shape);
         default: return vl_stage_blocks_per_cu_k<real, RS, 3>(shape);
     }
 }
@@ -465,6 +733,10 @@ void launch_vl_stage(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan,
         case 0: launch_vl_stage_k<real, RS, 0>(st, boxes, plan, a, shape); break;
         case 1: launch_vl_stage_k<real, RS, 1>(st, boxes, plan, a, shape); break;
         case 2: launch_vl_stage_k<real, RS, 2>(st, boxes, plan, a, shape); break;
+        case 4:   // FP32 only; an FP64 context runs the PLM corrector as kind 1
+            if constexpr (sizeof(real) == 4) launch_vl_stage_k<real, RS, 4>(st, boxes, plan, a, shape);
+            else launch_vl_stage_k<real, RS, 1>(st, boxes, plan, a, shape);
+            break;
         default: launch_vl_stage_k<real, RS, 3>(st, boxes, plan, a, shape); break;
     }
 }
