@@ -2064,9 +2064,7 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
         if (vs_shape_ok(g.precision, v)) c->vshape = v;
     }
     if (const char* e = getenv("IMHD_VSH")) c->vsh = atoi(e) != 0;
-    // the FP32 shared-slope PLM corrector runs 16 x 16 tiles at three workgroups per CU
-    // (168 registers); its 32 x 16 form holds one workgroup per CU
-    c->vshape_c = g.precision == 32 && g.recon == IMHD_PLM_PRIM && c->vsh && !getenv("IMHD_VSHAPE") ? 0 : c->vshape;
+    c->vshape_c = c->vshape;
     if (const char* e = getenv("IMHD_VSHAPE_C")) {   // the corrector's alone
         const int v = std::min(VS_SHAPES - 1, std::max(0, atoi(e)));
         if (vs_shape_ok(g.precision, v)) c->vshape_c = v;

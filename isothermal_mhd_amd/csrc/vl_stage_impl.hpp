@@ -30,11 +30,12 @@ namespace imhd {
 // (40 KB) fits 4 workgroups per CU and that of a 32 x 16 tile (80 KB) 2, 16 waves per CU
 // either way; FP64 needs more of both and gets 2 waves per SIMD; the PPM correctors
 // (KIND 2, 3: six-cell stencils, the characteristic one with 6 x 6 eigenvector matrices)
-// get 2 in FP32 too, rather than spill at 128 registers
+// get 2 in FP32 too, rather than spill at 128 registers; the shared-slope PLM corrector
+// (KIND 4) gets 3 on 16 x 16 tiles (three workgroups per CU) and 4 otherwise
 template <class real, int KIND, int N>
 constexpr int vs_waves() {
     if constexpr (sizeof(real) == 8) return 1;
-    if constexpr (KIND == 4) return N == 256 ? 3 : (N == 512 ? 2 : 4);
+    if constexpr (KIND == 4) return N == 256 ? 3 : 4;   // 16 x 16: 168 registers, three workgroups per CU
     return KIND == 2 || KIND == 3 ? 2 : 4;
 }
 #define VS_WAVES (vs_waves<real, KIND, TX * TY>())
@@ -181,8 +182,13 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     __shared__ real sEf[2][2][N];      // [plane parity][x face: E(2,0); y face: E(2,1)]
     __shared__ real sEz3[2][N];        // z face k+1/2: E(1,1), E(0,1)
     __shared__ ctype sC[2][5][N];      // [plane parity][centre Ex, Ey, Ez, m_x/rho, m_y/rho]
-    __shared__ double sEz[N];          // corner Ez of plane k
-    __shared__ double sEc[NB][2][N];   // [buffer][Ex, Ey] corner EMFs of index k (at k+1/2)
+    // corner EMFs, summed in FP64; SH stores them in FP32 (their inputs, the FP32 face and
+    // centre EMFs, carry FP32 rounding already) and keeps two buffers of Ex, Ey, so that the
+    // update reads the index k-1 corners instead of carrying them in registers
+    using etype = typename std::conditional<SH, float, double>::type;
+    constexpr int NBE = SH ? 2 : NB;
+    __shared__ etype sEz[N];           // corner Ez of plane k
+    __shared__ etype sEc[NBE][2][N];   // [buffer][Ex, Ey] corner EMFs of index k (at k+1/2)
     __shared__ real sW[SH ? 2 : 1][SH ? 7 : 1][SH ? N : 1];   // SH: [plane parity][primitive, global order]
 
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
@@ -228,6 +234,18 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     const int sh = a.ccb_lo_shift;
     const bool ccb_ij = i >= a.u_lo + sh && j >= a.u_lo + sh;
     const bool int_ij = i >= 1 && j >= 1 && i <= n0 && j <= n1;
+    // The per-lane conditions as bits of one vector register.  The shared-slope corrector
+    // re-reads it every plane (an empty asm marks it as rewritten), so that its many
+    // conditions are formed where used instead of being held as 64-bit lane masks in scalar
+    // registers across the march, which the allocator would spill.
+    enum { LB_CELL, LB_FX, LB_FY, LB_FZ, LB_C, LB_U, LB_CCB, LB_INT, LB_X0, LB_XE, LB_Y0, LB_YE, LB_XF, LB_YF };
+    unsigned lane_bits = (unsigned)cell_ij << LB_CELL | (unsigned)fx_ij << LB_FX | (unsigned)fy_ij << LB_FY |
+                         (unsigned)fz_ij << LB_FZ | (unsigned)c_ij << LB_C | (unsigned)u_ij << LB_U |
+                         (unsigned)ccb_ij << LB_CCB | (unsigned)int_ij << LB_INT | (unsigned)(tx == 0) << LB_X0 |
+                         (unsigned)(tx == TX - 1) << LB_XE | (unsigned)(ty == 0) << LB_Y0 |
+                         (unsigned)(ty == TY - 2) << LB_YE | (unsigned)(tx < TX - 1) << LB_XF |
+                         (unsigned)(ty < TY - 1) << LB_YF;
+    auto lb = [&](int bit) -> bool { return (lane_bits >> bit) & 1u; };
     const double dtd = a.factor * a.dt[b] / B.dx;
     const real dtdx = (real)dtd;
     const SweepArgs<real> sw0 = stage_sweep<real, 0, KIND>(a), sw1 = stage_sweep<real, 1, KIND>(a),
@@ -386,6 +404,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     // contraction is on.
 #pragma unroll 1
     for (int k = ks - 2; k < ke; k++) {
+        if constexpr (SH) asm volatile("" : "+v"(lane_bits));
         const bool live = k >= ks - 1;   // uniform over the workgroup
         // A: plane k+1 and the z-face k+1/2
         if constexpr (SH) {
@@ -394,17 +413,17 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             const bool pt = inr(p, a.ft_lo, n2 + a.ft_hi);
             const real beta = (real)a.mc_beta, cs = (real)a.cs;
             const long long c = cell_index(B, i, j, p);
-            const bool zlive = live && fz_ij && inr(k, a.fn_lo, n2 + a.fn_hi);
-            const bool ylive = fy_ij && pt && ty < TY - 1;   // row TY-1's y-faces are formed by the next tile row
-            const bool xlive = fx_ij && pt && tx < TX - 1;   // column TX-1's x-faces by the next tile
+            const bool zlive = live && lb(LB_FZ) && inr(k, a.fn_lo, n2 + a.fn_hi);
+            const bool ylive = lb(LB_FY) && pt && lb(LB_YF);   // row TY-1's y-faces are formed by the next tile row
+            const bool xlive = lb(LB_FX) && pt && lb(LB_XF);   // column TX-1's x-faces by the next tile
             // the phase's HBM loads first, all in flight together: the own cell of plane k+2,
             // the edge lanes' outer neighbours and the faces' normal field (index 0 for idle lanes)
             real u2[7], gy[6], gx[6];
             load_cell(i, j, k + 2, u2);
             gy[0] = gy[1] = gy[2] = gy[3] = gy[4] = gy[5] = real(0);
             gx[0] = gx[1] = gx[2] = gx[3] = gx[4] = gx[5] = real(0);
-            if (ty == 0 || ty == TY - 2) load_rot(std::integral_constant<int, 1>{}, i, ty == 0 ? j - 1 : j + 2, p, gy);
-            if (tx == 0 || tx == TX - 1) load_rot(std::integral_constant<int, 0>{}, tx == 0 ? i - 1 : i + 1, j, p, gx);
+            if (lb(LB_Y0) || lb(LB_YE)) load_rot(std::integral_constant<int, 1>{}, i, lb(LB_Y0) ? j - 1 : j + 2, p, gy);
+            if (lb(LB_X0) || lb(LB_XE)) load_rot(std::integral_constant<int, 0>{}, lb(LB_X0) ? i - 1 : i + 1, j, p, gx);
             const real wbz = a.bfr[2][zlive ? cell_index(B, i, j, k) : 0];
             const real wby = a.bfr[1][ylive ? c : 0];
             const real wbx = a.bfr[0][xlive ? c : 0];
@@ -435,7 +454,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 for (int r = 0; r < 6; r++) uLz[r] = uLn[r];
             }
             vz_hi = ctype(0);
-            if (cell_ij && inr(p, 1 - nb, n2 + nb)) {   // centre EMFs and velocities
+            if (lb(LB_CELL) && inr(p, 1 - nb, n2 + nb)) {   // centre EMFs and velocities
                 real W[7];   // own primitives of plane k+1
 #pragma unroll
                 for (int f = 0; f < 7; f++) W[f] = sW[q1][f][t];
@@ -454,8 +473,8 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 for (int r = 0; r < 6; r++) {
                     const int f = rslot<1>(r);
                     const real w0 = sW[q1][f][t], w1 = sW[q1][f][tp1], lm = sW[q1][f][tm], l2 = sW[q1][f][tp2];
-                    const real wm = ty == 0 ? gy[r] : lm;
-                    const real w2 = ty == TY - 2 ? gy[r] : l2;
+                    const real wm = lb(LB_Y0) ? gy[r] : lm;
+                    const real w2 = lb(LB_YE) ? gy[r] : l2;
                     sl[r] = w0 + real(0.5) * mc_slope(wm, w0, w1, beta);
                     sr[r] = w1 - real(0.5) * mc_slope(w0, w1, w2, beta);
                 }
@@ -477,8 +496,8 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 for (int r = 0; r < 6; r++) {
                     const int f = rslot<0>(r);
                     const real w0 = sW[q1][f][t], lp = lane_prev(w0), ln = lane_next(w0);
-                    const real wm = tx == 0 ? gx[r] : lp;
-                    const real wp = tx == TX - 1 ? gx[r] : ln;
+                    const real wm = lb(LB_X0) ? gx[r] : lp;
+                    const real wp = lb(LB_XE) ? gx[r] : ln;
                     const real d = mc_slope(wm, w0, wp, beta);
                     sl[r] = w0 + real(0.5) * d;
                     rr[r] = w0 - real(0.5) * d;
@@ -507,9 +526,9 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         __syncthreads();
         // D: corner EMFs of index k, x/y flux differences.  The update's inputs at t^n are
         // loaded first, so that their latency overlaps the corner work and the barrier.
-        const int q0 = k & 1, q1 = (k + 1) & 1, eq = ONE ? 0 : q0;
-        const bool upd = k >= ks && u_ij && inr(k, a.u_lo, n2 + a.u_hi);
-        const bool ccb = ccb_ij && k >= a.u_lo + sh;
+        const int q0 = k & 1, q1 = (k + 1) & 1, eq = NBE == 1 ? 0 : q0;
+        const bool upd = k >= ks && lb(LB_U) && inr(k, a.u_lo, n2 + a.u_hi);
+        const bool ccb = lb(LB_CCB) && k >= a.u_lo + sh;
         real pq[4] = {real(0), real(0), real(0), real(0)}, pc[4] = {real(0), real(0), real(0), real(0)};
         double pb[3] = {0.0, 0.0, 0.0}, pbm[3] = {0.0, 0.0, 0.0};
         if (upd) {
@@ -528,7 +547,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 if (!have_bzm) pbm[2] = sb[2][c - sz];
             }
         }
-        if (live && c_ij && inr(k, a.c_lo, n2 + a.c_hi)) {
+        if (live && lb(LB_C) && inr(k, a.c_lo, n2 + a.c_hi)) {
             const double vx = 0.5 * ((double)sC[q0][3][t] + (double)sC[q0][3][t + 1]);
             const double vy = 0.5 * ((double)sC[q0][4][t] + (double)sC[q0][4][t + TX]);
             const double vz = 0.5 * ((double)vz_lo + (double)vz_hi);
@@ -556,7 +575,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             }
         }
         if (ONE) {   // plane k+1, while its fluxes are in the single buffer
-            if (u_ij) {
+            if (lb(LB_U)) {
 #pragma unroll
                 for (int f = 0; f < 4; f++) {
                     const real dx = SH ? dxn[f] : sF[0][0][f][t] - sF[0][0][f][t - 1];
@@ -597,7 +616,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             }
             const double ex = sEc[eq][0][t], ey = sEc[eq][1][t], ez = sEz[t];
             const double ez_y = sEz[t - TX], ez_x = sEz[t - 1], ey_x = sEc[eq][1][t - 1], ex_y = sEc[eq][0][t - TX];
-            if (!ONE) {
+            if (NBE == 2) {
                 const int qm = (k - 1) & 1;
                 ey_m = sEc[qm][1][t]; ex_m = sEc[qm][0][t]; eyx_m = sEc[qm][1][t - 1]; exy_m = sEc[qm][0][t - TX];
             }
@@ -646,7 +665,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                     }
                 }
             }
-            if (KIND != 0 && a.cmax != nullptr && int_ij && k >= 1 && k <= n2) {   // corrector only
+            if (KIND != 0 && a.cmax != nullptr && lb(LB_INT) && k >= 1 && k <= n2) {   // corrector only
                 if constexpr (ONE) {
                     cfl = fmax(cfl, cell_fast_speed_f(nq, (float)a.cs));
                 } else {
@@ -657,7 +676,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             }
         } else {
             have_bzm = false;
-            if (ONE && u_ij) { ey_m = sEc[0][1][t]; ex_m = sEc[0][0][t]; eyx_m = sEc[0][1][t - 1]; exy_m = sEc[0][0][t - TX]; }
+            if (NBE == 1 && lb(LB_U)) { ey_m = sEc[0][1][t]; ex_m = sEc[0][0][t]; eyx_m = sEc[0][1][t - 1]; exy_m = sEc[0][0][t - TX]; }
         }
 #pragma unroll
         for (int f = 0; f < 4; f++) {
