@@ -28,15 +28,15 @@ namespace imhd {
 
 // waves per SIMD the register allocation must allow: in FP32 the LDS of a 16 x 16 tile
 // (40 KB) fits 4 workgroups per CU and that of a 32 x 16 tile (80 KB) 2, 16 waves per CU
-// either way; FP64 needs more of both and gets 2 waves per SIMD; the PPM correctors
-// (KIND 2, 3: six-cell stencils, the characteristic one with 6 x 6 eigenvector matrices)
-// get 2 in FP32 too, rather than spill at 128 registers; the shared-slope PLM corrector
+// either way; FP64 needs more of both and gets 2 waves per SIMD; the characteristic PPM
+// corrector (KIND 3: six-cell stencils and 6 x 6 eigenvector matrices, about 210
+// registers) gets 2 in FP32 too, rather than spill at 128; the shared-slope PLM corrector
 // (KIND 4) gets 3 on 16 x 16 tiles (three workgroups per CU) and 4 otherwise
 template <class real, int KIND, int N>
 constexpr int vs_waves() {
     if constexpr (sizeof(real) == 8) return 1;
     if constexpr (KIND == 4) return N == 256 ? 3 : 4;   // 16 x 16: 168 registers, three workgroups per CU
-    return KIND == 2 || KIND == 3 ? 2 : 4;
+    return KIND == 3 ? 2 : 4;
 }
 #define VS_WAVES (vs_waves<real, KIND, TX * TY>())
 
@@ -182,11 +182,11 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     __shared__ real sEf[2][2][N];      // [plane parity][x face: E(2,0); y face: E(2,1)]
     __shared__ real sEz3[2][N];        // z face k+1/2: E(1,1), E(0,1)
     __shared__ ctype sC[2][5][N];      // [plane parity][centre Ex, Ey, Ez, m_x/rho, m_y/rho]
-    // corner EMFs, summed in FP64; SH stores them in FP32 (their inputs, the FP32 face and
-    // centre EMFs, carry FP32 rounding already) and keeps two buffers of Ex, Ey, so that the
-    // update reads the index k-1 corners instead of carrying them in registers
-    using etype = typename std::conditional<SH, float, double>::type;
-    constexpr int NBE = SH ? 2 : NB;
+    // corner EMFs, summed in FP64; FP32 stores them in FP32 (their inputs, the FP32 face
+    // EMFs, carry FP32 rounding already) and keeps two buffers of Ex, Ey, so that the update
+    // reads the index k-1 corners instead of carrying them in registers
+    using etype = typename std::conditional<ONE, float, double>::type;
+    constexpr int NBE = ONE ? 2 : NB;
     __shared__ etype sEz[N];           // corner Ez of plane k
     __shared__ etype sEc[NBE][2][N];   // [buffer][Ex, Ey] corner EMFs of index k (at k+1/2)
     __shared__ real sW[SH ? 2 : 1][SH ? 7 : 1][SH ? N : 1];   // SH: [plane parity][primitive, global order]
@@ -234,10 +234,10 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     const int sh = a.ccb_lo_shift;
     const bool ccb_ij = i >= a.u_lo + sh && j >= a.u_lo + sh;
     const bool int_ij = i >= 1 && j >= 1 && i <= n0 && j <= n1;
-    // The per-lane conditions as bits of one vector register.  The shared-slope corrector
-    // re-reads it every plane (an empty asm marks it as rewritten), so that its many
-    // conditions are formed where used instead of being held as 64-bit lane masks in scalar
-    // registers across the march, which the allocator would spill.
+    // The per-lane conditions as bits of one vector register.  FP32 re-reads it every plane
+    // (an empty asm marks it as rewritten), so that the conditions are formed where used
+    // instead of being held as 64-bit lane masks in scalar registers across the march, which
+    // the allocator would spill.
     enum { LB_CELL, LB_FX, LB_FY, LB_FZ, LB_C, LB_U, LB_CCB, LB_INT, LB_X0, LB_XE, LB_Y0, LB_YE, LB_XF, LB_YF };
     unsigned lane_bits = (unsigned)cell_ij << LB_CELL | (unsigned)fx_ij << LB_FX | (unsigned)fy_ij << LB_FY |
                          (unsigned)fz_ij << LB_FZ | (unsigned)c_ij << LB_C | (unsigned)u_ij << LB_U |
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     // the thread's own cell of plane p of the flux state, global field order: loaded once
     // per plane for the centre values, the three faces and (predictor) the update
     auto own = [&](int p, real* U) {
-        if (cell_ij && inr(p, 1 - nb, n2 + nb)) {
+        if (lb(LB_CELL) && inr(p, 1 - nb, n2 + nb)) {
             const long long c = cell_index(B, i, j, p);
 #pragma unroll
             for (int f = 0; f < 7; f++) U[f] = a.qf[f][c];
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     auto cells = [&](int p, const real* U) -> double {
         double vz = 0.0;
         const int q = p & 1;
-        if (cell_ij && inr(p, 1 - nb, n2 + nb)) {
+        if (lb(LB_CELL) && inr(p, 1 - nb, n2 + nb)) {
             const double r = (double)U[0], m1 = (double)U[1], m2 = (double)U[2], m3 = (double)U[3],
                          b4 = (double)U[4], b5 = (double)U[5], b6 = (double)U[6];
             const double ir = 1.0 / r;
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         const long long c = cell_index(B, i, j, p);
         const bool pt = inr(p, a.ft_lo, n2 + a.ft_hi);
         real F[7];
-        if (fx_ij && pt) {
+        if (lb(LB_FX) && pt) {
             sm = fmax(sm, stage_face_own<real, RS, FK, 0>(sw0, U, c, sx, F));
             sF[fq][0][0][t] = F[0]; sF[fq][0][1][t] = F[1]; sF[fq][0][2][t] = F[2]; sF[fq][0][3][t] = F[3];
             sEf[q][0][t] = -F[5];
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             sEf[q][0][t] = real(0);
             ey_x = real(0);
         }
-        if (fy_ij && pt) {
+        if (lb(LB_FY) && pt) {
             sm = fmax(sm, stage_face_own<real, RS, FK, 1>(sw1, U, c, sy, F));
             sF[fq][FY][0][t] = F[0]; sF[fq][FY][2][t] = F[1]; sF[fq][FY][3][t] = F[2]; sF[fq][FY][1][t] = F[3];
             ex_y = -F[5];
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     };
     // z-face between planes p and p+1
     auto zface = [&](int p, bool live, const real* UL, const real* UR, real* fz, real& sm) {
-        if (live && fz_ij && inr(p, a.fn_lo, n2 + a.fn_hi)) {
+        if (live && lb(LB_FZ) && inr(p, a.fn_lo, n2 + a.fn_hi)) {
             real F[7];
             sm = fmax(sm, stage_face_z<real, RS, FK>(sw2, UL, UR, cell_index(B, i, j, p), sz, F));
             fz[0] = F[0]; fz[3] = F[1]; fz[1] = F[2]; fz[2] = F[3];
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     // contraction is on.
 #pragma unroll 1
     for (int k = ks - 2; k < ke; k++) {
-        if constexpr (SH) asm volatile("" : "+v"(lane_bits));
+        if constexpr (ONE) asm volatile("" : "+v"(lane_bits));
         const bool live = k >= ks - 1;   // uniform over the workgroup
         // A: plane k+1 and the z-face k+1/2
         if constexpr (SH) {
