@@ -203,8 +203,10 @@ __global__ __launch_bounds__(256) void k_ou_columns(int nbox, const BoxDesc* __r
 // and takes the rows (j, k) r = ry, ry + OU_RY, ...; the x-twiddles of its cell stay in
 // registers (KC = K <= 4; KC = 0: any K, loaded per row), and the row's mode sums D are
 // uniform over the workgroup.  Per cell the field a(x) = sum_slot 2 Re(e_x D) is summed in
-// slot order, as ou_field does.
-template <int KC>
+// slot order, as ou_field does (FAST = false, the FP64 state's arithmetic).  FAST (an FP32
+// state) forms each slot's term as two FMAs with the twiddles prescaled by 2 and -2: the
+// same field to FP64 rounding, in 30 instead of 75 operations per cell.
+template <int KC, bool FAST>
 struct OuRowField {
     static constexpr int WC = KC > 0 ? 2 * KC + 1 : 1;
     double xr[WC], xi[WC];
@@ -218,8 +220,8 @@ struct OuRowField {
 #pragma unroll
             for (int s = 0; s < WC; s++) {
                 const double* ex = tx + 2 * ((size_t)s * nx + (i - 1));
-                xr[s] = ex[0];
-                xi[s] = ex[1];
+                xr[s] = FAST ? 2.0 * ex[0] : ex[0];
+                xi[s] = FAST ? -2.0 * ex[1] : ex[1];
             }
         }
     }
@@ -229,12 +231,18 @@ struct OuRowField {
 #pragma unroll
             for (int s = 0; s < WC; s++)
 #pragma unroll
-                for (int d = 0; d < 3; d++) a[d] = a[d] + 2.0 * (xr[s] * D[s * 6 + 2 * d] - xi[s] * D[s * 6 + 2 * d + 1]);
+                for (int d = 0; d < 3; d++) {
+                    if constexpr (FAST) a[d] = fma(xr[s], D[s * 6 + 2 * d], fma(xi[s], D[s * 6 + 2 * d + 1], a[d]));
+                    else a[d] = a[d] + 2.0 * (xr[s] * D[s * 6 + 2 * d] - xi[s] * D[s * 6 + 2 * d + 1]);
+                }
         } else {
             for (int s = 0; s < W; s++) {
                 const double* ex = tx + 2 * ((size_t)s * nx + (i - 1));
                 const double er = ex[0], ei = ex[1];
-                for (int d = 0; d < 3; d++) a[d] = a[d] + 2.0 * (er * D[s * 6 + 2 * d] - ei * D[s * 6 + 2 * d + 1]);
+                for (int d = 0; d < 3; d++) {
+                    if constexpr (FAST) a[d] = fma(2.0 * er, D[s * 6 + 2 * d], fma(-2.0 * ei, D[s * 6 + 2 * d + 1], a[d]));
+                    else a[d] = a[d] + 2.0 * (er * D[s * 6 + 2 * d] - ei * D[s * 6 + 2 * d + 1]);
+                }
             }
         }
     }
@@ -253,7 +261,7 @@ __global__ __launch_bounds__(256) void k_ou_sums(const BoxDesc* __restrict__ box
     const double* col = a.ou.col + a.ou.coloff[b];
     double v[FORCE_NSUM] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (i <= B.n[0]) {
-        const OuRowField<KC> F(a.ou, B, b, i);
+        const OuRowField<KC, sizeof(real) == 4> F(a.ou, B, b, i);
 #pragma unroll 4
         for (int r = (int)blockIdx.y; r < nrow; r += OU_RY) {
             const int k = r / ny, j = r - k * ny;
@@ -285,7 +293,7 @@ __global__ __launch_bounds__(256) void k_ou_kick(const BoxDesc* __restrict__ box
     const double* mean = a.mean + 3 * b;
     double cfl = 0.0;
     if (i <= B.n[0]) {
-        const OuRowField<KC> F(a.ou, B, b, i);
+        const OuRowField<KC, sizeof(real) == 4> F(a.ou, B, b, i);
         const double m0 = mean[0], m1 = mean[1], m2 = mean[2];
 #pragma unroll 2
         for (int r = (int)blockIdx.y; r < nrow; r += OU_RY) {
