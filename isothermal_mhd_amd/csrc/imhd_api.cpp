@@ -324,11 +324,12 @@ StageRanges stage_ranges(const imhd_ctx* c, bool corr) {
 // width - 2); threads 1 .. width-2 update, threads 0 .. width-1 solve faces, threads
 // 0 .. width-2 form corners.  The tile count covers every face, corner and cell of the
 // stage's ranges; the march covers planes zlo-1 .. zhi+1.
-// kind of a fused stage's kernel: vs_kind, with the FP32 PLM corrector on its shared-slope
-// form unless IMHD_VSH=0
+// kind of a fused stage's kernel: vs_kind, with the FP32 PLM corrector (4) and the FP32
+// predictor (5) on their LDS-neighbour forms unless IMHD_VSH=0
 int stage_kind(const imhd_ctx* c, bool corr) {
     const int k = vs_kind(corr, c->cfg.recon);
-    return k == 1 && c->prec == 32 && c->vsh ? 4 : k;
+    if (c->prec == 32 && c->vsh) return k == 1 ? 4 : (k == 0 ? 5 : k);
+    return k;
 }
 int stage_shape(const imhd_ctx* c, bool corr) { return corr ? c->vshape_c : c->vshape; }
 

@@ -140,7 +140,7 @@ struct StageArgs {
 };
 // kind: 0 predictor, 1 PLM corrector, 2 PPM-primitive corrector, 3 PPM-characteristic
 // corrector (vs_kind), 4 FP32 PLM corrector reconstructing from LDS-held primitives with
-// shared slopes
+// shared slopes, 5 FP32 predictor with LDS-held and lane-shifted neighbours
 template <class real, int RS>
 void launch_vl_stage(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan, const StageArgs<real>& a,
                      int kind, int shape);

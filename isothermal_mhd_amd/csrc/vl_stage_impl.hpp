@@ -36,6 +36,7 @@ template <class real, int KIND, int N>
 constexpr int vs_waves() {
     if constexpr (sizeof(real) == 8) return 1;
     if constexpr (KIND == 4) return N == 256 ? 3 : 4;   // 16 x 16: 168 registers, three workgroups per CU
+    if constexpr (KIND == 5) return 4;
     return KIND == 3 ? 2 : 4;
 }
 #define VS_WAVES (vs_waves<real, KIND, TX * TY>())
@@ -146,15 +147,15 @@ __device__ __forceinline__ real plm_solve(const real* sl, const real* sr, real w
 // Direction d's sweep view of the flux state: the rotated slots are the global fields
 // permuted at compile time, so the three views share the seven field pointers (and their
 // scalar registers) instead of passing 21 copies.
-// The predictor's flux state is the state at t^n, whose faces are never the FP32 half-step
-// ones, so its view reads the FP64 faces without testing.
+// The predictor's (kinds 0 and 5) flux state is the state at t^n, whose faces are never the
+// FP32 half-step ones, so its view reads the FP64 faces without testing.
 template <class real, int D, int KIND>
 __device__ __forceinline__ SweepArgs<real> stage_sweep(const StageArgs<real>& a) {
     SweepArgs<real> s{};
 #pragma unroll
     for (int r = 0; r < 7; r++) s.w[r] = r == 4 ? nullptr : a.qf[perm(D, r)];
     s.bn = a.bf[D];
-    s.bnr = KIND == 0 ? nullptr : a.bfr[D];
+    s.bnr = KIND == 0 || KIND == 5 ? nullptr : a.bfr[D];
     s.dir = D;
     s.cs = a.cs;
     s.mc_beta = a.mc_beta;
@@ -175,10 +176,15 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     // differences by lane shift (no x slot in sF), centre values in FP32: 80 KB in all.  Its
     // flux state is the half step, whose normal faces are the FP32 ones in a.bfr.
     constexpr bool SH = ONE && KIND == 4;
-    constexpr int FK = KIND == 4 ? 1 : KIND;   // per-face arithmetic of the other paths
-    constexpr int FY = SH ? 0 : 1;   // sF slot of the y face
-    using ctype = typename std::conditional<SH, float, double>::type;
-    __shared__ real sF[NB][SH ? 1 : 2][4][N];   // [buffer][x, y face][flux, global order]
+    // FP32 predictor (KIND 5) the same way: the own cell of plane k+2 to a plane-parity LDS
+    // buffer, the x-neighbour by DPP lane shift, the y-neighbour from the buffer
+    constexpr bool SHP = ONE && KIND == 5;
+    constexpr bool SHX = SH || SHP;
+    constexpr bool PRED = KIND == 0 || KIND == 5;
+    constexpr int FK = KIND == 4 ? 1 : (KIND == 5 ? 0 : KIND);   // per-face arithmetic of the other paths
+    constexpr int FY = SHX ? 0 : 1;   // sF slot of the y face
+    using ctype = typename std::conditional<SHX, float, double>::type;
+    __shared__ real sF[NB][SHX ? 1 : 2][4][N];   // [buffer][x, y face][flux, global order]
     __shared__ real sEf[2][2][N];      // [plane parity][x face: E(2,0); y face: E(2,1)]
     __shared__ real sEz3[2][N];        // z face k+1/2: E(1,1), E(0,1)
     __shared__ ctype sC[2][5][N];      // [plane parity][centre Ex, Ey, Ez, m_x/rho, m_y/rho]
@@ -189,7 +195,8 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     constexpr int NBE = ONE ? 2 : NB;
     __shared__ etype sEz[N];           // corner Ez of plane k
     __shared__ etype sEc[NBE][2][N];   // [buffer][Ex, Ey] corner EMFs of index k (at k+1/2)
-    __shared__ real sW[SH ? 2 : 1][SH ? 7 : 1][SH ? N : 1];   // SH: [plane parity][primitive, global order]
+    // SH: [plane parity][primitive, global order]; SHP: [plane parity][conserved, global order]
+    __shared__ real sW[SHX ? 2 : 1][SHX ? 7 : 1][SHX ? N : 1];
 
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
     int b = 0;
@@ -398,6 +405,13 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         for (int r = 0; r < 6; r++) uLz[r] = real(0);
         __syncthreads();
     }
+    if constexpr (SHP) {
+        real W[7];
+        load_cell(i, j, ks - 1, W);
+#pragma unroll
+        for (int f = 0; f < 7; f++) sW[(ks - 1) & 1][f][t] = W[f];
+        __syncthreads();
+    }
     // The first pass (k = ks - 2) only forms plane ks - 1.  Every plane's faces and centre
     // values come from this one copy of the loop body, so a plane shared by two z-segments
     // (and the corner EMFs formed from it) gets the same bits in both, also where FMA
@@ -517,6 +531,99 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
 #pragma unroll
                 for (int f = 0; f < 4; f++) dxn[f] = fx[f] - lane_prev(fx[f]);
             }
+        } else if constexpr (SHP) {
+            const int q1 = (k + 1) & 1, q2 = k & 1;   // buffers of planes k+1 and k (refilled with k+2)
+            const int p = k + 1;
+            const bool pt = inr(p, a.ft_lo, n2 + a.ft_hi);
+            const real cs = (real)a.cs;
+            const long long c = cell_index(B, i, j, p);
+            {   // own cell of plane k+2 into the buffer plane k used
+                real u2[7];
+                load_cell(i, j, k + 2, u2);
+#pragma unroll
+                for (int f = 0; f < 7; f++) sW[q2][f][t] = u2[f];
+            }
+#pragma unroll
+            for (int f = 0; f < 7; f++) U1[f] = sW[q1][f][t];   // own cell of plane k+1
+            real F[7];
+            if (live && lb(LB_FZ) && inr(k, a.fn_lo, n2 + a.fn_hi)) {   // z-face k+1/2 from planes k and k+1
+                real uL[7], uR[7];
+#pragma unroll
+                for (int r = 0; r < 7; r++) {
+                    if (r == 4) continue;
+                    uL[r] = U0[perm(2, r)];
+                    uR[r] = U1[perm(2, r)];
+                }
+                uL[4] = uR[4] = (real)a.bf[2][cell_index(B, i, j, k)];
+                real smx = real(0);
+                riemann<RS>(uL, uR, cs, F, smx);
+                sm = fmax(sm, smx);
+                fz_hi[0] = F[0]; fz_hi[3] = F[1]; fz_hi[1] = F[2]; fz_hi[2] = F[3];
+                sEz3[0][t] = -F[5];
+                sEz3[1][t] = F[6];
+            } else {
+                fz_hi[0] = fz_hi[1] = fz_hi[2] = fz_hi[3] = real(0);
+                sEz3[0][t] = sEz3[1][t] = real(0);
+            }
+            vz_hi = ctype(0);
+            if (lb(LB_CELL) && inr(p, 1 - nb, n2 + nb)) {   // centre EMFs and velocities
+                const real ir = fdiv(real(1), U1[0]);
+                const real vx = U1[1] * ir, vy = U1[2] * ir, vz = U1[3] * ir;
+                sC[q1][0][t] = U1[5] * vz - U1[6] * vy;
+                sC[q1][1][t] = U1[6] * vx - U1[4] * vz;
+                sC[q1][2][t] = U1[4] * vy - U1[5] * vx;
+                sC[q1][3][t] = vx;
+                sC[q1][4][t] = vy;
+                vz_hi = vz;
+            }
+            if (lb(LB_FY) && pt && lb(LB_YF)) {   // y-face j+1/2, neighbour from the buffer
+                const int tp1 = t + TX;
+                real uL[7], uR[7];
+#pragma unroll
+                for (int r = 0; r < 7; r++) {
+                    if (r == 4) continue;
+                    uL[r] = U1[perm(1, r)];
+                    uR[r] = sW[q1][perm(1, r)][tp1];
+                }
+                uL[4] = uR[4] = (real)a.bf[1][c];
+                real smx = real(0);
+                riemann<RS>(uL, uR, cs, F, smx);
+                sm = fmax(sm, smx);
+                sF[0][FY][0][t] = F[0]; sF[0][FY][2][t] = F[1]; sF[0][FY][3][t] = F[2]; sF[0][FY][1][t] = F[3];
+                exy_hi = -F[5];
+                sEf[q1][1][t] = F[6];
+            } else {
+                sF[0][FY][0][t] = sF[0][FY][1][t] = sF[0][FY][2][t] = sF[0][FY][3][t] = real(0);
+                sEf[q1][1][t] = real(0);
+                exy_hi = real(0);
+            }
+            {   // x-face i+1/2, neighbour from lane + 1
+                real un[7], fx[4];
+#pragma unroll
+                for (int f = 0; f < 7; f++) un[f] = lane_next(U1[f]);
+                if (lb(LB_FX) && pt && lb(LB_XF)) {
+                    real uL[7], uR[7];
+#pragma unroll
+                    for (int r = 0; r < 7; r++) {
+                        if (r == 4) continue;
+                        uL[r] = U1[r];
+                        uR[r] = un[r];
+                    }
+                    uL[4] = uR[4] = (real)a.bf[0][c];
+                    real smx = real(0);
+                    riemann<RS>(uL, uR, cs, F, smx);
+                    sm = fmax(sm, smx);
+                    fx[0] = F[0]; fx[1] = F[1]; fx[2] = F[2]; fx[3] = F[3];
+                    sEf[q1][0][t] = -F[5];
+                    eyx_hi = F[6];
+                } else {
+                    fx[0] = fx[1] = fx[2] = fx[3] = real(0);
+                    sEf[q1][0][t] = real(0);
+                    eyx_hi = real(0);
+                }
+#pragma unroll
+                for (int f = 0; f < 4; f++) dxn[f] = fx[f] - lane_prev(fx[f]);
+            }
         } else {
             own(k + 1, U1);
             vz_hi = cells(k + 1, U1);
@@ -534,8 +641,8 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         if (upd) {
             const long long c = cell_index(B, i, j, k);
 #pragma unroll
-            for (int f = 0; f < 4; f++) pq[f] = KIND == 0 ? U0[f] : a.src.q[f][c];   // predictor: the flux state is the base
-            if (KIND != 0 && a.comp[0] != nullptr) {
+            for (int f = 0; f < 4; f++) pq[f] = PRED ? U0[f] : a.src.q[f][c];   // predictor: the flux state is the base
+            if (!PRED && a.comp[0] != nullptr) {
 #pragma unroll
                 for (int f = 0; f < 4; f++) pc[f] = a.comp[f][c];
             }
@@ -578,7 +685,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             if (lb(LB_U)) {
 #pragma unroll
                 for (int f = 0; f < 4; f++) {
-                    const real dx = SH ? dxn[f] : sF[0][0][f][t] - sF[0][0][f][t - 1];
+                    const real dx = SHX ? dxn[f] : sF[0][0][f][t] - sF[0][0][f][t - 1];
                     dxy_n[f] = dx + (sF[0][FY][f][t] - sF[0][FY][f][t - TX]);
                 }
             }
@@ -599,7 +706,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 const real q0v = pq[f];
                 oq[f] = q0v;
                 real v;
-                if (KIND != 0 && a.comp[0] != nullptr) {   // Kahan terms: corrector only
+                if (!PRED && a.comp[0] != nullptr) {   // Kahan terms: corrector only
                     real y = -(dtdx * div) - pc[f];
                     v = q0v + y;
                     a.comp[f][c] = (v - q0v) - y;
@@ -608,7 +715,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 }
                 if (f == 0 && v < (real)MIN_DENS) {
                     v = (real)MIN_DENS;
-                    if (KIND != 0 && a.comp[0] != nullptr) a.comp[0][c] = real(0);
+                    if (!PRED && a.comp[0] != nullptr) a.comp[0][c] = real(0);
                     atomicAdd(a.floor_count, 1u);
                 }
                 a.dst.q[f][c] = v;
@@ -623,7 +730,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             double bx = pb[0] - dtd * ((ez - ez_y) - (ey - ey_m));
             double by = pb[1] - dtd * ((ex - ex_m) - (ez - ez_x));
             double bz = pb[2] - dtd * ((ey - ey_x) - (ex - ex_y));
-            if (KIND == 0 && a.bhw[0] != nullptr) {   // FP32 half step: the corrector reads these as FP32
+            if (PRED && a.bhw[0] != nullptr) {   // FP32 half step: the corrector reads these as FP32
                 a.bhw[0][c] = (real)bx;
                 a.bhw[1][c] = (real)by;
                 a.bhw[2][c] = (real)bz;
@@ -641,9 +748,9 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 nq[5] = (real)(0.5 * (by + bym));
                 nq[6] = (real)(0.5 * (bz + bz_m));
             } else {
-                nq[4] = KIND == 0 ? U0[4] : a.src.q[4][c];
-                nq[5] = KIND == 0 ? U0[5] : a.src.q[5][c];
-                nq[6] = KIND == 0 ? U0[6] : a.src.q[6][c];
+                nq[4] = PRED ? U0[4] : a.src.q[4][c];
+                nq[5] = PRED ? U0[5] : a.src.q[5][c];
+                nq[6] = PRED ? U0[6] : a.src.q[6][c];
             }
             a.dst.q[4][c] = nq[4];
             a.dst.q[5][c] = nq[5];
@@ -653,7 +760,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             ey_m = ey; ex_m = ex; eyx_m = ey_x; exy_m = ex_y;
             if (a.pvel[0] != nullptr) {   // passive velocities (copy_velocity's arithmetic)
                 const real io = ONE ? (real)__builtin_amdgcn_rcpf((float)oq[0]) : real(1) / oq[0];
-                if (KIND == 0) {
+                if (PRED) {
 #pragma unroll
                     for (int d = 0; d < 3; d++) a.pvel[d][c] = oq[1 + d] * io;
                 } else {
@@ -665,7 +772,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                     }
                 }
             }
-            if (KIND != 0 && a.cmax != nullptr && lb(LB_INT) && k >= 1 && k <= n2) {   // corrector only
+            if (!PRED && a.cmax != nullptr && lb(LB_INT) && k >= 1 && k <= n2) {   // corrector only
                 if constexpr (ONE) {
                     cfl = fmax(cfl, cell_fast_speed_f(nq, (float)a.cs));
                 } else {
@@ -690,7 +797,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         vz_lo = vz_hi;
     }
     atomic_max_nonneg(a.smax, b, (double)sm);
-    if (KIND != 0 && a.cmax != nullptr) atomic_max_nonneg(a.cmax, b, (double)cfl);
+    if (!PRED && a.cmax != nullptr) atomic_max_nonneg(a.cmax, b, (double)cfl);
 }
 
 // kind: 0 predictor, 1 PLM corrector, 2 PPM-primitive corrector, 3 PPM-characteristic
@@ -740,6 +847,9 @@ int vl_stage_blocks_per_cu(int kind, int shape) {
         case 4:
             if constexpr (sizeof(real) == 4) return vl_stage_blocks_per_cu_k<real, RS, 4>(shape);
             return vl_stage_blocks_per_cu_k<real, RS, 1>(shape);
+        case 5:
+            if constexpr (sizeof(real) == 4) return vl_stage_blocks_per_cu_k<real, RS, 5>(shape);
+            return vl_stage_blocks_per_cu_k<real, RS, 0>(shape);
         default: return vl_stage_blocks_per_cu_k<real, RS, 3>(shape);
     }
 }
@@ -755,6 +865,10 @@ void launch_vl_stage(hipStream_t st, const BoxDesc* boxes, const TilePlan& plan,
         case 4:   // FP32 only; an FP64 context runs the PLM corrector as kind 1
             if constexpr (sizeof(real) == 4) launch_vl_stage_k<real, RS, 4>(st, boxes, plan, a, shape);
             else launch_vl_stage_k<real, RS, 1>(st, boxes, plan, a, shape);
+            break;
+        case 5:   // FP32 only; an FP64 context runs the predictor as kind 0
+            if constexpr (sizeof(real) == 4) launch_vl_stage_k<real, RS, 5>(st, boxes, plan, a, shape);
+            else launch_vl_stage_k<real, RS, 0>(st, boxes, plan, a, shape);
             break;
         default: launch_vl_stage_k<real, RS, 3>(st, boxes, plan, a, shape); break;
     }
