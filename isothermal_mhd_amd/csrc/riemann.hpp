@@ -82,9 +82,18 @@ __device__ __forceinline__ void hlld(const real* uL, const real* uR, real asqr, 
     real FHLL0 = (sR * FL[0] - sL * FR[0] + sR * sL * (uR[0] - uL[0])) * isRL;
     real FHLL1 = (sR * FL[1] - sL * FR[1] + sR * sL * (uR[1] - uL[1])) * isRL;
     const real rho_s = uHLL0, momx_s = uHLL1;
-    const real vx_s = fdiv(FHLL0, rho_s);
-    const real irho_s = fdiv(real(1), rho_s);
-    const real isrho_s = fsqrt(irho_s);
+    // 1/rho*, 1/sqrt(rho*) and sqrt(rho*): FP32 forms all three from one hardware rsqrt
+    real irho_s, isrho_s, srho_s;
+    if constexpr (sizeof(real) == 4) {
+        isrho_s = __builtin_amdgcn_rsqf(rho_s);
+        irho_s = isrho_s * isrho_s;
+        srho_s = rho_s * isrho_s;
+    } else {
+        irho_s = real(1) / rho_s;
+        isrho_s = sqrt(irho_s);
+        srho_s = sqrt(rho_s);
+    }
+    const real vx_s = sizeof(real) == 4 ? FHLL0 * irho_s : FHLL0 / rho_s;
     const real sLs = vx_s - fabs(Bx) * isrho_s;
     const real sRs = vx_s + fabs(Bx) * isrho_s;
     const real zero = real(ZERO);
@@ -126,8 +135,8 @@ __device__ __forceinline__ void hlld(const real* uL, const real* uR, real asqr, 
 #pragma unroll
         for (int m = 0; m < 7; m++) F[m] = FR[m] + sR * (uRs[m] - uR[m]);
     } else {
-        const real X = fsqrt(rho_s) * sgn1(Bx);
-        const real iX = fdiv(real(1), X);
+        const real X = srho_s * sgn1(Bx);
+        const real iX = sizeof(real) == 4 ? isrho_s * sgn1(Bx) : real(1) / X;
         real uC2 = real(0.5) * (uLs[2] + uRs[2] + X * (uRs[5] - uLs[5]));
         real uC3 = real(0.5) * (uLs[3] + uRs[3] + X * (uRs[6] - uLs[6]));
         real uC5 = real(0.5) * (uLs[5] + uRs[5] + iX * (uRs[2] - uLs[2]));
@@ -137,8 +146,8 @@ __device__ __forceinline__ void hlld(const real* uL, const real* uR, real asqr, 
         F[2] = uC2 * vx_s - Bx * uC5;
         F[3] = uC3 * vx_s - Bx * uC6;
         F[4] = real(0);
-        F[5] = vx_s * uC5 - fdiv(uC2 * Bx, rho_s);
-        F[6] = vx_s * uC6 - fdiv(uC3 * Bx, rho_s);
+        F[5] = vx_s * uC5 - (sizeof(real) == 4 ? (uC2 * Bx) * irho_s : (uC2 * Bx) / rho_s);
+        F[6] = vx_s * uC6 - (sizeof(real) == 4 ? (uC3 * Bx) * irho_s : (uC3 * Bx) / rho_s);
     }
 }
 
