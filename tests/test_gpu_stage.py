@@ -153,3 +153,20 @@ def test_fused_stage_fp32_32x32_tiles(monkeypatch):
     for x, y in zip(ref[0], got[0]):
         if x is not None:
             assert np.array_equal(x, y)
+
+
+def test_fused_fp32_lds_stencil_kernels_match_per_face_loads(monkeypatch):
+    """The FP32 predictor and PLM corrector that take their stencils from LDS and lane
+    shifts (the default) against the per-face-load kernels (IMHD_VSH=0), to rounding, with
+    the corrector on 32 x 16 and on 16 x 16 tiles (IMHD_VSHAPE_C=0: three workgroups per
+    CU), on a multi-tile periodic box and thin open ones with every solver; the
+    floor-reset count and the dt sequence agree too."""
+    for case in CASES[:3]:
+        say(f"[gpu] fp32 LDS-stencil kernels vs per-face loads {case}")
+        ref = run(monkeypatch, {"IMHD_VSH": "0"}, *case, precision=32)
+        for env in ({}, {"IMHD_VSHAPE_C": "0"}):
+            got = run(monkeypatch, env, *case, precision=32)
+            for x, y in zip(ref[0][:2], got[0][:2]):
+                assert np.max(np.abs(x - y)) <= 1e-5 * max(1.0, np.max(np.abs(x))), env
+            assert abs(ref[1][1][0] - got[1][1][0]) <= 1e-5 * ref[1][1][0]
+            assert ref[2] == got[2]
