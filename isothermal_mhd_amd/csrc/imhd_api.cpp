@@ -764,7 +764,7 @@ void tvd_step(imhd_ctx* c) {
         a.dt = c->dt;
         launch_tvd_ct_pred<real>(mk(c, c->pl_ghost), a);
     }
-    {
+    if (c->q[3][0] == nullptr) {   // else k_tvd_half3 forms the centre EMF from its own loads
         ProfScope ps(c, "tvd_center_corr", c->pl_tcenter.active);
         TvdCenterArgs<real> a{};
         for (int f = 0; f < 4; f++) a.q[f] = R<real>(c->q[s0][f]);
@@ -776,7 +776,7 @@ void tvd_step(imhd_ctx* c) {
         a.dt = c->dt;
         launch_tvd_center<real>(mk(c, c->pl_tcenter), a);
     }
-    if (c->q[3][0] != nullptr) {   // the three half states in one pass, then the sweeps
+    if (c->q[3][0] != nullptr) {   // the three half states (and centre EMF) in one pass, then the sweeps
         {
             ProfScope ps(c, "tvd_half3", c->pl_thalf.active);
             TvdHalfArgs<real> a{};
@@ -788,6 +788,8 @@ void tvd_step(imhd_ctx* c) {
                 a.Ec[e] = c->Ec[e];
                 for (int f = 0; f < 4; f++) a.Fb[e][f] = R<real>(c->F[e][f]);
                 for (int f = 0; f < 7; f++) o.dst[e][f] = R<real>(c->q[hs[e]][f]);
+                o.bh[e] = c->b[s1][e];
+                o.Ecc[e] = c->Ecc[e];
             }
             a.dt = c->dt;
             a.floor_count = c->floor_count;

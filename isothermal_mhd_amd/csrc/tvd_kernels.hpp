@@ -59,6 +59,10 @@ struct TvdHalfArgs {
 template <class real>
 struct TvdHalfDst {        // the three half states of k_tvd_half3, one set per sweep direction
     real* dst[3][7];
+    // non-null: k_tvd_half3 also forms the corrector centre EMF (k_tvd_center's arithmetic)
+    // on its range [4-nb, n+nb-3] from the half-step faces bh
+    const double* bh[3];
+    double* Ecc[3];
 };
 
 template <class real>

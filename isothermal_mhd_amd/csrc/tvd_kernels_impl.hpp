@@ -488,16 +488,22 @@ __global__ __launch_bounds__(256) void k_tvd_corner(const BoxDesc* __restrict__ 
     const double dx = B.dx, idx2 = 2.0 / dx;
     auto Q = [&](int f, long long p) -> double { return (double)a.q[f][p]; };
     auto QB = [&](int f, long long p) -> double { return (double)a.qb[f][p]; };
+    // x / rho of cell p: FP64 divides (the reference's arithmetic); an FP32 state multiplies
+    // by its hardware reciprocal, formed once per cell
+    auto divr = [&](double x, long long p) -> double {
+        if constexpr (sizeof(real) == 4) return x * (double)__builtin_amdgcn_rcpf(a.q[0][p]);
+        else return x / Q(0, p);
+    };
     auto Ecen = [&](int comp, long long p) -> double {
         if (a.Ecc[0] != nullptr) return a.Ecc[comp][p];
-        if (comp == 0) return (QB(1, p) * Q(3, p) - QB(2, p) * Q(2, p)) / Q(0, p);
-        if (comp == 1) return (QB(2, p) * Q(1, p) - QB(0, p) * Q(3, p)) / Q(0, p);
-        return (QB(0, p) * Q(2, p) - QB(1, p) * Q(1, p)) / Q(0, p);
+        if (comp == 0) return divr(QB(1, p) * Q(3, p) - QB(2, p) * Q(2, p), p);
+        if (comp == 1) return divr(QB(2, p) * Q(1, p) - QB(0, p) * Q(3, p), p);
+        return divr(QB(0, p) * Q(2, p) - QB(1, p) * Q(1, p), p);
     };
     auto E = [&](int comp, int slot, long long p) -> double { return (double)a.E[2 * comp + slot][p]; };
-    const double vx = 0.5 * (Q(1, c) / Q(0, c) + Q(1, c + sx) / Q(0, c + sx));
-    const double vy = 0.5 * (Q(2, c) / Q(0, c) + Q(2, c + sy) / Q(0, c + sy));
-    const double vz = 0.5 * (Q(3, c) / Q(0, c) + Q(3, c + sz) / Q(0, c + sz));
+    const double vx = 0.5 * (divr(Q(1, c), c) + divr(Q(1, c + sx), c + sx));
+    const double vy = 0.5 * (divr(Q(2, c), c) + divr(Q(2, c + sy), c + sy));
+    const double vz = 0.5 * (divr(Q(3, c), c) + divr(Q(3, c + sz), c + sz));
     double a14, a34, b14, b34;
     {   // Ez at (i+1/2, j+1/2, k)
         double e00 = Ecen(2, c), e10 = Ecen(2, c + sx), e01 = Ecen(2, c + sy), e11 = Ecen(2, c + sx + sy);
@@ -525,6 +531,23 @@ __global__ __launch_bounds__(256) void k_tvd_corner(const BoxDesc* __restrict__ 
 // Corrector centre EMF from the unsplit half update of rho, m with the predictor fluxes
 // and the face average of the half-step face fields (isotvd_mod.f90:876-906)
 template <class real>
+__device__ __forceinline__ void tvd_center_calc(const real* q, const real* const (*Fb)[4], const double* const* bh,
+                                                double* const* Ecc, long long c, long long sx, long long sy,
+                                                long long sz, real dtdx) {
+    real h[4];
+#pragma unroll
+    for (int f = 0; f < 4; f++)
+        h[f] = q[f] - dtdx * (Fb[0][f][c] - Fb[0][f][c - sx] + Fb[1][f][c] - Fb[1][f][c - sy] + Fb[2][f][c] -
+                              Fb[2][f][c - sz]);
+    double vx = (double)h[1] / (double)h[0], vy = (double)h[2] / (double)h[0], vz = (double)h[3] / (double)h[0];
+    double Bx = 0.5 * (bh[0][c] + bh[0][c - sx]);
+    double By = 0.5 * (bh[1][c] + bh[1][c - sy]);
+    double Bz = 0.5 * (bh[2][c] + bh[2][c - sz]);
+    Ecc[0][c] = By * vz - Bz * vy;
+    Ecc[1][c] = Bz * vx - Bx * vz;
+    Ecc[2][c] = Bx * vy - By * vx;
+}
+template <class real>
 __global__ __launch_bounds__(256) void k_tvd_center(const BoxDesc* __restrict__ boxes, Plan plan, TvdCenterArgs<real> a) {
     int b, i, j, k;
     if (!plan_item(plan, b, i, j, k)) return;
@@ -532,18 +555,8 @@ __global__ __launch_bounds__(256) void k_tvd_center(const BoxDesc* __restrict__ 
     const long long c = cell_index(B, i, j, k);
     const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
     const real dtdx = (real)(0.5 * a.dt[b] / B.dx);
-    real h[4];
-#pragma unroll
-    for (int f = 0; f < 4; f++)
-        h[f] = a.q[f][c] - dtdx * (a.Fb[0][f][c] - a.Fb[0][f][c - sx] + a.Fb[1][f][c] - a.Fb[1][f][c - sy] +
-                                   a.Fb[2][f][c] - a.Fb[2][f][c - sz]);
-    double vx = (double)h[1] / (double)h[0], vy = (double)h[2] / (double)h[0], vz = (double)h[3] / (double)h[0];
-    double Bx = 0.5 * (a.bh[0][c] + a.bh[0][c - sx]);
-    double By = 0.5 * (a.bh[1][c] + a.bh[1][c - sy]);
-    double Bz = 0.5 * (a.bh[2][c] + a.bh[2][c - sz]);
-    a.Ecc[0][c] = By * vz - Bz * vy;
-    a.Ecc[1][c] = Bz * vx - Bx * vz;
-    a.Ecc[2][c] = Bx * vy - By * vx;
+    const real q[4] = {a.q[0][c], a.q[1][c], a.q[2][c], a.q[3][c]};
+    tvd_center_calc(q, a.Fb, a.bh, a.Ecc, c, sx, sy, sz, dtdx);
 }
 
 // minmod-like multidimensional source of the half updates (isotvd_mod.f90:1208-1212)
@@ -664,6 +677,12 @@ __global__ __launch_bounds__(256) void k_tvd_half3(const BoxDesc* __restrict__ b
         tvd_half_bt_fields(d, f1, f2);
         o.dst[d][f1][c] = bt[d][0];
         o.dst[d][f2][c] = bt[d][1];
+    }
+    if (o.Ecc[0] != nullptr) {   // the corrector centre EMF from the same loads (case 22's range)
+        const int nb = B.nb;
+        auto in = [&](int v, int a) { return v >= 4 - nb && v <= B.n[a] + nb - 3; };
+        if (in(i, 0) && in(j, 1) && in(k, 2))
+            tvd_center_calc(q, a.Fb, o.bh, o.Ecc, c, st[0], st[1], st[2], (real)dtd);
     }
 }
 

@@ -112,11 +112,12 @@ __device__ __forceinline__ real stage_face_z(const SweepArgs<real>& a, const rea
 // lanes active).  A tile row is TX lanes (16 or 32, dividing 64), so within a row these are
 // the x-neighbours; at a row's first and last lane they are not, and those lanes use loaded
 // values instead.
+// (bound_ctrl: the lane without a source lane reads 0; no old value to initialise)
 __device__ __forceinline__ float lane_prev(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float lane_next(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x130, 0xf, 0xf, true));
 }
 
 // MC-beta limited slope sign(dC) min(beta|dL|, beta|dR|, |dC|) of one primitive from three
