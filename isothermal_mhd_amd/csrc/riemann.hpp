@@ -50,15 +50,25 @@ __device__ __forceinline__ void hlle(const real* uL, const real* uR, real asqr, 
     for (int m = 0; m < 7; m++) F[m] = (sR * FL[m] - sL * FR[m] + sR * sL * (uR[m] - uL[m])) * isRL;
 }
 
-// Isothermal HLLD of Mignone (2007) as the reference states it (vanleer_mod.f90:2986-3239)
-template <class real>
-__device__ __forceinline__ void hlld(const real* uL, const real* uR, real asqr, real* F, real& smax) {
+// Isothermal HLLD of Mignone (2007) as the reference states it (vanleer_mod.f90:2986-3239).
+// VGIVEN (FP32 PLM states): the caller passes the interface velocities it formed the
+// momenta from (vl, vr), instead of dividing the momenta by rho again.
+template <class real, bool VGIVEN = false>
+__device__ __forceinline__ void hlld(const real* uL, const real* uR, real asqr, real* F, real& smax,
+                                     const real* vl = nullptr, const real* vr = nullptr) {
     real uavg[7], FL[7], FR[7];
 #pragma unroll
     for (int m = 0; m < 7; m++) uavg[m] = real(0.5) * (uL[m] + uR[m]);
     const real Bx = uL[4];
-    real vxL = fdiv(uL[1], uL[0]), vyL = fdiv(uL[2], uL[0]), vzL = fdiv(uL[3], uL[0]);
-    real vxR = fdiv(uR[1], uR[0]), vyR = fdiv(uR[2], uR[0]), vzR = fdiv(uR[3], uR[0]);
+    if constexpr (sizeof(real) == 4) uavg[4] = Bx;   // both sides carry the face's B_n: exact
+    real vxL, vyL, vzL, vxR, vyR, vzR;
+    if constexpr (VGIVEN) {
+        vxL = vl[0]; vyL = vl[1]; vzL = vl[2];
+        vxR = vr[0]; vyR = vr[1]; vzR = vr[2];
+    } else {
+        vxL = fdiv(uL[1], uL[0]); vyL = fdiv(uL[2], uL[0]); vzL = fdiv(uL[3], uL[0]);
+        vxR = fdiv(uR[1], uR[0]); vyR = fdiv(uR[2], uR[0]); vzR = fdiv(uR[3], uR[0]);
+    }
     phys_flux(uL, asqr, vxL, vyL, vzL, FL);
     phys_flux(uR, asqr, vxR, vyR, vzR, FR);
     real cfL = fast_speed(uL, asqr), cfR = fast_speed(uR, asqr), cfa = fast_speed(uavg, asqr);

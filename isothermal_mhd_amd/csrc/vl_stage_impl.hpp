@@ -141,7 +141,8 @@ __device__ __forceinline__ real plm_solve(const real* sl, const real* sr, real w
     uL[1] = sl[1] * uL[0]; uL[2] = sl[2] * uL[0]; uL[3] = sl[3] * uL[0];
     uR[1] = sr[1] * uR[0]; uR[2] = sr[2] * uR[0]; uR[3] = sr[3] * uR[0];
     real smax = real(0);
-    riemann<RS>(uL, uR, cs, F, smax);
+    if constexpr (RS == RS_HLLD && sizeof(real) == 4) hlld<real, true>(uL, uR, cs * cs, F, smax, sl + 1, sr + 1);
+    else riemann<RS>(uL, uR, cs, F, smax);
     return smax;
 }
 
