@@ -65,7 +65,9 @@ struct OuApplyArgs {
     unsigned long long* cmax;// non-null: fused CFL speed of the forced cells
     double cs;
 };
-constexpr int OU_RY = 1024;  // row groups per x-block of the OU passes (and partial rows of <rho a>)
+// row groups per x-block of the OU passes (and partial rows of <rho a>): 256 x-blocks of
+// 256 cells at 1024^3, four waves per SIMD; more (1024) ran the 1024^3 kick 25 % slower
+constexpr int OU_RY = 256;
 
 void launch_ou_update(hipStream_t st, int nbox, const BoxDesc* boxes, const OuDev& ou, int init);
 void launch_ou_columns(hipStream_t st, int nbox, int maxcols, const BoxDesc* boxes, const OuDev& ou);
