@@ -1007,7 +1007,7 @@ def main():
         "config": {
             "workload": C5_WORKLOAD.format(n=n),
             "cells": n ** 3, "cfl": args.cour,
-            "precision": "FP32 bulk; FP64 face B, corner-EMF sum, CFL reduction, diagnostics; "
+            "precision": "FP32 bulk; FP64 face B and its CT update, CFL reduction, diagnostics; "
                          "Kahan-compensated rho and rho v",
             "step": "VL2 predictor/corrector + CT, OU kick with the CFL speed of the kicked cells, "
                     "ghost fill, device-side dt",

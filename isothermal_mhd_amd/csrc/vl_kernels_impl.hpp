@@ -217,11 +217,11 @@ __global__ __launch_bounds__(256) void k_flux3(const BoxDesc* __restrict__ boxes
     atomic_max_nonneg(a.d[0].smax, b, sm);
 }
 
-__device__ __forceinline__ void upw(double v, double L14, double R14, double L34, double R34, double& o14,
-                                    double& o34) {
-    if (v > ZERO) { o14 = L14; o34 = L34; }
-    else if (v < -ZERO) { o14 = R14; o34 = R34; }
-    else { o14 = 0.5 * (L14 + R14); o34 = 0.5 * (L34 + R34); }
+template <class T>
+__device__ __forceinline__ void upw(T v, T L14, T R14, T L34, T R34, T& o14, T& o34) {
+    if (v > T(ZERO)) { o14 = L14; o34 = L34; }
+    else if (v < -T(ZERO)) { o14 = R14; o34 = R34; }
+    else { o14 = T(0.5) * (L14 + R14); o34 = T(0.5) * (L34 + R34); }
 }
 
 // Gardiner-Stone upwind corner EMFs (vanleer_mod.f90:1884-2076), summed in FP64.  The

@@ -192,8 +192,12 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     __shared__ ctype sC[2][5][N];      // [plane parity][centre Ex, Ey, Ez, m_x/rho, m_y/rho]
     // corner EMFs, summed in FP64; FP32 stores them in FP32 (their inputs, the FP32 face
     // EMFs, carry FP32 rounding already) and keeps two buffers of Ex, Ey, so that the update
-    // reads the index k-1 corners instead of carrying them in registers
+    // reads the index k-1 corners instead of carrying them in registers.  The LDS-stencil
+    // stages, whose centre values are FP32 as well, also sum them in FP32 (mtype): an FP64
+    // sum of FP32 inputs that is rounded to FP32 for storage gains a few FP32 ulps at most
+    // and costs a conversion per input.  The face-field update stays FP64.
     using etype = typename std::conditional<ONE, float, double>::type;
+    using mtype = ctype;
     constexpr int NBE = ONE ? 2 : NB;
     __shared__ etype sEz[N];           // corner Ez of plane k
     __shared__ etype sEc[NBE][2][N];   // [buffer][Ex, Ey] corner EMFs of index k (at k+1/2)
@@ -657,33 +661,34 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             }
         }
         if (live && lb(LB_C) && inr(k, a.c_lo, n2 + a.c_hi)) {
-            const double vx = 0.5 * ((double)sC[q0][3][t] + (double)sC[q0][3][t + 1]);
-            const double vy = 0.5 * ((double)sC[q0][4][t] + (double)sC[q0][4][t + TX]);
-            const double vz = 0.5 * ((double)vz_lo + (double)vz_hi);
+            const mtype vx = mtype(0.5) * ((mtype)sC[q0][3][t] + (mtype)sC[q0][3][t + 1]);
+            const mtype vy = mtype(0.5) * ((mtype)sC[q0][4][t] + (mtype)sC[q0][4][t + TX]);
+            const mtype vz = mtype(0.5) * ((mtype)vz_lo + (mtype)vz_hi);
             // k_corner's corner sums with the factor 2 of its upwinded differences taken out:
             // upw commutes with the exact scaling by 2, so 0.25 (a - b) here has the bits of
             // 0.125 (2a - 2b) there
-            double a14, a34, b14, b34;
+            constexpr mtype qtr = mtype(0.25);
+            mtype a14, a34, b14, b34;
             {   // Ez at (i+1/2, j+1/2, k)
-                double e00 = sC[q0][2][t], e10 = sC[q0][2][t + 1], e01 = sC[q0][2][t + TX], e11 = sC[q0][2][t + TX + 1];
-                double y0 = sEf[q0][1][t], y1 = sEf[q0][1][t + 1], x0 = sEf[q0][0][t], x1 = sEf[q0][0][t + TX];
+                mtype e00 = sC[q0][2][t], e10 = sC[q0][2][t + 1], e01 = sC[q0][2][t + TX], e11 = sC[q0][2][t + TX + 1];
+                mtype y0 = sEf[q0][1][t], y1 = sEf[q0][1][t + 1], x0 = sEf[q0][0][t], x1 = sEf[q0][0][t + TX];
                 upw(vx, y0 - e00, y1 - e10, e01 - y0, e11 - y1, a14, a34);
                 upw(vy, x0 - e00, x1 - e01, e10 - x0, e11 - x1, b14, b34);
-                sEz[t] = 0.25 * (x0 + x1 + y0 + y1) + 0.25 * ((a14 - a34) + (b14 - b34));
+                sEz[t] = qtr * (x0 + x1 + y0 + y1) + qtr * ((a14 - a34) + (b14 - b34));
             }
             {   // Ey at (i+1/2, j, k+1/2)
-                double e00 = sC[q0][1][t], e10 = sC[q0][1][t + 1], e01 = sC[q1][1][t], e11 = sC[q1][1][t + 1];
-                double z0 = sEz3[0][t], z1 = sEz3[0][t + 1], x0 = eyx_lo, x1 = eyx_hi;
+                mtype e00 = sC[q0][1][t], e10 = sC[q0][1][t + 1], e01 = sC[q1][1][t], e11 = sC[q1][1][t + 1];
+                mtype z0 = sEz3[0][t], z1 = sEz3[0][t + 1], x0 = eyx_lo, x1 = eyx_hi;
                 upw(vx, z0 - e00, z1 - e10, e01 - z0, e11 - z1, a14, a34);
                 upw(vz, x0 - e00, x1 - e01, e10 - x0, e11 - x1, b14, b34);
-                sEc[eq][1][t] = 0.25 * (x0 + x1 + z0 + z1) + 0.25 * ((a14 - a34) + (b14 - b34));
+                sEc[eq][1][t] = qtr * (x0 + x1 + z0 + z1) + qtr * ((a14 - a34) + (b14 - b34));
             }
             {   // Ex at (i, j+1/2, k+1/2)
-                double e00 = sC[q0][0][t], e10 = sC[q0][0][t + TX], e01 = sC[q1][0][t], e11 = sC[q1][0][t + TX];
-                double z0 = sEz3[1][t], z1 = sEz3[1][t + TX], y0 = exy_lo, y1 = exy_hi;
+                mtype e00 = sC[q0][0][t], e10 = sC[q0][0][t + TX], e01 = sC[q1][0][t], e11 = sC[q1][0][t + TX];
+                mtype z0 = sEz3[1][t], z1 = sEz3[1][t + TX], y0 = exy_lo, y1 = exy_hi;
                 upw(vy, z0 - e00, z1 - e10, e01 - z0, e11 - z1, a14, a34);
                 upw(vz, y0 - e00, y1 - e01, e10 - y0, e11 - y1, b14, b34);
-                sEc[eq][0][t] = 0.25 * (y0 + y1 + z0 + z1) + 0.25 * ((a14 - a34) + (b14 - b34));
+                sEc[eq][0][t] = qtr * (y0 + y1 + z0 + z1) + qtr * ((a14 - a34) + (b14 - b34));
             }
         }
         if (ONE) {   // plane k+1, while its fluxes are in the single buffer
