@@ -177,14 +177,32 @@ __device__ __forceinline__ float cell_fast_speed_f(const float* q, float cs) {
 template <class real>
 __device__ __forceinline__ real sgn1(real x) { return copysign(real(1), x); }  // Fortran SIGN(1.d0, x)
 
+// Minima and maxima.  FP64 keeps the reference's comparisons, so that ties between +0 and
+// -0 resolve as there; FP32 uses the hardware min/max (v_min3_f32 and v_max3_f32 take three
+// operands in one instruction), which differ from the comparisons only in the sign of a
+// zero result and for NaN operands.
 template <class real>
-__device__ __forceinline__ real min3(real a, real b, real c) { real m = a < b ? a : b; return m < c ? m : c; }
+__device__ __forceinline__ real min3(real a, real b, real c) {
+    if constexpr (sizeof(real) == 4) return __builtin_fminf(__builtin_fminf(a, b), c);
+    real m = a < b ? a : b;
+    return m < c ? m : c;
+}
 template <class real>
-__device__ __forceinline__ real max3(real a, real b, real c) { real m = a > b ? a : b; return m > c ? m : c; }
+__device__ __forceinline__ real max3(real a, real b, real c) {
+    if constexpr (sizeof(real) == 4) return __builtin_fmaxf(__builtin_fmaxf(a, b), c);
+    real m = a > b ? a : b;
+    return m > c ? m : c;
+}
 template <class real>
-__device__ __forceinline__ real max2(real a, real b) { return a > b ? a : b; }
+__device__ __forceinline__ real max2(real a, real b) {
+    if constexpr (sizeof(real) == 4) return __builtin_fmaxf(a, b);
+    return a > b ? a : b;
+}
 template <class real>
-__device__ __forceinline__ real min2(real a, real b) { return a < b ? a : b; }
+__device__ __forceinline__ real min2(real a, real b) {
+    if constexpr (sizeof(real) == 4) return __builtin_fminf(a, b);
+    return a < b ? a : b;
+}
 
 // Non-negative doubles order like their bit patterns, so the CFL/smax maxima use an
 // integer atomicMax.  A wave reduces with shuffles; its lane 0 then updates one of
