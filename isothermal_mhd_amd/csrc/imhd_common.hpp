@@ -45,9 +45,12 @@ struct Plan {
     int sy;                // > 0: rows enumerated in y-strips of sy rows, all k per strip
 };
 
+// The (i, j) part is formed in 32 bits (a padded plane holds far fewer than 2^31 cells), the
+// plane part in 64: for a uniform k that is scalar work, and the per-lane part of a
+// z-marching loop is loop-invariant.
 __device__ __forceinline__ long long cell_index(const BoxDesc& b, int i, int j, int k) {
-    return b.off + (long long)(i + b.nb - 1) +
-           (long long)b.P[0] * ((long long)(j + b.nb - 1) + (long long)b.P[1] * (long long)(k + b.nb - 1));
+    const int in_plane = (i + b.nb - 1) + b.P[0] * (j + b.nb - 1);
+    return b.off + (long long)b.P[0] * (long long)b.P[1] * (long long)(k + b.nb - 1) + (long long)in_plane;
 }
 
 // Workgroup ids are dealt round-robin over the 8 XCDs; renumber them so that each XCD

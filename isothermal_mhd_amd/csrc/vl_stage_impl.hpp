@@ -125,7 +125,8 @@ __device__ __forceinline__ float lane_next(float x) {
 template <class real>
 __device__ __forceinline__ real mc_slope(real wm, real w0, real wp, real beta) {
     const real dL = w0 - wm, dR = wp - w0, dC = real(0.5) * (dL + dR);
-    return sgn1(dC) * min3(beta * fabs(dL), beta * fabs(dR), fabs(dC));
+    // sign(dC) m for m >= 0 is copysign(m, dC), bit for bit (a multiply by +-1 is exact)
+    return copysign(min3(beta * fabs(dL), beta * fabs(dR), fabs(dC)), dC);
 }
 // Primitive slot r (rho, v_n, v_t1, v_t2, B_t1, B_t2) of direction D in a primitive vector
 // in global order (rho, v_x, v_y, v_z, B_x, B_y, B_z)
