@@ -54,7 +54,7 @@ typedef struct imhd_config {
     int bc;             /* IMHD_BC_OPEN / IMHD_BC_PERIODIC */
     int precision;      /* 64: FP64 throughout; 32: FP32 bulk, FP64 face B, corner EMF, CFL, diagnostics */
     int compensated;    /* precision 32 only: Kahan-compensated accumulation of rho, rho v */
-    int flux_protection; /* VL + Roe + precision 64: the reference's flux_protection
+    int flux_protection; /* VL + Roe (precision 64 or 32): the reference's flux_protection
                             (vanleer_mod.f90:81, off there): cells whose updated density falls
                             below MIN_DENS take HLLE fluxes and corner EMFs (:2084-2200) */
     double cs;          /* isothermal sound speed (constants_mod.f90:35) */

@@ -543,8 +543,9 @@ __global__ __launch_bounds__(256) void k_pack_cube(BoxDesc B, CubeArgs<real> a, 
 
 // Flux protection (vanleer_mod.f90:2084-2200), whole padded box.  The first-pass update is
 // update_cell's expression without the floor (final_check_unphysical runs after the
-// recompute).
-__global__ __launch_bounds__(256) void k_prot_flag(const BoxDesc* __restrict__ boxes, Plan plan, ProtArgs a) {
+// recompute), in the context's precision (FP32: the FP32 update's dt/dx and sums).
+template <class real>
+__global__ __launch_bounds__(256) void k_prot_flag(const BoxDesc* __restrict__ boxes, Plan plan, ProtArgs<real> a) {
     int b, i, j, k;
     if (!plan_item(plan, b, i, j, k)) return;
     const BoxDesc& B = boxes[b];
@@ -555,15 +556,16 @@ __global__ __launch_bounds__(256) void k_prot_flag(const BoxDesc* __restrict__ b
     unsigned char f = 0;
     if (in) {
         const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
-        const double dtdx = a.factor * a.dt[b] / B.dx;
-        const double div = (a.F[0][0][c] - a.F[0][0][c - sx]) + (a.F[1][0][c] - a.F[1][0][c - sy]) +
-                           (a.F[2][0][c] - a.F[2][0][c - sz]);
-        f = (a.src[0][c] - dtdx * div) < a.thr ? 1 : 0;
+        const real dtdx = (real)(a.factor * a.dt[b] / B.dx);
+        const real div = (a.F[0][0][c] - a.F[0][0][c - sx]) + (a.F[1][0][c] - a.F[1][0][c - sy]) +
+                         (a.F[2][0][c] - a.F[2][0][c - sz]);
+        f = (a.src[0][c] - dtdx * div) < (real)a.thr ? 1 : 0;
     }
     a.flag[c] = f;
 }
 
-__global__ __launch_bounds__(256) void k_prot_fix(const BoxDesc* __restrict__ boxes, Plan plan, ProtArgs a) {
+template <class real>
+__global__ __launch_bounds__(256) void k_prot_fix(const BoxDesc* __restrict__ boxes, Plan plan, ProtArgs<real> a) {
     int b, i, j, k;
     if (!plan_item(plan, b, i, j, k)) return;
     const BoxDesc& B = boxes[b];
@@ -582,12 +584,20 @@ __global__ __launch_bounds__(256) void k_prot_fix(const BoxDesc* __restrict__ bo
         for (int d = 0; d < 3; d++) a.Ec[d][c] = a.Ecp[d][c];
 }
 
-void launch_prot_flag(const Launch& L, const ProtArgs& a) {
-    if (L.plan.total > 0) hipLaunchKernelGGL(k_prot_flag, dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+template <class real>
+void launch_prot_flag(const Launch& L, const ProtArgs<real>& a) {
+    if (L.plan.total > 0)
+        hipLaunchKernelGGL(k_prot_flag<real>, dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
 }
-void launch_prot_fix(const Launch& L, const ProtArgs& a) {
-    if (L.plan.total > 0) hipLaunchKernelGGL(k_prot_fix, dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
+template <class real>
+void launch_prot_fix(const Launch& L, const ProtArgs<real>& a) {
+    if (L.plan.total > 0)
+        hipLaunchKernelGGL(k_prot_fix<real>, dim3(L.blocks()), dim3(256), 0, L.stream, L.boxes, L.plan, a);
 }
+template void launch_prot_flag<double>(const Launch&, const ProtArgs<double>&);
+template void launch_prot_flag<float>(const Launch&, const ProtArgs<float>&);
+template void launch_prot_fix<double>(const Launch&, const ProtArgs<double>&);
+template void launch_prot_fix<float>(const Launch&, const ProtArgs<float>&);
 
 // ---------------------------------------------------------------------------------
 #define L256(k, ...) hipLaunchKernelGGL(k, dim3(L.blocks()), dim3(256), 0, L.stream, __VA_ARGS__)

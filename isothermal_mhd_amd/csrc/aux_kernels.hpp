@@ -97,15 +97,17 @@ struct CubeArgs {
 };
 
 // Flux protection of the VL Roe update (update_hydro_state_3d with flux_protection,
-// vanleer_mod.f90:2080-2207), FP64.  k_prot_flag marks the cells of the update range whose
+// vanleer_mod.f90:2080-2207), FP64 or FP32 fluxes (the corner EMFs are FP64 in both
+// contexts).  k_prot_flag marks the cells of the update range whose
 // first-pass density falls below thr; k_prot_fix gives every face of a marked cell the
 // protection (HLLE) fluxes and the corner-EMF slots (i,j,k), (i-1,j,k), (i,j-1,k),
 // (i,j,k-1) of a marked cell, all three components, the protection corner EMFs.  The
 // ordinary update then recomputes the range from t^n with the modified arrays.
+template <class real>
 struct ProtArgs {
-    const double* src[4];     // rho, m at the stage's base state (q_3d_prot)
-    double* F[3][4];          // hydro fluxes per direction, global component order
-    const double* Fp[3][4];   // protection fluxes, same layout
+    const real* src[4];       // rho, m at the stage's base state (q_3d_prot)
+    real* F[3][4];            // hydro fluxes per direction, global component order
+    const real* Fp[3][4];     // protection fluxes, same layout
     double* Ec[3];            // corner EMFs
     const double* Ecp[3];     // protection corner EMFs
     unsigned char* flag;      // per cell of the field pool
@@ -114,8 +116,8 @@ struct ProtArgs {
     double thr;               // density threshold (MIN_DENS in the reference)
     int lo, hi_off;           // update range [lo, n + hi_off] per axis
 };
-void launch_prot_flag(const Launch& L, const ProtArgs& a);
-void launch_prot_fix(const Launch& L, const ProtArgs& a);
+template <class real> void launch_prot_flag(const Launch& L, const ProtArgs<real>& a);
+template <class real> void launch_prot_fix(const Launch& L, const ProtArgs<real>& a);
 
 template <class real> void launch_restore_shell(const Launch& L, const ShellArgs<real>& a);
 template <class real> void launch_fill_ghosts(const Launch& L, const GhostArgs<real>& a);

@@ -608,12 +608,12 @@ void protection_corner(imhd_ctx* c, int s, const PlanDev& p) {
 template <class real>
 void protection_fix(imhd_ctx* c, int src, bool corrector) {
     ProfScope ps(c, corrector ? "prot_fix_corr" : "prot_fix_pred", c->pl_ghost.active);
-    ProtArgs a{};
-    for (int f = 0; f < 4; f++) a.src[f] = (const double*)c->q[src][f];
+    ProtArgs<real> a{};
+    for (int f = 0; f < 4; f++) a.src[f] = (const real*)c->q[src][f];
     for (int d = 0; d < 3; d++)
         for (int f = 0; f < 4; f++) {
-            a.F[d][f] = (double*)c->F[d][f];
-            a.Fp[d][f] = (const double*)c->Fpr[d][f];
+            a.F[d][f] = (real*)c->F[d][f];
+            a.Fp[d][f] = (const real*)c->Fpr[d][f];
         }
     for (int d = 0; d < 3; d++) { a.Ec[d] = c->Ec[d]; a.Ecp[d] = c->Ecpr[d]; }
     a.flag = c->pflag;
@@ -2073,9 +2073,10 @@ int imhd_create_ensemble(const imhd_config* cfg, int nbox, imhd_ctx** out) {
         if (vs_shape_ok(g.precision, v)) c->vshape_c = v;
     }
     if (g.flux_protection) {
-        // flux_protection of vanleer_mod.f90 (:81): Roe with HLLE protection fluxes, FP64
-        if (g.integrator != IMHD_VL || g.riemann != IMHD_ROE || g.precision != 64) {
-            int r = bad("flux_protection needs the VL integrator, the Roe solver and precision 64");
+        // flux_protection of vanleer_mod.f90 (:81): Roe with HLLE protection fluxes (FP64 or
+        // FP32; FP32 contexts run the three-kernel FP32 stages with FP64 corner EMFs)
+        if (g.integrator != IMHD_VL || g.riemann != IMHD_ROE) {
+            int r = bad("flux_protection needs the VL integrator and the Roe solver");
             *out = c;
             return r;
         }

@@ -3,7 +3,9 @@ reference's update_hydro_state_3d with flux_protection, vanleer_mod.f90:2080-220
 the oracle's restatement (pinned in tests/test_oracle_protection.py), FP64 bit for bit:
 a rarefaction that the unprotected Roe update drives below the density floor, partial
 marking at a raised threshold (IMHD_PROT_DENS, a test knob), the untriggered case (equal
-to the unprotected step) and the all-marked case (equal to an HLLE step)."""
+to the unprotected step) and the all-marked case (equal to an HLLE step).  FP32 contexts
+(the FP32 three-kernel stages): the same cases against the FP64 oracle in relative L1
+(tolerances at each test), and the two identities bit for bit within FP32."""
 import sys
 
 import numpy as np
@@ -29,12 +31,12 @@ def rarefaction(n, vx0, seed=0):
     return q, bf
 
 
-def gpu(monkeypatch, dims, q, bf, steps, dt, riemann="roe", prot=False, thr=None, recon="plm"):
+def gpu(monkeypatch, dims, q, bf, steps, dt, riemann="roe", prot=False, thr=None, recon="plm", precision=64):
     from isothermal_mhd_amd import VanLeerSolver
     if thr is not None:
         monkeypatch.setenv("IMHD_PROT_DENS", repr(thr))
     try:
-        S = VanLeerSolver(dims, npass=0, riemann=riemann, recon=recon, bc="periodic", precision=64,
+        S = VanLeerSolver(dims, npass=0, riemann=riemann, recon=recon, bc="periodic", precision=precision,
                           flux_protection=prot)
     finally:
         if thr is not None:
@@ -61,6 +63,23 @@ def orc(dims, q, bf, steps, dt, riemann=1, prot=False, thr=0.0, recon=0):
 
 def same(a, b):
     return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def l1_fields(a, b, dims):
+    """Relative L1 over the interior of each field of state a against state b: rho, m_x,
+    m_y, m_z, cell B (3), face B (3)."""
+    nx, ny, nz = dims
+    I = (slice(NB, NB + nx), slice(NB, NB + ny), slice(NB, NB + nz))
+    out = []
+    for x, y in zip(a, b):
+        for f in range(x.shape[-1]):
+            den = np.abs(y[I + (f,)]).sum()
+            out.append(np.abs(x[I + (f,)] - y[I + (f,)]).sum() / den if den > 0 else 0.0)
+    return np.array(out)
+
+
+def l1(a, b, dims):
+    return float(l1_fields(a, b, dims).max())
 
 
 def test_rarefaction_protected_equals_oracle(monkeypatch):
@@ -103,8 +122,56 @@ def test_untriggered_and_all_marked(monkeypatch):
     assert same(allm[0], hlle[0])
 
 
+def test_fp32_rarefaction_protected_vs_oracle(monkeypatch):
+    dims = (16, 8, 8)
+    say("[gpu] protection FP32: rarefaction")
+    q, bf = rarefaction(dims, 10.0)
+    dt = 0.3 / 12.0 / 16
+    g = gpu(monkeypatch, dims, q, bf, 3, dt, prot=True, precision=32)
+    o = orc(dims, q, bf, 3, dt, prot=True)
+    assert g[1] == o[1] == 0
+    e = l1_fields(g[0], o[0], dims)
+    say("[gpu] L1 per field", " ".join(f"{v:.1e}" for v in e))
+    # rho and m_x, which carry the Mach-10 flow: FP32 rounding; the transverse momenta and
+    # field are 10 % random perturbations that the rarefaction thins towards vacuum (the
+    # densest-to-thinnest ratio reaches 500), where their FP32 error grows to ~1e-4 with or
+    # without protection
+    assert e[0] <= 1e-5 and e[1] <= 1e-5
+    assert e.max() <= 1e-3
+    u = gpu(monkeypatch, dims, q, bf, 3, dt, precision=32)      # unprotected: density floor hit
+    assert u[1] > 0
+
+
+def test_fp32_partial_marking_vs_oracle(monkeypatch):
+    dims = (14, 12, 10)
+    say("[gpu] protection FP32: partial marking")
+    q, bf, _ = random_state(*dims, npass=0, seed=9)
+    thr = 0.95
+    g = gpu(monkeypatch, dims, q, bf, 2, 2e-3, prot=True, thr=thr, precision=32)
+    o = orc(dims, q, bf, 2, 2e-3, prot=True, thr=thr)
+    err = l1(g[0], o[0], dims)
+    assert err <= 1e-5
+    ref = gpu(monkeypatch, dims, q, bf, 2, 2e-3, precision=32)
+    assert l1(ref[0], o[0], dims) > 10 * err    # the marking changed the result
+
+
+def test_fp32_untriggered_and_all_marked(monkeypatch):
+    dims = (12, 10, 8)
+    say("[gpu] protection FP32: untriggered / all marked")
+    # a protected context runs the three-kernel stages; the unprotected runs compared with
+    # it take the same stages (in FP32 the fused stages round differently)
+    monkeypatch.setenv("IMHD_VSTAGE", "0")
+    q, bf, _ = random_state(*dims, npass=0, seed=3)
+    plain = gpu(monkeypatch, dims, q, bf, 2, 2e-3, precision=32)
+    prot = gpu(monkeypatch, dims, q, bf, 2, 2e-3, prot=True, precision=32)
+    assert same(plain[0], prot[0])
+    allm = gpu(monkeypatch, dims, q, bf, 2, 2e-3, prot=True, thr=1e300, precision=32)
+    hlle = gpu(monkeypatch, dims, q, bf, 2, 2e-3, riemann="hlle", precision=32)
+    assert same(allm[0], hlle[0])
+
+
 def test_protection_config_checks():
     from isothermal_mhd_amd import ImhdError, VanLeerSolver
-    for kw in (dict(riemann="hlld"), dict(riemann="roe", precision=32)):
+    for kw in (dict(riemann="hlld"), dict(riemann="hlle", precision=32)):
         with pytest.raises(ImhdError, match="IMHD_EINVAL"):
             VanLeerSolver((8, 8, 8), flux_protection=True, **kw)
