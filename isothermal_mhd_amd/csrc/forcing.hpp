@@ -65,8 +65,9 @@ struct OuApplyArgs {
     unsigned long long* cmax;// non-null: fused CFL speed of the forced cells
     double cs;
 };
-// row groups per x-block of the OU passes (and partial rows of <rho a>): 256 x-blocks of
-// 256 cells at 1024^3, four waves per SIMD; more (1024) ran the 1024^3 kick 25 % slower
+// row groups per x-block of the OU kick: 256 x-blocks of 256 cells at 1024^3, four waves
+// per SIMD; more (1024) ran the 1024^3 kick 25 % slower.  The <rho a> pass takes cpx
+// cells per thread and OU_RY cpx row groups.
 constexpr int OU_RY = 256;
 
 void launch_ou_update(hipStream_t st, int nbox, const BoxDesc* boxes, const OuDev& ou, int init);
@@ -75,7 +76,8 @@ template <class real>
 void launch_ou_sums(hipStream_t st, int nbox, int maxnx, const BoxDesc* boxes, const OuApplyArgs<real>& a);
 template <class real>
 void launch_ou_kick(hipStream_t st, int nbox, int maxnx, const BoxDesc* boxes, const OuApplyArgs<real>& a);
-void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, int nbox, int maxnx, const double* part, double* mean);
+void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, int nbox, int maxnx, int K, const double* part,
+                        double* mean);
 size_t ou_part_rows(int nbox, int maxnx);   // partial rows of <rho a> (3 doubles each)
 
 // ---- the reference's random force (generate_rand_force) ----

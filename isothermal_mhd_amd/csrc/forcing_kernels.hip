@@ -212,6 +212,7 @@ struct OuRowField {
     double xr[WC], xi[WC];
     const double* tx;
     int W, nx, i;
+    __device__ __forceinline__ OuRowField() {}
     __device__ __forceinline__ OuRowField(const OuDev& ou, const BoxDesc& B, int b, int i_) : i(i_) {
         W = 2 * ou.K + 1;
         nx = B.n[0];
@@ -249,33 +250,54 @@ struct OuRowField {
 };
 
 // <rho a> partial sums: one partial row of 3 sums per workgroup (box, x-block, row group),
-// reduced in a fixed order by k_ou_finalize
-template <class real, int KC>
-__global__ __launch_bounds__(256) void k_ou_sums(const BoxDesc* __restrict__ boxes, OuApplyArgs<real> a, int nxb) {
+// reduced in a fixed order by k_ou_finalize.  A thread takes CPX cells of each of its rows
+// (x, x + 256, ..), so one fetch of a row's mode sums D (scalar loads, a dependent round
+// trip) serves CPX independent density loads, and a workgroup of 256 CPX cells fetches D
+// once where 256-cell x-blocks fetched it CPX times; the row groups are CPX times as many,
+// so the grid keeps its size.  Per thread the sums run row by row, cells in x order.
+template <class real, int KC, int CPX>
+__global__ __launch_bounds__(256) void k_ou_sums(const BoxDesc* __restrict__ boxes, OuApplyArgs<real> a, int nxb,
+                                                 int nry) {
     __shared__ double sh[FORCE_NSUM][256];
     const int b = blockIdx.z;
     const BoxDesc& B = boxes[b];
-    const int i = (int)blockIdx.x * 256 + (int)threadIdx.x + 1;
+    const int i0 = (int)blockIdx.x * 256 * CPX + (int)threadIdx.x + 1;
     const int ny = B.n[1], nrow = B.n[1] * B.n[2];
     const int W = 2 * a.ou.K + 1;
     const double* col = a.ou.col + a.ou.coloff[b];
     double v[FORCE_NSUM] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (i <= B.n[0]) {
-        const OuRowField<KC, sizeof(real) == 4> F(a.ou, B, b, i);
-#pragma unroll 4
-        for (int r = (int)blockIdx.y; r < nrow; r += OU_RY) {
+    if (i0 <= B.n[0]) {
+        bool act[CPX];
+        int iu[CPX];   // a cell past the box end stands in for the first (its weight is 0)
+#pragma unroll
+        for (int u = 0; u < CPX; u++) {
+            act[u] = i0 + 256 * u <= B.n[0];
+            iu[u] = act[u] ? i0 + 256 * u : i0;
+        }
+        OuRowField<KC, sizeof(real) == 4> F[CPX];
+#pragma unroll
+        for (int u = 0; u < CPX; u++) F[u] = OuRowField<KC, sizeof(real) == 4>(a.ou, B, b, iu[u]);
+        for (int r = (int)blockIdx.y; r < nrow; r += nry) {
             const int k = r / ny, j = r - k * ny;
-            double f[3];
-            F.eval(col + (size_t)r * W * 6, f);
-            const double rho = (double)a.q[0][cell_index(B, i, j + 1, k + 1)];
-            v[0] = v[0] + rho * f[0];
-            v[1] = v[1] + rho * f[1];
-            v[2] = v[2] + rho * f[2];
+            double rho[CPX];
+#pragma unroll
+            for (int u = 0; u < CPX; u++) {
+                const double q0 = (double)a.q[0][cell_index(B, iu[u], j + 1, k + 1)];
+                rho[u] = act[u] ? q0 : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < CPX; u++) {
+                double f[3];
+                F[u].eval(col + (size_t)r * W * 6, f);
+                v[0] = v[0] + rho[u] * f[0];
+                v[1] = v[1] + rho[u] * f[1];
+                v[2] = v[2] + rho[u] * f[2];
+            }
         }
     }
     block_sum5(sh, v);
     if (threadIdx.x < 3)
-        a.part[(((size_t)b * nxb + blockIdx.x) * OU_RY + blockIdx.y) * 3 + threadIdx.x] = sh[threadIdx.x][0];
+        a.part[(((size_t)b * nxb + blockIdx.x) * nry + blockIdx.y) * 3 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // kick: m += dt (rho a - <rho a>) on every interior cell, and (cmax non-null) the CFL
@@ -324,12 +346,11 @@ __global__ __launch_bounds__(256) void k_ou_kick(const BoxDesc* __restrict__ box
 
 // per box: <rho a> = sum / cells, its nxb x OU_RY partial rows in a fixed order
 __global__ __launch_bounds__(256) void k_ou_finalize(const BoxDesc* __restrict__ boxes, const double* part,
-                                                     double* mean, int nxb) {
+                                                     double* mean, int nrows) {
     __shared__ double sh[FORCE_NSUM][256];
     const int b = blockIdx.x;
     const BoxDesc& B = boxes[b];
     const double ncell = (double)B.n[0] * (double)B.n[1] * (double)B.n[2];
-    const int nrows = nxb * OU_RY;
     double v[FORCE_NSUM] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int w = threadIdx.x; w < nrows; w += 256)
         for (int s = 0; s < 3; s++) v[s] = v[s] + part[((size_t)b * nrows + w) * 3 + s];
@@ -349,17 +370,31 @@ void launch_ou_columns(hipStream_t st, int nbox, int maxcols, const BoxDesc* box
 }
 // x-blocks of 256 cells covering the widest box
 static int ou_nxb(int maxnx) { return (maxnx + 255) / 256; }
+// the <rho a> pass: cells per thread (K <= 4: 4 for boxes 1024 cells wide or more, 2 from
+// 512), its x-blocks of 256 cpx cells and its row groups
+static int ou_cpx(int maxnx, int K) { return K < 1 || K > 4 ? 1 : (maxnx >= 1024 ? 4 : (maxnx >= 512 ? 2 : 1)); }
+static int ou_sum_nxb(int maxnx, int cpx) { return (maxnx + 256 * cpx - 1) / (256 * cpx); }
 
+template <class real, int KC>
+static void launch_ou_sums_k(hipStream_t st, dim3 g, int cpx, const BoxDesc* boxes, const OuApplyArgs<real>& a,
+                             int nxb, int nry) {
+    if (KC == 0 || cpx == 1)
+        hipLaunchKernelGGL((k_ou_sums<real, KC, 1>), g, dim3(256), 0, st, boxes, a, nxb, nry);
+    else if (cpx == 2)
+        hipLaunchKernelGGL((k_ou_sums<real, KC, 2>), g, dim3(256), 0, st, boxes, a, nxb, nry);
+    else
+        hipLaunchKernelGGL((k_ou_sums<real, KC, 4>), g, dim3(256), 0, st, boxes, a, nxb, nry);
+}
 template <class real>
 void launch_ou_sums(hipStream_t st, int nbox, int maxnx, const BoxDesc* boxes, const OuApplyArgs<real>& a) {
-    const int nxb = ou_nxb(maxnx);
-    const dim3 g((unsigned)nxb, OU_RY, (unsigned)nbox);
+    const int cpx = ou_cpx(maxnx, a.ou.K), nxb = ou_sum_nxb(maxnx, cpx), nry = OU_RY * cpx;
+    const dim3 g((unsigned)nxb, (unsigned)nry, (unsigned)nbox);
     switch (a.ou.K) {
-        case 1: hipLaunchKernelGGL((k_ou_sums<real, 1>), g, dim3(256), 0, st, boxes, a, nxb); break;
-        case 2: hipLaunchKernelGGL((k_ou_sums<real, 2>), g, dim3(256), 0, st, boxes, a, nxb); break;
-        case 3: hipLaunchKernelGGL((k_ou_sums<real, 3>), g, dim3(256), 0, st, boxes, a, nxb); break;
-        case 4: hipLaunchKernelGGL((k_ou_sums<real, 4>), g, dim3(256), 0, st, boxes, a, nxb); break;
-        default: hipLaunchKernelGGL((k_ou_sums<real, 0>), g, dim3(256), 0, st, boxes, a, nxb); break;
+        case 1: launch_ou_sums_k<real, 1>(st, g, cpx, boxes, a, nxb, nry); break;
+        case 2: launch_ou_sums_k<real, 2>(st, g, cpx, boxes, a, nxb, nry); break;
+        case 3: launch_ou_sums_k<real, 3>(st, g, cpx, boxes, a, nxb, nry); break;
+        case 4: launch_ou_sums_k<real, 4>(st, g, cpx, boxes, a, nxb, nry); break;
+        default: launch_ou_sums_k<real, 0>(st, g, 1, boxes, a, nxb, nry); break;
     }
 }
 template <class real>
@@ -373,10 +408,17 @@ void launch_ou_kick(hipStream_t st, int nbox, int maxnx, const BoxDesc* boxes, c
         default: hipLaunchKernelGGL((k_ou_kick<real, 0>), g, dim3(256), 0, st, boxes, a); break;
     }
 }
-void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, int nbox, int maxnx, const double* part, double* mean) {
-    hipLaunchKernelGGL(k_ou_finalize, dim3(nbox), dim3(256), 0, st, boxes, part, mean, ou_nxb(maxnx));
+void launch_ou_finalize(hipStream_t st, const BoxDesc* boxes, int nbox, int maxnx, int K, const double* part,
+                        double* mean) {
+    const int cpx = ou_cpx(maxnx, K);
+    const int nrows = ou_sum_nxb(maxnx, cpx) * OU_RY * cpx;
+    hipLaunchKernelGGL(k_ou_finalize, dim3(nbox), dim3(256), 0, st, boxes, part, mean, nrows);
 }
-size_t ou_part_rows(int nbox, int maxnx) { return (size_t)nbox * ou_nxb(maxnx) * OU_RY; }
+size_t ou_part_rows(int nbox, int maxnx) {
+    size_t r = 0;
+    for (int cpx = 1; cpx <= 4; cpx *= 2) r = std::max(r, (size_t)ou_sum_nxb(maxnx, cpx) * OU_RY * cpx);
+    return (size_t)nbox * r;
+}
 template void launch_ou_sums<double>(hipStream_t, int, int, const BoxDesc*, const OuApplyArgs<double>&);
 template void launch_ou_sums<float>(hipStream_t, int, int, const BoxDesc*, const OuApplyArgs<float>&);
 template void launch_ou_kick<double>(hipStream_t, int, int, const BoxDesc*, const OuApplyArgs<double>&);

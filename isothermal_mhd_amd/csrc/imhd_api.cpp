@@ -911,7 +911,7 @@ void ou_step(imhd_ctx* c, bool fuse_cmax) {
     int maxnx = 0;
     for (const BoxDesc& B : c->hb) maxnx = std::max(maxnx, B.n[0]);
     launch_ou_sums<real>(c->st, c->nbox, maxnx, c->db, a);
-    launch_ou_finalize(c->st, c->db, c->nbox, maxnx, c->ou_part, c->ou_mean);
+    launch_ou_finalize(c->st, c->db, c->nbox, maxnx, a.ou.K, c->ou_part, c->ou_mean);
     a.cmax = fuse_cmax ? c->cmax : nullptr;
     launch_ou_kick<real>(c->st, c->nbox, maxnx, c->db, a);
 }
