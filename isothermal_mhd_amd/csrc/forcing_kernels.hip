@@ -344,7 +344,8 @@ __global__ __launch_bounds__(256) void k_ou_kick(const BoxDesc* __restrict__ box
     if (a.cmax != nullptr) atomic_max_nonneg(a.cmax, b, cfl);
 }
 
-// per box: <rho a> = sum / cells, its nxb x OU_RY partial rows in a fixed order
+// per box: <rho a> = sum / cells, its nrows partial rows (x-blocks x row groups of the
+// <rho a> pass) in a fixed order
 __global__ __launch_bounds__(256) void k_ou_finalize(const BoxDesc* __restrict__ boxes, const double* part,
                                                      double* mean, int nrows) {
     __shared__ double sh[FORCE_NSUM][256];
