@@ -53,6 +53,27 @@ def gpu_loop(cls, dims, riemann, q, bf, qp, nsteps, graph):
     return out
 
 
+@pytest.mark.parametrize("nx", [512, 1024])
+def test_ou_wide_box_matches_oracle(nx):
+    """Boxes 512 and 1024 cells wide take the <rho a> pass with two and four cells per
+    thread (one fetch of a row's mode sums for 512 or 1024 cells)."""
+    from isothermal_mhd_amd import VanLeerSolver
+    dims, nsteps = (nx, 6, 6), 3
+    say(f"[gpu] OU native loop on a {nx} x 6 x 6 box")
+    q, bf, qp = random_state(*dims, npass=1, seed=92)
+    O, t, dt = oracle_loop(dims, 2, 2, q, bf, qp, nsteps)
+    (qg, bg, pg), (tg, dtg), (kg, ag) = gpu_loop(VanLeerSolver, dims, "hlld", q, bf, qp, nsteps, graph=False)
+    qo, bo, po = O.get_state()[:3]
+    ko, ao = O.ou_modes()
+    assert np.array_equal(kg, ko)
+    assert rel(ag, ao) <= 1e-13
+    for f in range(7):
+        assert rel(qg[..., f], qo[..., f]) <= 1e-10, f
+    for d in range(3):
+        assert rel(bg[..., d], bo[..., d]) <= 1e-10
+    assert abs(tg[0] - t) <= 1e-10 * t and abs(dtg[0] - dt) <= 1e-10 * dt
+
+
 @pytest.mark.parametrize("integ", ["vl", "tvd"])
 def test_ou_native_loop_matches_oracle(integ):
     from isothermal_mhd_amd import TvdSolver, VanLeerSolver
