@@ -25,10 +25,11 @@ HBM traffic (roofline.traffic) comes from two rocprofv3 counter runs of `--pmc-r
 (FETCH_SIZE, then WRITE_SIZE), scaled by what the same counters read for a plain copy of
 known size (k_convert_in<float> of an FP64 set_state) and checked against a read-only
 reduction of known size (the diagnostics pass), reduced by `--pmc-summarize` into
-profiles/pmc_traffic.json:
-  rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -- python3 bench.py --pmc-run
-  rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write -- python3 bench.py --pmc-run
-  python3 bench.py --pmc-summarize gpurun_out/pmc_fetch gpurun_out/pmc_write
+profiles/pmc_traffic.json (`--pmc-map`: where the run records which launch is which
+family, pmc_out/pmc_map.json by default):
+  rocprofv3 --pmc FETCH_SIZE --output-format csv -d pmc_out/pmc_fetch -- python3 bench.py --pmc-run
+  rocprofv3 --pmc WRITE_SIZE --output-format csv -d pmc_out/pmc_write -- python3 bench.py --pmc-run
+  python3 bench.py --pmc-summarize pmc_out/pmc_fetch pmc_out/pmc_write
 """
 import argparse
 import csv
@@ -39,6 +40,7 @@ import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+PMC_MAP = os.path.join(ROOT, "pmc_out", "pmc_map.json")   # --pmc-run writes it, the summaries read it
 sys.path.insert(0, ROOT)
 
 METRIC = "Mcell-updates/s + HBM GB/s % peak, FP32 vs FP64 path at stated L1 error"
@@ -326,7 +328,7 @@ FAMILY_SYMBOL = {"stage_pred": "k_vl_stage", "stage_corr_cfl": "k_vl_stage",
 FAMILY_DISPATCHES = {}
 
 
-def pmc_run(n=1024, steps=3, n_cal=256):
+def pmc_run(n=1024, steps=3, n_cal=256, map_path=None):
     """Short FP32 run of the main workload for counter collection, then the kernels of known
     traffic that calibrate and check the counters, on an n_cal^3 box: the k_convert_in<float>
     dispatches of an FP64 set_state (8 bytes read and 4 written per padded cell, no reuse:
@@ -357,8 +359,9 @@ def pmc_run(n=1024, steps=3, n_cal=256):
            "copy": {"symbol": "k_convert_in<float>", "grid": (nc + 255) // 256 * 256,
                     "bytes_read": nc * 8, "bytes_written": nc * 4},
            "check": {"symbol": "k_diag_part<float>", "bytes_read": 40 * n_cal ** 3}}
-    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
-    with open(os.path.join(ROOT, "gpurun_out", "pmc_map.json"), "w") as f:
+    map_path = map_path or PMC_MAP
+    os.makedirs(os.path.dirname(os.path.abspath(map_path)), exist_ok=True)
+    with open(map_path, "w") as f:
         json.dump(out, f)
     print(json.dumps({"pmc_run": "done", "families": len(fam)}))
     return 0
@@ -406,8 +409,8 @@ def _family_values(rows, mp):
     return out
 
 
-def pmc_summarize(fetch_dir, write_dir):
-    with open(os.path.join(ROOT, "gpurun_out", "pmc_map.json")) as f:
+def pmc_summarize(fetch_dir, write_dir, map_path=None):
+    with open(map_path or PMC_MAP) as f:
         mp = json.load(f)
     fr, wr = _read_counter(fetch_dir, "FETCH_SIZE"), _read_counter(write_dir, "WRITE_SIZE")
     cp = mp["copy"]
@@ -454,11 +457,11 @@ def pmc_summarize(fetch_dir, write_dir):
 VALU_PEAK_GWI = 256 * 4 / 4 * 2.4
 
 
-def sq_summarize(sq_dir):
+def sq_summarize(sq_dir, map_path=None):
     """VALU wave-instructions per launch of each family (SQ_INSTS_VALU from a rocprofv3 run
     of --pmc-run), matched to families as in pmc_summarize; written to
     profiles/sq_valu.json, from which bench.py reports each kernel's VALU issue rate."""
-    with open(os.path.join(ROOT, "gpurun_out", "pmc_map.json")) as f:
+    with open(map_path or PMC_MAP) as f:
         mp = json.load(f)
     per = {c: _family_values(_read_counter(sq_dir, c), mp)
            for c in ("SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES", "SQ_WAVES")}
@@ -960,21 +963,22 @@ def main():
     ap.add_argument("--extra-steps", type=int, default=50)
     ap.add_argument("--pmc-summarize", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--sq-summarize", metavar="SQ_DIR")
+    ap.add_argument("--pmc-map", metavar="PATH", help="launch-to-family map of --pmc-run (default pmc_out/pmc_map.json)")
     ap.add_argument("--big", metavar="N[,N]", help="only the large-box capacity lines, e.g. 512,1024")
     ap.add_argument("--io", type=int, metavar="N", help="only the restart/snapshot file timing of an N^3 box")
     ap.add_argument("--tvd", type=int, metavar="N", help="only the forced K98 Roe-TVD workload at N^3 (FP32, FP64)")
     args = ap.parse_args()
     if args.pmc_summarize:
-        return pmc_summarize(*args.pmc_summarize)
+        return pmc_summarize(*args.pmc_summarize, map_path=args.pmc_map)
     if args.sq_summarize:
-        return sq_summarize(args.sq_summarize)
+        return sq_summarize(args.sq_summarize, map_path=args.pmc_map)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the box would shard by 3-D blocks with an RCCL halo exchange, not implemented)",
               file=sys.stderr)
         return 2
     if args.pmc_run:
-        return pmc_run(args.n)
+        return pmc_run(args.n, map_path=args.pmc_map)
     if args.tvd:
         print(json.dumps({"tvd": run_tvd_big(args.tvd, args.steps if args.steps != 200 else 20, args.cour)}),
               flush=True)
