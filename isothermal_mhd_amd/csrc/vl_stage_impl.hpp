@@ -204,6 +204,10 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     __shared__ etype sEc[NBE][2][N];   // [buffer][Ex, Ey] corner EMFs of index k (at k+1/2)
     // SH: [plane parity][primitive, global order]; SHP: [plane parity][conserved, global order]
     __shared__ real sW[SHX ? 2 : 1][SHX ? 7 : 1][SHX ? N : 1];
+    // SH: the x-edge lanes' outer neighbours (rotated primitives, direction 0), loaded with
+    // the row loads of plane k+2, whose cache lines they share, and read back one plane
+    // later by the same lane: [plane parity][slot][2 ty + (lane at the high edge)]
+    __shared__ real sGx[SH ? 2 : 1][SH ? 6 : 1][SH ? 2 * TY : 1];
 
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
     int b = 0;
@@ -398,6 +402,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     // SH: x-flux differences of plane k+1 (by lane shift), and the upper PLM state of plane k
     // at the z-face k+1/2 (rotated primitives), formed with plane k's slope one step earlier
     real dxn[4] = {real(0), real(0), real(0), real(0)}, uLz[6];
+    const int gslot = SH ? 2 * ty + (lb(LB_XE) ? 1 : 0) : 0;
     if constexpr (SH) {
         real W[7];
         load_cell(i, j, ks - 2, W);
@@ -408,6 +413,13 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         to_prims(W);
 #pragma unroll
         for (int f = 0; f < 7; f++) sW[(ks - 1) & 1][f][t] = W[f];
+        if (lb(LB_X0) || lb(LB_XE)) {
+            real g[6] = {real(0), real(0), real(0), real(0), real(0), real(0)};
+            load_rot(std::integral_constant<int, 0>{}, lb(LB_X0) ? i - 1 : i + 1, j, ks - 1, g);
+            to_prims(g);
+#pragma unroll
+            for (int r = 0; r < 6; r++) sGx[(ks - 1) & 1][r][gslot] = g[r];
+        }
 #pragma unroll
         for (int r = 0; r < 6; r++) uLz[r] = real(0);
         __syncthreads();
@@ -439,12 +451,13 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
             const bool xlive = lb(LB_FX) && pt && lb(LB_XF);   // column TX-1's x-faces by the next tile
             // the phase's HBM loads first, all in flight together: the own cell of plane k+2,
             // the edge lanes' outer neighbours and the faces' normal field (index 0 for idle lanes)
-            real u2[7], gy[6], gx[6];
+            real u2[7], gy[6], gx2[6];
             load_cell(i, j, k + 2, u2);
             gy[0] = gy[1] = gy[2] = gy[3] = gy[4] = gy[5] = real(0);
-            gx[0] = gx[1] = gx[2] = gx[3] = gx[4] = gx[5] = real(0);
+            gx2[0] = gx2[1] = gx2[2] = gx2[3] = gx2[4] = gx2[5] = real(0);
+            const bool xedge = lb(LB_X0) || lb(LB_XE);
+            if (xedge) load_rot(std::integral_constant<int, 0>{}, lb(LB_X0) ? i - 1 : i + 1, j, k + 2, gx2);
             if (lb(LB_Y0) || lb(LB_YE)) load_rot(std::integral_constant<int, 1>{}, i, lb(LB_Y0) ? j - 1 : j + 2, p, gy);
-            if (lb(LB_X0) || lb(LB_XE)) load_rot(std::integral_constant<int, 0>{}, lb(LB_X0) ? i - 1 : i + 1, j, p, gx);
             const real wbz = a.bfr[2][zlive ? cell_index(B, i, j, k) : 0];
             const real wby = a.bfr[1][ylive ? c : 0];
             const real wbx = a.bfr[0][xlive ? c : 0];
@@ -462,6 +475,11 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 }
 #pragma unroll
                 for (int f = 0; f < 7; f++) sW[q2][f][t] = u2[f];
+                if (xedge) {
+                    to_prims(gx2);
+#pragma unroll
+                    for (int r = 0; r < 6; r++) sGx[q2][r][gslot] = gx2[r];
+                }
                 if (zlive) {
                     sm = fmax(sm, plm_solve<real, RS>(uLz, sr, wbz, cs, F));
                     fz_hi[0] = F[0]; fz_hi[3] = F[1]; fz_hi[1] = F[2]; fz_hi[2] = F[3];
@@ -511,8 +529,9 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                 }
             }
             {   // x-face i+1/2: each lane forms its cell's slope; the right state comes from lane + 1
-                real sl[6], rr[6], sr[6], fx[4];
-                to_prims(gx);
+                real sl[6], rr[6], sr[6], fx[4], gx[6];
+#pragma unroll
+                for (int r = 0; r < 6; r++) gx[r] = sGx[q1][r][gslot];   // used by the edge lanes only
 #pragma unroll
                 for (int r = 0; r < 6; r++) {
                     const int f = rslot<0>(r);
