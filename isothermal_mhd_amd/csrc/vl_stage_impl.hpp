@@ -119,6 +119,12 @@ __device__ __forceinline__ float lane_prev(float x) {
 __device__ __forceinline__ float lane_next(float x) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x130, 0xf, 0xf, true));
 }
+__device__ __forceinline__ double lane_prev_d(double x) {
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)v, 0x138, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(v >> 32), 0x138, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
 
 // MC-beta limited slope sign(dC) min(beta|dL|, beta|dR|, |dC|) of one primitive from three
 // cell values (plm_flux's arithmetic)
@@ -402,6 +408,9 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     // SH: x-flux differences of plane k+1 (by lane shift), and the upper PLM state of plane k
     // at the z-face k+1/2 (rotated primitives), formed with plane k's slope one step earlier
     real dxn[4] = {real(0), real(0), real(0), real(0)}, uLz[6];
+    // SHP: the predictor's flux state is its update base, so the face field its x- and
+    // y-faces of plane k+1 read is the update's of the next plane: carried (plane k)
+    double cbx = 0.0, cby = 0.0;
     const int gslot = SH ? 2 * ty + (lb(LB_XE) ? 1 : 0) : 0;
     if constexpr (SH) {
         real W[7];
@@ -439,6 +448,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
     for (int k = ks - 2; k < ke; k++) {
         if constexpr (ONE) asm volatile("" : "+v"(lane_bits));
         const bool live = k >= ks - 1;   // uniform over the workgroup
+        double fbx = 0.0, fby = 0.0, fbz = 0.0;   // SHP: face field at (i, j, k+1) x, y and (i, j, k) z
         // A: plane k+1 and the z-face k+1/2
         if constexpr (SH) {
             const int q1 = (k + 1) & 1, q2 = k & 1;   // buffers of planes k+1 and k (refilled with k+2)
@@ -569,6 +579,9 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
 #pragma unroll
                 for (int f = 0; f < 7; f++) sW[q2][f][t] = u2[f];
             }
+            fbz = a.bf[2][cell_ok(i, j, k) ? cell_index(B, i, j, k) : 0];
+            fbx = a.bf[0][cell_ok(i, j, p) ? c : 0];
+            fby = a.bf[1][cell_ok(i, j, p) ? c : 0];
 #pragma unroll
             for (int f = 0; f < 7; f++) U1[f] = sW[q1][f][t];   // own cell of plane k+1
             real F[7];
@@ -580,7 +593,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                     uL[r] = U0[perm(2, r)];
                     uR[r] = U1[perm(2, r)];
                 }
-                uL[4] = uR[4] = (real)a.bf[2][cell_index(B, i, j, k)];
+                uL[4] = uR[4] = (real)fbz;
                 real smx = real(0);
                 riemann<RS>(uL, uR, cs, F, smx);
                 sm = fmax(sm, smx);
@@ -611,7 +624,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                     uL[r] = U1[perm(1, r)];
                     uR[r] = sW[q1][perm(1, r)][tp1];
                 }
-                uL[4] = uR[4] = (real)a.bf[1][c];
+                uL[4] = uR[4] = (real)fby;
                 real smx = real(0);
                 riemann<RS>(uL, uR, cs, F, smx);
                 sm = fmax(sm, smx);
@@ -635,7 +648,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
                         uL[r] = U1[r];
                         uR[r] = un[r];
                     }
-                    uL[4] = uR[4] = (real)a.bf[0][c];
+                    uL[4] = uR[4] = (real)fbx;
                     real smx = real(0);
                     riemann<RS>(uL, uR, cs, F, smx);
                     sm = fmax(sm, smx);
@@ -664,6 +677,8 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         const bool ccb = lb(LB_CCB) && k >= a.u_lo + sh;
         real pq[4] = {real(0), real(0), real(0), real(0)}, pc[4] = {real(0), real(0), real(0), real(0)};
         double pb[3] = {0.0, 0.0, 0.0}, pbm[3] = {0.0, 0.0, 0.0};
+        double cbx_left = 0.0;   // SHP: lane - 1's carried x face field (all lanes take part)
+        if constexpr (SHP) cbx_left = lane_prev_d(cbx);
         if (upd) {
             const long long c = cell_index(B, i, j, k);
 #pragma unroll
@@ -672,10 +687,21 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
 #pragma unroll
                 for (int f = 0; f < 4; f++) pc[f] = a.comp[f][c];
             }
+            if constexpr (SHP) {
+                pb[0] = cbx;
+                pb[1] = cby;
+                pb[2] = fbz;
+            } else {
 #pragma unroll
-            for (int d = 0; d < 3; d++) pb[d] = sb[d][c];
+                for (int d = 0; d < 3; d++) pb[d] = sb[d][c];
+            }
             if (ccb) {
-                pbm[0] = sb[0][c - sx];
+                if constexpr (SHP) {
+                    if (tx == 0) pbm[0] = sb[0][c - sx];   // a row's first lane: lane - 1 is another row
+                    else pbm[0] = cbx_left;
+                } else {
+                    pbm[0] = sb[0][c - sx];
+                }
                 pbm[1] = sb[1][c - sy];
                 if (!have_bzm) pbm[2] = sb[2][c - sz];
             }
@@ -822,6 +848,7 @@ __global__ __launch_bounds__(TX * TY, VS_WAVES) void k_vl_stage(const BoxDesc* _
         }
 #pragma unroll
         for (int f = 0; f < 7; f++) U0[f] = U1[f];
+        if constexpr (SHP) { cbx = fbx; cby = fby; }
         eyx_lo = eyx_hi;
         exy_lo = exy_hi;
         vz_lo = vz_hi;
