@@ -486,42 +486,84 @@ __global__ __launch_bounds__(256) void k_tvd_corner(const BoxDesc* __restrict__ 
     const long long c = cell_index(B, i, j, k);
     const long long sx = 1, sy = B.P[0], sz = (long long)B.P[0] * B.P[1];
     const double dx = B.dx, idx2 = 2.0 / dx;
-    auto Q = [&](int f, long long p) -> double { return (double)a.q[f][p]; };
-    auto QB = [&](int f, long long p) -> double { return (double)a.qb[f][p]; };
-    // x / rho of cell p: FP64 divides (the reference's arithmetic); an FP32 state multiplies
-    // by its hardware reciprocal, formed once per cell
-    auto divr = [&](double x, long long p) -> double {
-        if constexpr (sizeof(real) == 4) return x * (double)__builtin_amdgcn_rcpf(a.q[0][p]);
-        else return x / Q(0, p);
+    // x / rho: FP64 divides (the reference's arithmetic); an FP32 state multiplies by its
+    // hardware reciprocal
+    auto divr = [](double x, real rho) -> double {
+        if constexpr (sizeof(real) == 4) return x * (double)__builtin_amdgcn_rcpf(rho);
+        else return x / (double)rho;
     };
-    auto Ecen = [&](int comp, long long p) -> double {
-        if (a.Ecc[0] != nullptr) return a.Ecc[comp][p];
-        if (comp == 0) return divr(QB(1, p) * Q(3, p) - QB(2, p) * Q(2, p), p);
-        if (comp == 1) return divr(QB(2, p) * Q(1, p) - QB(0, p) * Q(3, p), p);
-        return divr(QB(0, p) * Q(2, p) - QB(1, p) * Q(1, p), p);
-    };
-    auto E = [&](int comp, int slot, long long p) -> double { return (double)a.E[2 * comp + slot][p]; };
-    const double vx = 0.5 * (divr(Q(1, c), c) + divr(Q(1, c + sx), c + sx));
-    const double vy = 0.5 * (divr(Q(2, c), c) + divr(Q(2, c + sy), c + sy));
-    const double vz = 0.5 * (divr(Q(3, c), c) + divr(Q(3, c + sz), c + sz));
+    // Every load comes before the arithmetic (the only stores go to Ec), so the loads of a
+    // corner triple are in flight together: face EMFs, rho and m of the cells the upwinding
+    // velocities take, and the centre EMFs, read (corrector) or formed from the t^n state of
+    // the seven cells c, +x, +y, +z, +x+y, +x+z, +y+z (predictor, isotvd_mod.f90:947-953).
+    const double ez_y0 = (double)a.E[5][c], ez_y1 = (double)a.E[5][c + sx];
+    const double ez_x0 = (double)a.E[4][c], ez_x1 = (double)a.E[4][c + sy];
+    const double ey_z0 = (double)a.E[3][c], ey_z1 = (double)a.E[3][c + sx];
+    const double ey_x0 = (double)a.E[2][c], ey_x1 = (double)a.E[2][c + sz];
+    const double ex_z0 = (double)a.E[1][c], ex_z1 = (double)a.E[1][c + sy];
+    const double ex_y0 = (double)a.E[0][c], ex_y1 = (double)a.E[0][c + sz];
+    const real r0 = a.q[0][c], r1 = a.q[0][c + sx], r2 = a.q[0][c + sy], r3 = a.q[0][c + sz];
+    const real mx0 = a.q[1][c], mx1 = a.q[1][c + sx];
+    const real my0 = a.q[2][c], my2 = a.q[2][c + sy];
+    const real mz0 = a.q[3][c], mz3 = a.q[3][c + sz];
+    // centre EMFs: Ez at c, +x, +y, +x+y; Ey at c, +x, +z, +x+z; Ex at c, +y, +z, +y+z
+    double ez[4], ey[4], ex[4];
+    if (a.Ecc[0] != nullptr) {
+        const long long pz[4] = {c, c + sx, c + sy, c + sx + sy};
+        const long long py[4] = {c, c + sx, c + sz, c + sx + sz};
+        const long long px[4] = {c, c + sy, c + sz, c + sy + sz};
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            ez[n] = a.Ecc[2][pz[n]];
+            ey[n] = a.Ecc[1][py[n]];
+            ex[n] = a.Ecc[0][px[n]];
+        }
+    } else {
+        const long long p7[7] = {c, c + sx, c + sy, c + sz, c + sx + sy, c + sx + sz, c + sy + sz};
+        real w[7][7];   // rho, m, B of each cell (the values no centre EMF takes are not loaded)
+#pragma unroll
+        for (int n = 0; n < 7; n++) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) w[n][f] = a.q[f][p7[n]];
+#pragma unroll
+            for (int f = 0; f < 3; f++) w[n][4 + f] = a.qb[f][p7[n]];
+        }
+        // (B x m) / rho, component comp, of cell n
+        auto cen = [&](int comp, int n) -> double {
+            const real* u = w[n];
+            if (comp == 0) return divr((double)u[5] * (double)u[3] - (double)u[6] * (double)u[2], u[0]);
+            if (comp == 1) return divr((double)u[6] * (double)u[1] - (double)u[4] * (double)u[3], u[0]);
+            return divr((double)u[4] * (double)u[2] - (double)u[5] * (double)u[1], u[0]);
+        };
+        const int nz[4] = {0, 1, 2, 4}, ny[4] = {0, 1, 3, 5}, nx[4] = {0, 2, 3, 6};
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            ez[n] = cen(2, nz[n]);
+            ey[n] = cen(1, ny[n]);
+            ex[n] = cen(0, nx[n]);
+        }
+    }
+    const double vx = 0.5 * (divr((double)mx0, r0) + divr((double)mx1, r1));
+    const double vy = 0.5 * (divr((double)my0, r0) + divr((double)my2, r2));
+    const double vz = 0.5 * (divr((double)mz0, r0) + divr((double)mz3, r3));
     double a14, a34, b14, b34;
     {   // Ez at (i+1/2, j+1/2, k)
-        double e00 = Ecen(2, c), e10 = Ecen(2, c + sx), e01 = Ecen(2, c + sy), e11 = Ecen(2, c + sx + sy);
-        double y0 = E(2, 1, c), y1 = E(2, 1, c + sx), x0 = E(2, 0, c), x1 = E(2, 0, c + sy);
+        const double e00 = ez[0], e10 = ez[1], e01 = ez[2], e11 = ez[3];
+        const double y0 = ez_y0, y1 = ez_y1, x0 = ez_x0, x1 = ez_x1;
         tvd_upw(vx, idx2 * (y0 - e00), idx2 * (y1 - e10), idx2 * (e01 - y0), idx2 * (e11 - y1), a14, a34);
         tvd_upw(vy, idx2 * (x0 - e00), idx2 * (x1 - e01), idx2 * (e10 - x0), idx2 * (e11 - x1), b14, b34);
         a.Ec[2][c] = 0.25 * (x0 + x1 + y0 + y1) + 0.125 * dx * ((a14 - a34) + (b14 - b34));
     }
     {   // Ey at (i+1/2, j, k+1/2)
-        double e00 = Ecen(1, c), e10 = Ecen(1, c + sx), e01 = Ecen(1, c + sz), e11 = Ecen(1, c + sx + sz);
-        double z0 = E(1, 1, c), z1 = E(1, 1, c + sx), x0 = E(1, 0, c), x1 = E(1, 0, c + sz);
+        const double e00 = ey[0], e10 = ey[1], e01 = ey[2], e11 = ey[3];
+        const double z0 = ey_z0, z1 = ey_z1, x0 = ey_x0, x1 = ey_x1;
         tvd_upw(vx, idx2 * (z0 - e00), idx2 * (z1 - e10), idx2 * (e01 - z0), idx2 * (e11 - z1), a14, a34);
         tvd_upw(vz, idx2 * (x0 - e00), idx2 * (x1 - e01), idx2 * (e10 - x0), idx2 * (e11 - x1), b14, b34);
         a.Ec[1][c] = 0.25 * (x0 + x1 + z0 + z1) + 0.125 * dx * ((a14 - a34) + (b14 - b34));
     }
     {   // Ex at (i, j+1/2, k+1/2)
-        double e00 = Ecen(0, c), e10 = Ecen(0, c + sy), e01 = Ecen(0, c + sz), e11 = Ecen(0, c + sy + sz);
-        double z0 = E(0, 1, c), z1 = E(0, 1, c + sy), y0 = E(0, 0, c), y1 = E(0, 0, c + sz);
+        const double e00 = ex[0], e10 = ex[1], e01 = ex[2], e11 = ex[3];
+        const double z0 = ex_z0, z1 = ex_z1, y0 = ex_y0, y1 = ex_y1;
         tvd_upw(vy, idx2 * (z0 - e00), idx2 * (z1 - e10), idx2 * (e01 - z0), idx2 * (e11 - z1), a14, a34);
         tvd_upw(vz, idx2 * (y0 - e00), idx2 * (y1 - e01), idx2 * (e10 - y0), idx2 * (e11 - y1), b14, b34);
         a.Ec[0][c] = 0.25 * (y0 + y1 + z0 + z1) + 0.125 * dx * ((a14 - a34) + (b14 - b34));
