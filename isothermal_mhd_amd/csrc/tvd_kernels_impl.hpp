@@ -772,46 +772,82 @@ __global__ __launch_bounds__(256) void k_tvd_final(const BoxDesc* __restrict__ b
     const bool interior = i >= 1 && j >= 1 && k >= 1 && i <= B.n[0] && j <= B.n[1] && k <= B.n[2];
     const bool face = i >= 5 - nb && j >= 5 - nb && k >= 5 - nb && i <= B.n[0] + nb - 5 && j <= B.n[1] + nb - 5 &&
                       k <= B.n[2] + nb - 5;
-    const double* Ex = a.Ec[0];
-    const double* Ey = a.Ec[1];
-    const double* Ez = a.Ec[2];
-    auto ctx = [&](long long m) { return a.src.b[0][m] - dtd * ((Ez[m] - Ez[m - sy]) - (Ey[m] - Ey[m - sz])); };
-    auto cty = [&](long long m) { return a.src.b[1][m] - dtd * ((Ex[m] - Ex[m - sz]) - (Ez[m] - Ez[m - sx])); };
-    auto ctz = [&](long long m) { return a.src.b[2][m] - dtd * ((Ey[m] - Ey[m - sx]) - (Ex[m] - Ex[m - sy])); };
+    const bool comp = a.comp[0] != nullptr;
+    // Every load comes before the first store: the kernel reads the t^n set, the fluxes and
+    // the corner EMFs and writes the other set (the Kahan terms only at its own cell), so the
+    // loads of a cell are in flight together.
+    double bs[3];   // t^n faces at c
+#pragma unroll
+    for (int d = 0; d < 3; d++) bs[d] = a.src.b[d][c];
+    real qs[7];
+#pragma unroll
+    for (int f = 0; f < 7; f++) qs[f] = a.src.q[f][c];
+    // corner EMFs: Ez at c, c-y, c-x, c-x-y; Ey at c, c-z, c-x, c-x-z; Ex at c, c-z, c-y, c-y-z
+    double ez[4] = {0.0, 0.0, 0.0, 0.0}, ey[4] = {0.0, 0.0, 0.0, 0.0}, ex[4] = {0.0, 0.0, 0.0, 0.0};
+    if (face || interior) {
+        const double* Ex = a.Ec[0];
+        const double* Ey = a.Ec[1];
+        const double* Ez = a.Ec[2];
+        ez[0] = Ez[c]; ez[1] = Ez[c - sy]; ez[2] = Ez[c - sx]; ez[3] = Ez[c - sx - sy];
+        ey[0] = Ey[c]; ey[1] = Ey[c - sz]; ey[2] = Ey[c - sx]; ey[3] = Ey[c - sx - sz];
+        ex[0] = Ex[c]; ex[1] = Ex[c - sz]; ex[2] = Ex[c - sy]; ex[3] = Ex[c - sy - sz];
+    }
+    double bl[3] = {0.0, 0.0, 0.0};   // t^n lower faces c-x (x), c-y (y), c-z (z)
+    real fc[4][3], fm[4][3], cp[4];    // fluxes at c and at c minus the unit step; Kahan terms
+    if (interior) {
+        bl[0] = a.src.b[0][c - sx];
+        bl[1] = a.src.b[1][c - sy];
+        bl[2] = a.src.b[2][c - sz];
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            fc[f][0] = a.F[0][f][c]; fm[f][0] = a.F[0][f][c - sx];
+            fc[f][1] = a.F[1][f][c]; fm[f][1] = a.F[1][f][c - sy];
+            fc[f][2] = a.F[2][f][c]; fm[f][2] = a.F[2][f][c - sz];
+            cp[f] = comp ? a.comp[f][c] : real(0);
+        }
+    }
+    // update_CT_3d: b - dt/dx ((E1 - E1') - (E2 - E2'))
+    auto ct = [&](double b0, double e1, double e1m, double e2, double e2m) { return b0 - dtd * ((e1 - e1m) - (e2 - e2m)); };
     double bx, by, bz;
-    if (face) { bx = ctx(c); by = cty(c); bz = ctz(c); }
-    else { bx = a.src.b[0][c]; by = a.src.b[1][c]; bz = a.src.b[2][c]; }
+    if (face) {
+        bx = ct(bs[0], ez[0], ez[1], ey[0], ey[1]);
+        by = ct(bs[1], ex[0], ex[1], ez[0], ez[2]);
+        bz = ct(bs[2], ey[0], ey[2], ex[0], ex[2]);
+    } else {
+        bx = bs[0]; by = bs[1]; bz = bs[2];
+    }
     a.dst.b[0][c] = bx;
     a.dst.b[1][c] = by;
     a.dst.b[2][c] = bz;
     if (!interior) {
 #pragma unroll
-        for (int f = 0; f < 7; f++) a.dst.q[f][c] = a.src.q[f][c];
+        for (int f = 0; f < 7; f++) a.dst.q[f][c] = qs[f];
         return;
     }
 #pragma unroll
     for (int f = 0; f < 4; f++) {
-        real div = (a.F[0][f][c] - a.F[0][f][c - sx]) + (a.F[1][f][c] - a.F[1][f][c - sy]) +
-                   (a.F[2][f][c] - a.F[2][f][c - sz]);
-        real q0 = a.src.q[f][c];
+        real div = (fc[f][0] - fm[f][0]) + (fc[f][1] - fm[f][1]) + (fc[f][2] - fm[f][2]);
+        real q0 = qs[f];
         real v;
-        if (a.comp[0] != nullptr) {  // compensated (Kahan) accumulation of the FP32 state
-            real y = -(dtdx * div) - a.comp[f][c];
+        if (comp) {  // compensated (Kahan) accumulation of the FP32 state
+            real y = -(dtdx * div) - cp[f];
             v = q0 + y;
-            a.comp[f][c] = (v - q0) - y;
+            cp[f] = (v - q0) - y;
         } else {
             v = q0 - dtdx * div;
         }
         if (f == 0 && v < (real)MIN_DENS) {
             v = (real)MIN_DENS;
-            if (a.comp[0] != nullptr) a.comp[0][c] = real(0);
+            cp[0] = real(0);
             atomicAdd(a.floor_count, 1u);
         }
         a.dst.q[f][c] = v;
+        if (comp) a.comp[f][c] = cp[f];
     }
-    a.dst.q[4][c] = (real)(0.5 * (bx + ctx(c - sx)));
-    a.dst.q[5][c] = (real)(0.5 * (by + cty(c - sy)));
-    a.dst.q[6][c] = (real)(0.5 * (bz + ctz(c - sz)));
+    // cell-centred B from the new faces; the lower faces recomputed (never read back)
+    a.dst.q[4][c] = (real)(0.5 * (bx + ct(bl[0], ez[2], ez[3], ey[2], ey[3])));
+    a.dst.q[5][c] = (real)(0.5 * (by + ct(bl[1], ex[2], ex[3], ez[1], ez[3])));
+    a.dst.q[6][c] = (real)(0.5 * (bz + ct(bl[2], ey[1], ey[3], ex[1], ex[3])));
 }
 
 // get_max_speed_tvd (isotvd_mod.f90:2607-2673) over [0, n]: fast speed of the
