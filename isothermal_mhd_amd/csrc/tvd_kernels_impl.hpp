@@ -739,19 +739,24 @@ __global__ __launch_bounds__(256) void k_tvd_ct_pred(const BoxDesc* __restrict__
     const int nb = B.nb;
     const bool in = i >= 3 - nb && j >= 3 - nb && k >= 3 - nb && i <= B.n[0] + nb - 3 && j <= B.n[1] + nb - 3 &&
                     k <= B.n[2] + nb - 3;
+    // all loads before the first store (b1 and b0 are different sets)
+    const double b0x = a.b0[0][c], b0y = a.b0[1][c], b0z = a.b0[2][c];
     if (!in) {
-        a.b1[0][c] = a.b0[0][c];
-        a.b1[1][c] = a.b0[1][c];
-        a.b1[2][c] = a.b0[2][c];
+        a.b1[0][c] = b0x;
+        a.b1[1][c] = b0y;
+        a.b1[2][c] = b0z;
         return;
     }
     const double dtd = 0.5 * (a.dt[b] / B.dx);
     const double* Ex = a.Ec[0];
     const double* Ey = a.Ec[1];
     const double* Ez = a.Ec[2];
-    a.b1[0][c] = a.b0[0][c] - dtd * ((Ez[c] - Ez[c - sy]) - (Ey[c] - Ey[c - sz]));
-    a.b1[1][c] = a.b0[1][c] - dtd * ((Ex[c] - Ex[c - sz]) - (Ez[c] - Ez[c - sx]));
-    a.b1[2][c] = a.b0[2][c] - dtd * ((Ey[c] - Ey[c - sx]) - (Ex[c] - Ex[c - sy]));
+    const double ez0 = Ez[c], ez_y = Ez[c - sy], ez_x = Ez[c - sx];
+    const double ey0 = Ey[c], ey_z = Ey[c - sz], ey_x = Ey[c - sx];
+    const double ex0 = Ex[c], ex_z = Ex[c - sz], ex_y = Ex[c - sy];
+    a.b1[0][c] = b0x - dtd * ((ez0 - ez_y) - (ey0 - ey_z));
+    a.b1[1][c] = b0y - dtd * ((ex0 - ex_z) - (ez0 - ez_x));
+    a.b1[2][c] = b0z - dtd * ((ey0 - ey_x) - (ex0 - ex_y));
 }
 
 // update_state_vector_3d + update_CT_3d(.TRUE.) (isotvd_mod.f90:1655-1851): rho, m on the
