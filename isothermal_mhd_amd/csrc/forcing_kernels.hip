@@ -277,6 +277,7 @@ __global__ __launch_bounds__(256) void k_ou_sums(const BoxDesc* __restrict__ box
         OuRowField<KC, sizeof(real) == 4> F[CPX];
 #pragma unroll
         for (int u = 0; u < CPX; u++) F[u] = OuRowField<KC, sizeof(real) == 4>(a.ou, B, b, iu[u]);
+#pragma unroll 2
         for (int r = (int)blockIdx.y; r < nrow; r += nry) {
             const int k = r / ny, j = r - k * ny;
             double rho[CPX];
